@@ -1,0 +1,85 @@
+// Shared host/device helpers for the gfx950 HODLR / HSS engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hmat_amd.h"
+
+namespace hmat {
+
+// Error carried from deep inside the engine to the C ABI boundary.
+struct Error : std::runtime_error {
+  hmat_status code;
+  Error(hmat_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char* what, const char* file, int line) {
+  if (e != hipSuccess) {
+    hmat_status c = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+                        ? HMAT_ERR_OOM : HMAT_ERR_DEVICE;
+    throw Error(c, std::string(what) + ": " + hipGetErrorString(e) + " at " + file +
+                       ":" + std::to_string(line));
+  }
+}
+#define HIP_CHECK(x) ::hmat::hip_check((x), #x, __FILE__, __LINE__)
+#define HMAT_REQUIRE(cond, code, msg) \
+  do { if (!(cond)) throw ::hmat::Error((code), (msg)); } while (0)
+
+// Owning device buffer (typed, 256-byte aligned by hipMalloc).
+template <class T>
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { resize(n); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void resize(size_t n) {
+    if (n == n_) return;
+    release();
+    if (n) HIP_CHECK(hipMalloc(&p_, n * sizeof(T)));
+    n_ = n;
+  }
+  // grow-only resize (keeps the allocation when large enough)
+  void reserve(size_t n) { if (n > n_) resize(n); }
+  void zero(hipStream_t s) { if (n_) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(T), s)); }
+  void upload(const T* h, size_t n, hipStream_t s) {
+    reserve(n);
+    if (n) HIP_CHECK(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+  T* get() const { return p_; }
+  size_t size() const { return n_; }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+template <class T>
+inline std::vector<T> download(const T* d, size_t n, hipStream_t s) {
+  std::vector<T> h(n);
+  if (n) HIP_CHECK(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  return h;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int round_up(int a, int b) { return (int)((a + b - 1) / b) * b; }
+
+}  // namespace hmat

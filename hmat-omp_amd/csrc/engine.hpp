@@ -1,0 +1,123 @@
+// Tree topology and shared level-batched building blocks.
+#pragma once
+#include <cmath>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace hmat {
+
+// Binary cluster tree: mid = (s+e)/2, leaf when len <= leaf_size
+// (hodlr_routines_omp.h:36-43, hss_routines_omp.h:361-374).
+struct TNode {
+  int64_t start = 0, len = 0;
+  int depth = 0, parent = -1, left = -1, right = -1;
+  bool leaf = false;
+  int pos = 0;       // index within its depth's list
+  int leaf_idx = -1;
+  int ref_id = 0;    // HSSNode::id of the reference (both children numbered before recursing)
+};
+
+struct Tree {
+  std::vector<TNode> nodes;               // nodes[0] = root
+  std::vector<std::vector<int>> by_depth; // node ids per depth, in start order
+  std::vector<int> leaves;                // in start order
+  int max_depth = 0;
+  int64_t lmax = 0;
+
+  void build(int64_t N, int64_t leaf) {
+    nodes.clear();
+    by_depth.clear();
+    leaves.clear();
+    nodes.push_back(TNode{});
+    nodes[0].start = 0;
+    nodes[0].len = N;
+    int next_ref = 1;
+    grow(0, leaf, next_ref);
+    max_depth = 0;
+    for (auto& n : nodes) max_depth = std::max(max_depth, n.depth);
+    by_depth.assign(max_depth + 1, {});
+    // depth lists in start order: a pre-order walk visits each depth left to right
+    walk(0);
+    lmax = 0;
+    for (int id : leaves) lmax = std::max(lmax, nodes[id].len);
+  }
+
+ private:
+  void grow(int id, int64_t leaf, int& next_ref) {
+    TNode& n = nodes[id];
+    if (n.len <= leaf) {
+      nodes[id].leaf = true;
+      return;
+    }
+    const int64_t s = n.start, e = n.start + n.len, mid = (s + e) / 2;
+    const int d = n.depth;
+    TNode l, r;
+    l.start = s; l.len = mid - s; l.depth = d + 1; l.parent = id; l.ref_id = next_ref++;
+    r.start = mid; r.len = e - mid; r.depth = d + 1; r.parent = id; r.ref_id = next_ref++;
+    const int li = (int)nodes.size();
+    nodes.push_back(l);
+    nodes.push_back(r);
+    nodes[id].left = li;
+    nodes[id].right = li + 1;
+    grow(li, leaf, next_ref);
+    grow(li + 1, leaf, next_ref);
+  }
+  void walk(int id) {
+    TNode& n = nodes[id];
+    n.pos = (int)by_depth[n.depth].size();
+    by_depth[n.depth].push_back(id);
+    if (n.leaf) {
+      n.leaf_idx = (int)leaves.size();
+      leaves.push_back(id);
+      return;
+    }
+    walk(n.left);
+    walk(n.right);
+  }
+};
+
+inline GenDesc gen_from_kernel(const hmat_kernel& k, const double* dx, const double* dy) {
+  GenDesc g;
+  g.x = dx;
+  g.y = dy;
+  g.p0 = k.p0;
+  g.p1 = k.p1;
+  switch (k.type) {
+    case HMAT_KERNEL_GAUSSIAN: g.kind = GenKind::Gaussian; break;
+    case HMAT_KERNEL_CAUCHY: g.kind = GenKind::Cauchy; break;
+    case HMAT_KERNEL_REF_CAUCHY: g.kind = GenKind::RefCauchy; break;
+    case HMAT_KERNEL_REF_GAUSSIAN: g.kind = GenKind::RefGaussian; break;
+    case HMAT_KERNEL_REF_COULOMB: g.kind = GenKind::RefCoulomb; break;
+    default: throw Error(HMAT_ERR_ARG, "unknown kernel type");
+  }
+  return g;
+}
+
+// Segmented shifted CholeskyQR3 (Fukaya et al., SIAM J. Sci. Comput. 2020):
+// one shifted pass then CholeskyQR2, for each segment of a tall array.
+// Q overwrites `Y` (the result is copied back); optionally R = R3 R2 R1.
+struct OrthSeg {
+  int64_t off;  // row offset of the segment inside the N x kcap arrays
+  int32_t len, k;
+};
+
+class Orthogonalizer {
+ public:
+  // Y, T: device arrays with leading dimension ld, at least kcap columns.
+  // Rout (optional): nseg matrices of kcap x kcap (ld kcap).
+  void run(const std::vector<OrthSeg>& segs, double* Y, double* T, int64_t ld, int kcap,
+           double* Rout, hipStream_t s);
+
+ private:
+  kern::Workspace ws_;
+  DevBuf<double> G_, R_, Ri_, Racc_, Rtmp_, shift_;
+  DevBuf<int> info_;
+  DevBuf<SegNN> dnn_;
+  DevBuf<SegTN> dtn_;
+  DevBuf<SmallMat> dsm_;
+  void pass(const std::vector<OrthSeg>& segs, const double* in, double* out, int64_t ld, int kcap,
+            bool shifted, hipStream_t s);
+};
+
+}  // namespace hmat

@@ -1,0 +1,912 @@
+// gfx950 kernels of the HODLR / HSS engine (FP64 path).
+//
+// Wavefront = 64 lanes; workgroups are 256 threads (4 waves).  The dense
+// contraction of the build (the randomised sketch of every off-diagonal
+// block of a tree level) runs on FP64 MFMA (v_mfma_f64_16x16x4f64) with the
+// kernel-matrix entries generated in registers, so A never exists in HBM.
+// Everything else is tall-skinny and bandwidth-bound.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "kernels.hpp"
+#include "mfma.hpp"
+
+namespace hmat {
+namespace kern {
+
+constexpr int kWG = 256;
+
+// --------------------------------------------------------------------------
+// seg_gemm_nn: Y(seg) = alpha X(seg) T_s + beta Y(seg)
+// One workgroup owns R rows x (G*16) columns of one segment.  Lanes run
+// down the rows (coalesced X and Y); the small per-segment T is read with
+// wave-uniform addresses (scalar loads).
+// --------------------------------------------------------------------------
+struct NNTile {
+  int32_t seg, row0, c0, pad_;
+};
+
+template <int R>
+__global__ __launch_bounds__(kWG) void k_seg_gemm_nn(const SegNN* __restrict__ segs,
+                                                     const NNTile* __restrict__ tiles,
+                                                     const double* __restrict__ X, int64_t ldx,
+                                                     const double* __restrict__ T, int64_t ldt,
+                                                     double* __restrict__ Y, int64_t ldy,
+                                                     double alpha, double beta) {
+  constexpr int G = kWG / R;
+  constexpr int CPT = 16;
+  const NNTile tl = tiles[blockIdx.x];
+  const SegNN sg = segs[tl.seg];
+  const int r = threadIdx.x % R;
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / R);
+  const int row = tl.row0 + r;
+  const bool live = row < sg.len;
+  const int ncols = min(G * CPT, sg.q - tl.c0);
+  double acc[CPT];
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) acc[i] = 0.0;
+  const double* xr = X + sg.xoff + row;
+  const double* tb = T + sg.toff + (int64_t)(tl.c0 + g) * ldt;
+  for (int i = 0; i < sg.p; ++i) {
+    const double xv = live ? xr[(int64_t)i * ldx] : 0.0;
+#pragma unroll
+    for (int cc = 0; cc < CPT; ++cc) {
+      if (g + cc * G < ncols) acc[cc] = fma(xv, tb[i + (int64_t)(cc * G) * ldt], acc[cc]);
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int cc = 0; cc < CPT; ++cc) {
+    const int c = g + cc * G;
+    if (c < ncols) {
+      double* yp = Y + sg.yoff + row + (int64_t)(tl.c0 + c) * ldy;
+      double v = alpha * acc[cc];
+      if (beta != 0.0) v = fma(beta, *yp, v);
+      *yp = v;
+    }
+  }
+}
+
+void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X,
+                 int64_t ldx, const double* T, int64_t ldt, double* Y, int64_t ldy,
+                 double alpha, double beta, hipStream_t s) {
+  if (segs.empty()) return;
+  int qmax = 0;
+  for (auto& g : segs) qmax = std::max(qmax, g.q);
+  if (qmax == 0) return;
+  const int R = qmax <= 16 ? 256 : (qmax <= 32 ? 128 : 64);
+  const int cw = (kWG / R) * 16;
+  std::vector<NNTile> tiles;
+  for (size_t i = 0; i < segs.size(); ++i) {
+    const auto& g = segs[i];
+    if (g.len <= 0 || g.q <= 0) continue;
+    for (int r0 = 0; r0 < g.len; r0 += R)
+      for (int c0 = 0; c0 < g.q; c0 += cw) tiles.push_back({(int32_t)i, r0, c0, 0});
+  }
+  if (tiles.empty()) return;
+  dsegs.upload(segs, s);
+  DevBuf<NNTile> dt;
+  dt.upload(tiles, s);
+  dim3 grid((unsigned)tiles.size());
+  if (R == 256)
+    k_seg_gemm_nn<256><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
+  else if (R == 128)
+    k_seg_gemm_nn<128><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
+  else
+    k_seg_gemm_nn<64><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));  // dt is freed on return
+}
+
+// --------------------------------------------------------------------------
+// seg_gemm_tn: O_s = alpha X(seg)^T Y(seg) + beta O_s, two stages:
+// partial 64x64 products over row chunks, then a deterministic sum.
+// --------------------------------------------------------------------------
+struct TNItem {
+  int32_t seg, r0, r1, pt, qt, slot;
+};
+struct TNOut {
+  int32_t seg, pt, qt, slot0, nslot, pad_;
+};
+
+__global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part(const SegTN* __restrict__ segs,
+                                                          const TNItem* __restrict__ items,
+                                                          const double* __restrict__ X, int64_t ldx,
+                                                          const double* __restrict__ Y, int64_t ldy,
+                                                          double* __restrict__ part) {
+  __shared__ double xs[32][65];
+  __shared__ double ys[32][65];
+  const TNItem it = items[blockIdx.x];
+  const SegTN sg = segs[it.seg];
+  const int ti = threadIdx.x % 16, tj = threadIdx.x / 16;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  const int pc0 = it.pt * 64, qc0 = it.qt * 64;
+  for (int rb = it.r0; rb < it.r1; rb += 32) {
+    for (int e = threadIdx.x; e < 32 * 64; e += kWG) {
+      const int rr = e % 32, cc = e / 32, row = rb + rr;
+      const bool rok = row < it.r1;
+      xs[rr][cc] = (rok && pc0 + cc < sg.p) ? X[sg.xoff + row + (int64_t)(pc0 + cc) * ldx] : 0.0;
+      ys[rr][cc] = (rok && qc0 + cc < sg.q) ? Y[sg.yoff + row + (int64_t)(qc0 + cc) * ldy] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int rr = 0; rr < 32; ++rr) {
+      double xv[4], yv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) xv[a] = xs[rr][ti + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) yv[b] = ys[rr][tj + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = fma(xv[a], yv[b], acc[a][b]);
+    }
+    __syncthreads();
+  }
+  double* pp = part + (int64_t)it.slot * 4096;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) pp[(ti + 16 * a) + (tj + 16 * b) * 64] = acc[a][b];
+}
+
+__global__ __launch_bounds__(kWG) void k_seg_gemm_tn_reduce(const SegTN* __restrict__ segs,
+                                                            const TNOut* __restrict__ outs,
+                                                            const double* __restrict__ part,
+                                                            double* __restrict__ O, int64_t ldo,
+                                                            double alpha, double beta) {
+  const TNOut ot = outs[blockIdx.x];
+  const SegTN sg = segs[ot.seg];
+  for (int e = threadIdx.x; e < 4096; e += kWG) {
+    const int i = e % 64, j = e / 64;
+    const int gi = ot.pt * 64 + i, gj = ot.qt * 64 + j;
+    if (gi >= sg.p || gj >= sg.q) continue;
+    double sum = 0.0;
+    for (int c = 0; c < ot.nslot; ++c) sum += part[(int64_t)(ot.slot0 + c) * 4096 + e];
+    double* o = O + sg.ooff + gi + (int64_t)gj * ldo;
+    double v = alpha * sum;
+    if (beta != 0.0) v = fma(beta, *o, v);
+    *o = v;
+  }
+}
+
+void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const double* X,
+                 int64_t ldx, const double* Y, int64_t ldy, double* O, int64_t ldo,
+                 double alpha, double beta, Workspace& ws, hipStream_t s) {
+  if (segs.empty()) return;
+  int64_t total_rows = 0;
+  for (auto& g : segs) total_rows += std::max(g.len, 0);
+  // ~2048 partial items in total, chunks a multiple of 32 rows
+  int64_t ch = std::max<int64_t>(64, round_up((int)std::min<int64_t>(ceil_div(total_rows, 2048), 1 << 20), 32));
+  std::vector<TNItem> items;
+  std::vector<TNOut> outs;
+  int slot = 0;
+  for (size_t i = 0; i < segs.size(); ++i) {
+    const auto& g = segs[i];
+    if (g.p <= 0 || g.q <= 0) continue;
+    const int npt = (int)ceil_div(g.p, 64), nqt = (int)ceil_div(g.q, 64);
+    for (int pt = 0; pt < npt; ++pt)
+      for (int qt = 0; qt < nqt; ++qt) {
+        TNOut o{(int32_t)i, pt, qt, slot, 0, 0};
+        if (g.len <= 0) {
+          items.push_back({(int32_t)i, 0, 0, pt, qt, slot++});
+          o.nslot = 1;
+        } else {
+          for (int64_t r0 = 0; r0 < g.len; r0 += ch) {
+            items.push_back({(int32_t)i, (int32_t)r0, (int32_t)std::min<int64_t>(r0 + ch, g.len), pt, qt, slot++});
+            o.nslot++;
+          }
+        }
+        outs.push_back(o);
+      }
+  }
+  if (outs.empty()) return;
+  ws.part.reserve((size_t)slot * 4096);
+  dsegs.upload(segs, s);
+  DevBuf<TNItem> di;
+  di.upload(items, s);
+  DevBuf<TNOut> dout;
+  dout.upload(outs, s);
+  k_seg_gemm_tn_part<<<(unsigned)items.size(), kWG, 0, s>>>(dsegs.get(), di.get(), X, ldx, Y, ldy,
+                                                            ws.part.get());
+  HIP_CHECK(hipGetLastError());
+  k_seg_gemm_tn_reduce<<<(unsigned)outs.size(), kWG, 0, s>>>(dsegs.get(), dout.get(), ws.part.get(),
+                                                             O, ldo, alpha, beta);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// --------------------------------------------------------------------------
+// Entry generators.  row_pt/col_pt are hoisted out of the reduction loop.
+// --------------------------------------------------------------------------
+struct GDense {
+  const double* A;
+  int64_t lda;
+  __device__ double row_pt(int64_t) const { return 0.0; }
+  __device__ double col_pt(int64_t) const { return 0.0; }
+  __device__ double eval(double, double, int64_t i, int64_t j) const { return A[i + j * lda]; }
+};
+struct GGaussian {  // exp(-(x_i-x_j)^2 c) + p1 delta, c = 1/(2 sigma^2)
+  const double* x;
+  double c, p1;
+  __device__ double row_pt(int64_t i) const { return x[i]; }
+  __device__ double col_pt(int64_t j) const { return x[j]; }
+  __device__ double eval(double xi, double xj, int64_t i, int64_t j) const {
+    const double d = xi - xj;
+    return exp(-(d * d) * c) + (i == j ? p1 : 0.0);
+  }
+};
+struct GCauchy {  // 1/(x_i - y_j)
+  const double* x;
+  const double* y;
+  __device__ double row_pt(int64_t i) const { return x[i]; }
+  __device__ double col_pt(int64_t j) const { return y[j]; }
+  __device__ double eval(double xi, double yj, int64_t, int64_t) const { return 1.0 / (xi - yj); }
+};
+struct GRefCauchy {  // 1/(x_i + x_j + 1) + p1 delta     (kernel.h:30-48)
+  const double* x;
+  double p1;
+  __device__ double row_pt(int64_t i) const { return x[i]; }
+  __device__ double col_pt(int64_t j) const { return x[j]; }
+  __device__ double eval(double xi, double xj, int64_t i, int64_t j) const {
+    return 1.0 / (xi + xj + 1.0) + (i == j ? p1 : 0.0);
+  }
+};
+struct GRefGaussian {  // exp(-(x_i-x_j)^2 / h^2) + p1 delta   (kernel.h:52-65)
+  const double* x;
+  double h2, p1;
+  __device__ double row_pt(int64_t i) const { return x[i]; }
+  __device__ double col_pt(int64_t j) const { return x[j]; }
+  __device__ double eval(double xi, double xj, int64_t i, int64_t j) const {
+    const double d = xi - xj;
+    return exp(-(d * d) / h2) + (i == j ? p1 : 0.0);
+  }
+};
+struct GRefCoulomb {  // 1/|x_i-x_j|, 0 on the diagonal, + p1    (kernel.h:69-89)
+  const double* x;
+  double p1;
+  __device__ double row_pt(int64_t i) const { return x[i]; }
+  __device__ double col_pt(int64_t j) const { return x[j]; }
+  __device__ double eval(double xi, double xj, int64_t i, int64_t j) const {
+    return i == j ? p1 : 1.0 / fabs(xi - xj);
+  }
+};
+
+template <class F>
+static void with_gen(const GenDesc& g, F&& f) {
+  switch (g.kind) {
+    case GenKind::Dense: f(GDense{g.A, g.lda}); break;
+    case GenKind::Gaussian: f(GGaussian{g.x, 1.0 / (2.0 * g.p0 * g.p0), g.p1}); break;
+    case GenKind::Cauchy: f(GCauchy{g.x, g.y}); break;
+    case GenKind::RefCauchy: f(GRefCauchy{g.x, g.p1}); break;
+    case GenKind::RefGaussian: f(GRefGaussian{g.x, g.p0 * g.p0, g.p1}); break;
+    case GenKind::RefCoulomb: f(GRefCoulomb{g.x, g.p1}); break;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Sketch GEMM on FP64 MFMA.  A workgroup owns 128 output rows (4 waves x 2
+// row fragments of 16) and NT column tiles of 16; the reduction runs in
+// K-tiles of 32 whose X rows are staged in LDS.  Each lane generates one
+// A entry per fragment per MFMA step (v_mfma_f64_16x16x4f64 consumes a
+// 16x4 A fragment, one element per lane).
+// --------------------------------------------------------------------------
+struct SkTile {
+  int32_t seg, row0, rb, re;
+  int32_t split, pad_;
+};
+
+template <class Gen, bool TRANS, int NT>
+__global__ __launch_bounds__(kWG) void k_sketch(Gen gen, const SegSk* __restrict__ segs,
+                                                const SkTile* __restrict__ tiles,
+                                                const double* __restrict__ X, int64_t ldx,
+                                                double* __restrict__ Y, int64_t ldy,
+                                                int64_t split_stride) {
+  constexpr int RF = 2, KT = 32, NC = NT * 16;
+  __shared__ double xs[KT][NC + 1];
+  const SkTile tl = tiles[blockIdx.x];
+  const SegSk sg = segs[tl.seg];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int cbase = blockIdx.y * NC;
+  const int kcols = min(NC, sg.k - cbase);
+  mfma::v4d acc[RF][NT];
+#pragma unroll
+  for (int f = 0; f < RF; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[f][t] = mfma::v4d{0.0, 0.0, 0.0, 0.0};
+
+  int rloc[RF];
+  int64_t iglob[RF];
+  double opt[RF];
+#pragma unroll
+  for (int f = 0; f < RF; ++f) {
+    rloc[f] = tl.row0 + w * 32 + f * 16 + (lane % 16);
+    iglob[f] = sg.out0 + min(rloc[f], max(sg.outlen - 1, 0));
+    opt[f] = TRANS ? gen.col_pt(iglob[f]) : gen.row_pt(iglob[f]);
+  }
+  const int kq = lane / 16;
+  for (int kb = tl.rb; kb < tl.re; kb += KT) {
+    for (int e = threadIdx.x; e < KT * NC; e += kWG) {
+      const int kk = e % KT, cc = e / KT, j = kb + kk;
+      xs[kk][cc] = (j < tl.re && cc < kcols) ? X[sg.xoff + j + (int64_t)(cbase + cc) * ldx] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int ks = 0; ks < KT / 4; ++ks) {
+      const int jl = kb + ks * 4 + kq;
+      const bool inr = jl < tl.re;
+      // clamp so that the point fetch below stays inside the reduction range
+      const int64_t jg = sg.red0 + (inr ? jl : tl.re - 1);
+      const bool jok = inr && !((uint64_t)(jg - sg.excl0) < (uint64_t)sg.excllen);
+      const double rpt = TRANS ? gen.row_pt(jg) : gen.col_pt(jg);
+      double a[RF];
+#pragma unroll
+      for (int f = 0; f < RF; ++f) {
+        const bool ok = jok && rloc[f] < sg.outlen;
+        a[f] = ok ? (TRANS ? gen.eval(rpt, opt[f], jg, iglob[f]) : gen.eval(opt[f], rpt, iglob[f], jg))
+                  : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const double b = xs[ks * 4 + kq][t * 16 + (lane % 16)];
+#pragma unroll
+        for (int f = 0; f < RF; ++f) acc[f][t] = mfma::f64_16x16x4(a[f], b, acc[f][t]);
+      }
+    }
+    __syncthreads();
+  }
+  double* ybase = Y + (int64_t)tl.split * split_stride + sg.yoff;
+#pragma unroll
+  for (int f = 0; f < RF; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = tl.row0 + w * 32 + f * 16 + mfma::f64_row(lane, v);
+        const int col = t * 16 + mfma::f64_col(lane, v);
+        if (row < sg.outlen && col < kcols) ybase[row + (int64_t)(cbase + col) * ldy] = acc[f][t][v];
+      }
+}
+
+struct SumTile {
+  int32_t seg, row0;
+};
+__global__ __launch_bounds__(kWG) void k_sketch_sum(const SegSk* __restrict__ segs,
+                                                    const SumTile* __restrict__ tiles,
+                                                    const double* __restrict__ part, int nsplit,
+                                                    int64_t split_stride, double* __restrict__ Y,
+                                                    int64_t ldy) {
+  const SumTile tl = tiles[blockIdx.x];
+  const SegSk sg = segs[tl.seg];
+  const int r = tl.row0 + (threadIdx.x % 64);
+  if (r >= sg.outlen) return;
+  for (int c = threadIdx.x / 64; c < sg.k; c += 4) {
+    const int64_t o = sg.yoff + r + (int64_t)c * ldy;
+    double s = 0.0;
+    for (int p = 0; p < nsplit; ++p) s += part[(int64_t)p * split_stride + o];
+    Y[o] = s;
+  }
+}
+
+template <class Gen, bool TRANS>
+static void launch_sketch(const Gen& gen, int NT, dim3 grid, const SegSk* ds, const SkTile* dt,
+                          const double* X, int64_t ldx, double* Y, int64_t ldy, int64_t ss,
+                          hipStream_t s) {
+  switch (NT) {
+    case 1: k_sketch<Gen, TRANS, 1><<<grid, kWG, 0, s>>>(gen, ds, dt, X, ldx, Y, ldy, ss); break;
+    case 2: k_sketch<Gen, TRANS, 2><<<grid, kWG, 0, s>>>(gen, ds, dt, X, ldx, Y, ldy, ss); break;
+    case 3: k_sketch<Gen, TRANS, 3><<<grid, kWG, 0, s>>>(gen, ds, dt, X, ldx, Y, ldy, ss); break;
+    default: k_sketch<Gen, TRANS, 4><<<grid, kWG, 0, s>>>(gen, ds, dt, X, ldx, Y, ldy, ss); break;
+  }
+}
+
+void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs,
+            const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap, Workspace& ws,
+            hipStream_t s) {
+  if (segs.empty() || kcap <= 0) return;
+  const int kpad = round_up(kcap, 16);
+  const int NT = std::min(4, kpad / 16);
+  const int ncg = (int)ceil_div(kpad, NT * 16);
+  int64_t nrow_tiles = 0, redmax = 0;
+  for (auto& sg : segs) {
+    nrow_tiles += ceil_div(std::max(sg.outlen, 0), 128);
+    redmax = std::max<int64_t>(redmax, sg.redlen);
+  }
+  // split the reduction when the row tiles alone cannot fill 256 CUs
+  int nsplit = 1;
+  const int64_t want = 1024;
+  if (nrow_tiles * ncg < want) {
+    nsplit = (int)std::min<int64_t>(ceil_div(want, std::max<int64_t>(nrow_tiles * ncg, 1)),
+                                    std::max<int64_t>(1, redmax / 256));
+    nsplit = std::max(1, std::min(nsplit, 64));
+  }
+  std::vector<SkTile> tiles;
+  for (size_t i = 0; i < segs.size(); ++i) {
+    const auto& sg = segs[i];
+    if (sg.outlen <= 0 || sg.k <= 0) continue;
+    const int64_t chunk = round_up((int)ceil_div(std::max(sg.redlen, 1), nsplit), 32);
+    for (int r0 = 0; r0 < sg.outlen; r0 += 128)
+      for (int p = 0; p < nsplit; ++p) {
+        const int rb = (int)std::min<int64_t>((int64_t)p * chunk, sg.redlen);
+        const int re = (int)std::min<int64_t>((int64_t)(p + 1) * chunk, sg.redlen);
+        tiles.push_back({(int32_t)i, r0, rb, re, p, 0});
+      }
+  }
+  if (tiles.empty()) return;
+  dsegs.upload(segs, s);
+  DevBuf<SkTile> dt;
+  dt.upload(tiles, s);
+  double* out = Y;
+  int64_t ss = 0;
+  if (nsplit > 1) {
+    // partial sums go to a scratch copy laid out like Y
+    int64_t span = 0;
+    for (auto& sg : segs)
+      span = std::max<int64_t>(span, sg.yoff + sg.outlen + (int64_t)(sg.k - 1) * ldy + 1);
+    ss = span;
+    ws.part.reserve((size_t)(span * nsplit));
+    out = ws.part.get();
+  }
+  dim3 grid((unsigned)tiles.size(), (unsigned)ncg);
+  with_gen(g, [&](auto gen) {
+    if (trans)
+      launch_sketch<decltype(gen), true>(gen, NT, grid, dsegs.get(), dt.get(), X, ldx, out, ldy, ss, s);
+    else
+      launch_sketch<decltype(gen), false>(gen, NT, grid, dsegs.get(), dt.get(), X, ldx, out, ldy, ss, s);
+  });
+  HIP_CHECK(hipGetLastError());
+  if (nsplit > 1) {
+    std::vector<SumTile> st;
+    for (size_t i = 0; i < segs.size(); ++i)
+      for (int r0 = 0; r0 < segs[i].outlen; r0 += 64) st.push_back({(int32_t)i, r0});
+    DevBuf<SumTile> dst;
+    dst.upload(st, s);
+    k_sketch_sum<<<(unsigned)st.size(), kWG, 0, s>>>(dsegs.get(), dst.get(), ws.part.get(), nsplit, ss,
+                                                     Y, ldy);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
+// --------------------------------------------------------------------------
+// Batched small dense kernels: one workgroup per matrix, matrices in global
+// memory (they stay in L1/L2 for the sizes used here, n <= a few hundred).
+// --------------------------------------------------------------------------
+__device__ inline double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < kWG / 64; ++i) t += red[i];
+  return t;
+}
+
+// R^T R = G + shift I (upper R); Rinv = R^{-1}.  A zero Gram (all-zero
+// input) returns R = Rinv = 0 so that the basis built from it is zero.
+__global__ __launch_bounds__(kWG) void k_chol_rinv(const SmallMat* __restrict__ mats,
+                                                   const double* __restrict__ G,
+                                                   double* __restrict__ R, double* __restrict__ Ri,
+                                                   const double* __restrict__ shift_coef,
+                                                   int* __restrict__ info) {
+  __shared__ double red[kWG / 64];
+  __shared__ int bad;
+  const SmallMat m = mats[blockIdx.x];
+  const int n = m.n, ld = m.ld;
+  const double* g = G + m.off;
+  double* r = R + m.off;
+  double* ri = Ri + m.off;
+  double tr = 0.0;
+  for (int i = threadIdx.x; i < n; i += kWG) tr += g[i + (int64_t)i * ld];
+  tr = block_sum(tr, red);
+  const double shift = shift_coef[blockIdx.x] * tr;
+  if (threadIdx.x == 0) { bad = 0; info[blockIdx.x] = 0; }
+  for (int e = threadIdx.x; e < n * n; e += kWG) {
+    const int i = e % n, j = e / n;
+    r[i + (int64_t)j * ld] = (i <= j) ? g[i + (int64_t)j * ld] + (i == j ? shift : 0.0) : 0.0;
+    ri[i + (int64_t)j * ld] = 0.0;
+  }
+  __syncthreads();
+  if (!(tr > 0.0)) {
+    if (!(tr == 0.0) && threadIdx.x == 0) info[blockIdx.x] = -1;  // NaN
+    return;  // R = Rinv = 0
+  }
+  for (int j = 0; j < n; ++j) {
+    const double d = r[j + (int64_t)j * ld];
+    if (!(d > 0.0) || !isfinite(d)) {
+      if (threadIdx.x == 0) { bad = j + 1; info[blockIdx.x] = j + 1; }
+      break;
+    }
+    const double rjj = sqrt(d);
+    __syncthreads();
+    for (int c = j + 1 + threadIdx.x; c < n; c += kWG) r[j + (int64_t)c * ld] /= rjj;
+    if (threadIdx.x == 0) r[j + (int64_t)j * ld] = rjj;
+    __syncthreads();
+    const int rem = n - j - 1;
+    for (int e = threadIdx.x; e < rem * rem; e += kWG) {
+      const int a = j + 1 + e % rem, c = j + 1 + e / rem;
+      if (a <= c) r[a + (int64_t)c * ld] -= r[j + (int64_t)a * ld] * r[j + (int64_t)c * ld];
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (bad) return;
+  // R^{-1}: column c by thread c (back substitution, upper triangular)
+  for (int c = threadIdx.x; c < n; c += kWG) {
+    ri[c + (int64_t)c * ld] = 1.0 / r[c + (int64_t)c * ld];
+    for (int i = c - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int t = i + 1; t <= c; ++t) s += r[i + (int64_t)t * ld] * ri[t + (int64_t)c * ld];
+      ri[i + (int64_t)c * ld] = -s / r[i + (int64_t)i * ld];
+    }
+  }
+}
+
+void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* G, double* R,
+               double* Rinv, const std::vector<double>& shift_coef, DevBuf<double>& dshift,
+               int* info, hipStream_t s) {
+  if (mats.empty()) return;
+  dm.upload(mats, s);
+  dshift.upload(shift_coef, s);
+  k_chol_rinv<<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), G, R, Rinv, dshift.get(), info);
+  HIP_CHECK(hipGetLastError());
+}
+
+// One-sided Jacobi (Hestenes) with a round-robin pair schedule; the four
+// waves rotate disjoint column pairs in parallel.
+__global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__ mats,
+                                                    double* __restrict__ A, double* __restrict__ V,
+                                                    double* __restrict__ S, int64_t sstride,
+                                                    double* __restrict__ work) {
+  __shared__ int rotated;
+  __shared__ double sig[512];
+  __shared__ int perm[512];
+  const SmallMat m = mats[blockIdx.x];
+  const int n = m.n, ld = m.ld;
+  double* a = A + m.off;
+  double* v = V + m.off;
+  double* sv = S + (int64_t)blockIdx.x * sstride;
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  for (int e = threadIdx.x; e < n * n; e += kWG) {
+    const int i = e % n, j = e / n;
+    v[i + (int64_t)j * ld] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int np = n + (n & 1);  // even player count; column n is a phantom
+  const int circ = np - 1;
+  for (int sweep = 0; sweep < 60 && n > 1; ++sweep) {
+    if (threadIdx.x == 0) rotated = 0;
+    __syncthreads();
+    for (int step = 0; step < circ; ++step) {
+      for (int pi = w; pi < np / 2; pi += kWG / 64) {
+        int p, q;
+        if (pi == 0) {
+          p = 0;
+          q = 1 + step % circ;
+        } else {
+          p = 1 + (step + pi) % circ;
+          q = 1 + (step - pi + circ) % circ;
+        }
+        if (p >= n || q >= n) continue;
+        if (p > q) { const int t = p; p = q; q = t; }
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int i = lane; i < n; i += 64) {
+          const double x = a[i + (int64_t)p * ld], y = a[i + (int64_t)q * ld];
+          al += x * x;
+          be += y * y;
+          ga += x * y;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+          al += __shfl_xor(al, o);
+          be += __shfl_xor(be, o);
+          ga += __shfl_xor(ga, o);
+        }
+        if (!(fabs(ga) > 1e-15 * sqrt(al * be)) || ga == 0.0) continue;
+        const double zeta = (be - al) / (2.0 * ga);
+        double t;
+        if (fabs(zeta) > 1e150)
+          t = 0.5 / zeta;
+        else
+          t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+        for (int i = lane; i < n; i += 64) {
+          const double x = a[i + (int64_t)p * ld], y = a[i + (int64_t)q * ld];
+          a[i + (int64_t)p * ld] = c * x - sn * y;
+          a[i + (int64_t)q * ld] = sn * x + c * y;
+          const double vx = v[i + (int64_t)p * ld], vy = v[i + (int64_t)q * ld];
+          v[i + (int64_t)p * ld] = c * vx - sn * vy;
+          v[i + (int64_t)q * ld] = sn * vx + c * vy;
+        }
+        if (lane == 0) rotated = 1;
+      }
+      __syncthreads();
+    }
+    if (!rotated) break;
+    __syncthreads();
+  }
+  // singular values = column norms
+  for (int j = w; j < n; j += kWG / 64) {
+    double s2 = 0.0;
+    for (int i = lane; i < n; i += 64) s2 += a[i + (int64_t)j * ld] * a[i + (int64_t)j * ld];
+    for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o);
+    if (lane == 0) sig[j] = sqrt(s2);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < n; ++j) perm[j] = j;
+    for (int j = 0; j < n; ++j) {  // selection sort, descending
+      int best = j;
+      for (int t = j + 1; t < n; ++t)
+        if (sig[perm[t]] > sig[perm[best]]) best = t;
+      const int tmp = perm[j];
+      perm[j] = perm[best];
+      perm[best] = tmp;
+    }
+  }
+  __syncthreads();
+  double* wu = work + m.off * 2;
+  double* wv = wu + (int64_t)n * ld;
+  for (int e = threadIdx.x; e < n * n; e += kWG) {
+    const int i = e % n, j = e / n;
+    wu[i + (int64_t)j * ld] = a[i + (int64_t)j * ld];
+    wv[i + (int64_t)j * ld] = v[i + (int64_t)j * ld];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * n; e += kWG) {
+    const int i = e % n, j = e / n;
+    const int pj = perm[j];
+    const double sj = sig[pj];
+    a[i + (int64_t)j * ld] = sj > 0.0 ? wu[i + (int64_t)pj * ld] / sj : 0.0;
+    v[i + (int64_t)j * ld] = wv[i + (int64_t)pj * ld];
+  }
+  for (int j = threadIdx.x; j < n; j += kWG) sv[j] = sig[perm[j]];
+}
+
+void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* A, double* V,
+                double* S, int64_t sstride, double* work, hipStream_t s) {
+  if (mats.empty()) return;
+  for (auto& mm : mats)
+    HMAT_REQUIRE(mm.n <= 512, HMAT_ERR_ARG, "jacobi_svd: n > 512");
+  dm.upload(mats, s);
+  k_jacobi_svd<<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), A, V, S, sstride, work);
+  HIP_CHECK(hipGetLastError());
+}
+
+// Gauss-Jordan inversion with partial pivoting, in place.
+__global__ __launch_bounds__(kWG) void k_gj_inverse(const SmallMat* __restrict__ mats,
+                                                    double* __restrict__ A, int* __restrict__ info) {
+  __shared__ double bv[kWG];
+  __shared__ int bi[kWG];
+  __shared__ int piv_s;
+  __shared__ int perm[4096];
+  const SmallMat m = mats[blockIdx.x];
+  const int n = m.n, ld = m.ld;
+  double* a = A + m.off;
+  if (threadIdx.x == 0) info[blockIdx.x] = 0;
+  for (int k = 0; k < n; ++k) {
+    double best = -1.0;
+    int bidx = k;
+    for (int i = k + threadIdx.x; i < n; i += kWG) {
+      const double v = fabs(a[i + (int64_t)k * ld]);
+      if (v > best) { best = v; bidx = i; }
+    }
+    bv[threadIdx.x] = best;
+    bi[threadIdx.x] = bidx;
+    __syncthreads();
+    for (int o = kWG / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) {
+        const double v2 = bv[threadIdx.x + o];
+        const int i2 = bi[threadIdx.x + o];
+        if (v2 > bv[threadIdx.x] || (v2 == bv[threadIdx.x] && i2 < bi[threadIdx.x])) {
+          bv[threadIdx.x] = v2;
+          bi[threadIdx.x] = i2;
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      piv_s = bi[0];
+      perm[k] = bi[0];
+      if (!(bv[0] > 0.0) || !isfinite(bv[0])) info[blockIdx.x] = k + 1;
+    }
+    __syncthreads();
+    if (!(bv[0] > 0.0) || !isfinite(bv[0])) return;
+    const int p = piv_s;
+    if (p != k)
+      for (int j = threadIdx.x; j < n; j += kWG) {
+        const double t = a[k + (int64_t)j * ld];
+        a[k + (int64_t)j * ld] = a[p + (int64_t)j * ld];
+        a[p + (int64_t)j * ld] = t;
+      }
+    __syncthreads();
+    const double pv = a[k + (int64_t)k * ld];
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += kWG)
+      if (j != k) a[k + (int64_t)j * ld] /= pv;
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += kWG) {
+      const int i = e % n, j = e / n;
+      if (i != k && j != k) a[i + (int64_t)j * ld] -= a[i + (int64_t)k * ld] * a[k + (int64_t)j * ld];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kWG)
+      if (i != k) a[i + (int64_t)k * ld] = -a[i + (int64_t)k * ld] / pv;
+    if (threadIdx.x == 0) a[k + (int64_t)k * ld] = 1.0 / pv;
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    const int p = perm[k];
+    if (p != k)
+      for (int i = threadIdx.x; i < n; i += kWG) {
+        const double t = a[i + (int64_t)k * ld];
+        a[i + (int64_t)k * ld] = a[i + (int64_t)p * ld];
+        a[i + (int64_t)p * ld] = t;
+      }
+    __syncthreads();
+  }
+}
+
+void gj_inverse(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* A, int* info,
+                hipStream_t s) {
+  if (mats.empty()) return;
+  for (auto& mm : mats) HMAT_REQUIRE(mm.n <= 4096, HMAT_ERR_ARG, "gj_inverse: n > 4096");
+  dm.upload(mats, s);
+  k_gj_inverse<<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), A, info);
+  HIP_CHECK(hipGetLastError());
+}
+
+// --------------------------------------------------------------------------
+// Fill helpers
+// --------------------------------------------------------------------------
+__global__ void k_fill_identity(const SegNN* __restrict__ segs, double* __restrict__ Y, int64_t ldy) {
+  const SegNN sg = segs[blockIdx.y];
+  const int64_t total = (int64_t)sg.len * sg.q;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % sg.len), c = (int)(e / sg.len);
+    Y[sg.yoff + r + (int64_t)c * ldy] = (r == c) ? 1.0 : 0.0;
+  }
+}
+
+void fill_identity_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, double* Y,
+                        int64_t ldy, hipStream_t s) {
+  if (segs.empty()) return;
+  dsegs.upload(segs, s);
+  k_fill_identity<<<dim3(64, (unsigned)segs.size()), kWG, 0, s>>>(dsegs.get(), Y, ldy);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+__global__ void k_copy_segs(const SegNN* __restrict__ segs, const double* __restrict__ X, int64_t ldx,
+                            double* __restrict__ Y, int64_t ldy) {
+  const SegNN sg = segs[blockIdx.y];
+  const int64_t total = (int64_t)sg.len * sg.q;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % sg.len), c = (int)(e / sg.len);
+    Y[sg.yoff + r + (int64_t)c * ldy] = X[sg.xoff + r + (int64_t)c * ldx];
+  }
+}
+
+void copy_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X, int64_t ldx,
+               double* Y, int64_t ldy, hipStream_t s) {
+  if (segs.empty()) return;
+  dsegs.upload(segs, s);
+  k_copy_segs<<<dim3(64, (unsigned)segs.size()), kWG, 0, s>>>(dsegs.get(), X, ldx, Y, ldy);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+__global__ void k_trunc_prep(const SmallMat* __restrict__ mats, const double* __restrict__ Ur,
+                             const double* __restrict__ Vr, const double* __restrict__ S,
+                             int64_t sstride, const int* __restrict__ rank, int rmax,
+                             double* __restrict__ TU, double* __restrict__ TV) {
+  const SmallMat m = mats[blockIdx.x];
+  const int r = rank[blockIdx.x];
+  const double* sv = S + (int64_t)blockIdx.x * sstride;
+  for (int e = threadIdx.x; e < m.n * rmax; e += blockDim.x) {
+    const int i = e % m.n, j = e / m.n;
+    const int64_t o = m.off + i + (int64_t)j * m.ld;
+    TU[o] = j < r ? Vr[o] * sv[j] : 0.0;
+    TV[o] = j < r ? Ur[o] : 0.0;
+  }
+}
+
+void trunc_prep(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, const double* Ur,
+                const double* Vr, const double* S, int64_t sstride, const int* rank, int rmax,
+                double* TU, double* TV, hipStream_t s) {
+  if (mats.empty()) return;
+  dm.upload(mats, s);
+  k_trunc_prep<<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), Ur, Vr, S, sstride, rank, rmax, TU, TV);
+  HIP_CHECK(hipGetLastError());
+}
+
+__device__ inline uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ inline double u01(uint64_t z) { return (double)(z >> 11) * (1.0 / 9007199254740992.0); }
+
+// Omega entry (r, c) of stream `id` (the segment's toff field).
+__global__ void k_fill_random(const SegNN* __restrict__ segs, double* __restrict__ Y, int64_t ldy,
+                              uint64_t seed, int dist) {
+  const SegNN sg = segs[blockIdx.y];
+  const uint64_t key = splitmix64(seed ^ splitmix64((uint64_t)sg.toff + 0x51ED2701ull));
+  const int64_t total = (int64_t)sg.len * sg.q;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % sg.len), c = (int)(e / sg.len);
+    const uint64_t ctr = ((uint64_t)c << 40) ^ (uint64_t)r;
+    double val;
+    if (dist == HMAT_OMEGA_GAUSSIAN) {
+      const double u1 = u01(splitmix64(key ^ (ctr * 2 + 0x1234567ull)));
+      const double u2 = u01(splitmix64(key ^ (ctr * 2 + 1 + 0x7654321ull)));
+      val = sqrt(-2.0 * log(1.0 - u1)) * cospi(2.0 * u2);
+    } else {
+      val = u01(splitmix64(key ^ ctr));
+    }
+    Y[sg.yoff + r + (int64_t)c * ldy] = val;
+  }
+}
+
+void fill_random_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, double* Y,
+                      int64_t ldy, uint64_t seed, int dist, hipStream_t s) {
+  if (segs.empty()) return;
+  dsegs.upload(segs, s);
+  k_fill_random<<<dim3(64, (unsigned)segs.size()), kWG, 0, s>>>(dsegs.get(), Y, ldy, seed, dist);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <class Gen>
+__global__ void k_gen_blocks(Gen gen, const BlockDesc* __restrict__ blocks, double* __restrict__ out) {
+  const BlockDesc b = blocks[blockIdx.y];
+  const int64_t total = (int64_t)b.m * b.n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % b.m), j = (int)(e / b.m);
+    const int64_t gi = b.r0 + i, gj = b.c0 + j;
+    out[b.off + i + (int64_t)j * b.ld] = gen.eval(gen.row_pt(gi), gen.col_pt(gj), gi, gj);
+  }
+}
+
+void gen_blocks(const GenDesc& g, const std::vector<BlockDesc>& blocks, DevBuf<BlockDesc>& db,
+                double* out, hipStream_t s) {
+  if (blocks.empty()) return;
+  db.upload(blocks, s);
+  with_gen(g, [&](auto gen) {
+    k_gen_blocks<<<dim3(64, (unsigned)blocks.size()), kWG, 0, s>>>(gen, db.get(), out);
+  });
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+__global__ void k_col_norms2(const double* __restrict__ X, int64_t ldx, int64_t n,
+                             double* __restrict__ out) {
+  __shared__ double red[kWG / 64];
+  const double* x = X + (int64_t)blockIdx.x * ldx;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kWG) s += x[i] * x[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+void col_norms2(const double* X, int64_t ldx, int64_t n, int ncols, double* out, hipStream_t s) {
+  if (ncols <= 0) return;
+  k_col_norms2<<<(unsigned)ncols, kWG, 0, s>>>(X, ldx, n, out);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace kern
+}  // namespace hmat

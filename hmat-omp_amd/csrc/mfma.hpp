@@ -1,0 +1,23 @@
+// gfx950 matrix-core helpers.
+//
+// v_mfma_f64_16x16x4f64 (wave64): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; accumulator register v of lane l holds D[l/16 + 4v][l%16]
+// (rows interleaved by 4).  Layout measured on MI355X with a unique-value
+// probe (D[i][j] = i + 100 j).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace hmat {
+namespace mfma {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ inline v4d f64_16x16x4(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline int f64_row(int lane, int v) { return (lane >> 4) + 4 * v; }
+__device__ inline int f64_col(int lane, int) { return lane & 15; }
+
+}  // namespace mfma
+}  // namespace hmat

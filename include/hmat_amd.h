@@ -1,0 +1,204 @@
+/*
+ * hmat_amd.h — C ABI of the MI355X (gfx950) HODLR / HSS engine.
+ *
+ * Drop-in boundary for the OpenMP hot path of chenxinye/hmat-omp:
+ *   HODLRMatrixOMP  (include/hodlr_routines_omp.h:9-241)
+ *   HSSMatrixOMP    (include/hss_routines_omp.h:11-375)
+ * The reference is a header-only C++ library with no FFI layer; these entry
+ * points are what a C/C++/ctypes caller binds in place of the class methods.
+ * Each declaration names the method it replaces.
+ *
+ * Conventions
+ *   - Matrices are column-major doubles with an explicit leading dimension,
+ *     like the reference Matrix (include/matrix.h:36-37: data_[j*rows_+i]).
+ *   - Host-pointer entry points copy in/out; *_device entry points take
+ *     device pointers already resident in HBM and run on the handle's stream.
+ *   - Every call returns an hmat_status; hmat_last_error() gives a message
+ *     for the calling thread.  The reference ignores LAPACK info codes
+ *     (include/matrix.h:129-130,139-147); this ABI reports them instead.
+ *   - Calls on one handle are serialised by the caller; distinct handles
+ *     are independent.
+ */
+#ifndef HMAT_AMD_H
+#define HMAT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  HMAT_OK = 0,
+  HMAT_ERR_ARG = 1,         /* bad dimension / pointer / option            */
+  HMAT_ERR_DEVICE = 2,      /* HIP runtime error, or no HIP device         */
+  HMAT_ERR_OOM = 3,         /* device allocation failed                    */
+  HMAT_ERR_SINGULAR = 4,    /* zero pivot in a leaf or capacitance block   */
+  HMAT_ERR_NONFINITE = 5,   /* NaN/Inf in input or in a factor             */
+  HMAT_ERR_STATE = 6,       /* call out of order (e.g. solve before build) */
+  HMAT_ERR_NOT_CONVERGED = 7/* iterative refinement missed its target     */
+} hmat_status;
+
+typedef enum {
+  HMAT_PREC_FP64 = 0,       /* plain FP64 everywhere                       */
+  HMAT_PREC_MIXED = 1       /* low-precision sketch + FP32 factorisation,
+                               FP64 projection, residual and refinement    */
+} hmat_precision;
+
+typedef enum {
+  HMAT_OMEGA_UNIFORM = 0,   /* U[0,1), as Matrix::Random (matrix.h:50-57)  */
+  HMAT_OMEGA_GAUSSIAN = 1   /* N(0,1)                                      */
+} hmat_omega_dist;
+
+/* Compression options.  hmat_options_reference() gives the reference's
+ * behaviour: tolerance truncation S_i > tol*S_0 (compression.h:42-46),
+ * rsvd_rank 64 (hodlr_routines_omp.h:17), k = rank+10 (compression.h:74),
+ * no power iterations, exact SVD when a block side is < 256
+ * (compression.h:72), Omega ~ U[0,1). */
+typedef struct {
+  double tol;
+  int32_t rank;           /* rsvd_rank (tolerance mode) or the fixed rank */
+  int32_t fixed_rank;     /* 0: tolerance rule, 1: keep exactly `rank`    */
+  int32_t oversample;     /* p in k = rank + p                            */
+  int32_t power_iters;    /* q                                            */
+  int32_t exact_below;    /* exact SVD when a block side is below this    */
+  int32_t precision;      /* hmat_precision                               */
+  int32_t omega_dist;     /* hmat_omega_dist (device-generated Omega)     */
+  uint64_t seed;          /* device Omega seed                            */
+} hmat_options;
+
+/* Kernel matrices evaluated on the device (no dense A).  Entry (i,j):
+ *   HMAT_KERNEL_GAUSSIAN     exp(-(x_i-x_j)^2 / (2 p0^2)) + p1*delta_ij
+ *   HMAT_KERNEL_CAUCHY       1 / (x_i - y_j)
+ *   HMAT_KERNEL_REF_CAUCHY   1 / (x_i + x_j + 1) + p1*delta_ij   (kernel.h:30-48)
+ *   HMAT_KERNEL_REF_GAUSSIAN exp(-(x_i-x_j)^2 / p0^2) + p1*delta_ij (kernel.h:52-65)
+ *   HMAT_KERNEL_REF_COULOMB  1/|x_i-x_j| (0 on the diagonal) + p1*delta_ij
+ *                                                               (kernel.h:69-89)
+ * x (and y for CAUCHY) are host arrays of N doubles. */
+typedef enum {
+  HMAT_KERNEL_GAUSSIAN = 1,
+  HMAT_KERNEL_CAUCHY = 2,
+  HMAT_KERNEL_REF_CAUCHY = 3,
+  HMAT_KERNEL_REF_GAUSSIAN = 4,
+  HMAT_KERNEL_REF_COULOMB = 5
+} hmat_kernel_type;
+
+typedef struct {
+  int32_t type;
+  double p0;
+  double p1;
+  const double* x;
+  const double* y;
+} hmat_kernel;
+
+typedef struct hmat_hodlr_s* hmat_hodlr;
+typedef struct hmat_hss_s* hmat_hss;
+
+/* ------------------------------------------------------------- library -- */
+const char* hmat_last_error(void);
+const char* hmat_version(void);
+int32_t hmat_device_count(void);
+void hmat_options_reference(hmat_options* opts, double tol);
+
+/* ----------------------------------------------------------------- HODLR -- */
+/* HODLRMatrixOMP::HODLRMatrixOMP(double tol)  hodlr_routines_omp.h:17 */
+hmat_status hmat_hodlr_create(hmat_hodlr* h, const hmat_options* opts);
+/* HODLRMatrixOMP::~HODLRMatrixOMP()           hodlr_routines_omp.h:18 */
+hmat_status hmat_hodlr_destroy(hmat_hodlr h);
+
+/* HODLRMatrixOMP::build_from_dense(const Matrix& A, int leaf_size)
+ *   hodlr_routines_omp.h:21-79.  A: N x N host, column-major, lda >= N.
+ *   omega: optional host Omega blocks in the serial draw order (post-order,
+ *   A12 then A21, only blocks taking the randomised branch, each n x k
+ *   column-major; hodlr_routines.h:43-53), or NULL for device Omega.
+ *   omega_len must equal hmat_hodlr_omega_length() when omega != NULL. */
+hmat_status hmat_hodlr_build_dense(hmat_hodlr h, const double* A, int64_t lda,
+                                   int64_t N, int32_t leaf_size,
+                                   const double* omega, int64_t omega_len);
+/* Same, A already on the device. */
+hmat_status hmat_hodlr_build_dense_device(hmat_hodlr h, const double* dA,
+                                          int64_t lda, int64_t N,
+                                          int32_t leaf_size,
+                                          const double* omega,
+                                          int64_t omega_len);
+/* Matrix-free build: entries generated on the device (kernel.h:18-106 and
+ * the point kernels above).  No reference counterpart for N >= 46341
+ * (the reference overflows int at matrix.h:34). */
+hmat_status hmat_hodlr_build_kernel(hmat_hodlr h, const hmat_kernel* k,
+                                    int64_t N, int32_t leaf_size);
+/* Number of doubles of host Omega that build_dense expects. */
+int64_t hmat_hodlr_omega_length(const hmat_options* opts, int64_t N,
+                                int32_t leaf_size);
+
+/* HODLRMatrixOMP::multiply(const Matrix& x)   hodlr_routines_omp.h:81-135
+ *   Y = A_H X for nrhs columns (the reference uses column 0 only). */
+hmat_status hmat_hodlr_matvec(hmat_hodlr h, const double* X, int64_t ldx,
+                              int32_t nrhs, double* Y, int64_t ldy);
+hmat_status hmat_hodlr_matvec_device(hmat_hodlr h, const double* dX,
+                                     int64_t ldx, int32_t nrhs, double* dY,
+                                     int64_t ldy);
+
+/* Factor once for HODLRMatrixOMP::solve (hodlr_routines_omp.h:137-240);
+ * solve() factors on first use when this was not called. */
+hmat_status hmat_hodlr_factor(hmat_hodlr h);
+/* HODLRMatrixOMP::solve(const Matrix& b)     hodlr_routines_omp.h:137-145
+ *   X = A_H^{-1} B, nrhs columns.  In HMAT_PREC_MIXED the factorisation is
+ *   FP32 and the answer is refined in FP64 until ||B - A_H X|| <= rtol ||B||
+ *   per column (rtol from hmat_hodlr_set_refinement, default 1e-10). */
+hmat_status hmat_hodlr_solve(hmat_hodlr h, const double* B, int64_t ldb,
+                             int32_t nrhs, double* X, int64_t ldx);
+hmat_status hmat_hodlr_solve_device(hmat_hodlr h, const double* dB,
+                                    int64_t ldb, int32_t nrhs, double* dX,
+                                    int64_t ldx);
+hmat_status hmat_hodlr_set_refinement(hmat_hodlr h, double rtol,
+                                      int32_t max_iters);
+/* Iterations and final max relative residual of the last solve. */
+hmat_status hmat_hodlr_last_refinement(hmat_hodlr h, int32_t* iters,
+                                       double* resid);
+
+/* Introspection (HODLRMatrixOMP public members N, nodes; HODLRNode fields
+ * utils.h:6-29). */
+int64_t hmat_hodlr_size(hmat_hodlr h);
+int32_t hmat_hodlr_levels(hmat_hodlr h);
+/* ranks of the blocks at tree level lvl (1..levels), one per node at that
+ * level: rank of the block whose rows are that node. */
+hmat_status hmat_hodlr_level_ranks(hmat_hodlr h, int32_t lvl, int32_t* ranks,
+                                   int32_t cap, int32_t* count);
+/* Dense reconstruction of A_H (checking only; N x N host buffer). */
+hmat_status hmat_hodlr_to_dense(hmat_hodlr h, double* A, int64_t lda);
+/* Set the HIP stream (hipStream_t) used by the handle; NULL = own stream. */
+hmat_status hmat_hodlr_set_stream(hmat_hodlr h, void* stream);
+
+/* ------------------------------------------------------------------- HSS -- */
+/* HSSMatrixOMP::HSSMatrixOMP(double tol)      hss_routines_omp.h:19 */
+hmat_status hmat_hss_create(hmat_hss* h, const hmat_options* opts);
+hmat_status hmat_hss_destroy(hmat_hss h);
+/* HSSMatrixOMP::build_from_dense              hss_routines_omp.h:62-149
+ *   nested = 0: identity transfers, ranks add up (the reference operator).
+ *   nested = 1: compressed transfers (true nested bases).
+ *   omega: optional host Omega blocks in the serial draw order (reverse
+ *   node id over leaves, row block then column block, each N x k;
+ *   hss_routines.h:57-77), or NULL. */
+hmat_status hmat_hss_build_dense(hmat_hss h, const double* A, int64_t lda,
+                                 int64_t N, int32_t leaf_size, int32_t nested,
+                                 const double* omega, int64_t omega_len);
+hmat_status hmat_hss_build_kernel(hmat_hss h, const hmat_kernel* k, int64_t N,
+                                  int32_t leaf_size, int32_t nested);
+int64_t hmat_hss_omega_length(const hmat_options* opts, int64_t N,
+                              int32_t leaf_size);
+/* HSSMatrixOMP::multiply                      hss_routines_omp.h:154-230 */
+hmat_status hmat_hss_matvec(hmat_hss h, const double* X, int64_t ldx,
+                            int32_t nrhs, double* Y, int64_t ldy);
+hmat_status hmat_hss_factor(hmat_hss h);
+/* HSSMatrixOMP::solve                         hss_routines_omp.h:235-358 */
+hmat_status hmat_hss_solve(hmat_hss h, const double* B, int64_t ldb,
+                           int32_t nrhs, double* X, int64_t ldx);
+int64_t hmat_hss_size(hmat_hss h);
+/* rank_U of the root's children summed (reference: root->rank_U). */
+int32_t hmat_hss_root_rank(hmat_hss h);
+hmat_status hmat_hss_to_dense(hmat_hss h, double* A, int64_t lda);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HMAT_AMD_H */

@@ -1,0 +1,232 @@
+"""GPU parity of the HODLR engine against the CPU oracle and dense LAPACK.
+
+Bars:
+  * same host Omega as the oracle (the reference's serial draw order):
+    identical per-block ranks, operator within 1e-10, solution within
+    10 x tol of the oracle's solution (the stated drop-in bar);
+  * any Omega: matvec equals the dense reconstruction of the compressed
+    operator to 1e-13, solve equals dense LAPACK on that reconstruction to
+    1e-10 (FP64 Woodbury is exact algebra up to rounding);
+  * whole path vs dense LAPACK of the true matrix within the compression
+    tolerance times a conditioning margin.
+"""
+import numpy as np
+import pytest
+
+import hmat_amd
+from oracle import hodlr as oh, kernels, lowrank, rng
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def oracle_ranks_flat(h, depth):
+    return [r for pair in h.ranks_by_level()[depth - 1] for r in pair]
+
+
+@pytest.mark.parametrize("n,leaf,tol,seed", [(1024, 128, 1e-8, 7), (2048, 256, 1e-10, 11),
+                                             (1000, 60, 1e-9, 5)])
+def test_reference_mode_same_omega_parity(n, leaf, tol, seed):
+    """Reference Cauchy driver matrix, reference options, the oracle's Omega.
+
+    The trailing directions of each sketch Y = A Omega sit at the rounding
+    level, so they depend on summation order (MFMA vs BLAS): two exact FP64
+    implementations of the same RSVD agree to a fraction of their common
+    compression error, not bitwise.  Exact-SVD blocks have no such freedom
+    and must agree to rounding.
+    """
+    a = kernels.generate_matrix(kernels.CAUCHY, n)
+    s = rng.RandomStream(seed, record=True)
+    ho = oh.HODLRMatrix(tol)
+    ho.build_from_dense(a, leaf, s)
+    omega = s.flat_log()
+    hg = hmat_amd.HODLRMatrixOMP(tol)
+    assert hg.omega_length(n, leaf) == omega.size
+    hg.build_from_dense(a, leaf, omega=omega)
+    for d in range(1, hg.levels() + 1):
+        assert hg.level_ranks(d) == oracle_ranks_flat(ho, d), d
+    G, O = hg.to_dense(), ho.to_dense()
+    comp_err = rel(O, a)                      # the oracle's own compression error
+    assert rel(G, O) <= comp_err
+    assert rel(G, a) <= 1.05 * comp_err
+    # blocks below 256 take the exact SVD path (compression.h:72)
+    def walk(nd):
+        if nd.is_leaf:
+            return
+        s_, m, e = nd.start, nd.left.end, nd.end
+        if min(m - s_, e - m) < 256:
+            for r0, r1, c0, c1 in ((s_, m, m, e), (m, e, s_, m)):
+                assert rel(G[r0:r1, c0:c1], O[r0:r1, c0:c1]) < 1e-12
+        walk(nd.left)
+        walk(nd.right)
+    walk(ho.root)
+    x = rng.RandomStream(seed + 1).random(n, 1)
+    assert rel(hg.multiply(x), G @ x) < 1e-14
+    assert rel(hg.multiply(x), ho.multiply(x)) <= comp_err
+    b = a @ x
+    xs_g, xs_o = hg.solve(b), ho.solve(b)
+    assert rel(xs_g, xs_o) <= 10 * tol
+
+
+def test_fixed_rank_power_iterations_same_omega_parity():
+    """Fixed rank 16, oversample 10, q = 2 on the oracle's Omega (all blocks
+    randomised): same operator to a fraction of the compression error."""
+    n, leaf = 2048, 64
+    x = kernels.sorted_uniform_points(n, 17)
+    a = kernels.gaussian_points(x, 0.1, 1.0)
+    oo = lowrank.Options(tol=0.0, rank=16, fixed_rank=True, oversample=10, power_iters=2,
+                         exact_below=0)
+    s = rng.RandomStream(21, record=True)
+    ho = oh.HODLRMatrix(opts=oo)
+    ho.build_from_dense(a, leaf, s)
+    og = hmat_amd.make_options(0.0, rank=16, fixed_rank=True, oversample=10, power_iters=2,
+                               exact_below=0)
+    hg = hmat_amd.HODLRMatrixOMP(opts=og)
+    hg.build_from_dense(a, leaf, omega=s.flat_log())
+    G, O = hg.to_dense(), ho.to_dense()
+    assert rel(G, O) <= max(rel(O, a), 1e-13)
+    b = np.random.default_rng(5).standard_normal((n, 2))
+    assert rel(hg.solve(b), ho.solve_dense(b)) < 1e-9
+
+
+def test_operator_algebra_exact_fixed_rank():
+    n = 2048
+    x = kernels.sorted_uniform_points(n, 3)
+    a = kernels.gaussian_points(x, 0.1, 1.0)
+    o = hmat_amd.make_options(1e-8, rank=16, fixed_rank=True, oversample=10, power_iters=2,
+                              exact_below=0, omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=42)
+    h = hmat_amd.HODLRMatrixOMP(opts=o)
+    h.build_from_dense(a, 64)
+    ad = h.to_dense()
+    v = np.random.default_rng(1).standard_normal((n, 3))
+    assert rel(h.multiply(v), ad @ v) < 1e-13
+    assert rel(h.solve(v), np.linalg.solve(ad, v)) < 1e-10
+    assert all(r == 16 for d in range(1, h.levels() + 1) for r in h.level_ranks(d))
+
+
+def test_cfg1_gaussian_against_dense_lapack():
+    """N=4096 sorted U[0,1] points, sigma=0.1, lambda=1, leaf 64, rank 16,
+    oversample 10, q=2, b ~ N(0,1): build, matvec and solve vs dense LAPACK."""
+    n = 4096
+    x = kernels.sorted_uniform_points(n, 2024)
+    a = kernels.gaussian_points(x, 0.1, 1.0)
+    b = np.random.default_rng(0).standard_normal((n, 1))
+    o = hmat_amd.make_options(1e-8, rank=16, fixed_rank=True, oversample=10, power_iters=2,
+                              exact_below=0, omega_dist=hmat_amd.OMEGA_GAUSSIAN)
+    hk = hmat_amd.HODLRMatrixOMP(opts=o)
+    hk.build_kernel(hmat_amd.make_kernel(hmat_amd.KERNEL_GAUSSIAN, x, p0=0.1, p1=1.0), n, 64)
+    hd = hmat_amd.HODLRMatrixOMP(opts=o)
+    hd.build_from_dense(a, 64)
+    # generated and dense builds see the same Omega: same operator
+    assert rel(hk.to_dense(), hd.to_dense()) < 1e-12
+    assert rel(hk.to_dense(), a) < 1e-10
+    assert rel(hk.multiply(b), a @ b) < 1e-10
+    xd = np.linalg.solve(a, b)
+    assert rel(hk.solve(b), xd) < 1e-9
+
+
+@pytest.mark.parametrize("kind", ["ref_cauchy", "ref_gaussian", "ref_coulomb"])
+def test_reference_generators_on_device(kind):
+    n, leaf = 1024, 128
+    xg = kernels.generate_grid(n)
+    if kind == "ref_cauchy":
+        a, k = kernels.generate_cauchy(n, 1.0), hmat_amd.make_kernel(hmat_amd.KERNEL_REF_CAUCHY, xg, p1=1.0)
+    elif kind == "ref_gaussian":
+        a, k = kernels.generate_gaussian(n, 0.1, 1e-6), hmat_amd.make_kernel(hmat_amd.KERNEL_REF_GAUSSIAN, xg, p0=0.1, p1=1e-6)
+    else:
+        a, k = kernels.generate_coulomb(n, 10.0 * n), hmat_amd.make_kernel(hmat_amd.KERNEL_REF_COULOMB, xg, p1=10.0 * n)
+    hk = hmat_amd.HODLRMatrixOMP(1e-10)
+    hk.build_kernel(k, n, leaf)
+    hd = hmat_amd.HODLRMatrixOMP(1e-10)
+    hd.build_from_dense(a, leaf)
+    assert rel(hk.to_dense(), hd.to_dense()) < 1e-9
+    assert rel(hk.to_dense(), a) < 1e-8
+
+
+def test_multi_rhs_columnwise():
+    n = 1536
+    a = kernels.generate_matrix(kernels.CAUCHY, n)
+    h = hmat_amd.HODLRMatrixOMP(1e-9)
+    h.build_from_dense(a, 96)
+    v = np.random.default_rng(2).standard_normal((n, 16))
+    y = h.multiply(v)
+    z = h.solve(v)
+    for j in (0, 7, 15):
+        assert rel(y[:, j:j + 1], h.multiply(v[:, j])) < 1e-14
+        assert rel(z[:, j:j + 1], h.solve(v[:, j])) < 1e-13
+
+
+def test_cauchy_points_kernel_large():
+    """1/(x_i - y_j), x_i = i/N, y_j = (j+1/2)/N, leaf 256, rank 48."""
+    n = 8192
+    xc, yc = kernels.cauchy_grid_points(n)
+    o = hmat_amd.make_options(1e-12, rank=48, fixed_rank=True, oversample=10, power_iters=0,
+                              exact_below=0, omega_dist=hmat_amd.OMEGA_GAUSSIAN)
+    h = hmat_amd.HODLRMatrixOMP(opts=o)
+    h.build_kernel(hmat_amd.make_kernel(hmat_amd.KERNEL_CAUCHY, xc, yc), n, 256)
+    a = kernels.cauchy_points(xc, yc)
+    b = np.random.default_rng(0).standard_normal((n, 2))
+    assert rel(h.multiply(b), a @ b) < 1e-10
+    x = h.solve(b)
+    assert rel(h.multiply(x), b) < 1e-11
+    assert rel(x, np.linalg.solve(a, b)) < 1e-9
+
+
+def test_trivial_tree_is_dense_solve():
+    n = 50
+    a = kernels.gaussian_points(np.linspace(0, 1, n), 0.2, 1.0)
+    h = hmat_amd.HODLRMatrixOMP(1e-8)
+    h.build_from_dense(a, 64)
+    b = np.arange(n, dtype=float).reshape(-1, 1)
+    assert h.levels() == 0
+    assert rel(h.multiply(b), a @ b) < 1e-15
+    assert rel(h.solve(b), np.linalg.solve(a, b)) < 1e-12
+
+
+def test_block_diagonal_zero_offdiagonal():
+    n = 512
+    a = np.zeros((n, n))
+    for s in range(0, n, 64):
+        a[s:s + 64, s:s + 64] = kernels.gaussian_points(np.linspace(0, 1, 64), 0.3, 1.0)
+    h = hmat_amd.HODLRMatrixOMP(1e-8)
+    h.build_from_dense(a, 64)
+    b = np.random.default_rng(3).standard_normal((n, 1))
+    y, x = h.multiply(b), h.solve(b)
+    assert np.all(np.isfinite(y)) and np.all(np.isfinite(x))
+    assert rel(y, a @ b) < 1e-14
+    assert rel(x, np.linalg.solve(a, b)) < 1e-12
+
+
+def test_singular_leaf_reported():
+    n = 256
+    a = kernels.gaussian_points(np.linspace(0, 1, n), 0.1, 1.0)
+    a[5, :] = 0.0
+    a[:, 5] = 0.0
+    h = hmat_amd.HODLRMatrixOMP(1e-8)
+    h.build_from_dense(a, 64)
+    with pytest.raises(hmat_amd.HmatError) as e:
+        h.solve(np.ones(n))
+    assert e.value.status == 4  # HMAT_ERR_SINGULAR
+
+
+def test_nonfinite_input_reported():
+    n = 600
+    a = kernels.generate_matrix(kernels.CAUCHY, n)
+    a[400, 10] = np.nan
+    h = hmat_amd.HODLRMatrixOMP(1e-8)
+    with pytest.raises(hmat_amd.HmatError) as e:
+        h.build_from_dense(a, 100)
+    assert e.value.status == 5  # HMAT_ERR_NONFINITE
+
+
+def test_state_and_argument_errors():
+    h = hmat_amd.HODLRMatrixOMP(1e-8)
+    with pytest.raises(hmat_amd.HmatError) as e:
+        h._check_solve = h.solve(np.ones(4))
+    assert e.value.status == 6
+    with pytest.raises(hmat_amd.HmatError) as e:
+        h.build_from_dense(np.eye(8), 0)
+    assert e.value.status == 1
