@@ -78,6 +78,17 @@ int32_t hmat_device_count(void) {
   return n;
 }
 
+void hmat_profile_enable(int32_t on) { prof_set(on != 0); }
+void hmat_profile_reset(void) { prof_reset(); }
+int32_t hmat_profile_query(const char* name, int64_t* launches, double* ms, double* work) {
+  ProfEntry e;
+  if (!name || !prof_get(name, &e)) return 0;
+  if (launches) *launches = e.launches;
+  if (ms) *ms = e.ms;
+  if (work) *work = e.work;
+  return 1;
+}
+
 void hmat_options_reference(hmat_options* o, double tol) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
@@ -149,6 +160,37 @@ hmat_status hmat_hodlr_build_kernel(hmat_hodlr h, const hmat_kernel* k, int64_t 
     if (k->y) upload_cols(h->py, k->y, N, N, 1, h->eng.stream);
     GenDesc g = gen_from_kernel(*k, h->px.get(), k->y ? h->py.get() : nullptr);
     h->eng.build(g, N, leaf, nullptr, 0);
+  });
+}
+
+hmat_status hmat_hodlr_build_kernel_device(hmat_hodlr h, const hmat_kernel* k, int64_t N,
+                                           int32_t leaf) {
+  return guard([&] {
+    HMAT_REQUIRE(h && k && k->x && N > 0 && leaf > 0, HMAT_ERR_ARG, "bad build_kernel arguments");
+    HMAT_REQUIRE(k->type != HMAT_KERNEL_CAUCHY || k->y, HMAT_ERR_ARG, "Cauchy kernel needs y points");
+    GenDesc g = gen_from_kernel(*k, k->x, k->y);
+    h->eng.build(g, N, leaf, nullptr, 0);
+  });
+}
+
+hmat_status hmat_kernel_matvec_device(const hmat_kernel* k, int64_t N, const double* dX, int64_t ldx,
+                                      int32_t nrhs, double* dY, int64_t ldy) {
+  return guard([&] {
+    HMAT_REQUIRE(k && k->x && dX && dY && N > 0 && nrhs > 0 && ldx >= N && ldy >= N, HMAT_ERR_ARG,
+                 "bad kernel_matvec arguments");
+    check_device();
+    GenDesc g = gen_from_kernel(*k, k->x, k->y);
+    hipStream_t s;
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    kern::Workspace ws;
+    DevBuf<SegSk> ds;
+    for (int c0 = 0; c0 < nrhs; c0 += 64) {
+      const int nc = std::min(64, nrhs - c0);
+      std::vector<SegSk> seg{SegSk{0, 0, 0, (int64_t)c0 * ldx, (int64_t)c0 * ldy, (int32_t)N, (int32_t)N, 0, nc}};
+      kern::sketch(g, false, seg, ds, dX, ldx, dY, ldy, nc, ws, s);
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipStreamDestroy(s));
   });
 }
 

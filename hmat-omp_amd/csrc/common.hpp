@@ -79,6 +79,56 @@ inline std::vector<T> download(const T* d, size_t n, hipStream_t s) {
   return h;
 }
 
+// Opt-in kernel timing: HIP events around selected launches on the stream
+// they run on, plus the algorithmic work (flops or bytes) of each launch.
+struct ProfEntry {
+  int64_t launches = 0;
+  double ms = 0.0;
+  double work = 0.0;
+};
+bool prof_enabled();
+void prof_set(bool on);
+void prof_reset();
+void prof_add(const char* name, double ms, double work);
+bool prof_get(const char* name, ProfEntry* out);
+
+// RAII timer: records start at construction, stop + accumulate at finish().
+class ProfScope {
+ public:
+  ProfScope(const char* name, hipStream_t s, double work) : name_(name), s_(s), work_(work) {
+    on_ = prof_enabled();
+    if (on_) {
+      HIP_CHECK(hipEventCreate(&a_));
+      HIP_CHECK(hipEventCreate(&b_));
+      HIP_CHECK(hipEventRecord(a_, s_));
+    }
+  }
+  void finish() {
+    if (!on_) return;
+    HIP_CHECK(hipEventRecord(b_, s_));
+    HIP_CHECK(hipEventSynchronize(b_));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, a_, b_));
+    prof_add(name_, ms, work_);
+    (void)hipEventDestroy(a_);
+    (void)hipEventDestroy(b_);
+    on_ = false;
+  }
+  ~ProfScope() {
+    if (on_) {
+      (void)hipEventDestroy(a_);
+      (void)hipEventDestroy(b_);
+    }
+  }
+
+ private:
+  const char* name_;
+  hipStream_t s_;
+  double work_;
+  bool on_ = false;
+  hipEvent_t a_{}, b_{};
+};
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline int round_up(int a, int b) { return (int)((a + b - 1) / b) * b; }
 

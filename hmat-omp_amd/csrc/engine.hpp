@@ -94,9 +94,13 @@ inline GenDesc gen_from_kernel(const hmat_kernel& k, const double* dx, const dou
   return g;
 }
 
-// Segmented shifted CholeskyQR3 (Fukaya et al., SIAM J. Sci. Comput. 2020):
-// one shifted pass then CholeskyQR2, for each segment of a tall array.
-// Q overwrites `Y` (the result is copied back); optionally R = R3 R2 R1.
+// Segmented shifted CholeskyQR (Fukaya et al., SIAM J. Sci. Comput. 2020):
+// one shifted pass, then unshifted passes until every segment's Gram matrix
+// is the identity to kOrthTol = 1e-12 (two for well-conditioned input; sketches of
+// numerically low-rank blocks, cond ~1e16, take more).  Q overwrites `Y`;
+// optionally R = R_p ... R_2 R_1.
+constexpr double kOrthTol = 1e-12;
+constexpr int kOrthMaxPasses = 8;
 struct OrthSeg {
   int64_t off;  // row offset of the segment inside the N x kcap arrays
   int32_t len, k;
@@ -111,12 +115,14 @@ class Orthogonalizer {
 
  private:
   kern::Workspace ws_;
-  DevBuf<double> G_, R_, Ri_, Racc_, Rtmp_, shift_;
+  DevBuf<double> G_, R_, Ri_, Racc_, Rtmp_, shift_, gerr_;
   DevBuf<int> info_;
   DevBuf<SegNN> dnn_;
   DevBuf<SegTN> dtn_;
   DevBuf<SmallMat> dsm_;
-  void pass(const std::vector<OrthSeg>& segs, const double* in, double* out, int64_t ld, int kcap,
+  // One CholeskyQR pass in -> out.  Returns false (and leaves out alone)
+  // when `in` is already orthonormal to kOrthTol and `shifted` is false.
+  bool pass(const std::vector<OrthSeg>& segs, const double* in, double* out, int64_t ld, int kcap,
             bool shifted, hipStream_t s);
 };
 

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "hodlr.hpp"
 
@@ -20,6 +21,7 @@ namespace hmat {
 HodlrEngine::HodlrEngine(const hmat_options& o) : opt(o) {
   HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   own_stream = true;
+  if (opt.precision == HMAT_PREC_MIXED) refine_rtol = 1e-10;
 }
 
 HodlrEngine::~HodlrEngine() {
@@ -58,6 +60,8 @@ void HodlrEngine::build(const GenDesc& g, int64_t n, int leaf_size, const double
   HMAT_REQUIRE(opt.rank > 0 && opt.oversample >= 0 && opt.power_iters >= 0, HMAT_ERR_ARG,
                "bad rank/oversample/power_iters");
   built = factored = false;
+  f64_.clear();
+  f32_.clear();
   N = n;
   leaf = leaf_size;
   tree.build(N, leaf);
@@ -146,7 +150,7 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
     }
     kcap = std::max(kcap, k[s]);
   }
-  HMAT_REQUIRE(kcap <= 512, HMAT_ERR_ARG, "block sketch width above 512 is not supported");
+  HMAT_REQUIRE(kcap <= 300, HMAT_ERR_ARG, "block sketch width above 300 is not supported");
   const int64_t nk = N * (int64_t)kcap;
   Yb_.reserve(nk); Xb_.reserve(nk); Tb_.reserve(nk);
   HIP_CHECK(hipMemsetAsync(Yb_.get(), 0, nk * sizeof(double), stream));
@@ -210,12 +214,21 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
   }
 
   // Y = A Omega, power iterations, Q = orth(Y)      (compression.h:78-84)
-  kern::sketch(g, false, sk_fwd, dsk_, Xb_.get(), N, Yb_.get(), N, kcap, ws_, stream);
+  // Mixed precision runs these range-finding passes on BF16 MFMA; the
+  // projection B^T = A^T Q below stays FP64 in both modes.
+  const bool lp = (opt.precision == HMAT_PREC_MIXED);
+  auto range_pass = [&](bool trans, const std::vector<SegSk>& sk, const double* X, double* Y) {
+    if (lp)
+      kern::sketch_lp(g, trans, kLowPrecSplit, sk, dsk_, X, N, Y, N, kcap, ws_, stream);
+    else
+      kern::sketch(g, trans, sk, dsk_, X, N, Y, N, kcap, ws_, stream);
+  };
+  range_pass(false, sk_fwd, Xb_.get(), Yb_.get());
   for (int it = 0; it < opt.power_iters && !sk_fwd_rsvd.empty(); ++it) {
     orth_.run(orow_rsvd, Yb_.get(), Tb_.get(), N, kcap, nullptr, stream);
-    kern::sketch(g, true, sk_bwd_rsvd, dsk_, Yb_.get(), N, Xb_.get(), N, kcap, ws_, stream);
+    range_pass(true, sk_bwd_rsvd, Yb_.get(), Xb_.get());
     orth_.run(ocol_rsvd, Xb_.get(), Tb_.get(), N, kcap, nullptr, stream);
-    kern::sketch(g, false, sk_fwd_rsvd, dsk_, Xb_.get(), N, Yb_.get(), N, kcap, ws_, stream);
+    range_pass(false, sk_fwd_rsvd, Xb_.get(), Yb_.get());
   }
   orth_.run(orow, Yb_.get(), Tb_.get(), N, kcap, nullptr, stream);
   // B^T = A^T Q (n x k), then B^T = Qb R                 (compression.h:87)
@@ -267,22 +280,22 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 
-// y = D x over all leaves (beta = 0) or y += D x.
-void HodlrEngine::leaf_apply(const DevBuf<double>& Dm, const double* B, int64_t ldb, int ncols,
-                             double* Y, int64_t ldy, double beta) {
+// Y = D X over all leaves (leaf blocks stored lmax x lmax).
+template <class F>
+void HodlrEngine::leaf_apply_t(const F* Dm, const F* B, int64_t ldb, int ncols, F* Y, int64_t ldy) {
   const int64_t lm = tree.lmax;
   std::vector<SegNN> segs;
   for (size_t i = 0; i < tree.leaves.size(); ++i) {
     const TNode& nd = tree.nodes[tree.leaves[i]];
     segs.push_back(SegNN{(int64_t)i * lm * lm, nd.start, nd.start, (int32_t)nd.len, (int32_t)nd.len, ncols, 0});
   }
-  kern::seg_gemm_nn(segs, dnn_, Dm.get(), lm, B, ldb, Y, ldy, 1.0, beta, stream);
+  kern::seg_gemm_nn<F>(segs, dnn_, Dm, lm, B, ldb, Y, ldy, F(1), F(0), stream);
 }
 
 void HodlrEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "matvec before build");
   HMAT_REQUIRE(nrhs > 0 && ldx >= N && ldy >= N, HMAT_ERR_ARG, "bad matvec dimensions");
-  leaf_apply(D, dX, ldx, nrhs, dY, ldy, 0.0);
+  leaf_apply_t<double>(D.get(), dX, ldx, nrhs, dY, ldy);
   for (int c = 1; c <= tree.max_depth; ++c) {
     HLevel& L = lv[c];
     const int ns = (int)L.ids.size();
@@ -297,23 +310,25 @@ void HodlrEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, in
       // y_s += (U S)_s t_sib
       nn[s] = SegNN{nd.start, (int64_t)L.sib[s] * tstride, nd.start, (int32_t)nd.len, L.rank[s], nrhs, 0};
     }
-    kern::seg_gemm_tn(tn, dtn_, L.V.get(), N, dX, ldx, tvec_.get(), L.rmax, 1.0, 0.0, ws_, stream);
-    kern::seg_gemm_nn(nn, dnn_, L.US.get(), N, tvec_.get(), L.rmax, dY, ldy, 1.0, 1.0, stream);
+    kern::seg_gemm_tn<double>(tn, dtn_, L.V.get(), N, dX, ldx, tvec_.get(), L.rmax, 1.0, 0.0, ws_, stream);
+    kern::seg_gemm_nn<double>(nn, dnn_, L.US.get(), N, tvec_.get(), L.rmax, dY, ldy, 1.0, 1.0, stream);
   }
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 
-// W(rows of internal depth-j nodes, 0:ncols) -= Q C M^{-1} V^T W
-void HodlrEngine::correction(int j, double* W, int64_t ldw, int ncols) {
+// W(rows of internal depth-j nodes, 0:ncols) -= Q C M^{-1} V^T W, with
+//   M = [[I, V_l^T Q_l], [V_r^T Q_r, I]] and C swapping the two halves.
+template <class F>
+void HodlrEngine::correction_t(HFactor<F>& fa, int j, F* W, int64_t ldw, int ncols) {
   HParents& P = par[j];
   const HLevel& C = lv[j + 1];
   const int np = (int)P.ids.size();
   if (np == 0 || ncols == 0) return;
   const int64_t mm = P.mmax;
   const int64_t tst = mm * ncols;
-  tvec_.reserve((size_t)np * tst);
-  coef_.reserve((size_t)np * tst);
-  const double* Q = Wall.get() + colofs[j + 1] * N;
+  fa.tvec.reserve((size_t)np * tst);
+  fa.coef.reserve((size_t)np * tst);
+  const F* Q = fa.Wall.get() + colofs[j + 1] * N;
   std::vector<SegTN> tn;
   std::vector<SegNN> mn, up;
   for (int p = 0; p < np; ++p) {
@@ -326,25 +341,34 @@ void HodlrEngine::correction(int j, double* W, int64_t ldw, int ncols) {
     up.push_back(SegNN{l.start, (int64_t)p * tst + rr, l.start, (int32_t)l.len, rl, ncols, 0});
     up.push_back(SegNN{r.start, (int64_t)p * tst, r.start, (int32_t)r.len, rr, ncols, 0});
   }
-  kern::seg_gemm_tn(tn, dtn_, C.V.get(), N, W, ldw, tvec_.get(), mm, 1.0, 0.0, ws_, stream);
-  kern::seg_gemm_nn(mn, dnn_, P.Minv.get(), mm, tvec_.get(), mm, coef_.get(), mm, 1.0, 0.0, stream);
-  kern::seg_gemm_nn(up, dnn_, Q, N, coef_.get(), mm, W, ldw, -1.0, 1.0, stream);
+  kern::seg_gemm_tn<F>(tn, dtn_, fa.V[j + 1], N, W, ldw, fa.tvec.get(), mm, F(1), F(0), ws_, stream);
+  kern::seg_gemm_nn<F>(mn, dnn_, fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(0), stream);
+  kern::seg_gemm_nn<F>(up, dnn_, Q, N, fa.coef.get(), mm, W, ldw, F(-1), F(1), stream);
 }
 
-void HodlrEngine::factor() {
-  HMAT_REQUIRE(built, HMAT_ERR_STATE, "factor before build");
+template <class F>
+static void copy_or_convert(const double* src, F* dst, int64_t n, hipStream_t s) {
+  if constexpr (std::is_same<F, double>::value) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  } else {
+    kern::convert<F, double>(src, dst, n, s);
+  }
+}
+
+template <class F>
+void HodlrEngine::factor_t(HFactor<F>& fa) {
+  fa.clear();
   const int64_t lm = tree.lmax;
   const int nleaf = (int)tree.leaves.size();
   // leaf inverses (the reference refactorises D on every visit,
   // hodlr_routines_omp.h:150-154; here once)
-  Dinv.resize((size_t)nleaf * lm * lm);
-  HIP_CHECK(hipMemcpyAsync(Dinv.get(), D.get(), (size_t)nleaf * lm * lm * sizeof(double),
-                           hipMemcpyDeviceToDevice, stream));
+  fa.Dinv.resize((size_t)nleaf * lm * lm);
+  copy_or_convert<F>(D.get(), fa.Dinv.get(), (int64_t)nleaf * lm * lm, stream);
   std::vector<SmallMat> sm(nleaf);
   for (int i = 0; i < nleaf; ++i)
     sm[i] = SmallMat{(int64_t)i * lm * lm, (int32_t)tree.nodes[tree.leaves[i]].len, (int32_t)lm};
   info_.reserve(std::max(nleaf, 1));
-  kern::gj_inverse(sm, dsm_, Dinv.get(), info_.get(), stream);
+  kern::gj_inverse<F>(sm, dsm_, fa.Dinv.get(), info_.get(), stream);
   {
     auto info = download(info_.get(), nleaf, stream);
     for (int i = 0; i < nleaf; ++i)
@@ -356,26 +380,35 @@ void HodlrEngine::factor() {
   colofs.assign(dmax + 2, 0);
   for (int c = 1; c <= dmax; ++c) colofs[c + 1] = colofs[c] + lv[c].rmax;
   Rtot = colofs[dmax + 1];
-  par.clear();
-  par.resize(std::max(dmax, 1));
+  fa.V.assign(dmax + 1, nullptr);
+  fa.Vown.clear();
+  fa.Vown.resize(dmax + 1);
+  fa.Minv.clear();
+  fa.Minv.resize(std::max(dmax, 1));
+  par.assign(std::max(dmax, 1), HParents{});
   if (dmax == 0) {
-    factored = true;
+    fa.ready = true;
     return;
   }
+  for (int c = 1; c <= dmax; ++c) {
+    if constexpr (std::is_same<F, double>::value) {
+      fa.V[c] = lv[c].V.get();
+    } else {
+      fa.Vown[c].resize(lv[c].V.size());
+      kern::convert<F, double>(lv[c].V.get(), fa.Vown[c].get(), (int64_t)lv[c].V.size(), stream);
+      fa.V[c] = fa.Vown[c].get();
+    }
+  }
   // W_c = D^{-1} (U S)_c for every depth c, stacked by columns
-  Wall.resize((size_t)N * Rtot);
-  wtmp_.reserve((size_t)N * Rtot);
-  HIP_CHECK(hipMemsetAsync(wtmp_.get(), 0, (size_t)N * Rtot * sizeof(double), stream));
+  fa.Wall.resize((size_t)N * Rtot);
+  fa.wtmp.reserve((size_t)N * Rtot);
   for (int c = 1; c <= dmax; ++c)
-    if (lv[c].rmax)
-      HIP_CHECK(hipMemcpyAsync(wtmp_.get() + colofs[c] * N, lv[c].US.get(),
-                               (size_t)N * lv[c].rmax * sizeof(double), hipMemcpyDeviceToDevice, stream));
-  leaf_apply(Dinv, wtmp_.get(), N, (int)Rtot, Wall.get(), N, 0.0);
+    if (lv[c].rmax) copy_or_convert<F>(lv[c].US.get(), fa.wtmp.get() + colofs[c] * N, N * lv[c].rmax, stream);
+  leaf_apply_t<F>(fa.Dinv.get(), fa.wtmp.get(), N, (int)Rtot, fa.Wall.get(), N);
 
   for (int j = dmax - 1; j >= 0; --j) {
     HParents& P = par[j];
     const HLevel& C = lv[j + 1];
-    P.ids.clear(); P.lpos.clear(); P.rpos.clear(); P.msize.clear();
     for (int id : tree.by_depth[j]) {
       const TNode& nd = tree.nodes[id];
       if (nd.leaf) continue;
@@ -388,12 +421,12 @@ void HodlrEngine::factor() {
     P.mmax = 0;
     for (int m : P.msize) P.mmax = std::max(P.mmax, m);
     const int64_t mm = P.mmax;
-    P.Minv.resize((size_t)np * mm * mm);
+    fa.Minv[j].resize((size_t)np * mm * mm);
     // M = [[I, V_l^T Q_l], [V_r^T Q_r, I]]
     std::vector<SegNN> eye(np);
     std::vector<SegTN> kb;
     std::vector<SmallMat> ms(np);
-    const double* Q = Wall.get() + colofs[j + 1] * N;
+    const F* Q = fa.Wall.get() + colofs[j + 1] * N;
     for (int p = 0; p < np; ++p) {
       const TNode& l = tree.nodes[C.ids[P.lpos[p]]];
       const TNode& r = tree.nodes[C.ids[P.rpos[p]]];
@@ -403,32 +436,97 @@ void HodlrEngine::factor() {
       kb.push_back(SegTN{r.start, r.start, (int64_t)p * mm * mm + rr, (int32_t)r.len, rl, rr, 0});
       ms[p] = SmallMat{(int64_t)p * mm * mm, P.msize[p], (int32_t)mm};
     }
-    P.Minv.zero(stream);
-    kern::fill_identity_segs(eye, dnn_, P.Minv.get(), mm, stream);
-    kern::seg_gemm_tn(kb, dtn_, C.V.get(), N, Q, N, P.Minv.get(), mm, 1.0, 0.0, ws_, stream);
+    fa.Minv[j].zero(stream);
+    kern::fill_identity_segs<F>(eye, dnn_, fa.Minv[j].get(), mm, stream);
+    kern::seg_gemm_tn<F>(kb, dtn_, fa.V[j + 1], N, Q, N, fa.Minv[j].get(), mm, F(1), F(0), ws_, stream);
     info_.reserve(np);
-    kern::gj_inverse(ms, dsm_, P.Minv.get(), info_.get(), stream);
+    kern::gj_inverse<F>(ms, dsm_, fa.Minv[j].get(), info_.get(), stream);
     auto info = download(info_.get(), np, stream);
     for (int p = 0; p < np; ++p)
       if (info[p] != 0)
         throw Error(HMAT_ERR_SINGULAR, "singular capacitance matrix at depth " + std::to_string(j));
     // push the depth-j correction into the bases of shallower depths
-    if (colofs[j + 1] > 0) correction(j, Wall.get(), N, (int)colofs[j + 1]);
+    if (colofs[j + 1] > 0) correction_t<F>(fa, j, fa.Wall.get(), N, (int)colofs[j + 1]);
   }
   HIP_CHECK(hipStreamSynchronize(stream));
-  wtmp_.release();
+  fa.wtmp.release();
+  fa.ready = true;
+}
+
+template <class F>
+void HodlrEngine::apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int ncols, F* X,
+                                  int64_t ldx) {
+  leaf_apply_t<F>(fa.Dinv.get(), B, ldb, ncols, X, ldx);
+  for (int j = tree.max_depth - 1; j >= 0; --j) correction_t<F>(fa, j, X, ldx, ncols);
+}
+
+void HodlrEngine::factor() {
+  HMAT_REQUIRE(built, HMAT_ERR_STATE, "factor before build");
+  if (opt.precision == HMAT_PREC_MIXED) {
+    f64_.clear();
+    factor_t<float>(f32_);
+  } else {
+    f32_.clear();
+    factor_t<double>(f64_);
+  }
   factored = true;
 }
 
+// One application of the factored inverse, FP64 in and out.
+void HodlrEngine::apply_inverse(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx) {
+  if (opt.precision != HMAT_PREC_MIXED) {
+    apply_inverse_t<double>(f64_, B, ldb, ncols, X, ldx);
+    return;
+  }
+  f32_.b.reserve((size_t)N * ncols);
+  f32_.x.reserve((size_t)N * ncols);
+  kern::convert_cols<float, double>(N, ncols, B, ldb, f32_.b.get(), N, stream);
+  apply_inverse_t<float>(f32_, f32_.b.get(), N, ncols, f32_.x.get(), N);
+  kern::convert_cols<double, float>(N, ncols, f32_.x.get(), N, X, ldx, stream);
+}
+
+// X = A_H^{-1} B; then FP64 iterative refinement on the compressed operator
+// (residual through the FP64 matvec, correction accumulated in FP64) until
+// max_j ||b_j - A_H x_j|| / ||b_j|| <= refine_rtol.
 void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "solve before build");
   HMAT_REQUIRE(nrhs > 0 && ldb >= N && ldx >= N, HMAT_ERR_ARG, "bad solve dimensions");
   if (!factored) factor();
-  leaf_apply(Dinv, dB, ldb, nrhs, dX, ldx, 0.0);
-  for (int j = tree.max_depth - 1; j >= 0; --j) correction(j, dX, ldx, nrhs);
-  HIP_CHECK(hipStreamSynchronize(stream));
+  apply_inverse(dB, ldb, nrhs, dX, ldx);
   last_iters = 0;
-  last_resid = 0.0;
+  last_resid = -1.0;
+  if (!(refine_rtol > 0.0)) {
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return;
+  }
+  ir_y_.reserve((size_t)N * nrhs);
+  ir_r_.reserve((size_t)N * nrhs);
+  ir_n_.reserve((size_t)2 * nrhs);
+  kern::col_norms2(dB, ldb, N, nrhs, ir_n_.get(), stream);
+  auto bn = download(ir_n_.get(), nrhs, stream);
+  double resid = 0.0;
+  int it = 0;
+  for (;; ++it) {
+    matvec(dX, ldx, nrhs, ir_y_.get(), N);
+    kern::axpby_cols(N, nrhs, 1.0, dB, ldb, -1.0, ir_y_.get(), N, ir_r_.get(), N, stream);
+    kern::col_norms2(ir_r_.get(), N, N, nrhs, ir_n_.get(), stream);
+    auto rn = download(ir_n_.get(), nrhs, stream);
+    resid = 0.0;
+    for (int j = 0; j < nrhs; ++j)
+      resid = std::max(resid, bn[j] > 0.0 ? std::sqrt(rn[j] / bn[j]) : std::sqrt(rn[j]));
+    if (!std::isfinite(resid)) throw Error(HMAT_ERR_NONFINITE, "non-finite residual in refinement");
+    if (resid <= refine_rtol || it >= refine_max) break;
+    // correction d = A_H^{-1} r, accumulated into X in FP64
+    apply_inverse(ir_r_.get(), N, nrhs, ir_y_.get(), N);
+    kern::axpby_cols(N, nrhs, 1.0, dX, ldx, 1.0, ir_y_.get(), N, dX, ldx, stream);
+  }
+  HIP_CHECK(hipStreamSynchronize(stream));
+  last_iters = it;
+  last_resid = resid;
+  if (resid > refine_rtol && opt.precision == HMAT_PREC_MIXED)
+    throw Error(HMAT_ERR_NOT_CONVERGED, "refinement stopped at relative residual " +
+                                            std::to_string(resid) + " after " + std::to_string(it) +
+                                            " iterations");
 }
 
 void HodlrEngine::to_dense_host(double* A, int64_t lda) {

@@ -9,6 +9,10 @@ namespace hmat {
 
 enum class BlockMode { RSVD, ExactWide, ExactTall };
 
+// BF16 terms per operand in the mixed-precision range finder (3 terms carry
+// the 24 bits of an FP32 entry).
+constexpr int kLowPrecSplit = 3;
+
 // Does an m x n block take the randomised branch?  compression.h:72-75 in
 // tolerance mode; in fixed-rank mode also exact when k >= min(m, n).
 inline bool uses_rsvd(int64_t m, int64_t n, const hmat_options& o) {
@@ -33,12 +37,30 @@ struct HLevel {
   std::vector<double> S;   // nseg x rmax singular values (host copy)
 };
 
-// Internal nodes of depth j with their Woodbury capacitance inverses.
+// Internal nodes of depth j (children at depth j+1).
 struct HParents {
   std::vector<int> ids, lpos, rpos;
-  std::vector<int> msize;
+  std::vector<int> msize;  // rank(l) + rank(r): size of the capacitance M
   int mmax = 0;
-  DevBuf<double> Minv;  // np x mmax x mmax
+};
+
+// Factored inverse of the compressed operator in precision F:
+//   leaf inverses, Q_c = A_node^{-1} (U S)_c for every depth, and the
+//   capacitance inverses M^{-1} = (I + K C)^{-1} of every internal node.
+template <class F>
+struct HFactor {
+  DevBuf<F> Dinv;                 // nleaf x lmax x lmax
+  DevBuf<F> Wall;                 // N x Rtot
+  std::vector<DevBuf<F>> Vown;    // per depth, FP32 copies of V
+  std::vector<const F*> V;        // per depth
+  std::vector<DevBuf<F>> Minv;    // per parent depth: np x mmax x mmax
+  DevBuf<F> tvec, coef, wtmp, b, x;
+  bool ready = false;
+  void clear() {
+    Dinv.release(); Wall.release(); Vown.clear(); V.clear(); Minv.clear();
+    tvec.release(); coef.release(); wtmp.release(); b.release(); x.release();
+    ready = false;
+  }
 };
 
 class HodlrEngine {
@@ -62,23 +84,31 @@ class HodlrEngine {
   int leaf = 0;
   Tree tree;
   std::vector<HLevel> lv;  // index = depth (lv[0] unused)
-  DevBuf<double> D, Dinv;  // nleaf x lmax x lmax
+  DevBuf<double> D;        // nleaf x lmax x lmax
   bool built = false, factored = false;
   std::vector<HParents> par;  // index = depth of the parent
-  DevBuf<double> Wall;        // N x Rtot: Q_c = A_node^{-1} U_c S_c
   std::vector<int64_t> colofs;
   int64_t Rtot = 0;
-  double refine_rtol = 1e-10;
-  int refine_max = 50;
+  // FP64 refinement: on by default in mixed precision, opt-in in FP64
+  double refine_rtol = 0.0;
+  int refine_max = 30;
   int last_iters = 0;
-  double last_resid = 0.0;
+  double last_resid = -1.0;
 
  private:
   void build_level(int c, const GenDesc& g, const std::vector<int64_t>& omega_off,
                    const double* host_omega);
-  void correction(int j, double* W, int64_t ldw, int ncols);
-  void leaf_apply(const DevBuf<double>& Dm, const double* B, int64_t ldb, int ncols, double* Y,
-                  int64_t ldy, double beta);
+  template <class F> void factor_t(HFactor<F>& fa);
+  template <class F> void correction_t(HFactor<F>& fa, int j, F* W, int64_t ldw, int ncols);
+  template <class F> void apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int ncols, F* X,
+                                          int64_t ldx);
+  template <class F> void leaf_apply_t(const F* Dm, const F* B, int64_t ldb, int ncols, F* Y,
+                                       int64_t ldy);
+  void apply_inverse(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx);
+
+  HFactor<double> f64_;
+  HFactor<float> f32_;
+  DevBuf<double> ir_y_, ir_r_, ir_n_;
 
   kern::Workspace ws_;
   Orthogonalizer orth_;
@@ -88,7 +118,7 @@ class HodlrEngine {
   DevBuf<SmallMat> dsm_;
   DevBuf<BlockDesc> dbd_;
   DevBuf<double> Yb_, Xb_, Tb_, Rb_, Vr_, Sv_, Wk_, TU_, TV_;
-  DevBuf<double> tvec_, coef_, wtmp_;
+  DevBuf<double> tvec_;
   DevBuf<int> info_;
 };
 

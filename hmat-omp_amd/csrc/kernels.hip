@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "generators.hpp"
 #include "kernels.hpp"
 #include "mfma.hpp"
 
@@ -28,13 +29,13 @@ struct NNTile {
   int32_t seg, row0, c0, pad_;
 };
 
-template <int R>
+template <class F, int R>
 __global__ __launch_bounds__(kWG) void k_seg_gemm_nn(const SegNN* __restrict__ segs,
                                                      const NNTile* __restrict__ tiles,
-                                                     const double* __restrict__ X, int64_t ldx,
-                                                     const double* __restrict__ T, int64_t ldt,
-                                                     double* __restrict__ Y, int64_t ldy,
-                                                     double alpha, double beta) {
+                                                     const F* __restrict__ X, int64_t ldx,
+                                                     const F* __restrict__ T, int64_t ldt,
+                                                     F* __restrict__ Y, int64_t ldy,
+                                                     F alpha, F beta) {
   constexpr int G = kWG / R;
   constexpr int CPT = 16;
   const NNTile tl = tiles[blockIdx.x];
@@ -44,13 +45,13 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_nn(const SegNN* __restrict__ s
   const int row = tl.row0 + r;
   const bool live = row < sg.len;
   const int ncols = min(G * CPT, sg.q - tl.c0);
-  double acc[CPT];
+  F acc[CPT];
 #pragma unroll
-  for (int i = 0; i < CPT; ++i) acc[i] = 0.0;
-  const double* xr = X + sg.xoff + row;
-  const double* tb = T + sg.toff + (int64_t)(tl.c0 + g) * ldt;
+  for (int i = 0; i < CPT; ++i) acc[i] = F(0);
+  const F* xr = X + sg.xoff + row;
+  const F* tb = T + sg.toff + (int64_t)(tl.c0 + g) * ldt;
   for (int i = 0; i < sg.p; ++i) {
-    const double xv = live ? xr[(int64_t)i * ldx] : 0.0;
+    const F xv = live ? xr[(int64_t)i * ldx] : F(0);
 #pragma unroll
     for (int cc = 0; cc < CPT; ++cc) {
       if (g + cc * G < ncols) acc[cc] = fma(xv, tb[i + (int64_t)(cc * G) * ldt], acc[cc]);
@@ -61,17 +62,17 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_nn(const SegNN* __restrict__ s
   for (int cc = 0; cc < CPT; ++cc) {
     const int c = g + cc * G;
     if (c < ncols) {
-      double* yp = Y + sg.yoff + row + (int64_t)(tl.c0 + c) * ldy;
-      double v = alpha * acc[cc];
-      if (beta != 0.0) v = fma(beta, *yp, v);
+      F* yp = Y + sg.yoff + row + (int64_t)(tl.c0 + c) * ldy;
+      F v = alpha * acc[cc];
+      if (beta != F(0)) v = fma(beta, *yp, v);
       *yp = v;
     }
   }
 }
 
-void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X,
-                 int64_t ldx, const double* T, int64_t ldt, double* Y, int64_t ldy,
-                 double alpha, double beta, hipStream_t s) {
+template <class F>
+void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* X, int64_t ldx,
+                 const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha, F beta, hipStream_t s) {
   if (segs.empty()) return;
   int qmax = 0;
   for (auto& g : segs) qmax = std::max(qmax, g.q);
@@ -91,13 +92,67 @@ void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const dou
   dt.upload(tiles, s);
   dim3 grid((unsigned)tiles.size());
   if (R == 256)
-    k_seg_gemm_nn<256><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
+    k_seg_gemm_nn<F, 256><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
   else if (R == 128)
-    k_seg_gemm_nn<128><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
+    k_seg_gemm_nn<F, 128><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
   else
-    k_seg_gemm_nn<64><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
+    k_seg_gemm_nn<F, 64><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));  // dt is freed on return
+}
+template void seg_gemm_nn<double>(const std::vector<SegNN>&, DevBuf<SegNN>&, const double*, int64_t,
+                                  const double*, int64_t, double*, int64_t, double, double, hipStream_t);
+template void seg_gemm_nn<float>(const std::vector<SegNN>&, DevBuf<SegNN>&, const float*, int64_t,
+                                 const float*, int64_t, float*, int64_t, float, float, hipStream_t);
+
+// --------------------------------------------------------------------------
+// seg_trsm_upper: one thread per row, the row held in LDS (column-major so
+// that the 64 lanes touch 64 consecutive words), R read with wave-uniform
+// addresses.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_seg_trsm_upper(const SegNN* __restrict__ segs,
+                                                       const NNTile* __restrict__ tiles,
+                                                       const double* __restrict__ X, int64_t ldx,
+                                                       const double* __restrict__ R, int64_t ldr,
+                                                       double* __restrict__ Y, int64_t ldy) {
+  extern __shared__ double q[];  // [k][64]
+  const NNTile tl = tiles[blockIdx.x];
+  const SegNN sg = segs[tl.seg];
+  const int t = threadIdx.x;
+  const int row = tl.row0 + t;
+  const bool live = row < sg.len;
+  const int k = sg.q;
+  for (int j = 0; j < k; ++j) q[j * 64 + t] = live ? X[sg.xoff + row + (int64_t)j * ldx] : 0.0;
+  for (int j = 0; j < k; ++j) {
+    const double* rj = R + sg.toff + (int64_t)j * ldr;
+    double acc = q[j * 64 + t];
+    for (int i = 0; i < j; ++i) acc = fma(-q[i * 64 + t], rj[i], acc);
+    q[j * 64 + t] = rj[j] != 0.0 ? acc / rj[j] : 0.0;
+  }
+  if (!live) return;
+  for (int j = 0; j < k; ++j) Y[sg.yoff + row + (int64_t)j * ldy] = q[j * 64 + t];
+}
+
+void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X,
+                    int64_t ldx, const double* R, int64_t ldr, double* Y, int64_t ldy, hipStream_t s) {
+  if (segs.empty()) return;
+  int kmax = 0;
+  std::vector<NNTile> tiles;
+  for (size_t i = 0; i < segs.size(); ++i) {
+    const auto& g = segs[i];
+    if (g.len <= 0 || g.q <= 0) continue;
+    kmax = std::max(kmax, g.q);
+    for (int r0 = 0; r0 < g.len; r0 += 64) tiles.push_back({(int32_t)i, r0, 0, 0});
+  }
+  if (tiles.empty()) return;
+  HMAT_REQUIRE(kmax <= 300, HMAT_ERR_ARG, "seg_trsm_upper: k > 300");
+  dsegs.upload(segs, s);
+  DevBuf<NNTile> dt;
+  dt.upload(tiles, s);
+  const size_t lds = (size_t)kmax * 64 * sizeof(double);
+  k_seg_trsm_upper<<<(unsigned)tiles.size(), 64, lds, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // --------------------------------------------------------------------------
@@ -111,33 +166,34 @@ struct TNOut {
   int32_t seg, pt, qt, slot0, nslot, pad_;
 };
 
+template <class F>
 __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part(const SegTN* __restrict__ segs,
                                                           const TNItem* __restrict__ items,
-                                                          const double* __restrict__ X, int64_t ldx,
-                                                          const double* __restrict__ Y, int64_t ldy,
-                                                          double* __restrict__ part) {
-  __shared__ double xs[32][65];
-  __shared__ double ys[32][65];
+                                                          const F* __restrict__ X, int64_t ldx,
+                                                          const F* __restrict__ Y, int64_t ldy,
+                                                          F* __restrict__ part) {
+  __shared__ F xs[32][65];
+  __shared__ F ys[32][65];
   const TNItem it = items[blockIdx.x];
   const SegTN sg = segs[it.seg];
   const int ti = threadIdx.x % 16, tj = threadIdx.x / 16;
-  double acc[4][4];
+  F acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int b = 0; b < 4; ++b) acc[a][b] = F(0);
   const int pc0 = it.pt * 64, qc0 = it.qt * 64;
   for (int rb = it.r0; rb < it.r1; rb += 32) {
     for (int e = threadIdx.x; e < 32 * 64; e += kWG) {
       const int rr = e % 32, cc = e / 32, row = rb + rr;
       const bool rok = row < it.r1;
-      xs[rr][cc] = (rok && pc0 + cc < sg.p) ? X[sg.xoff + row + (int64_t)(pc0 + cc) * ldx] : 0.0;
-      ys[rr][cc] = (rok && qc0 + cc < sg.q) ? Y[sg.yoff + row + (int64_t)(qc0 + cc) * ldy] : 0.0;
+      xs[rr][cc] = (rok && pc0 + cc < sg.p) ? X[sg.xoff + row + (int64_t)(pc0 + cc) * ldx] : F(0);
+      ys[rr][cc] = (rok && qc0 + cc < sg.q) ? Y[sg.yoff + row + (int64_t)(qc0 + cc) * ldy] : F(0);
     }
     __syncthreads();
 #pragma unroll 4
     for (int rr = 0; rr < 32; ++rr) {
-      double xv[4], yv[4];
+      F xv[4], yv[4];
 #pragma unroll
       for (int a = 0; a < 4; ++a) xv[a] = xs[rr][ti + 16 * a];
 #pragma unroll
@@ -149,41 +205,43 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part(const SegTN* __restric
     }
     __syncthreads();
   }
-  double* pp = part + (int64_t)it.slot * 4096;
+  F* pp = part + (int64_t)it.slot * 4096;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) pp[(ti + 16 * a) + (tj + 16 * b) * 64] = acc[a][b];
 }
 
+template <class F>
 __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_reduce(const SegTN* __restrict__ segs,
                                                             const TNOut* __restrict__ outs,
-                                                            const double* __restrict__ part,
-                                                            double* __restrict__ O, int64_t ldo,
-                                                            double alpha, double beta) {
+                                                            const F* __restrict__ part,
+                                                            F* __restrict__ O, int64_t ldo,
+                                                            F alpha, F beta) {
+  // blockIdx.y picks 256 of the 4096 entries of the output tile
   const TNOut ot = outs[blockIdx.x];
   const SegTN sg = segs[ot.seg];
-  for (int e = threadIdx.x; e < 4096; e += kWG) {
-    const int i = e % 64, j = e / 64;
-    const int gi = ot.pt * 64 + i, gj = ot.qt * 64 + j;
-    if (gi >= sg.p || gj >= sg.q) continue;
-    double sum = 0.0;
-    for (int c = 0; c < ot.nslot; ++c) sum += part[(int64_t)(ot.slot0 + c) * 4096 + e];
-    double* o = O + sg.ooff + gi + (int64_t)gj * ldo;
-    double v = alpha * sum;
-    if (beta != 0.0) v = fma(beta, *o, v);
-    *o = v;
-  }
+  const int e = blockIdx.y * kWG + threadIdx.x;
+  const int i = e % 64, j = e / 64;
+  const int gi = ot.pt * 64 + i, gj = ot.qt * 64 + j;
+  if (gi >= sg.p || gj >= sg.q) return;
+  F sum = F(0);
+  for (int c = 0; c < ot.nslot; ++c) sum += part[(int64_t)(ot.slot0 + c) * 4096 + e];
+  F* o = O + sg.ooff + gi + (int64_t)gj * ldo;
+  F v = alpha * sum;
+  if (beta != F(0)) v = fma(beta, *o, v);
+  *o = v;
 }
 
-void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const double* X,
-                 int64_t ldx, const double* Y, int64_t ldy, double* O, int64_t ldo,
-                 double alpha, double beta, Workspace& ws, hipStream_t s) {
+template <class F>
+void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* X, int64_t ldx,
+                 const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta, Workspace& ws,
+                 hipStream_t s) {
   if (segs.empty()) return;
   int64_t total_rows = 0;
   for (auto& g : segs) total_rows += std::max(g.len, 0);
-  // ~2048 partial items in total, chunks a multiple of 32 rows
-  int64_t ch = std::max<int64_t>(64, round_up((int)std::min<int64_t>(ceil_div(total_rows, 2048), 1 << 20), 32));
+  // ~1024 partial items in total, chunks a multiple of 32 rows
+  int64_t ch = std::max<int64_t>(256, round_up((int)std::min<int64_t>(ceil_div(total_rows, 1024), 1 << 20), 32));
   std::vector<TNItem> items;
   std::vector<TNOut> outs;
   int slot = 0;
@@ -207,274 +265,27 @@ void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const dou
       }
   }
   if (outs.empty()) return;
-  ws.part.reserve((size_t)slot * 4096);
+  DevBuf<F>& part = ws.parts<F>();
+  part.reserve((size_t)slot * 4096);
   dsegs.upload(segs, s);
   DevBuf<TNItem> di;
   di.upload(items, s);
   DevBuf<TNOut> dout;
   dout.upload(outs, s);
-  k_seg_gemm_tn_part<<<(unsigned)items.size(), kWG, 0, s>>>(dsegs.get(), di.get(), X, ldx, Y, ldy,
-                                                            ws.part.get());
+  k_seg_gemm_tn_part<F><<<(unsigned)items.size(), kWG, 0, s>>>(dsegs.get(), di.get(), X, ldx, Y, ldy,
+                                                               part.get());
   HIP_CHECK(hipGetLastError());
-  k_seg_gemm_tn_reduce<<<(unsigned)outs.size(), kWG, 0, s>>>(dsegs.get(), dout.get(), ws.part.get(),
-                                                             O, ldo, alpha, beta);
+  k_seg_gemm_tn_reduce<F><<<dim3((unsigned)outs.size(), 4096 / kWG), kWG, 0, s>>>(
+      dsegs.get(), dout.get(), part.get(), O, ldo, alpha, beta);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));
 }
-
-// --------------------------------------------------------------------------
-// Entry generators.  row_pt/col_pt are hoisted out of the reduction loop.
-// --------------------------------------------------------------------------
-struct GDense {
-  const double* A;
-  int64_t lda;
-  __device__ double row_pt(int64_t) const { return 0.0; }
-  __device__ double col_pt(int64_t) const { return 0.0; }
-  __device__ double eval(double, double, int64_t i, int64_t j) const { return A[i + j * lda]; }
-};
-struct GGaussian {  // exp(-(x_i-x_j)^2 c) + p1 delta, c = 1/(2 sigma^2)
-  const double* x;
-  double c, p1;
-  __device__ double row_pt(int64_t i) const { return x[i]; }
-  __device__ double col_pt(int64_t j) const { return x[j]; }
-  __device__ double eval(double xi, double xj, int64_t i, int64_t j) const {
-    const double d = xi - xj;
-    return exp(-(d * d) * c) + (i == j ? p1 : 0.0);
-  }
-};
-struct GCauchy {  // 1/(x_i - y_j)
-  const double* x;
-  const double* y;
-  __device__ double row_pt(int64_t i) const { return x[i]; }
-  __device__ double col_pt(int64_t j) const { return y[j]; }
-  __device__ double eval(double xi, double yj, int64_t, int64_t) const { return 1.0 / (xi - yj); }
-};
-struct GRefCauchy {  // 1/(x_i + x_j + 1) + p1 delta     (kernel.h:30-48)
-  const double* x;
-  double p1;
-  __device__ double row_pt(int64_t i) const { return x[i]; }
-  __device__ double col_pt(int64_t j) const { return x[j]; }
-  __device__ double eval(double xi, double xj, int64_t i, int64_t j) const {
-    return 1.0 / (xi + xj + 1.0) + (i == j ? p1 : 0.0);
-  }
-};
-struct GRefGaussian {  // exp(-(x_i-x_j)^2 / h^2) + p1 delta   (kernel.h:52-65)
-  const double* x;
-  double h2, p1;
-  __device__ double row_pt(int64_t i) const { return x[i]; }
-  __device__ double col_pt(int64_t j) const { return x[j]; }
-  __device__ double eval(double xi, double xj, int64_t i, int64_t j) const {
-    const double d = xi - xj;
-    return exp(-(d * d) / h2) + (i == j ? p1 : 0.0);
-  }
-};
-struct GRefCoulomb {  // 1/|x_i-x_j|, 0 on the diagonal, + p1    (kernel.h:69-89)
-  const double* x;
-  double p1;
-  __device__ double row_pt(int64_t i) const { return x[i]; }
-  __device__ double col_pt(int64_t j) const { return x[j]; }
-  __device__ double eval(double xi, double xj, int64_t i, int64_t j) const {
-    return i == j ? p1 : 1.0 / fabs(xi - xj);
-  }
-};
-
-template <class F>
-static void with_gen(const GenDesc& g, F&& f) {
-  switch (g.kind) {
-    case GenKind::Dense: f(GDense{g.A, g.lda}); break;
-    case GenKind::Gaussian: f(GGaussian{g.x, 1.0 / (2.0 * g.p0 * g.p0), g.p1}); break;
-    case GenKind::Cauchy: f(GCauchy{g.x, g.y}); break;
-    case GenKind::RefCauchy: f(GRefCauchy{g.x, g.p1}); break;
-    case GenKind::RefGaussian: f(GRefGaussian{g.x, g.p0 * g.p0, g.p1}); break;
-    case GenKind::RefCoulomb: f(GRefCoulomb{g.x, g.p1}); break;
-  }
-}
-
-// --------------------------------------------------------------------------
-// Sketch GEMM on FP64 MFMA.  A workgroup owns 128 output rows (4 waves x 2
-// row fragments of 16) and NT column tiles of 16; the reduction runs in
-// K-tiles of 32 whose X rows are staged in LDS.  Each lane generates one
-// A entry per fragment per MFMA step (v_mfma_f64_16x16x4f64 consumes a
-// 16x4 A fragment, one element per lane).
-// --------------------------------------------------------------------------
-struct SkTile {
-  int32_t seg, row0, rb, re;
-  int32_t split, pad_;
-};
-
-template <class Gen, bool TRANS, int NT>
-__global__ __launch_bounds__(kWG) void k_sketch(Gen gen, const SegSk* __restrict__ segs,
-                                                const SkTile* __restrict__ tiles,
-                                                const double* __restrict__ X, int64_t ldx,
-                                                double* __restrict__ Y, int64_t ldy,
-                                                int64_t split_stride) {
-  constexpr int RF = 2, KT = 32, NC = NT * 16;
-  __shared__ double xs[KT][NC + 1];
-  const SkTile tl = tiles[blockIdx.x];
-  const SegSk sg = segs[tl.seg];
-  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
-  const int cbase = blockIdx.y * NC;
-  const int kcols = min(NC, sg.k - cbase);
-  mfma::v4d acc[RF][NT];
-#pragma unroll
-  for (int f = 0; f < RF; ++f)
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[f][t] = mfma::v4d{0.0, 0.0, 0.0, 0.0};
-
-  int rloc[RF];
-  int64_t iglob[RF];
-  double opt[RF];
-#pragma unroll
-  for (int f = 0; f < RF; ++f) {
-    rloc[f] = tl.row0 + w * 32 + f * 16 + (lane % 16);
-    iglob[f] = sg.out0 + min(rloc[f], max(sg.outlen - 1, 0));
-    opt[f] = TRANS ? gen.col_pt(iglob[f]) : gen.row_pt(iglob[f]);
-  }
-  const int kq = lane / 16;
-  for (int kb = tl.rb; kb < tl.re; kb += KT) {
-    for (int e = threadIdx.x; e < KT * NC; e += kWG) {
-      const int kk = e % KT, cc = e / KT, j = kb + kk;
-      xs[kk][cc] = (j < tl.re && cc < kcols) ? X[sg.xoff + j + (int64_t)(cbase + cc) * ldx] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll 2
-    for (int ks = 0; ks < KT / 4; ++ks) {
-      const int jl = kb + ks * 4 + kq;
-      const bool inr = jl < tl.re;
-      // clamp so that the point fetch below stays inside the reduction range
-      const int64_t jg = sg.red0 + (inr ? jl : tl.re - 1);
-      const bool jok = inr && !((uint64_t)(jg - sg.excl0) < (uint64_t)sg.excllen);
-      const double rpt = TRANS ? gen.row_pt(jg) : gen.col_pt(jg);
-      double a[RF];
-#pragma unroll
-      for (int f = 0; f < RF; ++f) {
-        const bool ok = jok && rloc[f] < sg.outlen;
-        a[f] = ok ? (TRANS ? gen.eval(rpt, opt[f], jg, iglob[f]) : gen.eval(opt[f], rpt, iglob[f], jg))
-                  : 0.0;
-      }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const double b = xs[ks * 4 + kq][t * 16 + (lane % 16)];
-#pragma unroll
-        for (int f = 0; f < RF; ++f) acc[f][t] = mfma::f64_16x16x4(a[f], b, acc[f][t]);
-      }
-    }
-    __syncthreads();
-  }
-  double* ybase = Y + (int64_t)tl.split * split_stride + sg.yoff;
-#pragma unroll
-  for (int f = 0; f < RF; ++f)
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = tl.row0 + w * 32 + f * 16 + mfma::f64_row(lane, v);
-        const int col = t * 16 + mfma::f64_col(lane, v);
-        if (row < sg.outlen && col < kcols) ybase[row + (int64_t)(cbase + col) * ldy] = acc[f][t][v];
-      }
-}
-
-struct SumTile {
-  int32_t seg, row0;
-};
-__global__ __launch_bounds__(kWG) void k_sketch_sum(const SegSk* __restrict__ segs,
-                                                    const SumTile* __restrict__ tiles,
-                                                    const double* __restrict__ part, int nsplit,
-                                                    int64_t split_stride, double* __restrict__ Y,
-                                                    int64_t ldy) {
-  const SumTile tl = tiles[blockIdx.x];
-  const SegSk sg = segs[tl.seg];
-  const int r = tl.row0 + (threadIdx.x % 64);
-  if (r >= sg.outlen) return;
-  for (int c = threadIdx.x / 64; c < sg.k; c += 4) {
-    const int64_t o = sg.yoff + r + (int64_t)c * ldy;
-    double s = 0.0;
-    for (int p = 0; p < nsplit; ++p) s += part[(int64_t)p * split_stride + o];
-    Y[o] = s;
-  }
-}
-
-template <class Gen, bool TRANS>
-static void launch_sketch(const Gen& gen, int NT, dim3 grid, const SegSk* ds, const SkTile* dt,
-                          const double* X, int64_t ldx, double* Y, int64_t ldy, int64_t ss,
-                          hipStream_t s) {
-  switch (NT) {
-    case 1: k_sketch<Gen, TRANS, 1><<<grid, kWG, 0, s>>>(gen, ds, dt, X, ldx, Y, ldy, ss); break;
-    case 2: k_sketch<Gen, TRANS, 2><<<grid, kWG, 0, s>>>(gen, ds, dt, X, ldx, Y, ldy, ss); break;
-    case 3: k_sketch<Gen, TRANS, 3><<<grid, kWG, 0, s>>>(gen, ds, dt, X, ldx, Y, ldy, ss); break;
-    default: k_sketch<Gen, TRANS, 4><<<grid, kWG, 0, s>>>(gen, ds, dt, X, ldx, Y, ldy, ss); break;
-  }
-}
-
-void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs,
-            const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap, Workspace& ws,
-            hipStream_t s) {
-  if (segs.empty() || kcap <= 0) return;
-  const int kpad = round_up(kcap, 16);
-  const int NT = std::min(4, kpad / 16);
-  const int ncg = (int)ceil_div(kpad, NT * 16);
-  int64_t nrow_tiles = 0, redmax = 0;
-  for (auto& sg : segs) {
-    nrow_tiles += ceil_div(std::max(sg.outlen, 0), 128);
-    redmax = std::max<int64_t>(redmax, sg.redlen);
-  }
-  // split the reduction when the row tiles alone cannot fill 256 CUs
-  int nsplit = 1;
-  const int64_t want = 1024;
-  if (nrow_tiles * ncg < want) {
-    nsplit = (int)std::min<int64_t>(ceil_div(want, std::max<int64_t>(nrow_tiles * ncg, 1)),
-                                    std::max<int64_t>(1, redmax / 256));
-    nsplit = std::max(1, std::min(nsplit, 64));
-  }
-  std::vector<SkTile> tiles;
-  for (size_t i = 0; i < segs.size(); ++i) {
-    const auto& sg = segs[i];
-    if (sg.outlen <= 0 || sg.k <= 0) continue;
-    const int64_t chunk = round_up((int)ceil_div(std::max(sg.redlen, 1), nsplit), 32);
-    for (int r0 = 0; r0 < sg.outlen; r0 += 128)
-      for (int p = 0; p < nsplit; ++p) {
-        const int rb = (int)std::min<int64_t>((int64_t)p * chunk, sg.redlen);
-        const int re = (int)std::min<int64_t>((int64_t)(p + 1) * chunk, sg.redlen);
-        tiles.push_back({(int32_t)i, r0, rb, re, p, 0});
-      }
-  }
-  if (tiles.empty()) return;
-  dsegs.upload(segs, s);
-  DevBuf<SkTile> dt;
-  dt.upload(tiles, s);
-  double* out = Y;
-  int64_t ss = 0;
-  if (nsplit > 1) {
-    // partial sums go to a scratch copy laid out like Y
-    int64_t span = 0;
-    for (auto& sg : segs)
-      span = std::max<int64_t>(span, sg.yoff + sg.outlen + (int64_t)(sg.k - 1) * ldy + 1);
-    ss = span;
-    ws.part.reserve((size_t)(span * nsplit));
-    out = ws.part.get();
-  }
-  dim3 grid((unsigned)tiles.size(), (unsigned)ncg);
-  with_gen(g, [&](auto gen) {
-    if (trans)
-      launch_sketch<decltype(gen), true>(gen, NT, grid, dsegs.get(), dt.get(), X, ldx, out, ldy, ss, s);
-    else
-      launch_sketch<decltype(gen), false>(gen, NT, grid, dsegs.get(), dt.get(), X, ldx, out, ldy, ss, s);
-  });
-  HIP_CHECK(hipGetLastError());
-  if (nsplit > 1) {
-    std::vector<SumTile> st;
-    for (size_t i = 0; i < segs.size(); ++i)
-      for (int r0 = 0; r0 < segs[i].outlen; r0 += 64) st.push_back({(int32_t)i, r0});
-    DevBuf<SumTile> dst;
-    dst.upload(st, s);
-    k_sketch_sum<<<(unsigned)st.size(), kWG, 0, s>>>(dsegs.get(), dst.get(), ws.part.get(), nsplit, ss,
-                                                     Y, ldy);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(s));
-  } else {
-    HIP_CHECK(hipStreamSynchronize(s));
-  }
-}
+template void seg_gemm_tn<double>(const std::vector<SegTN>&, DevBuf<SegTN>&, const double*, int64_t,
+                                  const double*, int64_t, double*, int64_t, double, double,
+                                  Workspace&, hipStream_t);
+template void seg_gemm_tn<float>(const std::vector<SegTN>&, DevBuf<SegTN>&, const float*, int64_t,
+                                 const float*, int64_t, float*, int64_t, float, float, Workspace&,
+                                 hipStream_t);
 
 // --------------------------------------------------------------------------
 // Batched small dense kernels: one workgroup per matrix, matrices in global
@@ -497,7 +308,7 @@ __global__ __launch_bounds__(kWG) void k_chol_rinv(const SmallMat* __restrict__ 
                                                    const double* __restrict__ G,
                                                    double* __restrict__ R, double* __restrict__ Ri,
                                                    const double* __restrict__ shift_coef,
-                                                   int* __restrict__ info) {
+                                                   int* __restrict__ info, double* __restrict__ gerr) {
   __shared__ double red[kWG / 64];
   __shared__ int bad;
   const SmallMat m = mats[blockIdx.x];
@@ -505,9 +316,24 @@ __global__ __launch_bounds__(kWG) void k_chol_rinv(const SmallMat* __restrict__ 
   const double* g = G + m.off;
   double* r = R + m.off;
   double* ri = Ri + m.off;
-  double tr = 0.0;
+  double tr = 0.0, dev = 0.0;
   for (int i = threadIdx.x; i < n; i += kWG) tr += g[i + (int64_t)i * ld];
+  for (int e = threadIdx.x; e < n * n; e += kWG) {
+    const int i = e % n, j = e / n;
+    dev = fmax(dev, fabs(g[i + (int64_t)j * ld] - (i == j ? 1.0 : 0.0)));
+  }
   tr = block_sum(tr, red);
+  {
+    for (int o = 32; o > 0; o >>= 1) dev = fmax(dev, __shfl_xor(dev, o));
+    __shared__ double dred[kWG / 64];
+    if (threadIdx.x % 64 == 0) dred[threadIdx.x / 64] = dev;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double d = 0.0;
+      for (int i = 0; i < kWG / 64; ++i) d = fmax(d, dred[i]);
+      gerr[blockIdx.x] = (tr == 0.0) ? 0.0 : d;
+    }
+  }
   const double shift = shift_coef[blockIdx.x] * tr;
   if (threadIdx.x == 0) { bad = 0; info[blockIdx.x] = 0; }
   for (int e = threadIdx.x; e < n * n; e += kWG) {
@@ -553,11 +379,11 @@ __global__ __launch_bounds__(kWG) void k_chol_rinv(const SmallMat* __restrict__ 
 
 void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* G, double* R,
                double* Rinv, const std::vector<double>& shift_coef, DevBuf<double>& dshift,
-               int* info, hipStream_t s) {
+               int* info, double* gerr, hipStream_t s) {
   if (mats.empty()) return;
   dm.upload(mats, s);
   dshift.upload(shift_coef, s);
-  k_chol_rinv<<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), G, R, Rinv, dshift.get(), info);
+  k_chol_rinv<<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), G, R, Rinv, dshift.get(), info, gerr);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -682,21 +508,22 @@ void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double*
 }
 
 // Gauss-Jordan inversion with partial pivoting, in place.
+template <class F>
 __global__ __launch_bounds__(kWG) void k_gj_inverse(const SmallMat* __restrict__ mats,
-                                                    double* __restrict__ A, int* __restrict__ info) {
-  __shared__ double bv[kWG];
+                                                    F* __restrict__ A, int* __restrict__ info) {
+  __shared__ F bv[kWG];
   __shared__ int bi[kWG];
   __shared__ int piv_s;
   __shared__ int perm[4096];
   const SmallMat m = mats[blockIdx.x];
   const int n = m.n, ld = m.ld;
-  double* a = A + m.off;
+  F* a = A + m.off;
   if (threadIdx.x == 0) info[blockIdx.x] = 0;
   for (int k = 0; k < n; ++k) {
-    double best = -1.0;
+    F best = F(-1);
     int bidx = k;
     for (int i = k + threadIdx.x; i < n; i += kWG) {
-      const double v = fabs(a[i + (int64_t)k * ld]);
+      const F v = fabs(a[i + (int64_t)k * ld]);
       if (v > best) { best = v; bidx = i; }
     }
     bv[threadIdx.x] = best;
@@ -704,7 +531,7 @@ __global__ __launch_bounds__(kWG) void k_gj_inverse(const SmallMat* __restrict__
     __syncthreads();
     for (int o = kWG / 2; o > 0; o >>= 1) {
       if (threadIdx.x < o) {
-        const double v2 = bv[threadIdx.x + o];
+        const F v2 = bv[threadIdx.x + o];
         const int i2 = bi[threadIdx.x + o];
         if (v2 > bv[threadIdx.x] || (v2 == bv[threadIdx.x] && i2 < bi[threadIdx.x])) {
           bv[threadIdx.x] = v2;
@@ -716,19 +543,19 @@ __global__ __launch_bounds__(kWG) void k_gj_inverse(const SmallMat* __restrict__
     if (threadIdx.x == 0) {
       piv_s = bi[0];
       perm[k] = bi[0];
-      if (!(bv[0] > 0.0) || !isfinite(bv[0])) info[blockIdx.x] = k + 1;
+      if (!(bv[0] > F(0)) || !isfinite(bv[0])) info[blockIdx.x] = k + 1;
     }
     __syncthreads();
-    if (!(bv[0] > 0.0) || !isfinite(bv[0])) return;
+    if (!(bv[0] > F(0)) || !isfinite(bv[0])) return;
     const int p = piv_s;
     if (p != k)
       for (int j = threadIdx.x; j < n; j += kWG) {
-        const double t = a[k + (int64_t)j * ld];
+        const F t = a[k + (int64_t)j * ld];
         a[k + (int64_t)j * ld] = a[p + (int64_t)j * ld];
         a[p + (int64_t)j * ld] = t;
       }
     __syncthreads();
-    const double pv = a[k + (int64_t)k * ld];
+    const F pv = a[k + (int64_t)k * ld];
     __syncthreads();
     for (int j = threadIdx.x; j < n; j += kWG)
       if (j != k) a[k + (int64_t)j * ld] /= pv;
@@ -740,14 +567,14 @@ __global__ __launch_bounds__(kWG) void k_gj_inverse(const SmallMat* __restrict__
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += kWG)
       if (i != k) a[i + (int64_t)k * ld] = -a[i + (int64_t)k * ld] / pv;
-    if (threadIdx.x == 0) a[k + (int64_t)k * ld] = 1.0 / pv;
+    if (threadIdx.x == 0) a[k + (int64_t)k * ld] = F(1) / pv;
     __syncthreads();
   }
   for (int k = n - 1; k >= 0; --k) {
     const int p = perm[k];
     if (p != k)
       for (int i = threadIdx.x; i < n; i += kWG) {
-        const double t = a[i + (int64_t)k * ld];
+        const F t = a[i + (int64_t)k * ld];
         a[i + (int64_t)k * ld] = a[i + (int64_t)p * ld];
         a[i + (int64_t)p * ld] = t;
       }
@@ -755,35 +582,89 @@ __global__ __launch_bounds__(kWG) void k_gj_inverse(const SmallMat* __restrict__
   }
 }
 
-void gj_inverse(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* A, int* info,
+template <class F>
+void gj_inverse(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, F* A, int* info,
                 hipStream_t s) {
   if (mats.empty()) return;
   for (auto& mm : mats) HMAT_REQUIRE(mm.n <= 4096, HMAT_ERR_ARG, "gj_inverse: n > 4096");
   dm.upload(mats, s);
-  k_gj_inverse<<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), A, info);
+  k_gj_inverse<F><<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), A, info);
   HIP_CHECK(hipGetLastError());
 }
+template void gj_inverse<double>(const std::vector<SmallMat>&, DevBuf<SmallMat>&, double*, int*, hipStream_t);
+template void gj_inverse<float>(const std::vector<SmallMat>&, DevBuf<SmallMat>&, float*, int*, hipStream_t);
 
 // --------------------------------------------------------------------------
 // Fill helpers
 // --------------------------------------------------------------------------
-__global__ void k_fill_identity(const SegNN* __restrict__ segs, double* __restrict__ Y, int64_t ldy) {
+template <class F>
+__global__ void k_fill_identity(const SegNN* __restrict__ segs, F* __restrict__ Y, int64_t ldy) {
   const SegNN sg = segs[blockIdx.y];
   const int64_t total = (int64_t)sg.len * sg.q;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int r = (int)(e % sg.len), c = (int)(e / sg.len);
-    Y[sg.yoff + r + (int64_t)c * ldy] = (r == c) ? 1.0 : 0.0;
+    Y[sg.yoff + r + (int64_t)c * ldy] = (r == c) ? F(1) : F(0);
   }
 }
 
-void fill_identity_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, double* Y,
-                        int64_t ldy, hipStream_t s) {
+template <class F>
+void fill_identity_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, F* Y, int64_t ldy,
+                        hipStream_t s) {
   if (segs.empty()) return;
   dsegs.upload(segs, s);
-  k_fill_identity<<<dim3(64, (unsigned)segs.size()), kWG, 0, s>>>(dsegs.get(), Y, ldy);
+  k_fill_identity<F><<<dim3(64, (unsigned)segs.size()), kWG, 0, s>>>(dsegs.get(), Y, ldy);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));
+}
+template void fill_identity_segs<double>(const std::vector<SegNN>&, DevBuf<SegNN>&, double*, int64_t, hipStream_t);
+template void fill_identity_segs<float>(const std::vector<SegNN>&, DevBuf<SegNN>&, float*, int64_t, hipStream_t);
+
+template <class To, class Ti>
+__global__ void k_convert(const Ti* __restrict__ X, To* __restrict__ Y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    Y[i] = (To)X[i];
+}
+template <class To, class Ti>
+void convert(const Ti* X, To* Y, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>(ceil_div(n, kWG), 4096);
+  k_convert<To, Ti><<<g, kWG, 0, s>>>(X, Y, n);
+  HIP_CHECK(hipGetLastError());
+}
+template void convert<float, double>(const double*, float*, int64_t, hipStream_t);
+template void convert<double, float>(const float*, double*, int64_t, hipStream_t);
+
+template <class To, class Ti>
+__global__ void k_convert_cols(int64_t n, const Ti* __restrict__ X, int64_t ldx, To* __restrict__ Y,
+                               int64_t ldy) {
+  const int c = blockIdx.y;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    Y[i + c * ldy] = (To)X[i + c * ldx];
+}
+template <class To, class Ti>
+void convert_cols(int64_t n, int ncols, const Ti* X, int64_t ldx, To* Y, int64_t ldy, hipStream_t s) {
+  if (n <= 0 || ncols <= 0) return;
+  const unsigned gx = (unsigned)std::min<int64_t>(ceil_div(n, kWG), 1024);
+  k_convert_cols<To, Ti><<<dim3(gx, (unsigned)ncols), kWG, 0, s>>>(n, X, ldx, Y, ldy);
+  HIP_CHECK(hipGetLastError());
+}
+template void convert_cols<float, double>(int64_t, int, const double*, int64_t, float*, int64_t, hipStream_t);
+template void convert_cols<double, float>(int64_t, int, const float*, int64_t, double*, int64_t, hipStream_t);
+template void convert_cols<double, double>(int64_t, int, const double*, int64_t, double*, int64_t, hipStream_t);
+
+__global__ void k_add_f32_cols(int64_t n, const float* __restrict__ D, int64_t ldd,
+                               double* __restrict__ X, int64_t ldx) {
+  const int c = blockIdx.y;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    X[i + c * ldx] += (double)D[i + c * ldd];
+}
+void add_f32_cols(int64_t n, int ncols, const float* D, int64_t ldd, double* X, int64_t ldx,
+                  hipStream_t s) {
+  if (n <= 0 || ncols <= 0) return;
+  const unsigned gx = (unsigned)std::min<int64_t>(ceil_div(n, kWG), 1024);
+  k_add_f32_cols<<<dim3(gx, (unsigned)ncols), kWG, 0, s>>>(n, D, ldd, X, ldx);
+  HIP_CHECK(hipGetLastError());
 }
 
 __global__ void k_copy_segs(const SegNN* __restrict__ segs, const double* __restrict__ X, int64_t ldx,
@@ -890,6 +771,22 @@ void gen_blocks(const GenDesc& g, const std::vector<BlockDesc>& blocks, DevBuf<B
   });
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));
+}
+
+__global__ void k_axpby_cols(int64_t n, double alpha, const double* __restrict__ X, int64_t ldx,
+                             double beta, const double* __restrict__ Y, int64_t ldy,
+                             double* __restrict__ Z, int64_t ldz) {
+  const int c = blockIdx.y;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    Z[i + c * ldz] = alpha * X[i + c * ldx] + beta * Y[i + c * ldy];
+}
+
+void axpby_cols(int64_t n, int ncols, double alpha, const double* X, int64_t ldx, double beta,
+                const double* Y, int64_t ldy, double* Z, int64_t ldz, hipStream_t s) {
+  if (n <= 0 || ncols <= 0) return;
+  const unsigned gx = (unsigned)std::min<int64_t>(ceil_div(n, kWG), 1024);
+  k_axpby_cols<<<dim3(gx, (unsigned)ncols), kWG, 0, s>>>(n, alpha, X, ldx, beta, Y, ldy, Z, ldz);
+  HIP_CHECK(hipGetLastError());
 }
 
 __global__ void k_col_norms2(const double* __restrict__ X, int64_t ldx, int64_t n,

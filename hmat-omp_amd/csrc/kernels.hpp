@@ -71,27 +71,50 @@ namespace kern {
 // scratch used by two-stage reductions
 struct Workspace {
   DevBuf<double> part;
+  DevBuf<float> part32;
+  DevBuf<__bf16> bf;  // BF16 planes of the mixed-precision sketch operand
   DevBuf<int64_t> aux;
+  template <class T> DevBuf<T>& parts();
 };
+template <> inline DevBuf<double>& Workspace::parts<double>() { return part; }
+template <> inline DevBuf<float>& Workspace::parts<float>() { return part32; }
 
-void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X,
-                 int64_t ldx, const double* T, int64_t ldt, double* Y, int64_t ldy,
-                 double alpha, double beta, hipStream_t s);
+// F = double or float (FP32 factorisation of the mixed-precision path)
+template <class F>
+void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* X, int64_t ldx,
+                 const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha, F beta, hipStream_t s);
 
-void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const double* X,
-                 int64_t ldx, const double* Y, int64_t ldy, double* O, int64_t ldo,
-                 double alpha, double beta, Workspace& ws, hipStream_t s);
+// Y(seg rows, 0:q) = X(seg rows, 0:q) R_s^{-1} for upper-triangular R_s
+// (q x q at toff, leading dimension ldt): row-wise forward substitution,
+// backward stable (unlike multiplying by an explicit inverse, whose error
+// grows with cond(R)).  Uses xoff/toff/yoff/len/q of each SegNN.
+void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X,
+                    int64_t ldx, const double* R, int64_t ldr, double* Y, int64_t ldy, hipStream_t s);
+
+template <class F>
+void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* X, int64_t ldx,
+                 const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta, Workspace& ws,
+                 hipStream_t s);
 
 void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs,
             const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap, Workspace& ws,
             hipStream_t s);
 
+// Same contraction on BF16 MFMA: entries generated in FP32 and split into
+// `split` (1..3) BF16 terms, the tall operand likewise; products of terms
+// whose split indices sum to < split are accumulated in FP32 and flushed to
+// FP64 every 512 reduction indices.  Y is FP64.
+void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>& segs,
+               DevBuf<SegSk>& dsegs, const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap,
+               Workspace& ws, hipStream_t s);
+
 // Gram G (n x n per entry) -> upper R with R^T R = G + shift_s I and
 // Rinv = R^{-1}.  shift_s = shift_coef * trace(G) when shift_coef > 0.
-// info[b] != 0 on breakdown.
+// info[b] != 0 on breakdown; gerr[b] = max |G - I| (0 for an all-zero G),
+// the orthogonality defect of the columns the Gram came from.
 void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* G, double* R,
                double* Rinv, const std::vector<double>& shift_coef, DevBuf<double>& dshift,
-               int* info, hipStream_t s);
+               int* info, double* gerr, hipStream_t s);
 
 // One-sided Jacobi SVD of n x n matrices A = U diag(S) V^T (A overwritten
 // by U).  S sorted descending, U/V columns permuted accordingly.
@@ -99,12 +122,14 @@ void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double*
                 double* S, int64_t sstride, double* work, hipStream_t s);
 
 // In-place inverse by Gauss-Jordan with partial pivoting.
-void gj_inverse(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* A,
-                int* info, hipStream_t s);
+template <class F>
+void gj_inverse(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, F* A, int* info,
+                hipStream_t s);
 
 // Y(rows, c) = 0 for all (rows,c) of the rectangle; identity helpers.
-void fill_identity_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, double* Y,
-                        int64_t ldy, hipStream_t s);
+template <class F>
+void fill_identity_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, F* Y, int64_t ldy,
+                        hipStream_t s);
 
 // Device Omega: Y[yoff + r + c*ldy] for r < len, c < k from a counter-based
 // generator keyed by (seed, stream id, r, c); uniform [0,1) or N(0,1).
@@ -123,6 +148,20 @@ void copy_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const doubl
 void trunc_prep(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, const double* Ur,
                 const double* Vr, const double* S, int64_t sstride, const int* rank, int rmax,
                 double* TU, double* TV, hipStream_t s);
+
+// Z(:, c) = alpha X(:, c) + beta Y(:, c) for n rows, ncols columns.
+void axpby_cols(int64_t n, int ncols, double alpha, const double* X, int64_t ldx, double beta,
+                const double* Y, int64_t ldy, double* Z, int64_t ldz, hipStream_t s);
+
+// Y = (To) X elementwise for n contiguous values.
+template <class To, class Ti>
+void convert(const Ti* X, To* Y, int64_t n, hipStream_t s);
+// Y(:, c) = (To) X(:, c), n rows, ncols columns, explicit leading dimensions.
+template <class To, class Ti>
+void convert_cols(int64_t n, int ncols, const Ti* X, int64_t ldx, To* Y, int64_t ldy, hipStream_t s);
+// X(:, c) += (double) D(:, c)
+void add_f32_cols(int64_t n, int ncols, const float* D, int64_t ldd, double* X, int64_t ldx,
+                  hipStream_t s);
 
 // Per-column 2-norms squared of an n-row, ncols column-major array.
 void col_norms2(const double* X, int64_t ldx, int64_t n, int ncols, double* out, hipStream_t s);

@@ -1,0 +1,473 @@
+// Block sketch GEMMs of the build: Y = A(rows, red) X or A(red, rows)^T X
+// for every off-diagonal block of a tree depth in one launch, with A's
+// entries generated in registers (or read from a dense input).
+//
+//   k_sketch     FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 entries.
+//   k_sketch_lp  BF16 MFMA (v_mfma_f32_16x16x32_bf16): FP32 entries split
+//                into SPLIT BF16 terms, X pre-split into BF16 planes,
+//                products whose term indices sum to < SPLIT, FP32
+//                accumulation flushed to FP64 every 512 reduction indices.
+//
+// Both stage a K-tile of 64 reduction rows of X (and their points) in LDS;
+// a tile whose reduction rows are all valid and not excluded and whose
+// output rows are all live takes a mask-free path (uniform branch).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "generators.hpp"
+#include "kernels.hpp"
+#include "mfma.hpp"
+
+namespace hmat {
+namespace kern {
+
+constexpr int kWGs = 256;
+
+struct SkTile {
+  int32_t seg, row0, rb, re;
+  int32_t split, pad_;
+};
+
+// rows [kb, kb+64) of the reduction range are all in range and outside the
+// excluded interval
+__device__ inline bool ktile_full(const SegSk& sg, int kb, int re) {
+  if (kb + 64 > re) return false;
+  if (sg.excllen <= 0) return true;
+  const int64_t lo = sg.red0 + kb, hi = lo + 64;
+  return hi <= sg.excl0 || lo >= sg.excl0 + sg.excllen;
+}
+
+template <class Gen, bool TRANS, int NT, int RF>
+__global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restrict__ segs,
+                                                 const SkTile* __restrict__ tiles,
+                                                 const double* __restrict__ X, int64_t ldx,
+                                                 double* __restrict__ Y, int64_t ldy,
+                                                 int64_t split_stride) {
+  constexpr int KT = 64, NC = NT * 16, ROWS = 4 * 16 * RF;
+  __shared__ double xs[KT][NC + 1];
+  __shared__ double rp[KT];
+  __shared__ double rm[KT];
+  const SkTile tl = tiles[blockIdx.x];
+  const SegSk sg = segs[tl.seg];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int cbase = blockIdx.y * NC;
+  const int kcols = min(NC, sg.k - cbase);
+  const bool rows_full = tl.row0 + ROWS <= sg.outlen;
+  mfma::v4d acc[RF][NT];
+#pragma unroll
+  for (int f = 0; f < RF; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[f][t] = mfma::v4d{0.0, 0.0, 0.0, 0.0};
+  int64_t iglob[RF];
+  double opt[RF], rmask[RF];
+#pragma unroll
+  for (int f = 0; f < RF; ++f) {
+    const int rloc = tl.row0 + w * 16 * RF + f * 16 + (lane % 16);
+    iglob[f] = sg.out0 + min(rloc, max(sg.outlen - 1, 0));
+    opt[f] = TRANS ? gen.col_pt(iglob[f]) : gen.row_pt(iglob[f]);
+    rmask[f] = rloc < sg.outlen ? 1.0 : 0.0;
+  }
+  const int kq = lane / 16;
+  for (int kb = tl.rb; kb < tl.re; kb += KT) {
+    for (int e = threadIdx.x; e < KT * NC; e += kWGs) {
+      const int kk = e % KT, cc = e / KT, j = kb + kk;
+      xs[kk][cc] = (j < tl.re && cc < kcols) ? X[sg.xoff + j + (int64_t)(cbase + cc) * ldx] : 0.0;
+    }
+    if (threadIdx.x < KT) {
+      const int j = kb + threadIdx.x;
+      const bool inr = j < tl.re;
+      const int64_t jg = sg.red0 + (inr ? j : tl.re - 1);
+      const bool ok = inr && !((uint64_t)(jg - sg.excl0) < (uint64_t)sg.excllen);
+      rp[threadIdx.x] = TRANS ? gen.row_pt(jg) : gen.col_pt(jg);
+      rm[threadIdx.x] = ok ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const bool full = rows_full && ktile_full(sg, kb, tl.re);
+#pragma unroll 4
+    for (int ks = 0; ks < KT / 4; ++ks) {
+      const int kk = ks * 4 + kq;
+      const int jl = min(kb + kk, tl.re - 1);
+      const int64_t jg = sg.red0 + jl;
+      const double rpt = rp[kk];
+      double a[RF];
+#pragma unroll
+      for (int f = 0; f < RF; ++f) {
+        a[f] = TRANS ? gen.eval(rpt, opt[f], jg, iglob[f]) : gen.eval(opt[f], rpt, iglob[f], jg);
+        if (!full) a[f] *= rm[kk] * rmask[f];
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const double b = xs[kk][t * 16 + (lane % 16)];
+#pragma unroll
+        for (int f = 0; f < RF; ++f) acc[f][t] = mfma::f64_16x16x4(a[f], b, acc[f][t]);
+      }
+    }
+    __syncthreads();
+  }
+  double* ybase = Y + (int64_t)tl.split * split_stride + sg.yoff;
+#pragma unroll
+  for (int f = 0; f < RF; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = tl.row0 + w * 16 * RF + f * 16 + mfma::f64_row(lane, v);
+        const int col = t * 16 + mfma::f64_col(lane, v);
+        if (row < sg.outlen && col < kcols) ybase[row + (int64_t)(cbase + col) * ldy] = acc[f][t][v];
+      }
+}
+
+// X (FP64) -> SPLIT BF16 planes with the same layout, plane stride `ps`.
+template <int SPLIT>
+__global__ void k_split_bf16(const double* __restrict__ X, int64_t n, __bf16* __restrict__ Xs,
+                             int64_t ps) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = (float)X[i];
+    const __bf16 h = (__bf16)v;
+    Xs[i] = h;
+    if constexpr (SPLIT >= 2) {
+      const float r1 = v - (float)h;
+      const __bf16 m = (__bf16)r1;
+      Xs[i + ps] = m;
+      if constexpr (SPLIT >= 3) Xs[i + 2 * ps] = (__bf16)(r1 - (float)m);
+    }
+  }
+}
+
+template <int SPLIT>
+__device__ inline void split_bf16(float v, __bf16 (&o)[3]) {
+  const __bf16 h = (__bf16)v;
+  o[0] = h;
+  if constexpr (SPLIT >= 2) {
+    const float r1 = v - (float)h;
+    const __bf16 m = (__bf16)r1;
+    o[1] = m;
+    if constexpr (SPLIT >= 3) o[2] = (__bf16)(r1 - (float)m);
+  }
+}
+
+template <class Gen, bool TRANS, int SPLIT>
+__global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __restrict__ segs,
+                                                    const SkTile* __restrict__ tiles,
+                                                    const __bf16* __restrict__ Xs, int64_t ldx,
+                                                    int64_t ps, double* __restrict__ Y, int64_t ldy,
+                                                    int64_t split_stride) {
+  constexpr int RF = 2, NT = 4, NC = 64, KT = 64, KP = KT + 8, FLUSH = 8;
+  __shared__ __attribute__((aligned(16))) __bf16 xs[SPLIT][NC][KP];
+  __shared__ __attribute__((aligned(16))) float rp[KT];
+  __shared__ __attribute__((aligned(16))) float rm[KT];
+  const SkTile tl = tiles[blockIdx.x];
+  const SegSk sg = segs[tl.seg];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int cbase = blockIdx.y * NC;
+  const int kcols = min(NC, sg.k - cbase);
+  const bool rows_full = tl.row0 + 128 <= sg.outlen;
+  const bool aligned = ((sg.xoff & 7) == 0) && ((ldx & 7) == 0);
+  mfma::v4d acc64[RF][NT];
+  mfma::v4f acc32[RF][NT];
+#pragma unroll
+  for (int f = 0; f < RF; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      acc64[f][t] = mfma::v4d{0.0, 0.0, 0.0, 0.0};
+      acc32[f][t] = mfma::v4f{0.f, 0.f, 0.f, 0.f};
+    }
+  int64_t iglob[RF];
+  float opt[RF], rmask[RF];
+#pragma unroll
+  for (int f = 0; f < RF; ++f) {
+    const int rloc = tl.row0 + w * 32 + f * 16 + (lane % 16);
+    iglob[f] = sg.out0 + min(rloc, max(sg.outlen - 1, 0));
+    opt[f] = TRANS ? gen.col_pt(iglob[f]) : gen.row_pt(iglob[f]);
+    rmask[f] = rloc < sg.outlen ? 1.f : 0.f;
+  }
+  const int kq = lane / 16;
+  int nt = 0;
+  for (int kb = tl.rb; kb < tl.re; kb += KT) {
+    const bool kfull = ktile_full(sg, kb, tl.re);
+    if (aligned && kfull) {
+      // 16-byte copies: 8 reduction rows of one column of one plane
+      for (int e = threadIdx.x; e < SPLIT * NC * (KT / 8); e += kWGs) {
+        const int ch = e % (KT / 8), cc = (e / (KT / 8)) % NC, sp = e / ((KT / 8) * NC);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (cc < kcols)
+          v = *reinterpret_cast<const uint4*>(Xs + sp * ps + sg.xoff + kb + 8 * ch + (int64_t)(cbase + cc) * ldx);
+        *reinterpret_cast<uint4*>(&xs[sp][cc][8 * ch]) = v;
+      }
+    } else {
+      for (int e = threadIdx.x; e < SPLIT * NC * KT; e += kWGs) {
+        const int kk = e % KT, cc = (e / KT) % NC, sp = e / (KT * NC), j = kb + kk;
+        xs[sp][cc][kk] = (j < tl.re && cc < kcols) ? Xs[sp * ps + sg.xoff + j + (int64_t)(cbase + cc) * ldx]
+                                                   : (__bf16)0.f;
+      }
+    }
+    if (threadIdx.x < KT) {
+      const int j = kb + threadIdx.x;
+      const bool inr = j < tl.re;
+      const int64_t jg = sg.red0 + (inr ? j : tl.re - 1);
+      const bool ok = inr && !((uint64_t)(jg - sg.excl0) < (uint64_t)sg.excllen);
+      rp[threadIdx.x] = TRANS ? gen.row_pt(jg) : gen.col_pt(jg);
+      rm[threadIdx.x] = ok ? 1.f : 0.f;
+    }
+    __syncthreads();
+    const bool full = rows_full && kfull;
+#pragma unroll
+    for (int ks = 0; ks < KT / 32; ++ks) {
+      const int k0 = ks * 32 + 8 * kq;
+      float pts[8], msk[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) pts[jj] = rp[k0 + jj];
+      if (!full) {
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) msk[jj] = rm[k0 + jj];
+      }
+      mfma::v8bf a[RF][SPLIT];
+#pragma unroll
+      for (int f = 0; f < RF; ++f) {
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const int64_t jg = sg.red0 + min(kb + k0 + jj, tl.re - 1);
+          float val = TRANS ? gen.eval(pts[jj], opt[f], jg, iglob[f]) : gen.eval(opt[f], pts[jj], iglob[f], jg);
+          if (!full) val *= msk[jj] * rmask[f];
+          __bf16 o[3];
+          split_bf16<SPLIT>(val, o);
+#pragma unroll
+          for (int sp = 0; sp < SPLIT; ++sp) a[f][sp][jj] = o[sp];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        mfma::v8bf b[SPLIT];
+#pragma unroll
+        for (int sp = 0; sp < SPLIT; ++sp)
+          b[sp] = *reinterpret_cast<const mfma::v8bf*>(&xs[sp][t * 16 + (lane % 16)][k0]);
+#pragma unroll
+        for (int sa = 0; sa < SPLIT; ++sa)
+#pragma unroll
+          for (int sb = 0; sb < SPLIT - sa; ++sb)
+#pragma unroll
+            for (int f = 0; f < RF; ++f) acc32[f][t] = mfma::bf16_16x16x32(a[f][sa], b[sb], acc32[f][t]);
+      }
+    }
+    __syncthreads();
+    if (++nt == FLUSH) {
+      nt = 0;
+#pragma unroll
+      for (int f = 0; f < RF; ++f)
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            acc64[f][t][v] += (double)acc32[f][t][v];
+            acc32[f][t][v] = 0.f;
+          }
+    }
+  }
+  double* ybase = Y + (int64_t)tl.split * split_stride + sg.yoff;
+#pragma unroll
+  for (int f = 0; f < RF; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = tl.row0 + w * 32 + f * 16 + mfma::bf16_row(lane, v);
+        const int col = t * 16 + mfma::bf16_col(lane, v);
+        if (row < sg.outlen && col < kcols)
+          ybase[row + (int64_t)(cbase + col) * ldy] = acc64[f][t][v] + (double)acc32[f][t][v];
+      }
+}
+
+struct SumTile {
+  int32_t seg, row0;
+};
+__global__ __launch_bounds__(kWGs) void k_sketch_sum(const SegSk* __restrict__ segs,
+                                                     const SumTile* __restrict__ tiles,
+                                                     const double* __restrict__ part, int nsplit,
+                                                     int64_t split_stride, double* __restrict__ Y,
+                                                     int64_t ldy) {
+  const SumTile tl = tiles[blockIdx.x];
+  const SegSk sg = segs[tl.seg];
+  const int r = tl.row0 + (threadIdx.x % 64);
+  if (r >= sg.outlen) return;
+  for (int c = threadIdx.x / 64; c < sg.k; c += 4) {
+    const int64_t o = sg.yoff + r + (int64_t)c * ldy;
+    double s = 0.0;
+    for (int p = 0; p < nsplit; ++p) s += part[(int64_t)p * split_stride + o];
+    Y[o] = s;
+  }
+}
+
+struct SketchPlan {
+  std::vector<SkTile> tiles;
+  int nsplit = 1, ncg = 1;
+  int64_t span = 0;
+};
+
+static SketchPlan plan_sketch(const std::vector<SegSk>& segs, int kcap, int ncols_per_wg, int rows_per_wg,
+                              int64_t ldy) {
+  SketchPlan p;
+  p.ncg = (int)ceil_div(round_up(kcap, 16), ncols_per_wg);
+  int64_t nrow_tiles = 0, redmax = 0;
+  for (auto& sg : segs) {
+    nrow_tiles += ceil_div(std::max(sg.outlen, 0), rows_per_wg);
+    redmax = std::max<int64_t>(redmax, sg.redlen);
+  }
+  // split the reduction when the row tiles alone cannot fill 256 CUs
+  const int64_t want = 1024;
+  if (nrow_tiles * p.ncg < want) {
+    p.nsplit = (int)std::min<int64_t>(ceil_div(want, std::max<int64_t>(nrow_tiles * p.ncg, 1)),
+                                      std::max<int64_t>(1, redmax / 256));
+    p.nsplit = std::max(1, std::min(p.nsplit, 64));
+  }
+  for (size_t i = 0; i < segs.size(); ++i) {
+    const auto& sg = segs[i];
+    if (sg.outlen <= 0 || sg.k <= 0) continue;
+    const int64_t chunk = round_up((int)ceil_div(std::max(sg.redlen, 1), p.nsplit), 64);
+    for (int r0 = 0; r0 < sg.outlen; r0 += rows_per_wg)
+      for (int q = 0; q < p.nsplit; ++q) {
+        const int rb = (int)std::min<int64_t>((int64_t)q * chunk, sg.redlen);
+        const int re = (int)std::min<int64_t>((int64_t)(q + 1) * chunk, sg.redlen);
+        p.tiles.push_back({(int32_t)i, r0, rb, re, q, 0});
+      }
+  }
+  if (p.nsplit > 1)
+    for (auto& sg : segs)
+      p.span = std::max<int64_t>(p.span, sg.yoff + sg.outlen + (int64_t)(sg.k - 1) * ldy + 1);
+  return p;
+}
+
+static void sum_splits(const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs, const SketchPlan& p,
+                       const double* part, double* Y, int64_t ldy, hipStream_t s) {
+  std::vector<SumTile> st;
+  for (size_t i = 0; i < segs.size(); ++i)
+    for (int r0 = 0; r0 < segs[i].outlen; r0 += 64) st.push_back({(int32_t)i, r0});
+  DevBuf<SumTile> dst;
+  dst.upload(st, s);
+  k_sketch_sum<<<(unsigned)st.size(), kWGs, 0, s>>>(dsegs.get(), dst.get(), part, p.nsplit, p.span, Y, ldy);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <class Gen, bool TRANS>
+static void launch_sketch(const Gen& gen, int NT, int RF, dim3 grid, const SegSk* ds, const SkTile* dt,
+                          const double* X, int64_t ldx, double* Y, int64_t ldy, int64_t ss,
+                          hipStream_t s) {
+#define HMAT_SK(NT_, RF_) k_sketch<Gen, TRANS, NT_, RF_><<<grid, kWGs, 0, s>>>(gen, ds, dt, X, ldx, Y, ldy, ss)
+  if (RF == 4) {
+    switch (NT) {
+      case 1: HMAT_SK(1, 4); break;
+      case 2: HMAT_SK(2, 4); break;
+      case 3: HMAT_SK(3, 4); break;
+      default: HMAT_SK(4, 4); break;
+    }
+  } else {
+    switch (NT) {
+      case 1: HMAT_SK(1, 2); break;
+      case 2: HMAT_SK(2, 2); break;
+      case 3: HMAT_SK(3, 2); break;
+      default: HMAT_SK(4, 2); break;
+    }
+  }
+#undef HMAT_SK
+}
+
+void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs,
+            const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap, Workspace& ws,
+            hipStream_t s) {
+  if (segs.empty() || kcap <= 0) return;
+  const int kpad = round_up(kcap, 16);
+  const int NT = std::min(4, kpad / 16);
+  int64_t rows = 0;
+  for (auto& sg : segs) rows += sg.outlen;
+  // 256-row tiles when the segments are long enough to fill the chip with them
+  const int RF = (rows / 256 >= 512) ? 4 : 2;
+  const int rows_per_wg = 64 * RF;
+  SketchPlan p = plan_sketch(segs, kcap, NT * 16, rows_per_wg, ldy);
+  if (p.tiles.empty()) return;
+  dsegs.upload(segs, s);
+  DevBuf<SkTile> dt;
+  dt.upload(p.tiles, s);
+  double* out = Y;
+  if (p.nsplit > 1) {
+    ws.part.reserve((size_t)(p.span * p.nsplit));
+    out = ws.part.get();
+  }
+  dim3 grid((unsigned)p.tiles.size(), (unsigned)p.ncg);
+  double flops = 0.0;
+  for (auto& sg : segs) flops += 2.0 * sg.outlen * (double)sg.redlen * sg.k;
+  ProfScope prof(g.kind == GenKind::Dense ? "sketch_dense" : "sketch_gen", s, flops);
+  with_gen(g, [&](auto gen) {
+    if (trans)
+      launch_sketch<decltype(gen), true>(gen, NT, RF, grid, dsegs.get(), dt.get(), X, ldx, out, ldy, p.span, s);
+    else
+      launch_sketch<decltype(gen), false>(gen, NT, RF, grid, dsegs.get(), dt.get(), X, ldx, out, ldy, p.span, s);
+  });
+  HIP_CHECK(hipGetLastError());
+  prof.finish();
+  if (p.nsplit > 1) sum_splits(segs, dsegs, p, ws.part.get(), Y, ldy, s);
+  else HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <class Gen, bool TRANS>
+static void launch_sketch_lp(const Gen& gen, int split, dim3 grid, const SegSk* ds, const SkTile* dt,
+                             const __bf16* Xs, int64_t ldx, int64_t ps, double* Y, int64_t ldy,
+                             int64_t ss, hipStream_t s) {
+  switch (split) {
+    case 1: k_sketch_lp<Gen, TRANS, 1><<<grid, kWGs, 0, s>>>(gen, ds, dt, Xs, ldx, ps, Y, ldy, ss); break;
+    case 2: k_sketch_lp<Gen, TRANS, 2><<<grid, kWGs, 0, s>>>(gen, ds, dt, Xs, ldx, ps, Y, ldy, ss); break;
+    default: k_sketch_lp<Gen, TRANS, 3><<<grid, kWGs, 0, s>>>(gen, ds, dt, Xs, ldx, ps, Y, ldy, ss); break;
+  }
+}
+
+void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>& segs,
+               DevBuf<SegSk>& dsegs, const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap,
+               Workspace& ws, hipStream_t s) {
+  if (segs.empty() || kcap <= 0) return;
+  split = std::max(1, std::min(split, 3));
+  // BF16 planes of X over the rows any segment reduces over
+  int64_t xend = 0;
+  for (auto& sg : segs) xend = std::max<int64_t>(xend, sg.xoff + sg.redlen + (int64_t)(kcap - 1) * ldx);
+  const int64_t ps = round_up((int)std::min<int64_t>(xend, (int64_t)1 << 30), 64);
+  HMAT_REQUIRE(xend < ((int64_t)1 << 30), HMAT_ERR_ARG, "sketch operand too large for BF16 planes");
+  ws.bf.reserve((size_t)(ps * split));
+  {
+    const unsigned gx = (unsigned)std::min<int64_t>(ceil_div(xend, kWGs), 8192);
+    switch (split) {
+      case 1: k_split_bf16<1><<<gx, kWGs, 0, s>>>(X, xend, ws.bf.get(), ps); break;
+      case 2: k_split_bf16<2><<<gx, kWGs, 0, s>>>(X, xend, ws.bf.get(), ps); break;
+      default: k_split_bf16<3><<<gx, kWGs, 0, s>>>(X, xend, ws.bf.get(), ps); break;
+    }
+    HIP_CHECK(hipGetLastError());
+  }
+  SketchPlan p = plan_sketch(segs, kcap, 64, 128, ldy);
+  if (p.tiles.empty()) return;
+  dsegs.upload(segs, s);
+  DevBuf<SkTile> dt;
+  dt.upload(p.tiles, s);
+  double* out = Y;
+  if (p.nsplit > 1) {
+    ws.part.reserve((size_t)(p.span * p.nsplit));
+    out = ws.part.get();
+  }
+  dim3 grid((unsigned)p.tiles.size(), (unsigned)p.ncg);
+  double flops = 0.0;
+  for (auto& sg : segs) flops += 2.0 * sg.outlen * (double)sg.redlen * sg.k;
+  ProfScope prof("sketch_lp", s, flops);
+  with_gen_lp(g, [&](auto gen) {
+    if (trans)
+      launch_sketch_lp<decltype(gen), true>(gen, split, grid, dsegs.get(), dt.get(), ws.bf.get(), ldx, ps, out,
+                                            ldy, p.span, s);
+    else
+      launch_sketch_lp<decltype(gen), false>(gen, split, grid, dsegs.get(), dt.get(), ws.bf.get(), ldx, ps, out,
+                                             ldy, p.span, s);
+  });
+  HIP_CHECK(hipGetLastError());
+  prof.finish();
+  if (p.nsplit > 1) sum_splits(segs, dsegs, p, ws.part.get(), Y, ldy, s);
+  else HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace kern
+}  // namespace hmat

@@ -55,11 +55,17 @@ def _proto(lib):
         "hmat_version": (C.c_char_p, []),
         "hmat_device_count": (i32, []),
         "hmat_options_reference": (None, [C.POINTER(Options), C.c_double]),
+        "hmat_profile_enable": (None, [i32]),
+        "hmat_profile_reset": (None, []),
+        "hmat_profile_query": (i32, [C.c_char_p, C.POINTER(i64), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
         "hmat_hodlr_create": (st, [C.POINTER(vp), C.POINTER(Options)]),
         "hmat_hodlr_destroy": (st, [vp]),
         "hmat_hodlr_build_dense": (st, [vp, _D, i64, i64, i32, _D, i64]),
         "hmat_hodlr_build_dense_device": (st, [vp, vp, i64, i64, i32, _D, i64]),
         "hmat_hodlr_build_kernel": (st, [vp, C.POINTER(Kernel), i64, i32]),
+        "hmat_hodlr_build_kernel_device": (st, [vp, C.POINTER(Kernel), i64, i32]),
+        "hmat_kernel_matvec_device": (st, [C.POINTER(Kernel), i64, vp, i64, i32, vp, i64]),
         "hmat_hodlr_omega_length": (i64, [C.POINTER(Options), i64, i32]),
         "hmat_hodlr_matvec": (st, [vp, _D, i64, i32, _D, i64]),
         "hmat_hodlr_matvec_device": (st, [vp, vp, i64, i32, vp, i64]),
@@ -131,12 +137,38 @@ def make_options(tol=1e-6, rank=64, fixed_rank=False, oversample=10, power_iters
                    omega_dist, seed)
 
 
+def make_kernel_device(ktype, x_ptr, y_ptr=None, p0=0.0, p1=0.0):
+    """Kernel descriptor whose points are device pointers (ints)."""
+    return Kernel(ktype, p0, p1, C.cast(C.c_void_p(x_ptr), _D),
+                  C.cast(C.c_void_p(y_ptr), _D) if y_ptr else None)
+
+
+def kernel_matvec_device(kernel, n, x_ptr, ldx, nrhs, y_ptr, ldy):
+    _check(load().hmat_kernel_matvec_device(C.byref(kernel), n, x_ptr, ldx, nrhs, y_ptr, ldy))
+
+
 def make_kernel(ktype, x, y=None, p0=0.0, p1=0.0):
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
     k = Kernel(ktype, p0, p1, _ptr(x), _ptr(y))
     k._keep = (x, y)
     return k
+
+
+def profile_enable(on=True):
+    load().hmat_profile_enable(int(on))
+
+
+def profile_reset():
+    load().hmat_profile_reset()
+
+
+def profile_query(name):
+    """(launches, total ms, total algorithmic work) or None."""
+    n, ms, w = C.c_int64(), C.c_double(), C.c_double()
+    if not load().hmat_profile_query(name.encode(), C.byref(n), C.byref(ms), C.byref(w)):
+        return None
+    return n.value, ms.value, w.value
 
 
 class _Base:
@@ -218,6 +250,17 @@ class HODLRMatrixOMP(_Base):
         self._sync_opts()
         _check(self._lib.hmat_hodlr_build_kernel(self._h, C.byref(kernel), n, leaf_size))
         self.N = n
+
+    def build_kernel_device(self, kernel, n, leaf_size):
+        self._sync_opts()
+        _check(self._lib.hmat_hodlr_build_kernel_device(self._h, C.byref(kernel), n, leaf_size))
+        self.N = n
+
+    def solve_device(self, b_ptr, ldb, nrhs, x_ptr, ldx):
+        _check(self._lib.hmat_hodlr_solve_device(self._h, b_ptr, ldb, nrhs, x_ptr, ldx))
+
+    def multiply_device(self, x_ptr, ldx, nrhs, y_ptr, ldy):
+        _check(self._lib.hmat_hodlr_matvec_device(self._h, x_ptr, ldx, nrhs, y_ptr, ldy))
 
     def omega_length(self, n, leaf_size):
         self._sync_opts()

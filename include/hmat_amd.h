@@ -100,6 +100,23 @@ const char* hmat_version(void);
 int32_t hmat_device_count(void);
 void hmat_options_reference(hmat_options* opts, double tol);
 
+/* Opt-in kernel timing (no reference counterpart; the reference has only a
+ * std::chrono Timer in its drivers, src/main_hodlr.cpp:14-24).  When enabled,
+ * named launches are bracketed by HIP events on their own stream and their
+ * algorithmic work is summed:
+ *   "sketch_gen" / "sketch_dense"  FP64 flops of the block sketch GEMMs. */
+void hmat_profile_enable(int32_t on);
+void hmat_profile_reset(void);
+/* returns 1 and fills the totals when `name` was recorded, else 0 */
+int32_t hmat_profile_query(const char* name, int64_t* launches, double* ms, double* work);
+
+/* Exact FP64 Y = A X for a generated kernel matrix, A never stored (checks
+ * the true-matrix residual at sizes where a dense A cannot exist; the
+ * reference forms A densely, kernel.h:30-106).  Device pointers. */
+hmat_status hmat_kernel_matvec_device(const hmat_kernel* k_device_points, int64_t N,
+                                      const double* dX, int64_t ldx, int32_t nrhs, double* dY,
+                                      int64_t ldy);
+
 /* ----------------------------------------------------------------- HODLR -- */
 /* HODLRMatrixOMP::HODLRMatrixOMP(double tol)  hodlr_routines_omp.h:17 */
 hmat_status hmat_hodlr_create(hmat_hodlr* h, const hmat_options* opts);
@@ -126,6 +143,10 @@ hmat_status hmat_hodlr_build_dense_device(hmat_hodlr h, const double* dA,
  * (the reference overflows int at matrix.h:34). */
 hmat_status hmat_hodlr_build_kernel(hmat_hodlr h, const hmat_kernel* k,
                                     int64_t N, int32_t leaf_size);
+/* Same with x / y already on the device (inputs resident in HBM). */
+hmat_status hmat_hodlr_build_kernel_device(hmat_hodlr h,
+                                           const hmat_kernel* k_device_points,
+                                           int64_t N, int32_t leaf_size);
 /* Number of doubles of host Omega that build_dense expects. */
 int64_t hmat_hodlr_omega_length(const hmat_options* opts, int64_t N,
                                 int32_t leaf_size);

@@ -32,11 +32,9 @@ def oracle_ranks_flat(h, depth):
 def test_reference_mode_same_omega_parity(n, leaf, tol, seed):
     """Reference Cauchy driver matrix, reference options, the oracle's Omega.
 
-    The trailing directions of each sketch Y = A Omega sit at the rounding
-    level, so they depend on summation order (MFMA vs BLAS): two exact FP64
-    implementations of the same RSVD agree to a fraction of their common
-    compression error, not bitwise.  Exact-SVD blocks have no such freedom
-    and must agree to rounding.
+    Same Omega, same ranks, and the compressed operators agree to rounding
+    (the orthogonalisation is iterated until Q^T Q = I to 1e-12, which is
+    what makes the projection Q^T A match the oracle's SVD-based basis).
     """
     a = kernels.generate_matrix(kernels.CAUCHY, n)
     s = rng.RandomStream(seed, record=True)
@@ -50,8 +48,8 @@ def test_reference_mode_same_omega_parity(n, leaf, tol, seed):
         assert hg.level_ranks(d) == oracle_ranks_flat(ho, d), d
     G, O = hg.to_dense(), ho.to_dense()
     comp_err = rel(O, a)                      # the oracle's own compression error
-    assert rel(G, O) <= comp_err
-    assert rel(G, a) <= 1.05 * comp_err
+    assert rel(G, O) < 1e-12
+    assert abs(rel(G, a) - comp_err) <= 1e-3 * comp_err
     # blocks below 256 take the exact SVD path (compression.h:72)
     def walk(nd):
         if nd.is_leaf:
@@ -65,10 +63,11 @@ def test_reference_mode_same_omega_parity(n, leaf, tol, seed):
     walk(ho.root)
     x = rng.RandomStream(seed + 1).random(n, 1)
     assert rel(hg.multiply(x), G @ x) < 1e-14
-    assert rel(hg.multiply(x), ho.multiply(x)) <= comp_err
+    assert rel(hg.multiply(x), ho.multiply(x)) < 1e-12
     b = a @ x
     xs_g, xs_o = hg.solve(b), ho.solve(b)
-    assert rel(xs_g, xs_o) <= 10 * tol
+    assert rel(xs_g, xs_o) <= 10 * tol        # the drop-in bar
+    assert rel(xs_g, xs_o) < 1e-9
 
 
 def test_fixed_rank_power_iterations_same_omega_parity():
@@ -87,7 +86,7 @@ def test_fixed_rank_power_iterations_same_omega_parity():
     hg = hmat_amd.HODLRMatrixOMP(opts=og)
     hg.build_from_dense(a, leaf, omega=s.flat_log())
     G, O = hg.to_dense(), ho.to_dense()
-    assert rel(G, O) <= max(rel(O, a), 1e-13)
+    assert rel(G, O) < 1e-12
     b = np.random.default_rng(5).standard_normal((n, 2))
     assert rel(hg.solve(b), ho.solve_dense(b)) < 1e-9
 
