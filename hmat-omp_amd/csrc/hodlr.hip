@@ -264,7 +264,7 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
   DevBuf<int> drank;
   drank.upload(L.rank, stream);
   kern::trunc_prep(sm, dsm_, Rb_.get(), Vr_.get(), Sv_.get(), kcap, drank.get(), L.rmax, TU_.get(),
-                   TV_.get(), stream);
+                   TV_.get(), 0, stream);
   const int64_t nr = N * (int64_t)L.rmax;
   L.US.resize(nr);
   L.V.resize(nr);

@@ -1,11 +1,34 @@
 // HSS engine: the MI355X counterpart of HSSMatrixOMP
 // (include/hss_routines_omp.h:11-375).
+//
+// Generators (per node p with children a, b):
+//   leaves:   D_i, U_i (len x rU_i), V_i (len x rV_i) — the reference's
+//             leaf bases: RSVD of the whole off-diagonal row block A(I, I^c)
+//             and column block A(I^c, I)^T (hss_routines_omp.h:69-95);
+//   internal: transfers R_p ((rU_a + rU_b) x rU_p), W_p (same for V), and
+//             couplings B_ab = U_a^T A(I_a, I_b) V_b, B_ba = U_b^T A(I_b, I_a) V_a
+//             with the full (nested) bases of the children.
+// nested = false reproduces the reference operator: R = W = I and ranks add
+// up (hss_routines_omp.h:110-119).  nested = true compresses the transfers
+// (tolerance or fixed rank), so ranks stay bounded and the operator costs
+// O(N r) to apply.
 #pragma once
 #include <vector>
 
 #include "hodlr.hpp"
 
 namespace hmat {
+
+struct SNode {
+  int rU = 0, rV = 0;
+  // offsets into the small-matrix pool and their leading dimensions
+  // (uniform over a depth so that one launch covers the depth)
+  int64_t oR = -1, oW = -1;      // transfers: (rU_a+rU_b) x rU, (rV_a+rV_b) x rV
+  int64_t oBab = -1, oBba = -1;  // couplings: rU_a x rV_b, rU_b x rV_a
+  int ldR = 1, ldW = 1, ldB = 1;
+  int64_t oMinv = -1, oG = -1, oK = -1;  // factor: M^{-1} (ms x ms), G (kR x rU), K_hat (rV x rU)
+  int ms = 0;                             // rV_a + rV_b
+};
 
 class HssEngine {
  public:
@@ -26,9 +49,43 @@ class HssEngine {
   bool own_stream = false;
   int64_t N = 0;
   int leaf = 0;
+  bool nested = false;
   bool built = false, factored = false;
   DevBuf<double> px, py;  // kernel points
   Tree tree;
+  std::vector<SNode> sn;  // per tree node
+  int rUmax = 0, rVmax = 0;
+
+ private:
+  void compress_leaves(const GenDesc& g, bool col_side, const double* host_omega,
+                       const std::vector<int64_t>& omega_off);
+  void build_internal();
+  int64_t pool_alloc(int64_t n);
+  void leaf_apply(const double* Dm, const double* B, int64_t ldb, int ncols, double* Y, int64_t ldy);
+
+  // leaf generators
+  DevBuf<double> D, Dinv, Ul, Vl, Ql;  // D/Dinv: nleaf x lmax^2; Ul, Vl, Ql: N x rmax
+  // projected blocks of the leaves (row side T^T, column side S^T), one
+  // N x r column block per leaf, and each leaf's column offset
+  DevBuf<double> leafT[2];
+  std::vector<int64_t> leafcol[2];
+  // small matrices (transfers, couplings, factors) in one device pool
+  DevBuf<double> pool;
+  int64_t pool_used = 0;
+  int ldv = 1;  // leading dimension of every per-node vector in the sweeps
+  DevBuf<double> va_, vb_, vc_, vd_, ve_;
+
+  kern::Workspace ws_;
+  Orthogonalizer orth_;
+  DevBuf<SegNN> dnn_;
+  DevBuf<SegTN> dtn_;
+  DevBuf<SegSk> dsk_;
+  DevBuf<SmallMat> dsm_;
+  DevBuf<BlockDesc> dbd_;
+  DevBuf<kern::BGemm> dbg_;
+  DevBuf<kern::ZeroRect> dzr_;
+  DevBuf<kern::Rect> drc_;
+  DevBuf<int> info_;
 };
 
 }  // namespace hmat

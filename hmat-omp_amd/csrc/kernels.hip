@@ -114,8 +114,9 @@ __global__ __launch_bounds__(64) void k_seg_trsm_upper(const SegNN* __restrict__
                                                        const NNTile* __restrict__ tiles,
                                                        const double* __restrict__ X, int64_t ldx,
                                                        const double* __restrict__ R, int64_t ldr,
-                                                       double* __restrict__ Y, int64_t ldy) {
-  extern __shared__ double q[];  // [k][64]
+                                                       double* __restrict__ Y, int64_t ldy,
+                                                       int r_in_lds) {
+  extern __shared__ double q[];  // [k][64], then R (k x k) when it fits
   const NNTile tl = tiles[blockIdx.x];
   const SegNN sg = segs[tl.seg];
   const int t = threadIdx.x;
@@ -123,8 +124,17 @@ __global__ __launch_bounds__(64) void k_seg_trsm_upper(const SegNN* __restrict__
   const bool live = row < sg.len;
   const int k = sg.q;
   for (int j = 0; j < k; ++j) q[j * 64 + t] = live ? X[sg.xoff + row + (int64_t)j * ldx] : 0.0;
+  const double* rbase = R + sg.toff;
+  int64_t lrr = ldr;
+  if (r_in_lds) {
+    double* rs = q + (int64_t)k * 64;
+    for (int e = t; e < k * k; e += 64) rs[e] = R[sg.toff + e % k + (int64_t)(e / k) * ldr];
+    __syncthreads();
+    rbase = rs;
+    lrr = k;
+  }
   for (int j = 0; j < k; ++j) {
-    const double* rj = R + sg.toff + (int64_t)j * ldr;
+    const double* rj = rbase + (int64_t)j * lrr;
     double acc = q[j * 64 + t];
     for (int i = 0; i < j; ++i) acc = fma(-q[i * 64 + t], rj[i], acc);
     q[j * 64 + t] = rj[j] != 0.0 ? acc / rj[j] : 0.0;
@@ -149,8 +159,12 @@ void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const 
   dsegs.upload(segs, s);
   DevBuf<NNTile> dt;
   dt.upload(tiles, s);
-  const size_t lds = (size_t)kmax * 64 * sizeof(double);
-  k_seg_trsm_upper<<<(unsigned)tiles.size(), 64, lds, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy);
+  size_t lds = (size_t)kmax * 64 * sizeof(double);
+  const size_t rlds = (size_t)kmax * kmax * sizeof(double);
+  const int r_in_lds = (lds + rlds <= 96 * 1024) ? 1 : 0;
+  if (r_in_lds) lds += rlds;
+  k_seg_trsm_upper<<<(unsigned)tiles.size(), 64, lds, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy,
+                                                           r_in_lds);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));
 }
@@ -302,20 +316,24 @@ __device__ inline double block_sum(double v, double* red) {
   return t;
 }
 
-// R^T R = G + shift I (upper R); Rinv = R^{-1}.  A zero Gram (all-zero
-// input) returns R = Rinv = 0 so that the basis built from it is zero.
-__global__ __launch_bounds__(kWG) void k_chol_rinv(const SmallMat* __restrict__ mats,
-                                                   const double* __restrict__ G,
-                                                   double* __restrict__ R, double* __restrict__ Ri,
-                                                   const double* __restrict__ shift_coef,
-                                                   int* __restrict__ info, double* __restrict__ gerr) {
+// R^T R = G + shift I (upper R).  A zero Gram (all-zero input) returns
+// R = 0, which the triangular solve turns into a zero basis.  The matrix is
+// factored in LDS (dynamic, n*(n+1) doubles) when it fits, else in place in
+// global memory.
+__global__ __launch_bounds__(kWG) void k_chol(const SmallMat* __restrict__ mats,
+                                              const double* __restrict__ G, double* __restrict__ R,
+                                              const double* __restrict__ shift_coef,
+                                              int* __restrict__ info, double* __restrict__ gerr,
+                                              int use_lds) {
+  extern __shared__ double sm[];
   __shared__ double red[kWG / 64];
-  __shared__ int bad;
+  __shared__ double dred[kWG / 64];
   const SmallMat m = mats[blockIdx.x];
   const int n = m.n, ld = m.ld;
   const double* g = G + m.off;
-  double* r = R + m.off;
-  double* ri = Ri + m.off;
+  double* rg = R + m.off;
+  double* r = use_lds ? sm : rg;
+  const int lr = use_lds ? n + 1 : ld;
   double tr = 0.0, dev = 0.0;
   for (int i = threadIdx.x; i < n; i += kWG) tr += g[i + (int64_t)i * ld];
   for (int e = threadIdx.x; e < n * n; e += kWG) {
@@ -323,88 +341,92 @@ __global__ __launch_bounds__(kWG) void k_chol_rinv(const SmallMat* __restrict__ 
     dev = fmax(dev, fabs(g[i + (int64_t)j * ld] - (i == j ? 1.0 : 0.0)));
   }
   tr = block_sum(tr, red);
-  {
-    for (int o = 32; o > 0; o >>= 1) dev = fmax(dev, __shfl_xor(dev, o));
-    __shared__ double dred[kWG / 64];
-    if (threadIdx.x % 64 == 0) dred[threadIdx.x / 64] = dev;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double d = 0.0;
-      for (int i = 0; i < kWG / 64; ++i) d = fmax(d, dred[i]);
-      gerr[blockIdx.x] = (tr == 0.0) ? 0.0 : d;
-    }
+  for (int o = 32; o > 0; o >>= 1) dev = fmax(dev, __shfl_xor(dev, o));
+  if (threadIdx.x % 64 == 0) dred[threadIdx.x / 64] = dev;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double d = 0.0;
+    for (int i = 0; i < kWG / 64; ++i) d = fmax(d, dred[i]);
+    gerr[blockIdx.x] = (tr == 0.0) ? 0.0 : d;
+    info[blockIdx.x] = 0;
   }
   const double shift = shift_coef[blockIdx.x] * tr;
-  if (threadIdx.x == 0) { bad = 0; info[blockIdx.x] = 0; }
   for (int e = threadIdx.x; e < n * n; e += kWG) {
     const int i = e % n, j = e / n;
-    r[i + (int64_t)j * ld] = (i <= j) ? g[i + (int64_t)j * ld] + (i == j ? shift : 0.0) : 0.0;
-    ri[i + (int64_t)j * ld] = 0.0;
+    r[i + (int64_t)j * lr] = (i <= j) ? g[i + (int64_t)j * ld] + (i == j ? shift : 0.0) : 0.0;
   }
   __syncthreads();
   if (!(tr > 0.0)) {
     if (!(tr == 0.0) && threadIdx.x == 0) info[blockIdx.x] = -1;  // NaN
-    return;  // R = Rinv = 0
+    for (int e = threadIdx.x; e < n * n; e += kWG) rg[e % n + (int64_t)(e / n) * ld] = 0.0;
+    return;
   }
   for (int j = 0; j < n; ++j) {
-    const double d = r[j + (int64_t)j * ld];
+    const double d = r[j + (int64_t)j * lr];
     if (!(d > 0.0) || !isfinite(d)) {
-      if (threadIdx.x == 0) { bad = j + 1; info[blockIdx.x] = j + 1; }
+      if (threadIdx.x == 0) info[blockIdx.x] = j + 1;
       break;
     }
     const double rjj = sqrt(d);
     __syncthreads();
-    for (int c = j + 1 + threadIdx.x; c < n; c += kWG) r[j + (int64_t)c * ld] /= rjj;
-    if (threadIdx.x == 0) r[j + (int64_t)j * ld] = rjj;
+    for (int c = j + 1 + threadIdx.x; c < n; c += kWG) r[j + (int64_t)c * lr] /= rjj;
+    if (threadIdx.x == 0) r[j + (int64_t)j * lr] = rjj;
     __syncthreads();
     const int rem = n - j - 1;
     for (int e = threadIdx.x; e < rem * rem; e += kWG) {
-      const int a = j + 1 + e % rem, c = j + 1 + e / rem;
-      if (a <= c) r[a + (int64_t)c * ld] -= r[j + (int64_t)a * ld] * r[j + (int64_t)c * ld];
+      const int a2 = j + 1 + e % rem, c = j + 1 + e / rem;
+      if (a2 <= c) r[a2 + (int64_t)c * lr] -= r[j + (int64_t)a2 * lr] * r[j + (int64_t)c * lr];
     }
     __syncthreads();
   }
   __syncthreads();
-  if (bad) return;
-  // R^{-1}: column c by thread c (back substitution, upper triangular)
-  for (int c = threadIdx.x; c < n; c += kWG) {
-    ri[c + (int64_t)c * ld] = 1.0 / r[c + (int64_t)c * ld];
-    for (int i = c - 1; i >= 0; --i) {
-      double s = 0.0;
-      for (int t = i + 1; t <= c; ++t) s += r[i + (int64_t)t * ld] * ri[t + (int64_t)c * ld];
-      ri[i + (int64_t)c * ld] = -s / r[i + (int64_t)i * ld];
+  if (use_lds)
+    for (int e = threadIdx.x; e < n * n; e += kWG) {
+      const int i = e % n, j = e / n;
+      rg[i + (int64_t)j * ld] = r[i + (int64_t)j * lr];
     }
-  }
 }
 
 void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* G, double* R,
                double* Rinv, const std::vector<double>& shift_coef, DevBuf<double>& dshift,
                int* info, double* gerr, hipStream_t s) {
+  (void)Rinv;
   if (mats.empty()) return;
+  int nmax = 0;
+  for (auto& mm : mats) nmax = std::max(nmax, mm.n);
+  const size_t lds = (size_t)nmax * (nmax + 1) * sizeof(double);
+  const int use_lds = lds <= 64 * 1024 ? 1 : 0;
   dm.upload(mats, s);
   dshift.upload(shift_coef, s);
-  k_chol_rinv<<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), G, R, Rinv, dshift.get(), info, gerr);
+  k_chol<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), G, R, dshift.get(), info, gerr,
+                                                                use_lds);
   HIP_CHECK(hipGetLastError());
 }
 
 // One-sided Jacobi (Hestenes) with a round-robin pair schedule; the four
-// waves rotate disjoint column pairs in parallel.
+// waves rotate disjoint column pairs in parallel.  A and V live in LDS
+// (dynamic, 2 n (n+1) doubles) when they fit, else in global memory.
 __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__ mats,
                                                     double* __restrict__ A, double* __restrict__ V,
                                                     double* __restrict__ S, int64_t sstride,
-                                                    double* __restrict__ work) {
+                                                    double* __restrict__ work, int use_lds) {
+  extern __shared__ double sm[];
   __shared__ int rotated;
   __shared__ double sig[512];
   __shared__ int perm[512];
   const SmallMat m = mats[blockIdx.x];
   const int n = m.n, ld = m.ld;
-  double* a = A + m.off;
-  double* v = V + m.off;
+  double* ag = A + m.off;
+  double* vg = V + m.off;
+  double* a = use_lds ? sm : ag;
+  double* v = use_lds ? sm + (int64_t)n * (n + 1) : vg;
+  const int la = use_lds ? n + 1 : ld;
   double* sv = S + (int64_t)blockIdx.x * sstride;
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
   for (int e = threadIdx.x; e < n * n; e += kWG) {
     const int i = e % n, j = e / n;
-    v[i + (int64_t)j * ld] = (i == j) ? 1.0 : 0.0;
+    if (use_lds) a[i + (int64_t)j * la] = ag[i + (int64_t)j * ld];
+    v[i + (int64_t)j * la] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
   const int np = n + (n & 1);  // even player count; column n is a phantom
@@ -426,7 +448,7 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
         if (p > q) { const int t = p; p = q; q = t; }
         double al = 0.0, be = 0.0, ga = 0.0;
         for (int i = lane; i < n; i += 64) {
-          const double x = a[i + (int64_t)p * ld], y = a[i + (int64_t)q * ld];
+          const double x = a[i + (int64_t)p * la], y = a[i + (int64_t)q * la];
           al += x * x;
           be += y * y;
           ga += x * y;
@@ -445,12 +467,12 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
           t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
         for (int i = lane; i < n; i += 64) {
-          const double x = a[i + (int64_t)p * ld], y = a[i + (int64_t)q * ld];
-          a[i + (int64_t)p * ld] = c * x - sn * y;
-          a[i + (int64_t)q * ld] = sn * x + c * y;
-          const double vx = v[i + (int64_t)p * ld], vy = v[i + (int64_t)q * ld];
-          v[i + (int64_t)p * ld] = c * vx - sn * vy;
-          v[i + (int64_t)q * ld] = sn * vx + c * vy;
+          const double x = a[i + (int64_t)p * la], y = a[i + (int64_t)q * la];
+          a[i + (int64_t)p * la] = c * x - sn * y;
+          a[i + (int64_t)q * la] = sn * x + c * y;
+          const double vx = v[i + (int64_t)p * la], vy = v[i + (int64_t)q * la];
+          v[i + (int64_t)p * la] = c * vx - sn * vy;
+          v[i + (int64_t)q * la] = sn * vx + c * vy;
         }
         if (lane == 0) rotated = 1;
       }
@@ -462,7 +484,7 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
   // singular values = column norms
   for (int j = w; j < n; j += kWG / 64) {
     double s2 = 0.0;
-    for (int i = lane; i < n; i += 64) s2 += a[i + (int64_t)j * ld] * a[i + (int64_t)j * ld];
+    for (int i = lane; i < n; i += 64) s2 += a[i + (int64_t)j * la] * a[i + (int64_t)j * la];
     for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o);
     if (lane == 0) sig[j] = sqrt(s2);
   }
@@ -479,20 +501,28 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
     }
   }
   __syncthreads();
-  double* wu = work + m.off * 2;
-  double* wv = wu + (int64_t)n * ld;
-  for (int e = threadIdx.x; e < n * n; e += kWG) {
-    const int i = e % n, j = e / n;
-    wu[i + (int64_t)j * ld] = a[i + (int64_t)j * ld];
-    wv[i + (int64_t)j * ld] = v[i + (int64_t)j * ld];
+  const double* su = a;
+  const double* svv = v;
+  int ls = la;
+  if (!use_lds) {
+    double* wu = work + m.off * 2;
+    double* wv = wu + (int64_t)n * ld;
+    for (int e = threadIdx.x; e < n * n; e += kWG) {
+      const int i = e % n, j = e / n;
+      wu[i + (int64_t)j * ld] = a[i + (int64_t)j * ld];
+      wv[i + (int64_t)j * ld] = v[i + (int64_t)j * ld];
+    }
+    __syncthreads();
+    su = wu;
+    svv = wv;
+    ls = ld;
   }
-  __syncthreads();
   for (int e = threadIdx.x; e < n * n; e += kWG) {
     const int i = e % n, j = e / n;
     const int pj = perm[j];
     const double sj = sig[pj];
-    a[i + (int64_t)j * ld] = sj > 0.0 ? wu[i + (int64_t)pj * ld] / sj : 0.0;
-    v[i + (int64_t)j * ld] = wv[i + (int64_t)pj * ld];
+    ag[i + (int64_t)j * ld] = sj > 0.0 ? su[i + (int64_t)pj * ls] / sj : 0.0;
+    vg[i + (int64_t)j * ld] = svv[i + (int64_t)pj * ls];
   }
   for (int j = threadIdx.x; j < n; j += kWG) sv[j] = sig[perm[j]];
 }
@@ -502,8 +532,13 @@ void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double*
   if (mats.empty()) return;
   for (auto& mm : mats)
     HMAT_REQUIRE(mm.n <= 512, HMAT_ERR_ARG, "jacobi_svd: n > 512");
+  int nmax = 0;
+  for (auto& mm : mats) nmax = std::max(nmax, mm.n);
+  const size_t lds = (size_t)2 * nmax * (nmax + 1) * sizeof(double);
+  const int use_lds = lds <= 128 * 1024 ? 1 : 0;
   dm.upload(mats, s);
-  k_jacobi_svd<<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), A, V, S, sstride, work);
+  k_jacobi_svd<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), A, V, S, sstride, work,
+                                                                     use_lds);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -690,25 +725,96 @@ void copy_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const doubl
 __global__ void k_trunc_prep(const SmallMat* __restrict__ mats, const double* __restrict__ Ur,
                              const double* __restrict__ Vr, const double* __restrict__ S,
                              int64_t sstride, const int* __restrict__ rank, int rmax,
-                             double* __restrict__ TU, double* __restrict__ TV) {
+                             double* __restrict__ TU, double* __restrict__ TV, int s_on_ur) {
   const SmallMat m = mats[blockIdx.x];
   const int r = rank[blockIdx.x];
   const double* sv = S + (int64_t)blockIdx.x * sstride;
   for (int e = threadIdx.x; e < m.n * rmax; e += blockDim.x) {
     const int i = e % m.n, j = e / m.n;
     const int64_t o = m.off + i + (int64_t)j * m.ld;
-    TU[o] = j < r ? Vr[o] * sv[j] : 0.0;
-    TV[o] = j < r ? Ur[o] : 0.0;
+    const double sj = j < r ? sv[j] : 0.0;
+    TU[o] = j < r ? (s_on_ur ? Vr[o] : Vr[o] * sj) : 0.0;
+    TV[o] = j < r ? (s_on_ur ? Ur[o] * sj : Ur[o]) : 0.0;
   }
 }
 
 void trunc_prep(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, const double* Ur,
                 const double* Vr, const double* S, int64_t sstride, const int* rank, int rmax,
-                double* TU, double* TV, hipStream_t s) {
+                double* TU, double* TV, int s_on_ur, hipStream_t s) {
   if (mats.empty()) return;
   dm.upload(mats, s);
-  k_trunc_prep<<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), Ur, Vr, S, sstride, rank, rmax, TU, TV);
+  k_trunc_prep<<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), Ur, Vr, S, sstride, rank, rmax, TU, TV,
+                                                     s_on_ur);
   HIP_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kWG) void k_bgemm(const BGemm* __restrict__ items) {
+  const BGemm g = items[blockIdx.x];
+  const int64_t total = (int64_t)g.m * g.n;
+  for (int64_t e = threadIdx.x; e < total; e += kWG) {
+    const int i = (int)(e % g.m), j = (int)(e / g.m);
+    double acc = 0.0;
+    for (int l = 0; l < g.k; ++l) {
+      const double a = g.ta ? g.A[l + (int64_t)i * g.lda] : g.A[i + (int64_t)l * g.lda];
+      const double b = g.tb ? g.B[j + (int64_t)l * g.ldb] : g.B[l + (int64_t)j * g.ldb];
+      acc = fma(a, b, acc);
+    }
+    double* c = g.C + i + (int64_t)j * g.ldc;
+    *c = g.alpha * acc + (g.beta != 0.0 ? g.beta * *c : 0.0);
+  }
+}
+
+void bgemm(const std::vector<BGemm>& items, DevBuf<BGemm>& d, hipStream_t s) {
+  if (items.empty()) return;
+  d.upload(items, s);
+  k_bgemm<<<(unsigned)items.size(), kWG, 0, s>>>(d.get());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+__global__ void k_rect_axpby(const Rect* __restrict__ items) {
+  const Rect r = items[blockIdx.y];
+  const int64_t total = (int64_t)r.m * r.n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % r.m), j = (int)(e / r.m);
+    double* d = r.dst + i + (int64_t)j * r.ldd;
+    const double sv = r.src ? r.src[i + (int64_t)j * r.lds] : 0.0;
+    *d = r.alpha * sv + (r.beta != 0.0 ? r.beta * *d : 0.0);
+  }
+}
+__global__ void k_rect_add_identity(const Rect* __restrict__ items) {
+  const Rect r = items[blockIdx.y];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < min(r.m, r.n); i += gridDim.x * blockDim.x)
+    r.dst[i + (int64_t)i * r.ldd] += r.alpha;
+}
+void rect_axpby(const std::vector<Rect>& items, DevBuf<Rect>& d, hipStream_t s) {
+  if (items.empty()) return;
+  d.upload(items, s);
+  k_rect_axpby<<<dim3(16, (unsigned)items.size()), kWG, 0, s>>>(d.get());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+void rect_add_identity(const std::vector<Rect>& items, DevBuf<Rect>& d, hipStream_t s) {
+  if (items.empty()) return;
+  d.upload(items, s);
+  k_rect_add_identity<<<dim3(4, (unsigned)items.size()), kWG, 0, s>>>(d.get());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+__global__ void k_zero_rects(const ZeroRect* __restrict__ items) {
+  const ZeroRect z = items[blockIdx.y];
+  const int64_t total = (int64_t)z.len * z.ncols;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x)
+    z.p[(z.r0 + e % z.len) + (z.c0 + e / z.len) * z.ld] = 0.0;
+}
+
+void zero_rects(const std::vector<ZeroRect>& items, DevBuf<ZeroRect>& d, hipStream_t s) {
+  if (items.empty()) return;
+  d.upload(items, s);
+  k_zero_rects<<<dim3(32, (unsigned)items.size()), kWG, 0, s>>>(d.get());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 __device__ inline uint64_t splitmix64(uint64_t z) {

@@ -143,15 +143,48 @@ void gen_blocks(const GenDesc& g, const std::vector<BlockDesc>& blocks, DevBuf<B
 void copy_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X, int64_t ldx,
                double* Y, int64_t ldy, hipStream_t s);
 
-// Truncation of a small SVD R = Ur diag(S) Vr^T to rank[b] columns:
-//   TU = Vr(:, :r) diag(S(:r)),  TV = Ur(:, :r), zero columns up to rmax.
+// Truncation of a small SVD R = Ur diag(S) Vr^T to rank[b] columns, zero
+// columns up to rmax:
+//   s_on_ur = 0:  TU = Vr(:, :r) diag(S(:r)),  TV = Ur(:, :r)
+//   s_on_ur = 1:  TU = Vr(:, :r),              TV = Ur(:, :r) diag(S(:r))
 void trunc_prep(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, const double* Ur,
                 const double* Vr, const double* S, int64_t sstride, const int* rank, int rmax,
-                double* TU, double* TV, hipStream_t s);
+                double* TU, double* TV, int s_on_ur, hipStream_t s);
+
+// dst(0:m, 0:n) = alpha src(0:m, 0:n) + beta dst (src may be null: beta dst),
+// and dst(i, i) += alpha for the identity items; per-item leading dimensions.
+struct Rect {
+  const double* src;
+  double* dst;
+  int64_t lds, ldd;
+  int32_t m, n;
+  double alpha, beta;
+};
+void rect_axpby(const std::vector<Rect>& items, DevBuf<Rect>& d, hipStream_t s);
+void rect_add_identity(const std::vector<Rect>& items, DevBuf<Rect>& d, hipStream_t s);
 
 // Z(:, c) = alpha X(:, c) + beta Y(:, c) for n rows, ncols columns.
 void axpby_cols(int64_t n, int ncols, double alpha, const double* X, int64_t ldx, double beta,
                 const double* Y, int64_t ldy, double* Z, int64_t ldz, hipStream_t s);
+
+// Batched small GEMM C = alpha op(A) op(B) + beta C, one workgroup per
+// entry (per-node work of the HSS sweeps and factorisation; sizes up to a
+// few hundred).
+struct BGemm {
+  const double* A;
+  const double* B;
+  double* C;
+  int32_t m, n, k, lda, ldb, ldc, ta, tb;
+  double alpha, beta;
+};
+void bgemm(const std::vector<BGemm>& items, DevBuf<BGemm>& d, hipStream_t s);
+// Zero rows [r0, r0 + len) of columns [c0, c0 + ncols) of a column-major array
+struct ZeroRect {
+  double* p;
+  int64_t ld;
+  int32_t r0, len, c0, ncols;
+};
+void zero_rects(const std::vector<ZeroRect>& items, DevBuf<ZeroRect>& d, hipStream_t s);
 
 // Y = (To) X elementwise for n contiguous values.
 template <class To, class Ti>

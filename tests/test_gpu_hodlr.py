@@ -2,7 +2,7 @@
 
 Bars:
   * same host Omega as the oracle (the reference's serial draw order):
-    identical per-block ranks, operator within 1e-10, solution within
+    identical per-block ranks, operator within 1e-12, solution within
     10 x tol of the oracle's solution (the stated drop-in bar);
   * any Omega: matvec equals the dense reconstruction of the compressed
     operator to 1e-13, solve equals dense LAPACK on that reconstruction to
