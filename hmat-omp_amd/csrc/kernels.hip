@@ -106,10 +106,70 @@ template void seg_gemm_nn<float>(const std::vector<SegNN>&, DevBuf<SegNN>&, cons
                                  const float*, int64_t, float*, int64_t, float, float, hipStream_t);
 
 // --------------------------------------------------------------------------
-// seg_trsm_upper: one thread per row, the row held in LDS (column-major so
-// that the 64 lanes touch 64 consecutive words), R read with wave-uniform
-// addresses.
+// seg_trsm_upper: Y(seg) = X(seg) R_s^{-1}, R_s upper triangular (zero
+// pivots give zero columns).  Forward substitution along each row.
+//
+// k <= 64: the row lives in registers (K = 16/32/64 doubles per lane, padded
+// with R = I beyond k), R in LDS and read with wave-uniform (broadcast)
+// addresses; 256 rows per workgroup, so X is read and Y written exactly once
+// with coalesced column accesses.  k > 64: the row lives in LDS.
 // --------------------------------------------------------------------------
+// Straight-line substitution, generated by template recursion so that the
+// K(K-1)/2 updates are fully unrolled for every K (the loop unroller gives
+// up at K = 64, which would put the row in scratch memory).
+template <int K, int J, int C0>
+__device__ __forceinline__ void trsm_chunk(double (&q)[K], const double* rt) {
+  if constexpr (C0 < K) {
+#pragma unroll
+    for (int c = C0; c < C0 + 8 && c < K; ++c) {
+      q[c] = fma(-q[J], rt[J * K + c], q[c]);
+      // finish this chunk before the next chunk's R loads are issued, so at
+      // most 8 R values are live (otherwise every load is hoisted up front)
+      asm volatile("" : "+v"(q[c])::"memory");
+    }
+    trsm_chunk<K, J, C0 + 8>(q, rt);
+  }
+}
+template <int K, int J>
+__device__ __forceinline__ void trsm_step(double (&q)[K], const double* rt) {
+  if constexpr (J < K) {
+    const double d = rt[J * K + J];
+    q[J] = d != 0.0 ? q[J] / d : 0.0;
+    trsm_chunk<K, J, J + 1>(q, rt);
+    trsm_step<K, J + 1>(q, rt);
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(kWG) void k_seg_trsm_reg(const SegNN* __restrict__ segs,
+                                                      const NNTile* __restrict__ tiles,
+                                                      const double* __restrict__ X, int64_t ldx,
+                                                      const double* __restrict__ R, int64_t ldr,
+                                                      double* __restrict__ Y, int64_t ldy) {
+  __shared__ double rt[K * K];  // R^T: row j of R at rt[j*K]
+  const NNTile tl = tiles[blockIdx.x];
+  const SegNN sg = segs[tl.seg];
+  const int k = sg.q;
+  for (int e = threadIdx.x; e < K * K; e += kWG) {
+    const int c = e % K, j = e / K;  // rt[j*K + c] = R(j, c)
+    rt[e] = (j < k && c < k) ? R[sg.toff + j + (int64_t)c * ldr] : (j == c ? 1.0 : 0.0);
+  }
+  const int row = tl.row0 + threadIdx.x;
+  const bool live = row < sg.len;
+  double q[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) q[j] = (live && j < k) ? X[sg.xoff + row + (int64_t)j * ldx] : 0.0;
+  __syncthreads();
+  // right-looking: once q_j is final, q_c -= q_j R(j, c) for c > j.  Each
+  // q_c sees the same operations in the same order as the left-looking
+  // recurrence q_c = (x_c - sum_{i<c} q_i R(i, c)) / R(c, c).
+  trsm_step<K, 0>(q, rt);
+  if (!live) return;
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+    if (j < k) Y[sg.yoff + row + (int64_t)j * ldy] = q[j];
+}
+
 __global__ __launch_bounds__(64) void k_seg_trsm_upper(const SegNN* __restrict__ segs,
                                                        const NNTile* __restrict__ tiles,
                                                        const double* __restrict__ X, int64_t ldx,
@@ -147,24 +207,36 @@ void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const 
                     int64_t ldx, const double* R, int64_t ldr, double* Y, int64_t ldy, hipStream_t s) {
   if (segs.empty()) return;
   int kmax = 0;
+  for (const auto& g : segs)
+    if (g.len > 0 && g.q > 0) kmax = std::max(kmax, g.q);
+  if (kmax == 0) return;
+  HMAT_REQUIRE(kmax <= 300, HMAT_ERR_ARG, "seg_trsm_upper: k > 300");
+  const int rows = kmax <= 64 ? kWG : 64;
   std::vector<NNTile> tiles;
   for (size_t i = 0; i < segs.size(); ++i) {
     const auto& g = segs[i];
     if (g.len <= 0 || g.q <= 0) continue;
-    kmax = std::max(kmax, g.q);
-    for (int r0 = 0; r0 < g.len; r0 += 64) tiles.push_back({(int32_t)i, r0, 0, 0});
+    for (int r0 = 0; r0 < g.len; r0 += rows) tiles.push_back({(int32_t)i, r0, 0, 0});
   }
-  if (tiles.empty()) return;
-  HMAT_REQUIRE(kmax <= 300, HMAT_ERR_ARG, "seg_trsm_upper: k > 300");
   dsegs.upload(segs, s);
   DevBuf<NNTile> dt;
   dt.upload(tiles, s);
-  size_t lds = (size_t)kmax * 64 * sizeof(double);
-  const size_t rlds = (size_t)kmax * kmax * sizeof(double);
-  const int r_in_lds = (lds + rlds <= 96 * 1024) ? 1 : 0;
-  if (r_in_lds) lds += rlds;
-  k_seg_trsm_upper<<<(unsigned)tiles.size(), 64, lds, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy,
-                                                           r_in_lds);
+  const unsigned nt = (unsigned)tiles.size();
+  if (kmax <= 16) {
+    k_seg_trsm_reg<16><<<nt, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy);
+  } else if (kmax <= 32) {
+    k_seg_trsm_reg<32><<<nt, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy);
+  } else if (kmax <= 48) {
+    k_seg_trsm_reg<48><<<nt, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy);
+  } else if (kmax <= 64) {
+    k_seg_trsm_reg<64><<<nt, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy);
+  } else {
+    size_t lds = (size_t)kmax * 64 * sizeof(double);
+    const size_t rlds = (size_t)kmax * kmax * sizeof(double);
+    const int r_in_lds = (lds + rlds <= 96 * 1024) ? 1 : 0;
+    if (r_in_lds) lds += rlds;
+    k_seg_trsm_upper<<<nt, 64, lds, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy, r_in_lds);
+  }
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));
 }
@@ -423,11 +495,19 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
   const int la = use_lds ? n + 1 : ld;
   double* sv = S + (int64_t)blockIdx.x * sstride;
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  double fro = 0.0;
   for (int e = threadIdx.x; e < n * n; e += kWG) {
     const int i = e % n, j = e / n;
-    if (use_lds) a[i + (int64_t)j * la] = ag[i + (int64_t)j * ld];
+    const double x = ag[i + (int64_t)j * ld];
+    fro += x * x;
+    if (use_lds) a[i + (int64_t)j * la] = x;
     v[i + (int64_t)j * la] = (i == j) ? 1.0 : 0.0;
   }
+  // ||A||_F^2: pairs whose inner product is below 1e-30 ||A||_F^2 (about
+  // 20 eps^2) are rounding noise of the rotations and are left alone, so
+  // columns at the noise level do not keep the sweep loop alive.
+  fro = block_sum(fro, sig);
+  const double floor2 = 1e-30 * fro;
   __syncthreads();
   const int np = n + (n & 1);  // even player count; column n is a phantom
   const int circ = np - 1;
@@ -458,7 +538,7 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
           be += __shfl_xor(be, o);
           ga += __shfl_xor(ga, o);
         }
-        if (!(fabs(ga) > 1e-15 * sqrt(al * be)) || ga == 0.0) continue;
+        if (!(fabs(ga) > 1e-15 * sqrt(al * be)) || !(fabs(ga) > floor2)) continue;
         const double zeta = (be - al) / (2.0 * ga);
         double t;
         if (fabs(zeta) > 1e150)
