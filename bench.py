@@ -33,6 +33,7 @@ import numpy as np
 METRIC = "build+solve s to 1e-10 resid, N=2^18: mixed-prec+IR vs plain FP64 GPU"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
+PEAK_HBM_GBS = 8000.0
 
 
 def log(msg):
@@ -86,8 +87,20 @@ def run_mode(hmat_amd, torch, mode, a):
         if q:
             prof[name] = {"launches": q[0], "ms": q[1], "work": q[2]}
     ranks = [max(h.level_ranks(d)) for d in range(1, h.levels() + 1)]
+    # matvec y = A_H x of the built operator (FP64 generators), 1 RHS
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        h.multiply_device(b.data_ptr(), n, 1, y.data_ptr(), n)
+    torch.cuda.synchronize()
+    hmat_amd.profile_reset()
+    hmat_amd.profile_enable(True)
+    for _ in range(a.matvec_reps):
+        h.multiply_device(b.data_ptr(), n, 1, y.data_ptr(), n)
+    hmat_amd.profile_enable(False)
+    q = hmat_amd.profile_query("matvec")
+    mv = {"launches": q[0], "ms": q[1], "work": q[2]} if q else None
     return {"times": times, "parts": parts, "iters": iters, "resid": resid, "prof": prof,
-            "x": x.clone(), "b": b, "dx": dx, "dy": dy, "ranks": ranks, "h": h}
+            "x": x.clone(), "b": b, "dx": dx, "dy": dy, "ranks": ranks, "h": h, "matvec": mv}
 
 
 def true_residual(hmat_amd, torch, res, n):
@@ -145,6 +158,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--matvec-reps", type=int, default=20)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the engine has no "
@@ -206,6 +220,15 @@ def main():
         for k2, p2 in prof.items():
             out[f"kernel_{k2}"] = {"launches": p2["launches"], "ms_total": round(p2["ms"], 2),
                                    "tflops": round(p2["work"] / (p2["ms"] * 1e-3) / 1e12, 2)}
+    mv = res[modes[0]]["matvec"] if "fp64" not in res else res["fp64"]["matvec"]
+    if mv:
+        avg = mv["ms"] / mv["launches"]
+        gbs = mv["work"] / mv["launches"] / (avg * 1e-3) / 1e9
+        out["matvec_roofline"] = {"kernel": "fused HODLR matvec (project + reduce + expand), FP64, 1 RHS",
+                                  "bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                                  "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                                  "bytes_per_launch": mv["work"] / mv["launches"],
+                                  "avg_launch_ms": round(avg, 4), "launches": mv["launches"]}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a)
     print(json.dumps(out), flush=True)

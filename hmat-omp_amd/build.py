@@ -20,7 +20,7 @@ LIB = os.path.join(LIB_DIR, "libhmat_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["kernels.hip", "sketch.hip", "engine.hip", "hodlr.hip", "hss.hip", "capi.hip"]
+SOURCES = ["kernels.hip", "sketch.hip", "matvec.hip", "engine.hip", "hodlr.hip", "hss.hip", "capi.hip"]
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

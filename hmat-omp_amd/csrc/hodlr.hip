@@ -115,7 +115,78 @@ void HodlrEngine::build(const GenDesc& g, int64_t n, int leaf_size, const double
   // free build scratch
   Yb_.release(); Xb_.release(); Tb_.release(); Rb_.release(); Vr_.release();
   Wk_.release(); TU_.release(); TV_.release();
+  prepare_matvec();
   built = true;
+}
+
+// Per-(leaf, depth) descriptors of the fused matvec: the depth-c segment
+// holding each leaf, its ranks and the global index of its sibling.
+void HodlrEngine::prepare_matvec() {
+  const int L = tree.max_depth;
+  const int nl = (int)tree.leaves.size();
+  std::vector<kern::MvLeaf> lf(nl);
+  std::vector<int64_t> starts(nl);
+  double bytes = 0.0;
+  for (int i = 0; i < nl; ++i) {
+    const TNode& nd = tree.nodes[tree.leaves[i]];
+    lf[i] = kern::MvLeaf{nd.start, (int32_t)nd.len, 0};
+    starts[i] = nd.start;
+    bytes += 8.0 * (double)nd.len * (double)nd.len;
+  }
+  int KM = 1;
+  for (int c = 1; c <= L; ++c) KM = std::max(KM, lv[c].rmax);
+  std::vector<kern::MvLevel> lvl(L);
+  std::vector<kern::MvEntry> ent((size_t)nl * L, kern::MvEntry{0, 0, 0, 0});
+  std::vector<kern::MvSeg> segs;
+  for (int c = 1; c <= L; ++c) {
+    const HLevel& Lc = lv[c];
+    lvl[c - 1] = kern::MvLevel{Lc.V.get(), Lc.US.get(), N};
+    const int base = (int)segs.size();
+    const int ns = (int)Lc.ids.size();
+    for (int s = 0; s < ns; ++s) {
+      const TNode& nd = tree.nodes[Lc.ids[s]];
+      const int l0 = (int)(std::lower_bound(starts.begin(), starts.end(), nd.start) - starts.begin());
+      const int l1 = (int)(std::lower_bound(starts.begin(), starts.end(), nd.start + nd.len) - starts.begin());
+      const int vcols = Lc.rank[Lc.sib[s]], ucols = Lc.rank[s];
+      segs.push_back(kern::MvSeg{c - 1, l0, l1, vcols});
+      for (int leaf = l0; leaf < l1; ++leaf)
+        ent[(size_t)leaf * L + (c - 1)] = kern::MvEntry{vcols, ucols, base + Lc.sib[s], 0};
+      bytes += 8.0 * (double)nd.len * (double)(vcols + ucols);
+    }
+  }
+  mv_leaves_.upload(lf, stream);
+  if (L > 0) {
+    mv_lvls_.upload(lvl, stream);
+    mv_ent_.upload(ent, stream);
+    mv_segs_.upload(segs, stream);
+  }
+  mv_.leaves = mv_leaves_.get();
+  mv_.lvls = mv_lvls_.get();
+  mv_.ent = mv_ent_.get();
+  mv_.segs = mv_segs_.get();
+  mv_.nleaf = nl;
+  mv_.L = L;
+  mv_.nseg = (int)segs.size();
+  mv_.KM = KM;
+  mv_bytes_base_ = bytes;
+  mv_part_.reserve(std::max<size_t>((size_t)nl * L * KM * 4, 1));
+  mv_t_.reserve(std::max<size_t>(segs.size() * KM * 4, 1));
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+double HodlrEngine::matvec_bytes(int nrhs) const { return mv_bytes_base_ + 16.0 * (double)N * nrhs; }
+
+// Fused path for up to 4 right-hand sides per pass (1, 2 or 4 at a time).
+void HodlrEngine::matvec_async(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
+  const int64_t lm = tree.lmax;
+  int done = 0;
+  while (done < nrhs) {
+    const int rem = nrhs - done;
+    const int nr = rem >= 4 ? 4 : (rem >= 2 ? 2 : 1);
+    kern::hodlr_matvec_fused(mv_, D.get(), lm, dX + (int64_t)done * ldx, ldx, nr, mv_part_.get(), mv_t_.get(),
+                             dY + (int64_t)done * ldy, ldy, stream);
+    done += nr;
+  }
 }
 
 void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t>& omega_off,
@@ -295,6 +366,13 @@ void HodlrEngine::leaf_apply_t(const F* Dm, const F* B, int64_t ldb, int ncols, 
 void HodlrEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "matvec before build");
   HMAT_REQUIRE(nrhs > 0 && ldx >= N && ldy >= N, HMAT_ERR_ARG, "bad matvec dimensions");
+  if (nrhs <= 4 && kern::mv_lds_bytes(mv_, tree.lmax, 4) <= 64 * 1024) {
+    ProfScope prof("matvec", stream, matvec_bytes(nrhs));
+    matvec_async(dX, ldx, nrhs, dY, ldy);
+    prof.finish();
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return;
+  }
   leaf_apply_t<double>(D.get(), dX, ldx, nrhs, dY, ldy);
   for (int c = 1; c <= tree.max_depth; ++c) {
     HLevel& L = lv[c];

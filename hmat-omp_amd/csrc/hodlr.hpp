@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "matvec.hpp"
 
 namespace hmat {
 
@@ -105,6 +106,10 @@ class HodlrEngine {
   template <class F> void leaf_apply_t(const F* Dm, const F* B, int64_t ldb, int ncols, F* Y,
                                        int64_t ldy);
   void apply_inverse(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx);
+  // fused matvec plan (built once per build) and its launch without a host sync
+  void prepare_matvec();
+  void matvec_async(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
+  double matvec_bytes(int nrhs) const;
 
   HFactor<double> f64_;
   HFactor<float> f32_;
@@ -120,6 +125,13 @@ class HodlrEngine {
   DevBuf<double> Yb_, Xb_, Tb_, Rb_, Vr_, Sv_, Wk_, TU_, TV_;
   DevBuf<double> tvec_;
   DevBuf<int> info_;
+  kern::MvPlan mv_;
+  DevBuf<kern::MvLeaf> mv_leaves_;
+  DevBuf<kern::MvLevel> mv_lvls_;
+  DevBuf<kern::MvEntry> mv_ent_;
+  DevBuf<kern::MvSeg> mv_segs_;
+  DevBuf<double> mv_part_, mv_t_;
+  double mv_bytes_base_ = 0.0;  // D + (U S) + V bytes of one pass
 };
 
 }  // namespace hmat

@@ -70,6 +70,88 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_nn(const SegNN* __restrict__ s
   }
 }
 
+// MFMA variant for wide segments (p >= 32 and q >= 16, e.g. the leaf
+// blocks D_i (len x len) applied to many columns): 64 x 64 output tile per
+// workgroup, each wave a 32 x 32 quarter as 2 x 2 MFMA 16x16 tiles; K-tiles
+// of 16 staged in LDS.  FP64: v_mfma_f64_16x16x4f64, FP32:
+// v_mfma_f32_16x16x4f32.
+template <class F>
+struct MfmaNN;
+template <>
+struct MfmaNN<double> {
+  typedef mfma::v4d acc_t;
+  static __device__ acc_t step(double a, double b, acc_t c) { return mfma::f64_16x16x4(a, b, c); }
+  static __device__ int row(int lane, int v) { return mfma::f64_row(lane, v); }
+};
+template <>
+struct MfmaNN<float> {
+  typedef mfma::v4f acc_t;
+  static __device__ acc_t step(float a, float b, acc_t c) { return mfma::f32_16x16x4(a, b, c); }
+  static __device__ int row(int lane, int v) { return mfma::f32_row(lane, v); }
+};
+
+template <class F>
+__global__ __launch_bounds__(kWG) void k_seg_gemm_nn_mfma(const SegNN* __restrict__ segs,
+                                                          const NNTile* __restrict__ tiles,
+                                                          const F* __restrict__ X, int64_t ldx,
+                                                          const F* __restrict__ T, int64_t ldt,
+                                                          F* __restrict__ Y, int64_t ldy, F alpha,
+                                                          F beta) {
+  constexpr int KT = 16, TM = 64, TN = 64;
+  __shared__ F xs[KT][TM + 1];  // xs[k][r] = X(row0 + r, k0 + k)
+  __shared__ F ts[KT][TN + 1];  // ts[k][c] = T(k0 + k, c0 + c)
+  const NNTile tl = tiles[blockIdx.x];
+  const SegNN sg = segs[tl.seg];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int wm = w & 1, wn = w >> 1;
+  typedef typename MfmaNN<F>::acc_t acc_t;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{F(0), F(0), F(0), F(0)};
+  for (int k0 = 0; k0 < sg.p; k0 += KT) {
+    for (int e = threadIdx.x; e < KT * TM; e += kWG) {
+      const int r = e % TM, k = e / TM;
+      xs[k][r] = (tl.row0 + r < sg.len && k0 + k < sg.p) ? X[sg.xoff + tl.row0 + r + (int64_t)(k0 + k) * ldx]
+                                                         : F(0);
+    }
+    for (int e = threadIdx.x; e < KT * TN; e += kWG) {
+      const int k = e % KT, c = e / KT;
+      ts[k][c] = (k0 + k < sg.p && tl.c0 + c < sg.q) ? T[sg.toff + k0 + k + (int64_t)(tl.c0 + c) * ldt] : F(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KT; kk += 4) {
+      F a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = xs[kk + lane / 16][wm * 32 + i * 16 + lane % 16];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[j] = ts[kk + lane / 16][wn * 32 + j * 16 + lane % 16];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = MfmaNN<F>::step(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = tl.row0 + wm * 32 + i * 16 + MfmaNN<F>::row(lane, v);
+        const int c = tl.c0 + wn * 32 + j * 16 + (lane & 15);
+        if (r < sg.len && c < sg.q) {
+          F* yp = Y + sg.yoff + r + (int64_t)c * ldy;
+          F val = alpha * acc[i][j][v];
+          if (beta != F(0)) val = fma(beta, *yp, val);
+          *yp = val;
+        }
+      }
+}
+
 template <class F>
 void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* X, int64_t ldx,
                  const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha, F beta, hipStream_t s) {
@@ -77,6 +159,26 @@ void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* 
   int qmax = 0;
   for (auto& g : segs) qmax = std::max(qmax, g.q);
   if (qmax == 0) return;
+  int pmin = 1 << 30;
+  for (auto& g : segs)
+    if (g.len > 0 && g.q > 0) pmin = std::min(pmin, g.p);
+  if (pmin >= 32 && qmax >= 16) {
+    std::vector<NNTile> tiles;
+    for (size_t i = 0; i < segs.size(); ++i) {
+      const auto& g = segs[i];
+      if (g.len <= 0 || g.q <= 0) continue;
+      for (int r0 = 0; r0 < g.len; r0 += 64)
+        for (int c0 = 0; c0 < g.q; c0 += 64) tiles.push_back({(int32_t)i, r0, c0, 0});
+    }
+    dsegs.upload(segs, s);
+    DevBuf<NNTile> dt;
+    dt.upload(tiles, s);
+    k_seg_gemm_nn_mfma<F><<<(unsigned)tiles.size(), kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy,
+                                                                  alpha, beta);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+    return;
+  }
   const int R = qmax <= 16 ? 256 : (qmax <= 32 ? 128 : 64);
   const int cw = (kWG / R) * 16;
   std::vector<NNTile> tiles;

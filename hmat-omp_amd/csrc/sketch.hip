@@ -24,6 +24,7 @@ namespace hmat {
 namespace kern {
 
 constexpr int kWGs = 256;
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
 struct SkTile {
   int32_t seg, row0, rb, re;
@@ -136,15 +137,19 @@ __global__ void k_split_bf16(const double* __restrict__ X, int64_t n, __bf16* __
   }
 }
 
+// Exact split of an FP32 value into SPLIT BF16 terms by truncation:
+// hi = top 8 significant bits, mid = top 8 bits of v - hi, lo = the rest,
+// each term's bits being the upper half of an FP32 word.  With SPLIT = 3
+// hi + mid + lo = v exactly (8 + 8 + 8 = 24 bits); no conversions needed.
 template <int SPLIT>
-__device__ inline void split_bf16(float v, __bf16 (&o)[3]) {
-  const __bf16 h = (__bf16)v;
+__device__ inline void split_bf16_bits(float v, uint32_t (&o)[3]) {
+  const uint32_t h = __float_as_uint(v) & 0xFFFF0000u;
   o[0] = h;
   if constexpr (SPLIT >= 2) {
-    const float r1 = v - (float)h;
-    const __bf16 m = (__bf16)r1;
+    const float r1 = v - __uint_as_float(h);
+    const uint32_t m = __float_as_uint(r1) & 0xFFFF0000u;
     o[1] = m;
-    if constexpr (SPLIT >= 3) o[2] = (__bf16)(r1 - (float)m);
+    if constexpr (SPLIT >= 3) o[2] = __float_as_uint(r1 - __uint_as_float(m));
   }
 }
 
@@ -223,19 +228,28 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
 #pragma unroll
         for (int jj = 0; jj < 8; ++jj) msk[jj] = rm[k0 + jj];
       }
+      // generated operand: two entries per 32-bit word (upper halves of
+      // their split terms), word jp of plane sp holds entries 2jp, 2jp+1
       mfma::v8bf a[RF][SPLIT];
 #pragma unroll
       for (int f = 0; f < RF; ++f) {
+        v4u w[SPLIT];
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) {
-          const int64_t jg = sg.red0 + min(kb + k0 + jj, tl.re - 1);
-          float val = TRANS ? gen.eval(pts[jj], opt[f], jg, iglob[f]) : gen.eval(opt[f], pts[jj], iglob[f], jg);
-          if (!full) val *= msk[jj] * rmask[f];
-          __bf16 o[3];
-          split_bf16<SPLIT>(val, o);
+        for (int jp = 0; jp < 4; ++jp) {
+          uint32_t o[2][3];
 #pragma unroll
-          for (int sp = 0; sp < SPLIT; ++sp) a[f][sp][jj] = o[sp];
+          for (int h = 0; h < 2; ++h) {
+            const int jj = 2 * jp + h;
+            const int64_t jg = sg.red0 + min(kb + k0 + jj, tl.re - 1);
+            float val = TRANS ? gen.eval(pts[jj], opt[f], jg, iglob[f]) : gen.eval(opt[f], pts[jj], iglob[f], jg);
+            if (!full) val *= msk[jj] * rmask[f];
+            split_bf16_bits<SPLIT>(val, o[h]);
+          }
+#pragma unroll
+          for (int sp = 0; sp < SPLIT; ++sp) w[sp][jp] = __builtin_amdgcn_perm(o[1][sp], o[0][sp], 0x07060302u);
         }
+#pragma unroll
+        for (int sp = 0; sp < SPLIT; ++sp) a[f][sp] = __builtin_bit_cast(mfma::v8bf, w[sp]);
       }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
