@@ -3,8 +3,9 @@
     python hmat-omp_amd/build.py [--jobs N]
 
 Each HIP source is compiled to an object in hmat-omp_amd/_build/, then the
-shared library is linked into hmat-omp_amd/lib/.  Packed FP32 instructions
-are disabled for every device compile.
+shared library is linked into hmat-omp_amd/lib/ and the C++ drop-in example
+(examples/drop_in_driver.cpp) into hmat-omp_amd/bin/.  Packed FP32
+instructions are disabled for every device compile.
 """
 import argparse
 import concurrent.futures as cf
@@ -17,6 +18,9 @@ SRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libhmat_amd.so")
+BIN_DIR = os.path.join(HERE, "bin")
+DRIVER_SRC = os.path.join(HERE, "examples", "drop_in_driver.cpp")
+DRIVER = os.path.join(BIN_DIR, "drop_in_driver")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -56,6 +60,13 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    # the C++ drop-in example links the library through include/hmat_amd.hpp
+    os.makedirs(BIN_DIR, exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"), DRIVER_SRC,
+           "-L", LIB_DIR, "-lhmat_amd", "-Wl,-rpath,$ORIGIN/../lib", "-o", DRIVER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("drop-in driver build failed:\n" + r.stderr)
     return LIB
 
 
