@@ -82,10 +82,13 @@ def run_mode(hmat_amd, torch, mode, a):
             resid.append(r)
     hmat_amd.profile_enable(False)
     prof = {}
-    for name in ("sketch_gen", "sketch_lp"):
+    for name in ("sketch_gen", "sketch_lp", "jacobi_sweeps", "jacobi_mats"):
         q = hmat_amd.profile_query(name)
         if q:
             prof[name] = {"launches": q[0], "ms": q[1], "work": q[2]}
+    sweeps = prof.pop("jacobi_sweeps", None)
+    mats = prof.pop("jacobi_mats", None)
+    jac = round(sweeps["work"] / mats["work"], 2) if sweeps and mats and mats["work"] else None
     ranks = [max(h.level_ranks(d)) for d in range(1, h.levels() + 1)]
     # matvec y = A_H x of the built operator (FP64 generators), 1 RHS
     y = torch.empty(n, dtype=torch.float64, device="cuda")
@@ -100,7 +103,8 @@ def run_mode(hmat_amd, torch, mode, a):
     q = hmat_amd.profile_query("matvec")
     mv = {"launches": q[0], "ms": q[1], "work": q[2]} if q else None
     return {"times": times, "parts": parts, "iters": iters, "resid": resid, "prof": prof,
-            "x": x.clone(), "b": b, "dx": dx, "dy": dy, "ranks": ranks, "h": h, "matvec": mv}
+            "x": x.clone(), "b": b, "dx": dx, "dy": dy, "ranks": ranks, "h": h, "matvec": mv,
+            "jacobi_avg_sweeps": jac}
 
 
 def true_residual(hmat_amd, torch, res, n):
@@ -200,6 +204,7 @@ def main():
         out[f"{m}_resid_AH"] = res[m]["resid"][-1]
         out[f"{m}_resid_trueA"] = true_residual(hmat_amd, torch, res[m], a.n)
         out[f"{m}_max_rank_per_level"] = res[m]["ranks"]
+        out[f"{m}_jacobi_avg_sweeps"] = res[m]["jacobi_avg_sweeps"]
     if "fp64" in res and "mixed" in res:
         out["speedup_mixed_vs_fp64"] = round(med["fp64"] / med["mixed"], 3)
         xf, xm = res["fp64"]["x"], res["mixed"]["x"]

@@ -583,7 +583,8 @@ void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* 
 __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__ mats,
                                                     double* __restrict__ A, double* __restrict__ V,
                                                     double* __restrict__ S, int64_t sstride,
-                                                    double* __restrict__ work, int use_lds) {
+                                                    double* __restrict__ work, int use_lds,
+                                                    int* __restrict__ sweeps_out) {
   extern __shared__ double sm[];
   __shared__ int rotated;
   __shared__ double sig[512];
@@ -613,7 +614,8 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
   __syncthreads();
   const int np = n + (n & 1);  // even player count; column n is a phantom
   const int circ = np - 1;
-  for (int sweep = 0; sweep < 60 && n > 1; ++sweep) {
+  int sweep = 0;
+  for (; sweep < 60 && n > 1; ++sweep) {
     if (threadIdx.x == 0) rotated = 0;
     __syncthreads();
     for (int step = 0; step < circ; ++step) {
@@ -663,6 +665,7 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
     if (!rotated) break;
     __syncthreads();
   }
+  if (sweeps_out && threadIdx.x == 0) sweeps_out[blockIdx.x] = sweep;
   // singular values = column norms
   for (int j = w; j < n; j += kWG / 64) {
     double s2 = 0.0;
@@ -719,9 +722,21 @@ void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double*
   const size_t lds = (size_t)2 * nmax * (nmax + 1) * sizeof(double);
   const int use_lds = lds <= 128 * 1024 ? 1 : 0;
   dm.upload(mats, s);
+  // with profiling on, the sweep counts are recorded as "jacobi_sweeps"
+  // (work = total sweeps, launches = calls) and "jacobi_mats"
+  DevBuf<int> sw;
+  const bool count = prof_enabled();
+  if (count) sw.resize(mats.size());
   k_jacobi_svd<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), A, V, S, sstride, work,
-                                                                     use_lds);
+                                                                     use_lds, count ? sw.get() : nullptr);
   HIP_CHECK(hipGetLastError());
+  if (count) {
+    auto h = download(sw.get(), mats.size(), s);
+    double tot = 0.0;
+    for (int v : h) tot += v;
+    prof_add("jacobi_sweeps", 0.0, tot);
+    prof_add("jacobi_mats", 0.0, (double)mats.size());
+  }
 }
 
 // Gauss-Jordan inversion with partial pivoting, in place.

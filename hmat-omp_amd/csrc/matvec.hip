@@ -27,12 +27,16 @@ namespace kern {
 
 constexpr int kMvWG = 256;
 
-// Lane l ends with the sum over the wave of v[l] (64 values per lane in,
-// the lane's own total out): six halving steps of 32 + 16 + ... + 1
-// shuffles.  Template recursion keeps every index static, so v stays in
-// registers.
+// Global-address-space views: loads through them are global_load (not
+// flat_load, which also counts against the LDS wait counter).
+typedef const __attribute__((address_space(1))) double gdouble;
+
+// 16 values per lane in; lane l ends with the wave total of v[l & 15]:
+// four halving steps inside each group of 16 lanes (8 + 4 + 2 + 1
+// shuffles), then the four groups are added (2 shuffles).  Template
+// recursion keeps the indices static, so v stays in registers.
 template <int S>
-__device__ __forceinline__ void halving_step(double (&v)[64], int lane) {
+__device__ __forceinline__ void halving_step(double (&v)[16], int lane) {
   if constexpr (S >= 1) {
     const bool upper = (lane & S) != 0;
 #pragma unroll
@@ -44,56 +48,68 @@ __device__ __forceinline__ void halving_step(double (&v)[64], int lane) {
     halving_step<S / 2>(v, lane);
   }
 }
-__device__ inline double wave_reduce_scatter64(double (&v)[64]) {
-  halving_step<32>(v, threadIdx.x & 63);
-  return v[0];
+__device__ inline double wave_reduce_scatter16(double (&v)[16]) {
+  const int lane = threadIdx.x & 63;
+  halving_step<8>(v, lane);
+  double s = v[0];
+  s += __shfl_xor(s, 16);
+  s += __shfl_xor(s, 32);
+  return s;
 }
 
-// partial[((leaf * L + c) * KM + k) * nrhs + col]
+// partial[((leaf * L + c) * KM + k) * nrhs + col].  Per depth, groups of
+// KG = 16 / NR columns of V: every lane loads its row of the group
+// (unconditionally, the column index clamped), the wave reduces, and the
+// four waves meet in LDS once per depth.
 template <int NR>
 __global__ __launch_bounds__(kMvWG) void k_mv_project(const MvLeaf* __restrict__ leaves,
                                                       const MvLevel* __restrict__ lvls,
                                                       const MvEntry* __restrict__ ent, int L, int KM,
                                                       const double* __restrict__ X, int64_t ldx,
                                                       double* __restrict__ part) {
-  constexpr int KG = 64 / NR;  // columns of V per reduction group
-  __shared__ double red[kMvWG / 64][64];
+  constexpr int KG = 16 / NR;
+  // [4][KMP * NR]: KMP = KM rounded up to 16, so the last group's 16 lanes
+  // stay inside their wave's section
+  extern __shared__ double red[];
+  const int KMP = (KM + 15) / 16 * 16;
   const MvLeaf lf = leaves[blockIdx.x];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  gdouble* Xg = (gdouble*)X;
   for (int c = 0; c < L; ++c) {
     const MvEntry e = ent[(int64_t)blockIdx.x * L + c];
     if (e.vcols <= 0) continue;
-    const double* V = lvls[c].V;
+    gdouble* V = (gdouble*)lvls[c].V;
     const int64_t N = lvls[c].ld;
     for (int kb = 0; kb < e.vcols; kb += KG) {
-      double v[64];
+      double v[16];
 #pragma unroll
-      for (int i = 0; i < 64; ++i) v[i] = 0.0;
+      for (int i = 0; i < 16; ++i) v[i] = 0.0;
       for (int r = threadIdx.x; r < lf.len; r += kMvWG) {
         const int64_t row = lf.start + r;
         double xr[NR];
 #pragma unroll
-        for (int q = 0; q < NR; ++q) xr[q] = X[row + (int64_t)q * ldx];
+        for (int q = 0; q < NR; ++q) xr[q] = Xg[row + (int64_t)q * ldx];
+        double a[KG];
+#pragma unroll
+        for (int kk = 0; kk < KG; ++kk) a[kk] = V[row + (int64_t)min(kb + kk, e.vcols - 1) * N];
 #pragma unroll
         for (int kk = 0; kk < KG; ++kk) {
-          const int k = kb + kk;
-          const double a = k < e.vcols ? V[row + (int64_t)k * N] : 0.0;
+          const double ak = kb + kk < e.vcols ? a[kk] : 0.0;
 #pragma unroll
-          for (int q = 0; q < NR; ++q) v[kk * NR + q] = fma(a, xr[q], v[kk * NR + q]);
+          for (int q = 0; q < NR; ++q) v[kk * NR + q] = fma(ak, xr[q], v[kk * NR + q]);
         }
       }
-      const double s = wave_reduce_scatter64(v);  // lane = kk * NR + q
-      red[w][lane] = s;
-      __syncthreads();
-      if (threadIdx.x < 64) {
-        double t = 0.0;
-#pragma unroll
-        for (int i = 0; i < kMvWG / 64; ++i) t += red[i][threadIdx.x];
-        const int kk = threadIdx.x / NR, q = threadIdx.x % NR, k = kb + kk;
-        if (k < e.vcols) part[(((int64_t)blockIdx.x * L + c) * KM + k) * NR + q] = t;
-      }
-      __syncthreads();
+      const double s = wave_reduce_scatter16(v);  // lane & 15 = kk * NR + q
+      if (lane < 16) red[w * KMP * NR + kb * NR + lane] = s;
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < e.vcols * NR; i += kMvWG) {
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < kMvWG / 64; ++j) t += red[j * KMP * NR + i];
+      part[((int64_t)blockIdx.x * L + c) * KM * NR + i] = t;
+    }
+    __syncthreads();
   }
 }
 
@@ -147,7 +163,7 @@ __global__ __launch_bounds__(kMvWG) void k_mv_expand(const MvLeaf* __restrict__ 
       ts[(int64_t)c * KM * NR + e] = t[(int64_t)en.tin * KM * NR + e];
   }
   __syncthreads();
-  const double* Dl = Dm + (int64_t)blockIdx.x * lm * lm;
+  gdouble* Dl = (gdouble*)(Dm + (int64_t)blockIdx.x * lm * lm);
   for (int r = threadIdx.x; r < lf.len; r += kMvWG) {
     double acc[NR];
 #pragma unroll
@@ -170,7 +186,7 @@ __global__ __launch_bounds__(kMvWG) void k_mv_expand(const MvLeaf* __restrict__ 
     const int64_t row = lf.start + r;
     for (int c = 0; c < L; ++c) {
       const MvEntry en = ent[(int64_t)blockIdx.x * L + c];
-      const double* U = lvls[c].US;
+      gdouble* U = (gdouble*)lvls[c].US;
       const int64_t N = lvls[c].ld;
       const double* tc = ts + (int64_t)c * KM * NR;
       int k = 0;
@@ -199,7 +215,8 @@ static void launch_mv(const MvPlan& p, const double* Dm, int64_t lm, const doubl
                       double* part, double* t, double* Y, int64_t ldy, hipStream_t s) {
   const unsigned nl = (unsigned)p.nleaf;
   if (p.L > 0) {
-    k_mv_project<NR><<<nl, kMvWG, 0, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
+    const size_t plds = (size_t)(kMvWG / 64) * ((p.KM + 15) / 16 * 16) * NR * sizeof(double);
+    k_mv_project<NR><<<nl, kMvWG, plds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
     HIP_CHECK(hipGetLastError());
     k_mv_reduce<NR><<<(unsigned)p.nseg, kMvWG, 0, s>>>(p.segs, p.L, p.KM, part, t);
     HIP_CHECK(hipGetLastError());

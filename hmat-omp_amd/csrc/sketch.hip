@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "generators.hpp"
 #include "kernels.hpp"
@@ -71,20 +72,48 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
     rmask[f] = rloc < sg.outlen ? 1.0 : 0.0;
   }
   const int kq = lane / 16;
-  for (int kb = tl.rb; kb < tl.re; kb += KT) {
-    for (int e = threadIdx.x; e < KT * NC; e += kWGs) {
+  // Software pipeline: the next K-tile of X and its points are loaded into
+  // registers while the current tile is computed from LDS.
+  constexpr int NPRE = KT * NC / kWGs;
+  static_assert(KT * NC % kWGs == 0, "prefetch layout");
+  double pre[NPRE];
+  double ppt = 0.0, pok = 0.0;
+  auto issue = [&](int kb) {
+#pragma unroll
+    for (int i = 0; i < NPRE; ++i) {
+      const int e = threadIdx.x + i * kWGs;
       const int kk = e % KT, cc = e / KT, j = kb + kk;
-      xs[kk][cc] = (j < tl.re && cc < kcols) ? X[sg.xoff + j + (int64_t)(cbase + cc) * ldx] : 0.0;
+      pre[i] = (j < tl.re && cc < kcols) ? X[sg.xoff + j + (int64_t)(cbase + cc) * ldx] : 0.0;
     }
     if (threadIdx.x < KT) {
       const int j = kb + threadIdx.x;
       const bool inr = j < tl.re;
       const int64_t jg = sg.red0 + (inr ? j : tl.re - 1);
       const bool ok = inr && !((uint64_t)(jg - sg.excl0) < (uint64_t)sg.excllen);
-      rp[threadIdx.x] = TRANS ? gen.row_pt(jg) : gen.col_pt(jg);
-      rm[threadIdx.x] = ok ? 1.0 : 0.0;
+      ppt = TRANS ? gen.row_pt(jg) : gen.col_pt(jg);
+      pok = ok ? 1.0 : 0.0;
     }
-    __syncthreads();
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int i = 0; i < NPRE; ++i) {
+      const int e = threadIdx.x + i * kWGs;
+      xs[e % KT][e / KT] = pre[i];
+    }
+    if (threadIdx.x < KT) {
+      rp[threadIdx.x] = ppt;
+      rm[threadIdx.x] = pok;
+    }
+  };
+  // Dense input reads A from memory in the inner loop already; there the
+  // next tile is fetched after the compute, keeping its registers free.
+  constexpr bool PF = !std::is_same<Gen, GDense>::value;
+  issue(tl.rb);
+  commit();
+  __syncthreads();
+  for (int kb = tl.rb; kb < tl.re; kb += KT) {
+    const bool has_next = kb + KT < tl.re;
+    if (PF && has_next) issue(kb + KT);
     const bool full = rows_full && ktile_full(sg, kb, tl.re);
 #pragma unroll 4
     for (int ks = 0; ks < KT / 4; ++ks) {
@@ -106,6 +135,11 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
       }
     }
     __syncthreads();
+    if (has_next) {
+      if (!PF) issue(kb + KT);
+      commit();
+      __syncthreads();
+    }
   }
   double* ybase = Y + (int64_t)tl.split * split_stride + sg.yoff;
 #pragma unroll
@@ -190,16 +224,42 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
   }
   const int kq = lane / 16;
   int nt = 0;
-  for (int kb = tl.rb; kb < tl.re; kb += KT) {
-    const bool kfull = ktile_full(sg, kb, tl.re);
-    if (aligned && kfull) {
-      // 16-byte copies: 8 reduction rows of one column of one plane
-      for (int e = threadIdx.x; e < SPLIT * NC * (KT / 8); e += kWGs) {
+  // Software pipeline: the next K-tile's BF16 planes and points are loaded
+  // into registers while the current tile is computed from LDS.  Tiles that
+  // are partial or unaligned are loaded element-wise at commit time.
+  constexpr int NPRE = SPLIT * NC * (KT / 8) / kWGs;
+  static_assert(SPLIT * NC * (KT / 8) % kWGs == 0, "prefetch layout");
+  uint4 pre[NPRE];
+  float ppt = 0.f, pok = 0.f;
+  auto issue = [&](int kb) -> bool {
+    const bool vec = aligned && ktile_full(sg, kb, tl.re);
+    if (vec) {
+#pragma unroll
+      for (int i = 0; i < NPRE; ++i) {
+        const int e = threadIdx.x + i * kWGs;
         const int ch = e % (KT / 8), cc = (e / (KT / 8)) % NC, sp = e / ((KT / 8) * NC);
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (cc < kcols)
-          v = *reinterpret_cast<const uint4*>(Xs + sp * ps + sg.xoff + kb + 8 * ch + (int64_t)(cbase + cc) * ldx);
-        *reinterpret_cast<uint4*>(&xs[sp][cc][8 * ch]) = v;
+        pre[i] = cc < kcols ? *reinterpret_cast<const uint4*>(Xs + sp * ps + sg.xoff + kb + 8 * ch +
+                                                              (int64_t)(cbase + cc) * ldx)
+                            : make_uint4(0, 0, 0, 0);
+      }
+    }
+    if (threadIdx.x < KT) {
+      const int j = kb + threadIdx.x;
+      const bool inr = j < tl.re;
+      const int64_t jg = sg.red0 + (inr ? j : tl.re - 1);
+      const bool ok = inr && !((uint64_t)(jg - sg.excl0) < (uint64_t)sg.excllen);
+      ppt = TRANS ? gen.row_pt(jg) : gen.col_pt(jg);
+      pok = ok ? 1.f : 0.f;
+    }
+    return vec;
+  };
+  auto commit = [&](int kb, bool vec) {
+    if (vec) {
+#pragma unroll
+      for (int i = 0; i < NPRE; ++i) {
+        const int e = threadIdx.x + i * kWGs;
+        const int ch = e % (KT / 8), cc = (e / (KT / 8)) % NC, sp = e / ((KT / 8) * NC);
+        *reinterpret_cast<uint4*>(&xs[sp][cc][8 * ch]) = pre[i];
       }
     } else {
       for (int e = threadIdx.x; e < SPLIT * NC * KT; e += kWGs) {
@@ -209,14 +269,16 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
       }
     }
     if (threadIdx.x < KT) {
-      const int j = kb + threadIdx.x;
-      const bool inr = j < tl.re;
-      const int64_t jg = sg.red0 + (inr ? j : tl.re - 1);
-      const bool ok = inr && !((uint64_t)(jg - sg.excl0) < (uint64_t)sg.excllen);
-      rp[threadIdx.x] = TRANS ? gen.row_pt(jg) : gen.col_pt(jg);
-      rm[threadIdx.x] = ok ? 1.f : 0.f;
+      rp[threadIdx.x] = ppt;
+      rm[threadIdx.x] = pok;
     }
-    __syncthreads();
+  };
+  commit(tl.rb, issue(tl.rb));
+  __syncthreads();
+  for (int kb = tl.rb; kb < tl.re; kb += KT) {
+    const bool kfull = ktile_full(sg, kb, tl.re);
+    const bool has_next = kb + KT < tl.re;
+    const bool vec_next = has_next ? issue(kb + KT) : false;
     const bool full = rows_full && kfull;
 #pragma unroll
     for (int ks = 0; ks < KT / 32; ++ks) {
@@ -266,6 +328,10 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
       }
     }
     __syncthreads();
+    if (has_next) {
+      commit(kb + KT, vec_next);
+      __syncthreads();
+    }
     if (++nt == FLUSH) {
       nt = 0;
 #pragma unroll
