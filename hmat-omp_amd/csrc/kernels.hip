@@ -561,6 +561,65 @@ __global__ __launch_bounds__(kWG) void k_chol(const SmallMat* __restrict__ mats,
     }
 }
 
+// n <= 64: one wave per matrix, lane c owns column c of R (in LDS); the
+// column loop needs no cross-wave barriers.  Same outputs and breakdown
+// rules as k_chol.
+__global__ __launch_bounds__(64) void k_chol64(const SmallMat* __restrict__ mats,
+                                               const double* __restrict__ G, double* __restrict__ R,
+                                               const double* __restrict__ shift_coef,
+                                               int* __restrict__ info, double* __restrict__ gerr) {
+  extern __shared__ double r[];  // n x (n+1), column c at r[c * (n+1)]
+  const SmallMat m = mats[blockIdx.x];
+  const int n = m.n, ld = m.ld, lr = n + 1;
+  const double* g = G + m.off;
+  double* rg = R + m.off;
+  const int c = threadIdx.x;
+  double tr = 0.0, dev = 0.0;
+  if (c < n) {
+    tr = g[c + (int64_t)c * ld];
+    for (int i = 0; i < n; ++i) dev = fmax(dev, fabs(g[i + (int64_t)c * ld] - (i == c ? 1.0 : 0.0)));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    tr += __shfl_xor(tr, o);
+    dev = fmax(dev, __shfl_xor(dev, o));
+  }
+  if (c == 0) {
+    gerr[blockIdx.x] = (tr == 0.0) ? 0.0 : dev;
+    info[blockIdx.x] = 0;
+  }
+  const double shift = shift_coef[blockIdx.x] * tr;
+  if (c < n)
+    for (int i = 0; i < n; ++i) r[i + c * lr] = (i <= c) ? g[i + (int64_t)c * ld] + (i == c ? shift : 0.0) : 0.0;
+  if (!(tr > 0.0)) {
+    if (!(tr == 0.0) && c == 0) info[blockIdx.x] = -1;  // NaN
+    if (c < n)
+      for (int i = 0; i < n; ++i) rg[i + (int64_t)c * ld] = 0.0;
+    return;
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    const double d = r[j + j * lr];
+    if (!(d > 0.0) || !isfinite(d)) {
+      if (c == 0) info[blockIdx.x] = j + 1;
+      break;
+    }
+    const double rjj = sqrt(d);
+    double rjc = 0.0;
+    if (c > j && c < n) {
+      rjc = r[j + c * lr] / rjj;
+      r[j + c * lr] = rjc;
+    }
+    if (c == j) r[j + j * lr] = rjj;
+    __syncthreads();
+    // column c: r(a, c) -= r(j, a) r(j, c) for j < a <= c
+    if (c > j && c < n)
+      for (int a2 = j + 1; a2 <= c; ++a2) r[a2 + c * lr] -= r[j + a2 * lr] * rjc;
+    __syncthreads();
+  }
+  if (c < n)
+    for (int i = 0; i < n; ++i) rg[i + (int64_t)c * ld] = r[i + c * lr];
+}
+
 void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* G, double* R,
                double* Rinv, const std::vector<double>& shift_coef, DevBuf<double>& dshift,
                int* info, double* gerr, hipStream_t s) {
@@ -572,8 +631,11 @@ void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* 
   const int use_lds = lds <= 64 * 1024 ? 1 : 0;
   dm.upload(mats, s);
   dshift.upload(shift_coef, s);
-  k_chol<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), G, R, dshift.get(), info, gerr,
-                                                                use_lds);
+  if (nmax <= 64)
+    k_chol64<<<(unsigned)mats.size(), 64, lds, s>>>(dm.get(), G, R, dshift.get(), info, gerr);
+  else
+    k_chol<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), G, R, dshift.get(), info, gerr,
+                                                                  use_lds);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -712,6 +774,122 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
   for (int j = threadIdx.x; j < n; j += kWG) sv[j] = sig[perm[j]];
 }
 
+// n <= 62: 1024 threads, one column pair per half-wave (two rows per lane),
+// so the n/2 disjoint pairs of a round-robin step rotate at the same time.
+// A and V live in LDS (dynamic, 2 n (n+1) doubles, under 64 KB).  Same
+// rotation and stopping rule as k_jacobi_svd.
+constexpr int kJWG = 1024;
+__global__ __launch_bounds__(kJWG) void k_jacobi_svd64(const SmallMat* __restrict__ mats,
+                                                       double* __restrict__ A, double* __restrict__ V,
+                                                       double* __restrict__ S, int64_t sstride,
+                                                       int* __restrict__ sweeps_out) {
+  extern __shared__ double sm[];
+  __shared__ double sig[64];
+  __shared__ double red[kJWG / 64];
+  __shared__ int perm[64];
+  __shared__ int rotated;
+  const SmallMat m = mats[blockIdx.x];
+  const int n = m.n, ld = m.ld;
+  double* ag = A + m.off;
+  double* vg = V + m.off;
+  double* sv = S + (int64_t)blockIdx.x * sstride;
+  const int LA = n + 1;
+  double* a = sm;
+  double* v = sm + n * LA;
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int hl = lane & 31, half = lane >> 5;
+  double fro = 0.0;
+  for (int e = threadIdx.x; e < n * n; e += kJWG) {
+    const int i = e % n, j = e / n;
+    const double x = ag[i + (int64_t)j * ld];
+    fro += x * x;
+    a[i + j * LA] = x;
+    v[i + j * LA] = (i == j) ? 1.0 : 0.0;
+  }
+  for (int o = 32; o > 0; o >>= 1) fro += __shfl_xor(fro, o);
+  if (lane == 0) red[w] = fro;
+  __syncthreads();
+  fro = 0.0;
+  for (int i = 0; i < kJWG / 64; ++i) fro += red[i];
+  const double floor2 = 1e-30 * fro;
+  const int np = n + (n & 1);
+  const int circ = np - 1;
+  int sweep = 0;
+  for (; sweep < 60 && n > 1; ++sweep) {
+    if (threadIdx.x == 0) rotated = 0;
+    __syncthreads();
+    for (int step = 0; step < circ; ++step) {
+      for (int pi = w * 2 + half; pi < np / 2; pi += kJWG / 32) {
+        int p, q;
+        if (pi == 0) {
+          p = 0;
+          q = 1 + step % circ;
+        } else {
+          p = 1 + (step + pi) % circ;
+          q = 1 + (step - pi + circ) % circ;
+        }
+        if (p >= n || q >= n) continue;
+        if (p > q) { const int t = p; p = q; q = t; }
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int i = hl; i < n; i += 32) {
+          const double x = a[i + p * LA], y = a[i + q * LA];
+          al += x * x;
+          be += y * y;
+          ga += x * y;
+        }
+        for (int o = 16; o > 0; o >>= 1) {
+          al += __shfl_xor(al, o);
+          be += __shfl_xor(be, o);
+          ga += __shfl_xor(ga, o);
+        }
+        if (!(fabs(ga) > 1e-15 * sqrt(al * be)) || !(fabs(ga) > floor2)) continue;
+        const double zeta = (be - al) / (2.0 * ga);
+        double t;
+        if (fabs(zeta) > 1e150)
+          t = 0.5 / zeta;
+        else
+          t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+        for (int i = hl; i < n; i += 32) {
+          const double x = a[i + p * LA], y = a[i + q * LA];
+          a[i + p * LA] = c * x - sn * y;
+          a[i + q * LA] = sn * x + c * y;
+          const double vx = v[i + p * LA], vy = v[i + q * LA];
+          v[i + p * LA] = c * vx - sn * vy;
+          v[i + q * LA] = sn * vx + c * vy;
+        }
+        if (hl == 0) rotated = 1;
+      }
+      __syncthreads();
+    }
+    if (!rotated) break;
+    __syncthreads();
+  }
+  if (sweeps_out && threadIdx.x == 0) sweeps_out[blockIdx.x] = sweep;
+  for (int j = w * 2 + half; j < n; j += kJWG / 32) {
+    double s2 = 0.0;
+    for (int i = hl; i < n; i += 32) s2 += a[i + j * LA] * a[i + j * LA];
+    for (int o = 16; o > 0; o >>= 1) s2 += __shfl_xor(s2, o);
+    if (hl == 0) sig[j] = sqrt(s2);
+  }
+  __syncthreads();
+  if (threadIdx.x < n) {  // rank of column j in descending order (ties by index)
+    const int j = threadIdx.x;
+    int r = 0;
+    for (int t = 0; t < n; ++t) r += (sig[t] > sig[j] || (sig[t] == sig[j] && t < j)) ? 1 : 0;
+    perm[r] = j;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * n; e += kJWG) {
+    const int i = e % n, j = e / n;
+    const int pj = perm[j];
+    const double sj = sig[pj];
+    ag[i + (int64_t)j * ld] = sj > 0.0 ? a[i + pj * LA] / sj : 0.0;
+    vg[i + (int64_t)j * ld] = v[i + pj * LA];
+  }
+  for (int j = threadIdx.x; j < n; j += kJWG) sv[j] = sig[perm[j]];
+}
+
 void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* A, double* V,
                 double* S, int64_t sstride, double* work, hipStream_t s) {
   if (mats.empty()) return;
@@ -727,8 +905,12 @@ void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double*
   DevBuf<int> sw;
   const bool count = prof_enabled();
   if (count) sw.resize(mats.size());
-  k_jacobi_svd<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), A, V, S, sstride, work,
-                                                                     use_lds, count ? sw.get() : nullptr);
+  if (nmax <= 62)
+    k_jacobi_svd64<<<(unsigned)mats.size(), kJWG, (size_t)2 * nmax * (nmax + 1) * sizeof(double), s>>>(
+        dm.get(), A, V, S, sstride, count ? sw.get() : nullptr);
+  else
+    k_jacobi_svd<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), A, V, S, sstride, work,
+                                                                       use_lds, count ? sw.get() : nullptr);
   HIP_CHECK(hipGetLastError());
   if (count) {
     auto h = download(sw.get(), mats.size(), s);
