@@ -261,6 +261,43 @@ hmat_status hmat_hodlr_last_refinement(hmat_hodlr h, int32_t* iters, double* res
   });
 }
 
+hmat_status hmat_hodlr_solve_transpose(hmat_hodlr h, const double* B, int64_t ldb, int32_t nrhs, double* X,
+                                       int64_t ldx) {
+  return guard([&] {
+    HMAT_REQUIRE(h && B && X && nrhs > 0, HMAT_ERR_ARG, "bad solve_transpose arguments");
+    const int64_t N = h->eng.N;
+    HMAT_REQUIRE(h->eng.built, HMAT_ERR_STATE, "solve before build");
+    HMAT_REQUIRE(ldb >= N && ldx >= N, HMAT_ERR_ARG, "leading dimension below N");
+    upload_cols(h->x, B, ldb, N, nrhs, h->eng.stream);
+    h->y.reserve((size_t)(N * nrhs));
+    h->eng.solve_transpose(h->x.get(), N, nrhs, h->y.get(), N);
+    download_cols(X, ldx, h->y, N, nrhs, h->eng.stream);
+  });
+}
+
+hmat_status hmat_hodlr_solve_transpose_device(hmat_hodlr h, const double* dB, int64_t ldb, int32_t nrhs,
+                                              double* dX, int64_t ldx) {
+  return guard([&] {
+    HMAT_REQUIRE(h && dB && dX && nrhs > 0, HMAT_ERR_ARG, "bad solve_transpose arguments");
+    HMAT_REQUIRE(ldb >= h->eng.N && ldx >= h->eng.N, HMAT_ERR_ARG, "leading dimension below N");
+    h->eng.solve_transpose(dB, ldb, nrhs, dX, ldx);
+  });
+}
+
+hmat_status hmat_hodlr_matvec_transpose(hmat_hodlr h, const double* X, int64_t ldx, int32_t nrhs, double* Y,
+                                        int64_t ldy) {
+  return guard([&] {
+    HMAT_REQUIRE(h && X && Y && nrhs > 0, HMAT_ERR_ARG, "bad matvec_transpose arguments");
+    const int64_t N = h->eng.N;
+    HMAT_REQUIRE(h->eng.built, HMAT_ERR_STATE, "matvec before build");
+    HMAT_REQUIRE(ldx >= N && ldy >= N, HMAT_ERR_ARG, "leading dimension below N");
+    upload_cols(h->x, X, ldx, N, nrhs, h->eng.stream);
+    h->y.reserve((size_t)(N * nrhs));
+    h->eng.matvec_transpose(h->x.get(), N, nrhs, h->y.get(), N);
+    download_cols(Y, ldy, h->y, N, nrhs, h->eng.stream);
+  });
+}
+
 int64_t hmat_hodlr_size(hmat_hodlr h) { return h ? h->eng.N : -1; }
 int32_t hmat_hodlr_levels(hmat_hodlr h) { return h && h->eng.built ? h->eng.tree.max_depth : -1; }
 

@@ -89,6 +89,7 @@ inline GenDesc gen_from_kernel(const hmat_kernel& k, const double* dx, const dou
     case HMAT_KERNEL_REF_CAUCHY: g.kind = GenKind::RefCauchy; break;
     case HMAT_KERNEL_REF_GAUSSIAN: g.kind = GenKind::RefGaussian; break;
     case HMAT_KERNEL_REF_COULOMB: g.kind = GenKind::RefCoulomb; break;
+    case HMAT_KERNEL_GAUSSIAN_DSIGMA: g.kind = GenKind::GaussianDSigma; break;
     default: throw Error(HMAT_ERR_ARG, "unknown kernel type");
   }
   return g;

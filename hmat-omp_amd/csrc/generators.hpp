@@ -74,6 +74,17 @@ struct GRefCoulomb {  // 1/|x_i-x_j|, 0 on the diagonal, + p1    (kernel.h:69-89
   }
 };
 
+struct GGaussDSigma {  // d/dsigma exp(-d^2/(2 sigma^2)) = exp(-d^2 c) d^2 / sigma^3
+  const double* x;
+  double c, is3;
+  __device__ double row_pt(int64_t i) const { return x[i]; }
+  __device__ double col_pt(int64_t j) const { return x[j]; }
+  __device__ double eval(double xi, double xj, int64_t, int64_t) const {
+    const double d2 = (xi - xj) * (xi - xj);
+    return exp(-d2 * c) * d2 * is3;
+  }
+};
+
 template <class F>
 inline void with_gen(const GenDesc& g, F&& f) {
   switch (g.kind) {
@@ -83,6 +94,9 @@ inline void with_gen(const GenDesc& g, F&& f) {
     case GenKind::RefCauchy: f(GRefCauchy{g.x, g.p1}); break;
     case GenKind::RefGaussian: f(GRefGaussian{g.x, 1.0 / (g.p0 * g.p0), g.p1}); break;
     case GenKind::RefCoulomb: f(GRefCoulomb{g.x, g.p1}); break;
+    case GenKind::GaussianDSigma:
+      f(GGaussDSigma{g.x, 1.0 / (2.0 * g.p0 * g.p0), 1.0 / (g.p0 * g.p0 * g.p0)});
+      break;
   }
 }
 
@@ -152,6 +166,7 @@ inline void with_gen_lp(const GenDesc& g, F&& f) {
     case GenKind::RefCauchy: f(LRefCauchy{g.x, (float)g.p1}); break;
     case GenKind::RefGaussian: f(LRefGaussian{g.x, (float)(1.0 / (g.p0 * g.p0)), (float)g.p1}); break;
     case GenKind::RefCoulomb: f(LRefCoulomb{g.x, (float)g.p1}); break;
+    default: throw Error(HMAT_ERR_ARG, "kernel has no low-precision generator");
   }
 }
 

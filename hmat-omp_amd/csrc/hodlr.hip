@@ -62,6 +62,9 @@ void HodlrEngine::build(const GenDesc& g, int64_t n, int leaf_size, const double
   built = factored = false;
   f64_.clear();
   f32_.clear();
+  f64T_.clear();
+  f32T_.clear();
+  factoredT_ = transposed_ = false;
   N = n;
   leaf = leaf_size;
   tree.build(N, leaf);
@@ -633,6 +636,55 @@ void HodlrEngine::to_dense_host(double* A, int64_t lda) {
         }
     }
   }
+}
+
+}  // namespace hmat
+
+namespace hmat {
+
+// Exchange (U S) and V at every depth, take ranks from the sibling,
+// transpose the leaf blocks, and swap in the other factorisation.  Applying
+// it twice restores the original state (sib is an involution).
+void HodlrEngine::flip() {
+  for (int c = 1; c <= tree.max_depth; ++c) {
+    HLevel& L = lv[c];
+    std::swap(L.US, L.V);
+    std::vector<int> r(L.rank.size());
+    for (size_t s = 0; s < r.size(); ++s) r[s] = L.rank[L.sib[s]];
+    L.rank.swap(r);
+  }
+  std::vector<int32_t> lens;
+  for (int id : tree.leaves) lens.push_back((int32_t)tree.nodes[id].len);
+  kern::transpose_blocks(D.get(), tree.lmax, lens, stream);
+  std::swap(f64_, f64T_);
+  std::swap(f32_, f32T_);
+  std::swap(factored, factoredT_);
+  transposed_ = !transposed_;
+  prepare_matvec();
+}
+
+void HodlrEngine::matvec_transpose(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
+  HMAT_REQUIRE(built, HMAT_ERR_STATE, "matvec before build");
+  flip();
+  try {
+    matvec(dX, ldx, nrhs, dY, ldy);
+  } catch (...) {
+    flip();
+    throw;
+  }
+  flip();
+}
+
+void HodlrEngine::solve_transpose(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
+  HMAT_REQUIRE(built, HMAT_ERR_STATE, "solve before build");
+  flip();
+  try {
+    solve(dB, ldb, nrhs, dX, ldx);
+  } catch (...) {
+    flip();
+    throw;
+  }
+  flip();
 }
 
 }  // namespace hmat

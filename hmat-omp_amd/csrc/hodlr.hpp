@@ -74,6 +74,11 @@ class HodlrEngine {
   void factor();
   void solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
   void to_dense_host(double* A, int64_t lda);
+  // A_H^T x and A_H^{-1 T} b (adjoints): the transposed operator is a
+  // HODLR matrix with (U S) and V exchanged, ranks taken from the sibling
+  // and the leaf blocks transposed.  Its factorisation is kept separately.
+  void matvec_transpose(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
+  void solve_transpose(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
 
   static std::vector<std::pair<int64_t, int>> omega_schedule(const hmat_options& o, int64_t N,
                                                              int leaf);
@@ -108,6 +113,10 @@ class HodlrEngine {
   void apply_inverse(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx);
   // fused matvec plan (built once per build) and its launch without a host sync
   void prepare_matvec();
+  void flip();  // switch between A_H and A_H^T in place (involution)
+  bool transposed_ = false, factoredT_ = false;
+  HFactor<double> f64T_;
+  HFactor<float> f32T_;
   void matvec_async(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
   double matvec_bytes(int nrhs) const;
 

@@ -1092,6 +1092,28 @@ __global__ void k_copy_segs(const SegNN* __restrict__ segs, const double* __rest
   }
 }
 
+__global__ void k_transpose_blocks(double* __restrict__ base, int64_t ld, const int32_t* __restrict__ lens) {
+  const int n = lens[blockIdx.x];
+  double* a = base + (int64_t)blockIdx.x * ld * ld;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    if (i < j) {  // each pair once
+      const double t = a[i + (int64_t)j * ld];
+      a[i + (int64_t)j * ld] = a[j + (int64_t)i * ld];
+      a[j + (int64_t)i * ld] = t;
+    }
+  }
+}
+
+void transpose_blocks(double* base, int64_t ld, const std::vector<int32_t>& lens, hipStream_t s) {
+  if (lens.empty()) return;
+  DevBuf<int32_t> dl;
+  dl.upload(lens, s);
+  k_transpose_blocks<<<(unsigned)lens.size(), kWG, 0, s>>>(base, ld, dl.get());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
 void copy_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X, int64_t ldx,
                double* Y, int64_t ldy, hipStream_t s) {
   if (segs.empty()) return;

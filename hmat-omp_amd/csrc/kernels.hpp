@@ -48,7 +48,8 @@ enum class GenKind : int {
   Cauchy = 2,       // 1/(x_i - y_j)
   RefCauchy = 3,    // 1/(x_i + x_j + 1) + p1 delta
   RefGaussian = 4,  // exp(-(x_i-x_j)^2/p0^2) + p1 delta
-  RefCoulomb = 5    // 1/|x_i-x_j|, diag 0, + p1 delta
+  RefCoulomb = 5,   // 1/|x_i-x_j|, diag 0, + p1 delta
+  GaussianDSigma = 6  // d/dsigma of Gaussian: exp(-(x_i-x_j)^2/(2 p0^2)) (x_i-x_j)^2 / p0^3
 };
 
 struct GenDesc {
@@ -138,6 +139,9 @@ void fill_random_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, doub
 
 void gen_blocks(const GenDesc& g, const std::vector<BlockDesc>& blocks, DevBuf<BlockDesc>& db,
                 double* out, hipStream_t s);
+
+// In-place transpose of block i (lens[i] x lens[i] at base + i*ld*ld, ld ld).
+void transpose_blocks(double* base, int64_t ld, const std::vector<int32_t>& lens, hipStream_t s);
 
 // Y(seg rows, 0:q) = X(seg rows, 0:q) (xoff/yoff/len/q of each SegNN)
 void copy_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X, int64_t ldx,

@@ -23,6 +23,7 @@ STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_OOM", 4: "ERR_SI
 PREC_FP64, PREC_MIXED = 0, 1
 OMEGA_UNIFORM, OMEGA_GAUSSIAN = 0, 1
 KERNEL_GAUSSIAN, KERNEL_CAUCHY, KERNEL_REF_CAUCHY, KERNEL_REF_GAUSSIAN, KERNEL_REF_COULOMB = 1, 2, 3, 4, 5
+KERNEL_GAUSSIAN_DSIGMA = 6
 
 
 class HmatError(RuntimeError):
@@ -72,6 +73,9 @@ def _proto(lib):
         "hmat_hodlr_factor": (st, [vp]),
         "hmat_hodlr_solve": (st, [vp, _D, i64, i32, _D, i64]),
         "hmat_hodlr_solve_device": (st, [vp, vp, i64, i32, vp, i64]),
+        "hmat_hodlr_solve_transpose": (st, [vp, _D, i64, i32, _D, i64]),
+        "hmat_hodlr_solve_transpose_device": (st, [vp, vp, i64, i32, vp, i64]),
+        "hmat_hodlr_matvec_transpose": (st, [vp, _D, i64, i32, _D, i64]),
         "hmat_hodlr_set_refinement": (st, [vp, C.c_double, i32]),
         "hmat_hodlr_last_refinement": (st, [vp, C.POINTER(i32), C.POINTER(C.c_double)]),
         "hmat_hodlr_size": (i64, [vp]),
@@ -261,6 +265,23 @@ class HODLRMatrixOMP(_Base):
 
     def multiply_device(self, x_ptr, ldx, nrhs, y_ptr, ldy):
         _check(self._lib.hmat_hodlr_matvec_device(self._h, x_ptr, ldx, nrhs, y_ptr, ldy))
+
+    def solve_transpose(self, b):
+        """X = A_H^{-T} B (adjoint solve; no reference counterpart)."""
+        b = _fortran(b)
+        x = np.zeros((self.N, b.shape[1]), order="F")
+        _check(self._lib.hmat_hodlr_solve_transpose(self._h, _ptr(b), b.shape[0], b.shape[1], _ptr(x), self.N))
+        return x
+
+    def solve_transpose_device(self, b_ptr, ldb, nrhs, x_ptr, ldx):
+        _check(self._lib.hmat_hodlr_solve_transpose_device(self._h, b_ptr, ldb, nrhs, x_ptr, ldx))
+
+    def multiply_transpose(self, x):
+        """Y = A_H^T X."""
+        x = _fortran(x)
+        y = np.zeros((self.N, x.shape[1]), order="F")
+        _check(self._lib.hmat_hodlr_matvec_transpose(self._h, _ptr(x), x.shape[0], x.shape[1], _ptr(y), self.N))
+        return y
 
     def omega_length(self, n, leaf_size):
         self._sync_opts()

@@ -74,13 +74,17 @@ typedef struct {
  *   HMAT_KERNEL_REF_GAUSSIAN exp(-(x_i-x_j)^2 / p0^2) + p1*delta_ij (kernel.h:52-65)
  *   HMAT_KERNEL_REF_COULOMB  1/|x_i-x_j| (0 on the diagonal) + p1*delta_ij
  *                                                               (kernel.h:69-89)
+ *   HMAT_KERNEL_GAUSSIAN_DSIGMA  d/dp0 of HMAT_KERNEL_GAUSSIAN (no delta term):
+ *                            exp(-(x_i-x_j)^2/(2 p0^2)) (x_i-x_j)^2 / p0^3
+ *                            (hmat_kernel_matvec_device only; for adjoints)
  * x (and y for CAUCHY) are host arrays of N doubles. */
 typedef enum {
   HMAT_KERNEL_GAUSSIAN = 1,
   HMAT_KERNEL_CAUCHY = 2,
   HMAT_KERNEL_REF_CAUCHY = 3,
   HMAT_KERNEL_REF_GAUSSIAN = 4,
-  HMAT_KERNEL_REF_COULOMB = 5
+  HMAT_KERNEL_REF_COULOMB = 5,
+  HMAT_KERNEL_GAUSSIAN_DSIGMA = 6
 } hmat_kernel_type;
 
 typedef struct {
@@ -171,6 +175,18 @@ hmat_status hmat_hodlr_solve(hmat_hodlr h, const double* B, int64_t ldb,
 hmat_status hmat_hodlr_solve_device(hmat_hodlr h, const double* dB,
                                     int64_t ldb, int32_t nrhs, double* dX,
                                     int64_t ldx);
+/* Transposed operator, for adjoints (no reference counterpart; the
+ * reference has no transpose solve).  A_H^T has the same HODLR structure
+ * with the roles of (U S) and V exchanged; its factorisation is kept beside
+ * the one of A_H, so repeated adjoint solves factor once. */
+hmat_status hmat_hodlr_solve_transpose(hmat_hodlr h, const double* B, int64_t ldb,
+                                       int32_t nrhs, double* X, int64_t ldx);
+hmat_status hmat_hodlr_solve_transpose_device(hmat_hodlr h, const double* dB,
+                                              int64_t ldb, int32_t nrhs,
+                                              double* dX, int64_t ldx);
+hmat_status hmat_hodlr_matvec_transpose(hmat_hodlr h, const double* X,
+                                        int64_t ldx, int32_t nrhs, double* Y,
+                                        int64_t ldy);
 hmat_status hmat_hodlr_set_refinement(hmat_hodlr h, double rtol,
                                       int32_t max_iters);
 /* Iterations and final max relative residual of the last solve. */
