@@ -34,6 +34,12 @@ METRIC = "build+solve s to 1e-10 resid, N=2^18: mixed-prec+IR vs plain FP64 GPU"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
+COPY_PROBE_BYTES = 256 << 20
+# HBM bytes per dispatch from the two calibrated rocprofv3 --pmc passes
+# (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, from a 256 MiB copy) recorded in
+# profiles/pmc_traffic_hodlr_cauchy_n262144_mixed.txt
+PMC_TRAFFIC = {"sketch_lp": 1.121e9, "matvec": 1.0396e9 + 1.5501e9}
+PMC_SOURCE = "profiles/pmc_traffic_hodlr_cauchy_n262144_mixed.txt"
 
 
 def log(msg):
@@ -163,6 +169,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--matvec-reps", type=int, default=20)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 256 MiB device copy first (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the engine has no "
@@ -173,6 +181,15 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if a.copy_probe:
+        # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
+        # calibrate the per-dispatch HBM traffic of the engine's kernels
+        src = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     modes = a.modes.split(",")
     res = {m: run_mode(hmat_amd, torch, m, a) for m in modes}
     med = {m: statistics.median(r["times"]) for m, r in res.items()}
@@ -219,7 +236,9 @@ def main():
         out["roofline"] = {"kernel": "k_sketch (FP64 MFMA, generated entries)" if name == "sketch_gen"
                            else "k_sketch_lp (BF16 MFMA)", "bound": "mfma",
                            "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-                           "frac": round(achieved / peak, 4), "traffic": None,
+                           "frac": round(achieved / peak, 4),
+                           "traffic": PMC_TRAFFIC.get(name) if a.n == 1 << 18 else None,
+                           "traffic_source": PMC_SOURCE,
                            "launches": p["launches"], "avg_launch_ms": round(p["ms"] / p["launches"], 4),
                            "step_share": round(p["ms"] / 1000 / a.steps / med[head], 3)}
         for k2, p2 in prof.items():
@@ -231,7 +250,9 @@ def main():
         gbs = mv["work"] / mv["launches"] / (avg * 1e-3) / 1e9
         out["matvec_roofline"] = {"kernel": "fused HODLR matvec (project + reduce + expand), FP64, 1 RHS",
                                   "bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
-                                  "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                                  "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+                                  "traffic": PMC_TRAFFIC["matvec"] if a.n == 1 << 18 else None,
+                                  "traffic_source": PMC_SOURCE,
                                   "bytes_per_launch": mv["work"] / mv["launches"],
                                   "avg_launch_ms": round(avg, 4), "launches": mv["launches"]}
     if not a.no_cpu:
