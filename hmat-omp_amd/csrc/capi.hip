@@ -298,6 +298,33 @@ hmat_status hmat_hodlr_matvec_transpose(hmat_hodlr h, const double* X, int64_t l
   });
 }
 
+hmat_status hmat_hodlr_build_kernel_batch_device(hmat_hodlr h, const hmat_kernel* k, const int64_t* sizes,
+                                                 int32_t nsys, int32_t leaf) {
+  return guard([&] {
+    HMAT_REQUIRE(h && k && k->x && sizes && nsys > 0 && leaf > 0, HMAT_ERR_ARG,
+                 "bad build_kernel_batch arguments");
+    HMAT_REQUIRE(k->type != HMAT_KERNEL_CAUCHY || k->y, HMAT_ERR_ARG, "Cauchy kernel needs y points");
+    HMAT_REQUIRE(k->type != HMAT_KERNEL_GAUSSIAN_DSIGMA, HMAT_ERR_ARG, "derivative kernel cannot be compressed");
+    std::vector<int64_t> sz(sizes, sizes + nsys);
+    GenDesc g = gen_from_kernel(*k, k->x, k->y);
+    h->eng.build_forest(g, sz, leaf);
+  });
+}
+
+hmat_status hmat_hodlr_save(hmat_hodlr h, const char* path) {
+  return guard([&] {
+    HMAT_REQUIRE(h && path, HMAT_ERR_ARG, "bad save arguments");
+    h->eng.save(path);
+  });
+}
+
+hmat_status hmat_hodlr_load(hmat_hodlr h, const char* path) {
+  return guard([&] {
+    HMAT_REQUIRE(h && path, HMAT_ERR_ARG, "bad load arguments");
+    h->eng.load(path);
+  });
+}
+
 int64_t hmat_hodlr_size(hmat_hodlr h) { return h ? h->eng.N : -1; }
 int32_t hmat_hodlr_levels(hmat_hodlr h) { return h && h->eng.built ? h->eng.tree.max_depth : -1; }
 

@@ -43,6 +43,35 @@ struct Tree {
     for (int id : leaves) lmax = std::max(lmax, nodes[id].len);
   }
 
+  // Forest of independent systems (batched solves): system s owns rows
+  // [sum_{t<s} sizes[t], ... + sizes[s]) and is bisected on its own; the
+  // roots are the depth-0 nodes, so every depth holds the nodes of all
+  // systems and a level still takes one launch.
+  void build_forest(const std::vector<int64_t>& sizes, int64_t leaf) {
+    nodes.clear();
+    by_depth.clear();
+    leaves.clear();
+    int next_ref = 0;
+    std::vector<int> roots;
+    int64_t off = 0;
+    for (int64_t n : sizes) {
+      TNode r;
+      r.start = off;
+      r.len = n;
+      r.ref_id = next_ref++;
+      roots.push_back((int)nodes.size());
+      nodes.push_back(r);
+      off += n;
+    }
+    for (int id : roots) grow(id, leaf, next_ref);
+    max_depth = 0;
+    for (auto& n : nodes) max_depth = std::max(max_depth, n.depth);
+    by_depth.assign(max_depth + 1, {});
+    for (int id : roots) walk(id);
+    lmax = 0;
+    for (int id : leaves) lmax = std::max(lmax, nodes[id].len);
+  }
+
  private:
   void grow(int id, int64_t leaf, int& next_ref) {
     TNode& n = nodes[id];

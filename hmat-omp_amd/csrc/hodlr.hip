@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <type_traits>
 
 #include "hodlr.hpp"
@@ -53,9 +54,7 @@ std::vector<std::pair<int64_t, int>> HodlrEngine::omega_schedule(const hmat_opti
   return out;
 }
 
-void HodlrEngine::build(const GenDesc& g, int64_t n, int leaf_size, const double* host_omega,
-                        int64_t omega_len) {
-  HMAT_REQUIRE(n > 0, HMAT_ERR_ARG, "N must be positive");
+void HodlrEngine::reset_for_build(int leaf_size) {
   HMAT_REQUIRE(leaf_size > 0, HMAT_ERR_ARG, "leaf_size must be positive");
   HMAT_REQUIRE(opt.rank > 0 && opt.oversample >= 0 && opt.power_iters >= 0, HMAT_ERR_ARG,
                "bad rank/oversample/power_iters");
@@ -65,9 +64,32 @@ void HodlrEngine::build(const GenDesc& g, int64_t n, int leaf_size, const double
   f64T_.clear();
   f32T_.clear();
   factoredT_ = transposed_ = false;
-  N = n;
   leaf = leaf_size;
+}
+
+void HodlrEngine::build(const GenDesc& g, int64_t n, int leaf_size, const double* host_omega,
+                        int64_t omega_len) {
+  HMAT_REQUIRE(n > 0, HMAT_ERR_ARG, "N must be positive");
+  reset_for_build(leaf_size);
+  N = n;
   tree.build(N, leaf);
+  build_levels(g, host_omega, omega_len);
+}
+
+void HodlrEngine::build_forest(const GenDesc& g, const std::vector<int64_t>& sizes, int leaf_size) {
+  HMAT_REQUIRE(!sizes.empty(), HMAT_ERR_ARG, "empty batch");
+  int64_t tot = 0;
+  for (int64_t n : sizes) {
+    HMAT_REQUIRE(n > 0, HMAT_ERR_ARG, "every system size must be positive");
+    tot += n;
+  }
+  reset_for_build(leaf_size);
+  N = tot;
+  tree.build_forest(sizes, leaf);
+  build_levels(g, nullptr, 0);
+}
+
+void HodlrEngine::build_levels(const GenDesc& g, const double* host_omega, int64_t omega_len) {
   lv.clear();
   lv.resize(tree.max_depth + 1);
   par.clear();
@@ -685,6 +707,145 @@ void HodlrEngine::solve_transpose(const double* dB, int64_t ldb, int nrhs, doubl
     throw;
   }
   flip();
+}
+
+}  // namespace hmat
+
+namespace hmat {
+
+// Operator file (little-endian, version 1):
+//   magic "HMATHODLR\0" (10 bytes), int32 version
+//   hmat_options opt, int64 N, int32 leaf, int32 nsys, int64 sizes[nsys]
+//   int32 max_depth, int64 lmax, int32 nleaf
+//   per depth c = 1..max_depth: int32 nseg, int32 rmax, int32 rank[nseg],
+//     double US[N * rmax], double V[N * rmax]   (column-major, ld N)
+//   double D[nleaf * lmax * lmax]
+// The tree is rebuilt from (sizes, leaf); factorisations are not stored.
+static const char kMagic[10] = {'H', 'M', 'A', 'T', 'H', 'O', 'D', 'L', 'R', 0};
+
+void HodlrEngine::save(const std::string& path) {
+  HMAT_REQUIRE(built, HMAT_ERR_STATE, "save before build");
+  HMAT_REQUIRE(!transposed_, HMAT_ERR_STATE, "save while transposed");
+  std::ofstream f(path, std::ios::binary);
+  HMAT_REQUIRE(f.good(), HMAT_ERR_ARG, "cannot open " + path + " for writing");
+  auto put = [&](const void* p, size_t n) { f.write(reinterpret_cast<const char*>(p), (std::streamsize)n); };
+  const int32_t version = 1;
+  put(kMagic, sizeof(kMagic));
+  put(&version, 4);
+  put(&opt, sizeof(opt));
+  put(&N, 8);
+  const int32_t lf = leaf;
+  put(&lf, 4);
+  std::vector<int64_t> sizes;
+  for (int id : tree.by_depth[0]) sizes.push_back(tree.nodes[id].len);
+  const int32_t nsys = (int32_t)sizes.size();
+  put(&nsys, 4);
+  put(sizes.data(), 8 * sizes.size());
+  const int32_t md = tree.max_depth;
+  put(&md, 4);
+  put(&tree.lmax, 8);
+  const int32_t nl = (int32_t)tree.leaves.size();
+  put(&nl, 4);
+  for (int c = 1; c <= tree.max_depth; ++c) {
+    const HLevel& L = lv[c];
+    const int32_t ns = (int32_t)L.ids.size(), rm = L.rmax;
+    put(&ns, 4);
+    put(&rm, 4);
+    std::vector<int32_t> rk(L.rank.begin(), L.rank.end());
+    put(rk.data(), 4 * rk.size());
+    auto us = download(L.US.get(), (size_t)N * rm, stream);
+    auto v = download(L.V.get(), (size_t)N * rm, stream);
+    put(us.data(), 8 * us.size());
+    put(v.data(), 8 * v.size());
+  }
+  auto d = download(D.get(), (size_t)nl * tree.lmax * tree.lmax, stream);
+  put(d.data(), 8 * d.size());
+  HMAT_REQUIRE(f.good(), HMAT_ERR_ARG, "write to " + path + " failed");
+}
+
+void HodlrEngine::load(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  HMAT_REQUIRE(f.good(), HMAT_ERR_ARG, "cannot open " + path);
+  auto get = [&](void* p, size_t n) {
+    f.read(reinterpret_cast<char*>(p), (std::streamsize)n);
+    HMAT_REQUIRE(f.good(), HMAT_ERR_ARG, path + ": truncated operator file");
+  };
+  char magic[10];
+  int32_t version = 0;
+  get(magic, 10);
+  get(&version, 4);
+  HMAT_REQUIRE(std::memcmp(magic, kMagic, 10) == 0 && version == 1, HMAT_ERR_ARG,
+               path + ": not a version-1 HODLR operator file");
+  hmat_options o;
+  get(&o, sizeof(o));
+  int64_t n = 0;
+  int32_t lf = 0, nsys = 0;
+  get(&n, 8);
+  get(&lf, 4);
+  get(&nsys, 4);
+  HMAT_REQUIRE(n > 0 && lf > 0 && nsys > 0 && nsys <= n, HMAT_ERR_ARG, path + ": bad header");
+  std::vector<int64_t> sizes(nsys);
+  get(sizes.data(), 8 * (size_t)nsys);
+  int64_t tot = 0;
+  for (int64_t v : sizes) {
+    HMAT_REQUIRE(v > 0, HMAT_ERR_ARG, path + ": bad system size");
+    tot += v;
+  }
+  HMAT_REQUIRE(tot == n, HMAT_ERR_ARG, path + ": sizes do not add up to N");
+  // the file's options replace the handle's; a mixed-precision operator
+  // refines to the default target unless one was set on this handle
+  if (o.precision == HMAT_PREC_MIXED && !(refine_rtol > 0.0)) refine_rtol = 1e-10;
+  opt = o;
+  reset_for_build(lf);
+  N = n;
+  if (nsys == 1)
+    tree.build(N, leaf);
+  else
+    tree.build_forest(sizes, leaf);
+  int32_t md = 0, nl = 0;
+  int64_t lm = 0;
+  get(&md, 4);
+  get(&lm, 8);
+  get(&nl, 4);
+  HMAT_REQUIRE(md == tree.max_depth && lm == tree.lmax && nl == (int32_t)tree.leaves.size(), HMAT_ERR_ARG,
+               path + ": tree does not match the header");
+  lv.clear();
+  lv.resize(tree.max_depth + 1);
+  par.clear();
+  for (int c = 1; c <= tree.max_depth; ++c) {
+    HLevel& L = lv[c];
+    L.ids = tree.by_depth[c];
+    const int ns = (int)L.ids.size();
+    L.sib.assign(ns, 0);
+    for (int s = 0; s < ns; ++s) {
+      const TNode& nd = tree.nodes[L.ids[s]];
+      const TNode& par_ = tree.nodes[nd.parent];
+      const int sib_id = (par_.left == L.ids[s]) ? par_.right : par_.left;
+      L.sib[s] = tree.nodes[sib_id].pos;
+    }
+    int32_t fns = 0, rm = 0;
+    get(&fns, 4);
+    get(&rm, 4);
+    HMAT_REQUIRE(fns == ns && rm >= 0, HMAT_ERR_ARG, path + ": level layout does not match the tree");
+    std::vector<int32_t> rk(ns);
+    get(rk.data(), 4 * (size_t)ns);
+    L.rank.assign(rk.begin(), rk.end());
+    L.rmax = rm;
+    for (int r : L.rank) HMAT_REQUIRE(r >= 0 && r <= rm, HMAT_ERR_ARG, path + ": bad rank");
+    std::vector<double> buf((size_t)N * rm);
+    get(buf.data(), 8 * buf.size());
+    L.US.upload(buf, stream);
+    get(buf.data(), 8 * buf.size());
+    L.V.upload(buf, stream);
+    HIP_CHECK(hipStreamSynchronize(stream));
+    L.S.assign((size_t)ns * rm, 0.0);
+  }
+  std::vector<double> d((size_t)nl * lm * lm);
+  get(d.data(), 8 * d.size());
+  D.upload(d, stream);
+  HIP_CHECK(hipStreamSynchronize(stream));
+  prepare_matvec();
+  built = true;
 }
 
 }  // namespace hmat

@@ -1,6 +1,7 @@
 // HODLR engine: the MI355X counterpart of HODLRMatrixOMP
 // (include/hodlr_routines_omp.h:9-241).
 #pragma once
+#include <string>
 #include <vector>
 
 #include "engine.hpp"
@@ -70,10 +71,18 @@ class HodlrEngine {
   ~HodlrEngine();
 
   void build(const GenDesc& g, int64_t N, int leaf, const double* host_omega, int64_t omega_len);
+  // Independent systems of the given sizes on consecutive rows (a forest):
+  // one operator, block diagonal across systems, built and solved level by
+  // level for all systems at once.
+  void build_forest(const GenDesc& g, const std::vector<int64_t>& sizes, int leaf);
   void matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
   void factor();
   void solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
   void to_dense_host(double* A, int64_t lda);
+  // Operator file: options, tree sizes, per-depth ranks, (U S) and V, leaf
+  // blocks.  load() replaces the handle's operator and options.
+  void save(const std::string& path);
+  void load(const std::string& path);
   // A_H^T x and A_H^{-1 T} b (adjoints): the transposed operator is a
   // HODLR matrix with (U S) and V exchanged, ranks taken from the sibling
   // and the leaf blocks transposed.  Its factorisation is kept separately.
@@ -102,6 +111,8 @@ class HodlrEngine {
   double last_resid = -1.0;
 
  private:
+  void reset_for_build(int leaf);
+  void build_levels(const GenDesc& g, const double* host_omega, int64_t omega_len);
   void build_level(int c, const GenDesc& g, const std::vector<int64_t>& omega_off,
                    const double* host_omega);
   template <class F> void factor_t(HFactor<F>& fa);

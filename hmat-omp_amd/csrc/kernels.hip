@@ -996,13 +996,182 @@ __global__ __launch_bounds__(kWG) void k_gj_inverse(const SmallMat* __restrict__
   }
 }
 
+// n <= 256: blocked Gauss-Jordan, the same pivoting and breakdown rules as
+// k_gj_inverse.  Thread i owns row i.  Per block of NB = 16 columns K:
+//   1. the panel A(:, K) is inverted in registers (pivot search by a block
+//      argmax, row exchange between the two owning threads, pivot row
+//      broadcast through LDS);
+//   2. the block's row swaps are applied to the other columns;
+//   3. the other columns get the whole block's transformation at once,
+//      A(:, c) = [rows not in K] A(:, c) + panel A(K, c), A(K, c) staged in LDS.
+// The matrix is read and written once per block instead of once per column.
+template <class F, int J>
+__device__ __forceinline__ bool gj_panel_col(F (&pan)[16], int kb, int nbk, int n, int lane, int w,
+                                             F* bv, int* bi, F* rowk, F* xk, F* xp, int* perm, int* piv_s,
+                                             int* bad, int* info) {
+  constexpr int NB = 16;
+  if (J >= nbk) return true;
+  const int i = threadIdx.x;
+  const int k = kb + J;
+  F v = (i < n && i >= k) ? fabs(pan[J]) : F(-1);
+  int idx = i;
+  for (int o = 32; o > 0; o >>= 1) {
+    const F v2 = __shfl_xor(v, o);
+    const int i2 = __shfl_xor(idx, o);
+    if (v2 > v || (v2 == v && i2 < idx)) {
+      v = v2;
+      idx = i2;
+    }
+  }
+  if (lane == 0) {
+    bv[w] = v;
+    bi[w] = idx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    F best = bv[0];
+    int bidx = bi[0];
+    for (int q = 1; q < kWG / 64; ++q)
+      if (bv[q] > best || (bv[q] == best && bi[q] < bidx)) {
+        best = bv[q];
+        bidx = bi[q];
+      }
+    *piv_s = bidx;
+    perm[k] = bidx;
+    *bad = (!(best > F(0)) || !isfinite(best)) ? 1 : 0;
+    if (*bad) info[blockIdx.x] = k + 1;
+  }
+  __syncthreads();
+  if (*bad) return false;
+  const int p = *piv_s;
+  if (p != k) {  // exchange rows p and k of the panel
+    if (i == k)
+#pragma unroll
+      for (int c = 0; c < NB; ++c) xk[c] = pan[c];
+    if (i == p)
+#pragma unroll
+      for (int c = 0; c < NB; ++c) xp[c] = pan[c];
+    __syncthreads();
+    if (i == k)
+#pragma unroll
+      for (int c = 0; c < NB; ++c) pan[c] = xp[c];
+    if (i == p)
+#pragma unroll
+      for (int c = 0; c < NB; ++c) pan[c] = xk[c];
+  }
+  if (i == k) {
+    const F pv = pan[J];
+    pan[J] = F(1);
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      pan[c] /= pv;
+      rowk[c] = pan[c];
+    }
+  }
+  __syncthreads();
+  if (i != k && i < n) {
+    const F f = pan[J];
+    pan[J] = F(0);
+#pragma unroll
+    for (int c = 0; c < NB; ++c) pan[c] -= f * rowk[c];
+  }
+  return true;
+}
+
+template <class F, int J>
+__device__ __forceinline__ bool gj_panel(F (&pan)[16], int kb, int nbk, int n, int lane, int w, F* bv,
+                                         int* bi, F* rowk, F* xk, F* xp, int* perm, int* piv_s, int* bad,
+                                         int* info) {
+  if constexpr (J < 16) {
+    if (!gj_panel_col<F, J>(pan, kb, nbk, n, lane, w, bv, bi, rowk, xk, xp, perm, piv_s, bad, info))
+      return false;
+    return gj_panel<F, J + 1>(pan, kb, nbk, n, lane, w, bv, bi, rowk, xk, xp, perm, piv_s, bad, info);
+  }
+  return true;
+}
+
+template <class F>
+__global__ __launch_bounds__(kWG) void k_gj_blocked(const SmallMat* __restrict__ mats, F* __restrict__ A,
+                                                    int* __restrict__ info) {
+  constexpr int NB = 16;
+  __shared__ F R[NB * 256];  // R[j * 256 + c] = A(kb + j, c) after the block's swaps
+  __shared__ F rowk[NB], xk[NB], xp[NB];
+  __shared__ F bv[kWG / 64];
+  __shared__ int bi[kWG / 64];
+  __shared__ int perm[256];
+  __shared__ int piv_s, bad;
+  const SmallMat m = mats[blockIdx.x];
+  const int n = m.n, ld = m.ld;
+  F* a = A + m.off;
+  const int i = threadIdx.x, lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  if (i == 0) info[blockIdx.x] = 0;
+  for (int kb = 0; kb < n; kb += NB) {
+    const int nbk = min(NB, n - kb);
+    F pan[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) pan[j] = (i < n && j < nbk) ? a[i + (int64_t)(kb + j) * ld] : F(0);
+    if (!gj_panel<F, 0>(pan, kb, nbk, n, lane, w, bv, bi, rowk, xk, xp, perm, &piv_s, &bad, info))
+      return;  // info holds the failing column + 1
+    // 2. this block's row swaps on the other columns (one thread per column)
+    if (i < n && (i < kb || i >= kb + nbk))
+      for (int j = 0; j < nbk; ++j) {
+        const int k = kb + j, p = perm[k];
+        if (p != k) {
+          const F t = a[k + (int64_t)i * ld];
+          a[k + (int64_t)i * ld] = a[p + (int64_t)i * ld];
+          a[p + (int64_t)i * ld] = t;
+        }
+      }
+    __syncthreads();
+    // rows of R beyond nbk are zero (the update runs over all NB rows)
+    if (i < n && (i < kb || i >= kb + nbk))
+      for (int j = 0; j < NB; ++j) R[j * 256 + i] = j < nbk ? a[kb + j + (int64_t)i * ld] : F(0);
+    __syncthreads();
+    // 3. other columns c: A(i, c) = [i not in K] A(i, c) + sum_j M(i, j) R(j, c)
+    //    (the pivot rows are replaced, not updated: forming M - I there would
+    //    cancel badly when the pivots are large)
+    if (i < n) {
+      const bool in_k = i >= kb && i < kb + nbk;
+      F wv[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) wv[j] = (j < nbk) ? pan[j] : F(0);
+      for (int c = 0; c < n; ++c) {
+        if (c >= kb && c < kb + nbk) continue;
+        F acc = in_k ? F(0) : a[i + (int64_t)c * ld];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc = fma(wv[j], R[j * 256 + c], acc);
+        a[i + (int64_t)c * ld] = acc;
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        if (j < nbk) a[i + (int64_t)(kb + j) * ld] = pan[j];
+    }
+    __syncthreads();
+  }
+  // column swaps in reverse order (each thread its own row)
+  if (i < n)
+    for (int k = n - 1; k >= 0; --k) {
+      const int p = perm[k];
+      if (p != k) {
+        const F t = a[i + (int64_t)k * ld];
+        a[i + (int64_t)k * ld] = a[i + (int64_t)p * ld];
+        a[i + (int64_t)p * ld] = t;
+      }
+    }
+}
+
 template <class F>
 void gj_inverse(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, F* A, int* info,
                 hipStream_t s) {
   if (mats.empty()) return;
   for (auto& mm : mats) HMAT_REQUIRE(mm.n <= 4096, HMAT_ERR_ARG, "gj_inverse: n > 4096");
+  int nmax = 0;
+  for (auto& mm : mats) nmax = std::max(nmax, mm.n);
   dm.upload(mats, s);
-  k_gj_inverse<F><<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), A, info);
+  if (nmax <= 256)
+    k_gj_blocked<F><<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), A, info);
+  else
+    k_gj_inverse<F><<<(unsigned)mats.size(), kWG, 0, s>>>(dm.get(), A, info);
   HIP_CHECK(hipGetLastError());
 }
 template void gj_inverse<double>(const std::vector<SmallMat>&, DevBuf<SmallMat>&, double*, int*, hipStream_t);

@@ -68,6 +68,7 @@ def _proto(lib):
         "hmat_hodlr_build_kernel_device": (st, [vp, C.POINTER(Kernel), i64, i32]),
         "hmat_kernel_matvec_device": (st, [C.POINTER(Kernel), i64, vp, i64, i32, vp, i64]),
         "hmat_hodlr_omega_length": (i64, [C.POINTER(Options), i64, i32]),
+        "hmat_hodlr_build_kernel_batch_device": (st, [vp, C.POINTER(Kernel), C.POINTER(i64), i32, i32]),
         "hmat_hodlr_matvec": (st, [vp, _D, i64, i32, _D, i64]),
         "hmat_hodlr_matvec_device": (st, [vp, vp, i64, i32, vp, i64]),
         "hmat_hodlr_factor": (st, [vp]),
@@ -83,6 +84,8 @@ def _proto(lib):
         "hmat_hodlr_level_ranks": (st, [vp, i32, C.POINTER(i32), i32, C.POINTER(i32)]),
         "hmat_hodlr_to_dense": (st, [vp, _D, i64]),
         "hmat_hodlr_set_stream": (st, [vp, vp]),
+        "hmat_hodlr_save": (st, [vp, C.c_char_p]),
+        "hmat_hodlr_load": (st, [vp, C.c_char_p]),
         "hmat_hss_create": (st, [C.POINTER(vp), C.POINTER(Options)]),
         "hmat_hss_destroy": (st, [vp]),
         "hmat_hss_build_dense": (st, [vp, _D, i64, i64, i32, i32, _D, i64]),
@@ -262,6 +265,24 @@ class HODLRMatrixOMP(_Base):
 
     def solve_device(self, b_ptr, ldb, nrhs, x_ptr, ldx):
         _check(self._lib.hmat_hodlr_solve_device(self._h, b_ptr, ldb, nrhs, x_ptr, ldx))
+
+    def save(self, path):
+        """Write the built operator to `path` (see hmat_hodlr_save)."""
+        _check(self._lib.hmat_hodlr_save(self._h, str(path).encode()))
+
+    def load(self, path):
+        """Replace this handle's operator and options with the file's."""
+        _check(self._lib.hmat_hodlr_load(self._h, str(path).encode()))
+        self.N = self._lib.hmat_hodlr_size(self._h)
+
+    def build_kernel_batch_device(self, kernel, sizes, leaf_size):
+        """Independent systems of the given sizes on consecutive points (a
+        block-diagonal operator, built and solved for all systems at once)."""
+        self._sync_opts()
+        sz = (C.c_int64 * len(sizes))(*[int(v) for v in sizes])
+        _check(self._lib.hmat_hodlr_build_kernel_batch_device(self._h, C.byref(kernel), sz, len(sizes),
+                                                              leaf_size))
+        self.N = int(sum(int(v) for v in sizes))
 
     def multiply_device(self, x_ptr, ldx, nrhs, y_ptr, ldy):
         _check(self._lib.hmat_hodlr_matvec_device(self._h, x_ptr, ldx, nrhs, y_ptr, ldy))

@@ -151,6 +151,18 @@ hmat_status hmat_hodlr_build_kernel(hmat_hodlr h, const hmat_kernel* k,
 hmat_status hmat_hodlr_build_kernel_device(hmat_hodlr h,
                                            const hmat_kernel* k_device_points,
                                            int64_t N, int32_t leaf_size);
+/* Batch of independent systems (no reference counterpart): system s owns
+ * rows [off_s, off_s + sizes[s]) of the concatenated points (off_s = sum of
+ * the sizes before s) and gets its own bisection tree; the handle then holds
+ * one block-diagonal operator whose levels are built, factored and solved
+ * for all systems at once.  matvec / solve / solve_transpose take
+ * sum(sizes) rows; per-system right-hand-side counts are expressed by zero
+ * columns.  Kernel parameters are shared: for per-system Gaussian widths,
+ * scale the points of system s by 1/sigma_s and use p0 = 1. */
+hmat_status hmat_hodlr_build_kernel_batch_device(hmat_hodlr h,
+                                                 const hmat_kernel* k_device_points,
+                                                 const int64_t* sizes,
+                                                 int32_t nsys, int32_t leaf_size);
 /* Number of doubles of host Omega that build_dense expects. */
 int64_t hmat_hodlr_omega_length(const hmat_options* opts, int64_t N,
                                 int32_t leaf_size);
@@ -203,6 +215,13 @@ hmat_status hmat_hodlr_level_ranks(hmat_hodlr h, int32_t lvl, int32_t* ranks,
                                    int32_t cap, int32_t* count);
 /* Dense reconstruction of A_H (checking only; N x N host buffer). */
 hmat_status hmat_hodlr_to_dense(hmat_hodlr h, double* A, int64_t lda);
+/* Operator files (no reference counterpart: the reference has no
+ * serialisation).  save writes the built operator (options, tree sizes,
+ * ranks, (U S) and V per level, leaf blocks) to `path`; load replaces the
+ * handle's operator and options with the file's.  The factorisation is
+ * recomputed on the first solve after a load. */
+hmat_status hmat_hodlr_save(hmat_hodlr h, const char* path);
+hmat_status hmat_hodlr_load(hmat_hodlr h, const char* path);
 /* Set the HIP stream (hipStream_t) used by the handle; NULL = own stream. */
 hmat_status hmat_hodlr_set_stream(hmat_hodlr h, void* stream);
 

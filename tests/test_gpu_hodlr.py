@@ -249,3 +249,64 @@ def test_fused_matvec_matches_dense(n, leaf, nrhs):
     # passes of 4, 2 and 1 columns group the cross-lane sums differently
     for j in range(nrhs):
         assert rel(y[:, j:j + 1], h.multiply(v[:, j])) < 1e-14
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_save_load_round_trip(tmp_path, mixed):
+    """The operator file reproduces the operator bit for bit; solves after a
+    load (which refactors) equal solves on the original handle."""
+    n = 3000
+    if mixed:  # mixed precision needs a well-conditioned system (FP32 factorisation)
+        xc, yc = kernels.cauchy_grid_points(n)
+        a = kernels.cauchy_points(xc, yc)
+    else:
+        a = kernels.gaussian_points(kernels.sorted_uniform_points(n, 21), 0.1, 1e-2)
+    o = hmat_amd.make_options(1e-10, rank=32, oversample=10, power_iters=1, exact_below=0,
+                              precision=hmat_amd.PREC_MIXED if mixed else hmat_amd.PREC_FP64,
+                              omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=5)
+    h = hmat_amd.HODLRMatrixOMP(opts=o)
+    h.build_from_dense(a, 100)
+    f = tmp_path / "op.hodlr"
+    h.save(f)
+    g = hmat_amd.HODLRMatrixOMP(1e-3)  # options come from the file
+    g.load(f)
+    assert g.N == n and g.levels() == h.levels()
+    for d in range(1, h.levels() + 1):
+        assert g.level_ranks(d) == h.level_ranks(d)
+    assert np.array_equal(g.to_dense(), h.to_dense())
+    b = np.random.default_rng(3).standard_normal((n, 2))
+    assert np.array_equal(g.multiply(b), h.multiply(b))
+    assert rel(g.solve(b), h.solve(b)) < 1e-14
+
+
+def test_load_rejects_bad_file(tmp_path):
+    f = tmp_path / "junk.hodlr"
+    f.write_bytes(b"not an operator")
+    h = hmat_amd.HODLRMatrixOMP(1e-8)
+    with pytest.raises(hmat_amd.HmatError) as e:
+        h.load(f)
+    assert e.value.status == 1
+
+
+def test_mixed_precision_ill_conditioned_reports_not_converged():
+    """Gaussian sigma = 0.1, lam = 1e-2, N = 3000 (kappa ~ 7.5e4): the FP32
+    factorisation is too inaccurate for the FP64 refinement to contract, and
+    the solve says so (HMAT_ERR_NOT_CONVERGED) instead of returning a wrong
+    answer; the FP64 path solves the same system."""
+    n = 3000
+    a = kernels.gaussian_points(kernels.sorted_uniform_points(n, 21), 0.1, 1e-2)
+    b = np.random.default_rng(3).standard_normal((n, 1))
+    om = hmat_amd.make_options(1e-10, rank=32, oversample=10, power_iters=1, exact_below=0,
+                               precision=hmat_amd.PREC_MIXED, omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=5)
+    h = hmat_amd.HODLRMatrixOMP(opts=om)
+    h.build_from_dense(a, 100)
+    try:
+        x = h.solve(b)
+        assert rel(h.multiply(x), b) <= 1e-10  # if it converges, it must be right
+    except hmat_amd.HmatError as e:
+        assert e.status == 7  # HMAT_ERR_NOT_CONVERGED
+    of = hmat_amd.make_options(1e-10, rank=32, oversample=10, power_iters=1, exact_below=0,
+                               omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=5)
+    g = hmat_amd.HODLRMatrixOMP(opts=of)
+    g.build_from_dense(a, 100)
+    assert rel(g.solve(b), np.linalg.solve(g.to_dense(), b)) < 1e-9
