@@ -222,13 +222,18 @@ template void seg_gemm_nn<float>(const std::vector<SegNN>&, DevBuf<SegNN>&, cons
 template <int K, int J, int C0>
 __device__ __forceinline__ void trsm_chunk(double (&q)[K], const double* rt) {
   if constexpr (C0 < K) {
+    constexpr int CE = (C0 + 8 < K) ? C0 + 8 : K;
+    double r[8];
 #pragma unroll
-    for (int c = C0; c < C0 + 8 && c < K; ++c) {
-      q[c] = fma(-q[J], rt[J * K + c], q[c]);
-      // finish this chunk before the next chunk's R loads are issued, so at
-      // most 8 R values are live (otherwise every load is hoisted up front)
-      asm volatile("" : "+v"(q[c])::"memory");
-    }
+    for (int c = C0; c < CE; ++c) r[c - C0] = rt[J * K + c];  // the chunk's loads go out together
+#pragma unroll
+    for (int c = C0; c < CE; ++c) q[c] = fma(-q[J], r[c - C0], q[c]);
+    // the chunk's results are final before the next chunk's R loads are
+    // issued, so at most 8 R values are live (otherwise every load is
+    // hoisted up front)
+#pragma unroll
+    for (int c = C0; c < CE; ++c) asm volatile("" : "+v"(q[c]));
+    asm volatile("" ::: "memory");
     trsm_chunk<K, J, C0 + 8>(q, rt);
   }
 }
@@ -561,6 +566,92 @@ __global__ __launch_bounds__(kWG) void k_chol(const SmallMat* __restrict__ mats,
     }
 }
 
+// n <= 64, registers: one wave per matrix, lane c holds column c of R
+// (col[i] = R(i, c)).  Step j: R(j, j) = sqrt of lane j's col[j] (read by
+// every lane with readlane), lanes c > j scale their col[j], and each
+// col[a] (a > j) takes -R(j, a) R(j, c) with R(j, a) read from lane a.
+// All indices are static (template recursion), so the column stays in
+// registers and the loop needs no barrier.  Same outputs and breakdown
+// rules as k_chol.
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+template <int J, int A>
+__device__ __forceinline__ void chol_update(double (&col)[64], double rjc) {
+  if constexpr (A < 64) {
+    const double rja = readlane_f64(col[J], A);  // R(J, A) from lane A
+    col[A] = fma(-rja, rjc, col[A]);
+    chol_update<J, A + 1>(col, rjc);
+  }
+}
+
+template <int J>
+__device__ __forceinline__ int chol_step(double (&col)[64], int n, int c) {
+  if constexpr (J < 64) {
+    if (J >= n) return 0;
+    const double d = readlane_f64(col[J], J);
+    if (!(d > 0.0) || !isfinite(d)) return J + 1;
+    const double rjj = sqrt(d);
+    double rjc = 0.0;
+    if (c == J) col[J] = rjj;
+    if (c > J) {
+      col[J] /= rjj;
+      rjc = col[J];
+    }
+    chol_update<J, J + 1>(col, rjc);
+    return chol_step<J + 1>(col, n, c);
+  }
+  return 0;
+}
+
+__global__ __launch_bounds__(64) void k_chol64r(const SmallMat* __restrict__ mats,
+                                                const double* __restrict__ G, double* __restrict__ R,
+                                                const double* __restrict__ shift_coef,
+                                                int* __restrict__ info, double* __restrict__ gerr) {
+  const SmallMat m = mats[blockIdx.x];
+  const int n = m.n, ld = m.ld;
+  const double* g = G + m.off;
+  double* rg = R + m.off;
+  const int c = threadIdx.x;
+  double tr = 0.0, dev = 0.0;
+  if (c < n) {
+    tr = g[c + (int64_t)c * ld];
+    for (int i = 0; i < n; ++i) dev = fmax(dev, fabs(g[i + (int64_t)c * ld] - (i == c ? 1.0 : 0.0)));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    tr += __shfl_xor(tr, o);
+    dev = fmax(dev, __shfl_xor(dev, o));
+  }
+  if (c == 0) {
+    gerr[blockIdx.x] = (tr == 0.0) ? 0.0 : dev;
+    info[blockIdx.x] = 0;
+  }
+  const double shift = shift_coef[blockIdx.x] * tr;
+  double col[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i)
+    col[i] = (c < n && i <= c) ? g[i + (int64_t)c * ld] + (i == c ? shift : 0.0) : 0.0;
+  if (!(tr > 0.0)) {
+    if (!(tr == 0.0) && c == 0) info[blockIdx.x] = -1;  // NaN
+    if (c < n)
+      for (int i = 0; i < n; ++i) rg[i + (int64_t)c * ld] = 0.0;
+    return;
+  }
+  const int bad = chol_step<0>(col, n, c);
+  if (bad) {
+    if (c == 0) info[blockIdx.x] = bad;
+    return;
+  }
+  if (c < n) {
+#pragma unroll
+    for (int i = 0; i < 64; ++i)
+      if (i < n) rg[i + (int64_t)c * ld] = i <= c ? col[i] : 0.0;
+  }
+}
+
 // n <= 64: one wave per matrix, lane c owns column c of R (in LDS); the
 // column loop needs no cross-wave barriers.  Same outputs and breakdown
 // rules as k_chol.
@@ -632,7 +723,7 @@ void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* 
   dm.upload(mats, s);
   dshift.upload(shift_coef, s);
   if (nmax <= 64)
-    k_chol64<<<(unsigned)mats.size(), 64, lds, s>>>(dm.get(), G, R, dshift.get(), info, gerr);
+    k_chol64r<<<(unsigned)mats.size(), 64, 0, s>>>(dm.get(), G, R, dshift.get(), info, gerr);
   else
     k_chol<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), G, R, dshift.get(), info, gerr,
                                                                   use_lds);
