@@ -426,6 +426,48 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_reduce(const SegTN* __restr
   *o = v;
 }
 
+// MFMA variant of the partial products: the same 64 x 64 output tile and
+// 32-row LDS staging; wave w computes output rows 16w..16w+15 (columns of X)
+// against all four 16-column groups of Y with 16x16x4 MFMAs.
+template <class F>
+__global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part_mfma(const SegTN* __restrict__ segs,
+                                                               const TNItem* __restrict__ items,
+                                                               const F* __restrict__ X, int64_t ldx,
+                                                               const F* __restrict__ Y, int64_t ldy,
+                                                               F* __restrict__ part) {
+  __shared__ F xs[32][65];
+  __shared__ F ys[32][65];
+  const TNItem it = items[blockIdx.x];
+  const SegTN sg = segs[it.seg];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  typedef typename MfmaNN<F>::acc_t acc_t;
+  acc_t acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = acc_t{F(0), F(0), F(0), F(0)};
+  const int pc0 = it.pt * 64, qc0 = it.qt * 64;
+  for (int rb = it.r0; rb < it.r1; rb += 32) {
+    for (int e = threadIdx.x; e < 32 * 64; e += kWG) {
+      const int rr = e % 32, cc = e / 32, row = rb + rr;
+      const bool rok = row < it.r1;
+      xs[rr][cc] = (rok && pc0 + cc < sg.p) ? X[sg.xoff + row + (int64_t)(pc0 + cc) * ldx] : F(0);
+      ys[rr][cc] = (rok && qc0 + cc < sg.q) ? Y[sg.yoff + row + (int64_t)(qc0 + cc) * ldy] : F(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 32; kk += 4) {
+      const F a = xs[kk + lane / 16][16 * w + lane % 16];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = MfmaNN<F>::step(a, ys[kk + lane / 16][16 * t + lane % 16], acc[t]);
+    }
+    __syncthreads();
+  }
+  F* pp = part + (int64_t)it.slot * 4096;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) pp[(16 * w + MfmaNN<F>::row(lane, v)) + (16 * t + (lane & 15)) * 64] = acc[t][v];
+}
+
 template <class F>
 void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* X, int64_t ldx,
                  const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta, Workspace& ws,
@@ -465,8 +507,8 @@ void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* 
   di.upload(items, s);
   DevBuf<TNOut> dout;
   dout.upload(outs, s);
-  k_seg_gemm_tn_part<F><<<(unsigned)items.size(), kWG, 0, s>>>(dsegs.get(), di.get(), X, ldx, Y, ldy,
-                                                               part.get());
+  k_seg_gemm_tn_part_mfma<F><<<(unsigned)items.size(), kWG, 0, s>>>(dsegs.get(), di.get(), X, ldx, Y, ldy,
+                                                                    part.get());
   HIP_CHECK(hipGetLastError());
   k_seg_gemm_tn_reduce<F><<<dim3((unsigned)outs.size(), 4096 / kWG), kWG, 0, s>>>(
       dsegs.get(), dout.get(), part.get(), O, ldo, alpha, beta);
