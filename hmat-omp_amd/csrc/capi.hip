@@ -325,6 +325,13 @@ hmat_status hmat_hodlr_load(hmat_hodlr h, const char* path) {
   });
 }
 
+hmat_status hmat_hodlr_set_solve_graph(hmat_hodlr h, int32_t on) {
+  return guard([&] {
+    HMAT_REQUIRE(h, HMAT_ERR_ARG, "NULL handle");
+    h->eng.solve_graph = on != 0;
+  });
+}
+
 int64_t hmat_hodlr_size(hmat_hodlr h) { return h ? h->eng.N : -1; }
 int32_t hmat_hodlr_levels(hmat_hodlr h) { return h && h->eng.built ? h->eng.tree.max_depth : -1; }
 

@@ -578,15 +578,107 @@ void HodlrEngine::factor() {
 // One application of the factored inverse, FP64 in and out.
 void HodlrEngine::apply_inverse(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx) {
   if (opt.precision != HMAT_PREC_MIXED) {
-    apply_inverse_t<double>(f64_, B, ldb, ncols, X, ldx);
+    if (!solve_graph) {
+      apply_inverse_t<double>(f64_, B, ldb, ncols, X, ldx);
+      return;
+    }
+    plan_solve<double>(f64_, ncols);
+    kern::convert_cols<double, double>(N, ncols, B, ldb, f64_.b.get(), N, stream);
+    run_solve_graph<double>(f64_, ncols);
+    kern::convert_cols<double, double>(N, ncols, f64_.x.get(), N, X, ldx, stream);
     return;
   }
+  if (solve_graph) plan_solve<float>(f32_, ncols);
   f32_.b.reserve((size_t)N * ncols);
   f32_.x.reserve((size_t)N * ncols);
   kern::convert_cols<float, double>(N, ncols, B, ldb, f32_.b.get(), N, stream);
-  apply_inverse_t<float>(f32_, f32_.b.get(), N, ncols, f32_.x.get(), N);
+  if (solve_graph)
+    run_solve_graph<float>(f32_, ncols);
+  else
+    apply_inverse_t<float>(f32_, f32_.b.get(), N, ncols, f32_.x.get(), N);
   kern::convert_cols<double, float>(N, ncols, f32_.x.get(), N, X, ldx, stream);
 }
+
+// Plan every launch of the apply-inverse for `ncols` columns and record them
+// into a graph (the same launches, in the same order, as apply_inverse_t).
+template <class F>
+void HodlrEngine::plan_solve(HFactor<F>& fa, int ncols) {
+  SolvePlan<F>& sp = fa.plan;
+  if (sp.ncols == ncols && sp.graph.e) return;
+  sp.clear();
+  const int64_t lm = tree.lmax;
+  fa.b.reserve((size_t)N * ncols);
+  fa.x.reserve((size_t)N * ncols);
+  {
+    std::vector<SegNN> segs;
+    for (size_t i = 0; i < tree.leaves.size(); ++i) {
+      const TNode& nd = tree.nodes[tree.leaves[i]];
+      segs.push_back(SegNN{(int64_t)i * lm * lm, nd.start, nd.start, (int32_t)nd.len, (int32_t)nd.len, ncols, 0});
+    }
+    kern::plan_nn(segs, sp.leaf, stream);
+  }
+  const int dmax = tree.max_depth;
+  sp.tn.resize(std::max(dmax, 1));
+  sp.mn.resize(std::max(dmax, 1));
+  sp.up.resize(std::max(dmax, 1));
+  sp.mm.assign(std::max(dmax, 1), 0);
+  size_t tv = 1, slots = 1;
+  for (int j = dmax - 1; j >= 0; --j) {
+    HParents& P = par[j];
+    const HLevel& C = lv[j + 1];
+    const int np = (int)P.ids.size();
+    if (np == 0) continue;
+    const int64_t mm = P.mmax;
+    const int64_t tst = mm * ncols;
+    sp.mm[j] = mm;
+    tv = std::max(tv, (size_t)np * tst);
+    std::vector<SegTN> tn;
+    std::vector<SegNN> mn, up;
+    for (int p = 0; p < np; ++p) {
+      const TNode& l = tree.nodes[C.ids[P.lpos[p]]];
+      const TNode& r = tree.nodes[C.ids[P.rpos[p]]];
+      const int rl = C.rank[P.lpos[p]], rr = C.rank[P.rpos[p]];
+      tn.push_back(SegTN{l.start, l.start, (int64_t)p * tst, (int32_t)l.len, rr, ncols, 0});
+      tn.push_back(SegTN{r.start, r.start, (int64_t)p * tst + rr, (int32_t)r.len, rl, ncols, 0});
+      mn.push_back(SegNN{(int64_t)p * mm * mm, (int64_t)p * tst, (int64_t)p * tst, P.msize[p], P.msize[p], ncols, 0});
+      up.push_back(SegNN{l.start, (int64_t)p * tst + rr, l.start, (int32_t)l.len, rl, ncols, 0});
+      up.push_back(SegNN{r.start, (int64_t)p * tst, r.start, (int32_t)r.len, rr, ncols, 0});
+    }
+    kern::plan_tn(tn, sp.tn[j], stream);
+    kern::plan_nn(mn, sp.mn[j], stream);
+    kern::plan_nn(up, sp.up[j], stream);
+    slots = std::max(slots, (size_t)sp.tn[j].nslots);
+  }
+  fa.tvec.reserve(tv);
+  fa.coef.reserve(tv);
+  sp.part.reserve(slots * 4096);
+  HIP_CHECK(hipStreamSynchronize(stream));
+  // record
+  HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  kern::launch_nn<F>(sp.leaf, fa.Dinv.get(), lm, fa.b.get(), N, fa.x.get(), N, F(1), F(0), stream);
+  for (int j = dmax - 1; j >= 0; --j) {
+    if (par[j].ids.empty()) continue;
+    const int64_t mm = sp.mm[j];
+    const F* Q = fa.Wall.get() + colofs[j + 1] * N;
+    kern::launch_tn<F>(sp.tn[j], fa.V[j + 1], N, fa.x.get(), N, fa.tvec.get(), mm, F(1), F(0), sp.part.get(),
+                       stream);
+    kern::launch_nn<F>(sp.mn[j], fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(0), stream);
+    kern::launch_nn<F>(sp.up[j], Q, N, fa.coef.get(), mm, fa.x.get(), N, F(-1), F(1), stream);
+  }
+  HIP_CHECK(hipStreamEndCapture(stream, &sp.graph.g));
+  HIP_CHECK(hipGraphInstantiate(&sp.graph.e, sp.graph.g, nullptr, nullptr, 0));
+  sp.ncols = ncols;
+}
+
+template <class F>
+void HodlrEngine::run_solve_graph(HFactor<F>& fa, int ncols) {
+  HMAT_REQUIRE(fa.plan.ncols == ncols && fa.plan.graph.e, HMAT_ERR_STATE, "solve graph not planned");
+  HIP_CHECK(hipGraphLaunch(fa.plan.graph.e, stream));
+}
+template void HodlrEngine::plan_solve<double>(HFactor<double>&, int);
+template void HodlrEngine::plan_solve<float>(HFactor<float>&, int);
+template void HodlrEngine::run_solve_graph<double>(HFactor<double>&, int);
+template void HodlrEngine::run_solve_graph<float>(HFactor<float>&, int);
 
 // X = A_H^{-1} B; then FP64 iterative refinement on the compressed operator
 // (residual through the FP64 matvec, correction accumulated in FP64) until

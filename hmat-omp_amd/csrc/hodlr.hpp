@@ -49,6 +49,56 @@ struct HParents {
 // Factored inverse of the compressed operator in precision F:
 //   leaf inverses, Q_c = A_node^{-1} (U S)_c for every depth, and the
 //   capacitance inverses M^{-1} = (I + K C)^{-1} of every internal node.
+// An instantiated HIP graph; moving leaves the source empty.
+struct GraphExec {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  GraphExec() = default;
+  GraphExec(const GraphExec&) = delete;
+  GraphExec& operator=(const GraphExec&) = delete;
+  GraphExec(GraphExec&& o) noexcept : g(o.g), e(o.e) { o.g = nullptr; o.e = nullptr; }
+  GraphExec& operator=(GraphExec&& o) noexcept {
+    if (this != &o) {
+      reset();
+      g = o.g;
+      e = o.e;
+      o.g = nullptr;
+      o.e = nullptr;
+    }
+    return *this;
+  }
+  ~GraphExec() { reset(); }
+  void reset() {
+    if (e) (void)hipGraphExecDestroy(e);
+    if (g) (void)hipGraphDestroy(g);
+    e = nullptr;
+    g = nullptr;
+  }
+};
+
+// The apply-inverse of one factorisation for a fixed column count: every
+// launch planned (descriptors on the device, buffers sized) and recorded
+// into one HIP graph that reads fa.b and writes fa.x.
+template <class F>
+struct SolvePlan {
+  int ncols = 0;
+  kern::NNPlan leaf;
+  std::vector<kern::TNPlan> tn;
+  std::vector<kern::NNPlan> mn, up;
+  std::vector<int64_t> mm;  // per parent depth: capacitance leading dimension
+  DevBuf<F> part;
+  GraphExec graph;
+  void clear() {
+    graph.reset();
+    tn.clear();
+    mn.clear();
+    up.clear();
+    mm.clear();
+    part.release();
+    ncols = 0;
+  }
+};
+
 template <class F>
 struct HFactor {
   DevBuf<F> Dinv;                 // nleaf x lmax x lmax
@@ -57,8 +107,10 @@ struct HFactor {
   std::vector<const F*> V;        // per depth
   std::vector<DevBuf<F>> Minv;    // per parent depth: np x mmax x mmax
   DevBuf<F> tvec, coef, wtmp, b, x;
+  SolvePlan<F> plan;
   bool ready = false;
   void clear() {
+    plan.clear();
     Dinv.release(); Wall.release(); Vown.clear(); V.clear(); Minv.clear();
     tvec.release(); coef.release(); wtmp.release(); b.release(); x.release();
     ready = false;
@@ -107,6 +159,7 @@ class HodlrEngine {
   // FP64 refinement: on by default in mixed precision, opt-in in FP64
   double refine_rtol = 0.0;
   int refine_max = 30;
+  bool solve_graph = true;  // replay the apply-inverse from a captured HIP graph
   int last_iters = 0;
   double last_resid = -1.0;
 
@@ -121,6 +174,9 @@ class HodlrEngine {
                                           int64_t ldx);
   template <class F> void leaf_apply_t(const F* Dm, const F* B, int64_t ldb, int ncols, F* Y,
                                        int64_t ldy);
+  // graph-captured apply-inverse (fa.b -> fa.x, N x ncols, ld N)
+  template <class F> void plan_solve(HFactor<F>& fa, int ncols);
+  template <class F> void run_solve_graph(HFactor<F>& fa, int ncols);
   void apply_inverse(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx);
   // fused matvec plan (built once per build) and its launch without a host sync
   void prepare_matvec();

@@ -25,9 +25,6 @@ constexpr int kWG = 256;
 // down the rows (coalesced X and Y); the small per-segment T is read with
 // wave-uniform addresses (scalar loads).
 // --------------------------------------------------------------------------
-struct NNTile {
-  int32_t seg, row0, c0, pad_;
-};
 
 template <class F, int R>
 __global__ __launch_bounds__(kWG) void k_seg_gemm_nn(const SegNN* __restrict__ segs,
@@ -152,9 +149,9 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_nn_mfma(const SegNN* __restric
       }
 }
 
-template <class F>
-void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* X, int64_t ldx,
-                 const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha, F beta, hipStream_t s) {
+void plan_nn(const std::vector<SegNN>& segs, NNPlan& p, hipStream_t s) {
+  p.ntiles = 0;
+  p.mode = 0;
   if (segs.empty()) return;
   int qmax = 0;
   for (auto& g : segs) qmax = std::max(qmax, g.q);
@@ -162,25 +159,16 @@ void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* 
   int pmin = 1 << 30;
   for (auto& g : segs)
     if (g.len > 0 && g.q > 0) pmin = std::min(pmin, g.p);
+  int R, cw;
   if (pmin >= 32 && qmax >= 16) {
-    std::vector<NNTile> tiles;
-    for (size_t i = 0; i < segs.size(); ++i) {
-      const auto& g = segs[i];
-      if (g.len <= 0 || g.q <= 0) continue;
-      for (int r0 = 0; r0 < g.len; r0 += 64)
-        for (int c0 = 0; c0 < g.q; c0 += 64) tiles.push_back({(int32_t)i, r0, c0, 0});
-    }
-    dsegs.upload(segs, s);
-    DevBuf<NNTile> dt;
-    dt.upload(tiles, s);
-    k_seg_gemm_nn_mfma<F><<<(unsigned)tiles.size(), kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy,
-                                                                  alpha, beta);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(s));
-    return;
+    p.mode = 1;  // MFMA, 64 x 64 tiles
+    R = 64;
+    cw = 64;
+  } else {
+    R = qmax <= 16 ? 256 : (qmax <= 32 ? 128 : 64);
+    p.mode = R;
+    cw = (kWG / R) * 16;
   }
-  const int R = qmax <= 16 ? 256 : (qmax <= 32 ? 128 : 64);
-  const int cw = (kWG / R) * 16;
   std::vector<NNTile> tiles;
   for (size_t i = 0; i < segs.size(); ++i) {
     const auto& g = segs[i];
@@ -188,19 +176,41 @@ void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* 
     for (int r0 = 0; r0 < g.len; r0 += R)
       for (int c0 = 0; c0 < g.q; c0 += cw) tiles.push_back({(int32_t)i, r0, c0, 0});
   }
-  if (tiles.empty()) return;
-  dsegs.upload(segs, s);
-  DevBuf<NNTile> dt;
-  dt.upload(tiles, s);
-  dim3 grid((unsigned)tiles.size());
-  if (R == 256)
-    k_seg_gemm_nn<F, 256><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
-  else if (R == 128)
-    k_seg_gemm_nn<F, 128><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
+  p.ntiles = (int)tiles.size();
+  if (!p.ntiles) return;
+  p.segs.upload(segs, s);
+  p.tiles.upload(tiles, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <class F>
+void launch_nn(const NNPlan& p, const F* X, int64_t ldx, const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha,
+               F beta, hipStream_t s) {
+  if (!p.ntiles) return;
+  const unsigned g = (unsigned)p.ntiles;
+  if (p.mode == 1)
+    k_seg_gemm_nn_mfma<F><<<g, kWG, 0, s>>>(p.segs.get(), p.tiles.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
+  else if (p.mode == 256)
+    k_seg_gemm_nn<F, 256><<<g, kWG, 0, s>>>(p.segs.get(), p.tiles.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
+  else if (p.mode == 128)
+    k_seg_gemm_nn<F, 128><<<g, kWG, 0, s>>>(p.segs.get(), p.tiles.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
   else
-    k_seg_gemm_nn<F, 64><<<grid, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
+    k_seg_gemm_nn<F, 64><<<g, kWG, 0, s>>>(p.segs.get(), p.tiles.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));  // dt is freed on return
+}
+template void launch_nn<double>(const NNPlan&, const double*, int64_t, const double*, int64_t, double*, int64_t,
+                                double, double, hipStream_t);
+template void launch_nn<float>(const NNPlan&, const float*, int64_t, const float*, int64_t, float*, int64_t, float,
+                               float, hipStream_t);
+
+template <class F>
+void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* X, int64_t ldx,
+                 const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha, F beta, hipStream_t s) {
+  (void)dsegs;
+  NNPlan p;
+  plan_nn(segs, p, s);
+  launch_nn<F>(p, X, ldx, T, ldt, Y, ldy, alpha, beta, s);
+  HIP_CHECK(hipStreamSynchronize(s));  // the plan's buffers are freed on return
 }
 template void seg_gemm_nn<double>(const std::vector<SegNN>&, DevBuf<SegNN>&, const double*, int64_t,
                                   const double*, int64_t, double*, int64_t, double, double, hipStream_t);
@@ -352,12 +362,6 @@ void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const 
 // seg_gemm_tn: O_s = alpha X(seg)^T Y(seg) + beta O_s, two stages:
 // partial 64x64 products over row chunks, then a deterministic sum.
 // --------------------------------------------------------------------------
-struct TNItem {
-  int32_t seg, r0, r1, pt, qt, slot;
-};
-struct TNOut {
-  int32_t seg, pt, qt, slot0, nslot, pad_;
-};
 
 template <class F>
 __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part(const SegTN* __restrict__ segs,
@@ -468,10 +472,8 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part_mfma(const SegTN* __re
     for (int v = 0; v < 4; ++v) pp[(16 * w + MfmaNN<F>::row(lane, v)) + (16 * t + (lane & 15)) * 64] = acc[t][v];
 }
 
-template <class F>
-void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* X, int64_t ldx,
-                 const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta, Workspace& ws,
-                 hipStream_t s) {
+void plan_tn(const std::vector<SegTN>& segs, TNPlan& p, hipStream_t s) {
+  p.nitems = p.nouts = p.nslots = 0;
   if (segs.empty()) return;
   int64_t total_rows = 0;
   for (auto& g : segs) total_rows += std::max(g.len, 0);
@@ -500,19 +502,42 @@ void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* 
       }
   }
   if (outs.empty()) return;
+  p.nitems = (int)items.size();
+  p.nouts = (int)outs.size();
+  p.nslots = slot;
+  p.segs.upload(segs, s);
+  p.items.upload(items, s);
+  p.outs.upload(outs, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <class F>
+void launch_tn(const TNPlan& p, const F* X, int64_t ldx, const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha,
+               F beta, F* part, hipStream_t s) {
+  if (!p.nouts) return;
+  k_seg_gemm_tn_part_mfma<F><<<(unsigned)p.nitems, kWG, 0, s>>>(p.segs.get(), p.items.get(), X, ldx, Y, ldy,
+                                                                 part);
+  HIP_CHECK(hipGetLastError());
+  k_seg_gemm_tn_reduce<F><<<dim3((unsigned)p.nouts, 4096 / kWG), kWG, 0, s>>>(p.segs.get(), p.outs.get(), part, O,
+                                                                             ldo, alpha, beta);
+  HIP_CHECK(hipGetLastError());
+}
+template void launch_tn<double>(const TNPlan&, const double*, int64_t, const double*, int64_t, double*, int64_t,
+                                double, double, double*, hipStream_t);
+template void launch_tn<float>(const TNPlan&, const float*, int64_t, const float*, int64_t, float*, int64_t,
+                               float, float, float*, hipStream_t);
+
+template <class F>
+void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* X, int64_t ldx,
+                 const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta, Workspace& ws,
+                 hipStream_t s) {
+  (void)dsegs;
+  TNPlan p;
+  plan_tn(segs, p, s);
+  if (!p.nouts) return;
   DevBuf<F>& part = ws.parts<F>();
-  part.reserve((size_t)slot * 4096);
-  dsegs.upload(segs, s);
-  DevBuf<TNItem> di;
-  di.upload(items, s);
-  DevBuf<TNOut> dout;
-  dout.upload(outs, s);
-  k_seg_gemm_tn_part_mfma<F><<<(unsigned)items.size(), kWG, 0, s>>>(dsegs.get(), di.get(), X, ldx, Y, ldy,
-                                                                    part.get());
-  HIP_CHECK(hipGetLastError());
-  k_seg_gemm_tn_reduce<F><<<dim3((unsigned)outs.size(), 4096 / kWG), kWG, 0, s>>>(
-      dsegs.get(), dout.get(), part.get(), O, ldo, alpha, beta);
-  HIP_CHECK(hipGetLastError());
+  part.reserve((size_t)p.nslots * 4096);
+  launch_tn<F>(p, X, ldx, Y, ldy, O, ldo, alpha, beta, part.get(), s);
   HIP_CHECK(hipStreamSynchronize(s));
 }
 template void seg_gemm_tn<double>(const std::vector<SegTN>&, DevBuf<SegTN>&, const double*, int64_t,

@@ -80,6 +80,38 @@ struct Workspace {
 template <> inline DevBuf<double>& Workspace::parts<double>() { return part; }
 template <> inline DevBuf<float>& Workspace::parts<float>() { return part32; }
 
+struct NNTile {
+  int32_t seg, row0, c0, pad_;
+};
+struct TNItem {
+  int32_t seg, r0, r1, pt, qt, slot;
+};
+struct TNOut {
+  int32_t seg, pt, qt, slot0, nslot, pad_;
+};
+
+// Launch plans: descriptors built and uploaded once, launched any number of
+// times with no allocation, copy or host synchronisation (graph capture).
+struct NNPlan {
+  DevBuf<SegNN> segs;
+  DevBuf<NNTile> tiles;
+  int ntiles = 0, mode = 0;
+};
+struct TNPlan {
+  DevBuf<SegTN> segs;
+  DevBuf<TNItem> items;
+  DevBuf<TNOut> outs;
+  int nitems = 0, nouts = 0, nslots = 0;  // launch_tn needs nslots * 4096 partials
+};
+void plan_nn(const std::vector<SegNN>& segs, NNPlan& p, hipStream_t s);
+template <class F>
+void launch_nn(const NNPlan& p, const F* X, int64_t ldx, const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha,
+               F beta, hipStream_t s);
+void plan_tn(const std::vector<SegTN>& segs, TNPlan& p, hipStream_t s);
+template <class F>
+void launch_tn(const TNPlan& p, const F* X, int64_t ldx, const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha,
+               F beta, F* part, hipStream_t s);
+
 // F = double or float (FP32 factorisation of the mixed-precision path)
 template <class F>
 void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* X, int64_t ldx,

@@ -85,6 +85,7 @@ def _proto(lib):
         "hmat_hodlr_to_dense": (st, [vp, _D, i64]),
         "hmat_hodlr_set_stream": (st, [vp, vp]),
         "hmat_hodlr_save": (st, [vp, C.c_char_p]),
+        "hmat_hodlr_set_solve_graph": (st, [vp, i32]),
         "hmat_hodlr_load": (st, [vp, C.c_char_p]),
         "hmat_hss_create": (st, [C.POINTER(vp), C.POINTER(Options)]),
         "hmat_hss_destroy": (st, [vp]),
@@ -265,6 +266,10 @@ class HODLRMatrixOMP(_Base):
 
     def solve_device(self, b_ptr, ldb, nrhs, x_ptr, ldx):
         _check(self._lib.hmat_hodlr_solve_device(self._h, b_ptr, ldb, nrhs, x_ptr, ldx))
+
+    def set_solve_graph(self, on):
+        """Replay solves from a captured HIP graph (default) or launch eagerly."""
+        _check(self._lib.hmat_hodlr_set_solve_graph(self._h, int(bool(on))))
 
     def save(self, path):
         """Write the built operator to `path` (see hmat_hodlr_save)."""

@@ -222,6 +222,10 @@ hmat_status hmat_hodlr_to_dense(hmat_hodlr h, double* A, int64_t lda);
  * recomputed on the first solve after a load. */
 hmat_status hmat_hodlr_save(hmat_hodlr h, const char* path);
 hmat_status hmat_hodlr_load(hmat_hodlr h, const char* path);
+/* Solves replay their apply-inverse from a HIP graph recorded once per
+ * factorisation and column count (default on); off launches the same
+ * kernels one by one.  No reference counterpart. */
+hmat_status hmat_hodlr_set_solve_graph(hmat_hodlr h, int32_t on);
 /* Set the HIP stream (hipStream_t) used by the handle; NULL = own stream. */
 hmat_status hmat_hodlr_set_stream(hmat_hodlr h, void* stream);
 

@@ -310,3 +310,32 @@ def test_mixed_precision_ill_conditioned_reports_not_converged():
     g = hmat_amd.HODLRMatrixOMP(opts=of)
     g.build_from_dense(a, 100)
     assert rel(g.solve(b), np.linalg.solve(g.to_dense(), b)) < 1e-9
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_solve_graph_replays_match_eager(mixed):
+    """Solves replayed from the captured graph equal the eagerly launched
+    solve bit for bit, over several replays with eager work in between."""
+    n = 4096
+    xc, yc = kernels.cauchy_grid_points(n)
+    o = hmat_amd.make_options(1e-10, rank=32, oversample=10, power_iters=1, exact_below=0,
+                              precision=hmat_amd.PREC_MIXED if mixed else hmat_amd.PREC_FP64,
+                              omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=9)
+    h = hmat_amd.HODLRMatrixOMP(opts=o)
+    h.build_kernel(hmat_amd.make_kernel(hmat_amd.KERNEL_CAUCHY, xc, yc), n, 128)
+    rng = np.random.default_rng(8)
+    bs = [rng.standard_normal((n, k)) for k in (1, 3, 1, 16, 3)]
+    h.set_solve_graph(False)
+    eager = [h.solve(b) for b in bs]
+    h.set_solve_graph(True)
+    for rep in range(3):
+        for b, xe in zip(bs, eager):
+            h.multiply(b)  # eager work between replays
+            assert np.array_equal(h.solve(b), xe)
+    # the transposed solve keeps its own graph
+    h.set_solve_graph(False)
+    te = h.solve_transpose(bs[1])
+    h.set_solve_graph(True)
+    for rep in range(2):
+        assert np.array_equal(h.solve_transpose(bs[1]), te)
+        assert np.array_equal(h.solve(bs[1]), eager[1])
