@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--matvec-reps", type=int, default=20)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tilings (the autotuner runs in the warm-up step otherwise)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (counter calibration)")
     a = ap.parse_args()
@@ -190,6 +192,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if not a.no_autotune:
+        hmat_amd.autotune_enable(True)
     modes = a.modes.split(",")
     res = {m: run_mode(hmat_amd, torch, m, a) for m in modes}
     med = {m: statistics.median(r["times"]) for m, r in res.items()}
@@ -211,7 +215,8 @@ def main():
         "config": {"workload": f"HODLR Cauchy 1/(x_i-y_j), N={a.n}, leaf {a.leaf}, rank {a.rank}, "
                                f"p {a.oversample}, q {a.q}, 1 RHS, resid<={a.rtol:g}",
                    "N": a.n, "leaf": a.leaf, "rank": a.rank, "oversample": a.oversample,
-                   "power_iters": a.q, "rtol": a.rtol, "parallelism": "single GPU"},
+                   "power_iters": a.q, "rtol": a.rtol, "parallelism": "single GPU",
+                   "autotune": not a.no_autotune},
     }
     for m in modes:
         out[f"{m}_s"] = round(med[m], 5)

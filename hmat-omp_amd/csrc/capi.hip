@@ -80,6 +80,8 @@ int32_t hmat_device_count(void) {
 
 void hmat_profile_enable(int32_t on) { prof_set(on != 0); }
 void hmat_profile_reset(void) { prof_reset(); }
+void hmat_autotune_enable(int32_t on) { autotune_enable(on != 0); }
+int64_t hmat_autotune_entries(void) { return (int64_t)tune_entries(); }
 int32_t hmat_profile_query(const char* name, int64_t* launches, double* ms, double* work) {
   ProfEntry e;
   if (!name || !prof_get(name, &e)) return 0;

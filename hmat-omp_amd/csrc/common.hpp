@@ -114,6 +114,12 @@ class ProfScope {
     (void)hipEventDestroy(b_);
     on_ = false;
   }
+  void cancel() {  // drop this scope's record (untimed work such as autotuning)
+    if (!on_) return;
+    (void)hipEventDestroy(a_);
+    (void)hipEventDestroy(b_);
+    on_ = false;
+  }
   ~ProfScope() {
     if (on_) {
       (void)hipEventDestroy(a_);
@@ -128,6 +134,26 @@ class ProfScope {
   bool on_ = false;
   hipEvent_t a_{}, b_{};
 };
+
+// Tile autotuner: a process-wide table from a launch's shape class to the
+// index of the fastest candidate tiling, filled on first use by timing the
+// candidates (off by default; tuned launches may round differently from the
+// default tiling).
+struct TuneKey {
+  int kind, trans, kpad, rows_lg, red_lg, nseg_lg;
+  bool operator<(const TuneKey& o) const {
+    const int a[6] = {kind, trans, kpad, rows_lg, red_lg, nseg_lg};
+    const int b[6] = {o.kind, o.trans, o.kpad, o.rows_lg, o.red_lg, o.nseg_lg};
+    for (int i = 0; i < 6; ++i)
+      if (a[i] != b[i]) return a[i] < b[i];
+    return false;
+  }
+};
+bool autotune_enabled();
+void autotune_enable(bool on);
+bool tune_lookup(const TuneKey& k, int& choice);
+void tune_store(const TuneKey& k, int choice);
+size_t tune_entries();
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline int round_up(int a, int b) { return (int)((a + b - 1) / b) * b; }

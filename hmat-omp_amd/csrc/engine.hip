@@ -19,6 +19,33 @@ std::map<std::string, ProfEntry>& prof_map() {
 }
 }  // namespace
 
+namespace {
+std::mutex g_tune_mu;
+bool g_tune_on = false;
+std::map<TuneKey, int>& tune_map() {
+  static std::map<TuneKey, int> m;
+  return m;
+}
+}  // namespace
+
+bool autotune_enabled() { return g_tune_on; }
+void autotune_enable(bool on) { g_tune_on = on; }
+bool tune_lookup(const TuneKey& k, int& choice) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = tune_map().find(k);
+  if (it == tune_map().end()) return false;
+  choice = it->second;
+  return true;
+}
+void tune_store(const TuneKey& k, int choice) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_map()[k] = choice;
+}
+size_t tune_entries() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return tune_map().size();
+}
+
 bool prof_enabled() { return g_prof_on; }
 void prof_set(bool on) { g_prof_on = on; }
 void prof_reset() {

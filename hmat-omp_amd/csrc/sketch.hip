@@ -386,7 +386,7 @@ struct SketchPlan {
 };
 
 static SketchPlan plan_sketch(const std::vector<SegSk>& segs, int kcap, int ncols_per_wg, int rows_per_wg,
-                              int64_t ldy) {
+                              int64_t ldy, int64_t want) {
   SketchPlan p;
   p.ncg = (int)ceil_div(round_up(kcap, 16), ncols_per_wg);
   int64_t nrow_tiles = 0, redmax = 0;
@@ -394,8 +394,8 @@ static SketchPlan plan_sketch(const std::vector<SegSk>& segs, int kcap, int ncol
     nrow_tiles += ceil_div(std::max(sg.outlen, 0), rows_per_wg);
     redmax = std::max<int64_t>(redmax, sg.redlen);
   }
-  // split the reduction when the row tiles alone cannot fill 256 CUs
-  const int64_t want = 1024;
+  // split the reduction when the row tiles alone cannot fill the chip
+  // (`want` workgroups: 1024 by default, or the autotuner's choice)
   if (nrow_tiles * p.ncg < want) {
     p.nsplit = (int)std::min<int64_t>(ceil_div(want, std::max<int64_t>(nrow_tiles * p.ncg, 1)),
                                       std::max<int64_t>(1, redmax / 256));
@@ -453,18 +453,14 @@ static void launch_sketch(const Gen& gen, int NT, int RF, dim3 grid, const SegSk
 #undef HMAT_SK
 }
 
-void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs,
-            const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap, Workspace& ws,
-            hipStream_t s) {
-  if (segs.empty() || kcap <= 0) return;
+// One sketch with a given tiling: rows per workgroup 64 * RF, split-K target.
+static void run_sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs,
+                       const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap, Workspace& ws, int RF,
+                       int64_t want, bool prof_on, hipStream_t s) {
   const int kpad = round_up(kcap, 16);
   const int NT = std::min(4, kpad / 16);
-  int64_t rows = 0;
-  for (auto& sg : segs) rows += sg.outlen;
-  // 256-row tiles when the segments are long enough to fill the chip with them
-  const int RF = (rows / 256 >= 512) ? 4 : 2;
   const int rows_per_wg = 64 * RF;
-  SketchPlan p = plan_sketch(segs, kcap, NT * 16, rows_per_wg, ldy);
+  SketchPlan p = plan_sketch(segs, kcap, NT * 16, rows_per_wg, ldy, want);
   if (p.tiles.empty()) return;
   dsegs.upload(segs, s);
   DevBuf<SkTile> dt;
@@ -477,7 +473,8 @@ void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf
   dim3 grid((unsigned)p.tiles.size(), (unsigned)p.ncg);
   double flops = 0.0;
   for (auto& sg : segs) flops += 2.0 * sg.outlen * (double)sg.redlen * sg.k;
-  ProfScope prof(g.kind == GenKind::Dense ? "sketch_dense" : "sketch_gen", s, flops);
+  ProfScope prof(g.kind == GenKind::Dense ? "sketch_dense" : "sketch_gen", s, prof_on ? flops : 0.0);
+  if (!prof_on) prof.cancel();
   with_gen(g, [&](auto gen) {
     if (trans)
       launch_sketch<decltype(gen), true>(gen, NT, RF, grid, dsegs.get(), dt.get(), X, ldx, out, ldy, p.span, s);
@@ -488,6 +485,73 @@ void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf
   prof.finish();
   if (p.nsplit > 1) sum_splits(segs, dsegs, p, ws.part.get(), Y, ldy, s);
   else HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Shape class of a sketch launch for the autotuner.
+static TuneKey sketch_key(int kind, bool trans, const std::vector<SegSk>& segs, int kcap) {
+  int64_t rows = 0, red = 0;
+  for (auto& sg : segs) {
+    rows += sg.outlen;
+    red += sg.redlen;
+  }
+  auto lg = [](int64_t v) { return v > 0 ? 64 - __builtin_clzll((unsigned long long)v) : 0; };
+  return TuneKey{kind, trans ? 1 : 0, round_up(kcap, 16), lg(rows), lg(red / std::max<size_t>(segs.size(), 1)),
+                 lg((int64_t)segs.size())};
+}
+
+// Time `ncand` variants of one launch on the stream and return the fastest
+// (the last variant run leaves its result in place; the caller re-runs the
+// winner when it was not the last).
+template <class Run>
+static int time_candidates(int ncand, Run&& run, hipStream_t s) {
+  hipEvent_t a, b;
+  HIP_CHECK(hipEventCreate(&a));
+  HIP_CHECK(hipEventCreate(&b));
+  int best = 0;
+  float best_ms = 0.f;
+  for (int c = 0; c < ncand; ++c) {
+    HIP_CHECK(hipEventRecord(a, s));
+    run(c);
+    HIP_CHECK(hipEventRecord(b, s));
+    HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    if (c == 0 || ms < best_ms) {
+      best = c;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return best;
+}
+
+void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs,
+            const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap, Workspace& ws,
+            hipStream_t s) {
+  if (segs.empty() || kcap <= 0) return;
+  int64_t rows = 0;
+  for (auto& sg : segs) rows += sg.outlen;
+  // default: 256-row tiles when the segments are long enough to fill the
+  // chip with them, split-K up to 1024 workgroups
+  int RF = (rows / 256 >= 512) ? 4 : 2;
+  int64_t want = 1024;
+  if (autotune_enabled()) {
+    static const int kRF[2] = {2, 4};
+    static const int64_t kWant[3] = {512, 1024, 2048};
+    const TuneKey key = sketch_key(g.kind == GenKind::Dense ? 0 : 1, trans, segs, kcap);
+    int choice = -1;
+    if (!tune_lookup(key, choice)) {
+      choice = time_candidates(6, [&](int c) {
+        run_sketch(g, trans, segs, dsegs, X, ldx, Y, ldy, kcap, ws, kRF[c / 3], kWant[c % 3], false, s);
+      }, s);
+      tune_store(key, choice);
+      if (choice == 5) return;  // the last candidate's result is in Y
+    }
+    RF = kRF[choice / 3];
+    want = kWant[choice % 3];
+  }
+  run_sketch(g, trans, segs, dsegs, X, ldx, Y, ldy, kcap, ws, RF, want, true, s);
 }
 
 template <class Gen, bool TRANS>
@@ -521,32 +585,48 @@ void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>
     }
     HIP_CHECK(hipGetLastError());
   }
-  SketchPlan p = plan_sketch(segs, kcap, 64, 128, ldy);
-  if (p.tiles.empty()) return;
-  dsegs.upload(segs, s);
-  DevBuf<SkTile> dt;
-  dt.upload(p.tiles, s);
-  double* out = Y;
-  if (p.nsplit > 1) {
-    ws.part.reserve((size_t)(p.span * p.nsplit));
-    out = ws.part.get();
+  auto run = [&](int64_t want, bool prof_on) {
+    SketchPlan p = plan_sketch(segs, kcap, 64, 128, ldy, want);
+    if (p.tiles.empty()) return;
+    dsegs.upload(segs, s);
+    DevBuf<SkTile> dt;
+    dt.upload(p.tiles, s);
+    double* out = Y;
+    if (p.nsplit > 1) {
+      ws.part.reserve((size_t)(p.span * p.nsplit));
+      out = ws.part.get();
+    }
+    dim3 grid((unsigned)p.tiles.size(), (unsigned)p.ncg);
+    double flops = 0.0;
+    for (auto& sg : segs) flops += 2.0 * sg.outlen * (double)sg.redlen * sg.k;
+    ProfScope prof("sketch_lp", s, flops);
+    if (!prof_on) prof.cancel();
+    with_gen_lp(g, [&](auto gen) {
+      if (trans)
+        launch_sketch_lp<decltype(gen), true>(gen, split, grid, dsegs.get(), dt.get(), ws.bf.get(), ldx, ps, out,
+                                              ldy, p.span, s);
+      else
+        launch_sketch_lp<decltype(gen), false>(gen, split, grid, dsegs.get(), dt.get(), ws.bf.get(), ldx, ps,
+                                               out, ldy, p.span, s);
+    });
+    HIP_CHECK(hipGetLastError());
+    prof.finish();
+    if (p.nsplit > 1) sum_splits(segs, dsegs, p, ws.part.get(), Y, ldy, s);
+    else HIP_CHECK(hipStreamSynchronize(s));
+  };
+  int64_t want = 1024;
+  if (autotune_enabled()) {
+    static const int64_t kWant[3] = {512, 1024, 2048};
+    const TuneKey key = sketch_key(2 + split, trans, segs, kcap);
+    int choice = -1;
+    if (!tune_lookup(key, choice)) {
+      choice = time_candidates(3, [&](int c) { run(kWant[c], false); }, s);
+      tune_store(key, choice);
+      if (choice == 2) return;  // the last candidate's result is in Y
+    }
+    want = kWant[choice];
   }
-  dim3 grid((unsigned)p.tiles.size(), (unsigned)p.ncg);
-  double flops = 0.0;
-  for (auto& sg : segs) flops += 2.0 * sg.outlen * (double)sg.redlen * sg.k;
-  ProfScope prof("sketch_lp", s, flops);
-  with_gen_lp(g, [&](auto gen) {
-    if (trans)
-      launch_sketch_lp<decltype(gen), true>(gen, split, grid, dsegs.get(), dt.get(), ws.bf.get(), ldx, ps, out,
-                                            ldy, p.span, s);
-    else
-      launch_sketch_lp<decltype(gen), false>(gen, split, grid, dsegs.get(), dt.get(), ws.bf.get(), ldx, ps, out,
-                                             ldy, p.span, s);
-  });
-  HIP_CHECK(hipGetLastError());
-  prof.finish();
-  if (p.nsplit > 1) sum_splits(segs, dsegs, p, ws.part.get(), Y, ldy, s);
-  else HIP_CHECK(hipStreamSynchronize(s));
+  run(want, true);
 }
 
 }  // namespace kern

@@ -58,6 +58,8 @@ def _proto(lib):
         "hmat_options_reference": (None, [C.POINTER(Options), C.c_double]),
         "hmat_profile_enable": (None, [i32]),
         "hmat_profile_reset": (None, []),
+        "hmat_autotune_enable": (None, [i32]),
+        "hmat_autotune_entries": (i64, []),
         "hmat_profile_query": (i32, [C.c_char_p, C.POINTER(i64), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
         "hmat_hodlr_create": (st, [C.POINTER(vp), C.POINTER(Options)]),
@@ -161,6 +163,15 @@ def make_kernel(ktype, x, y=None, p0=0.0, p1=0.0):
     k = Kernel(ktype, p0, p1, _ptr(x), _ptr(y))
     k._keep = (x, y)
     return k
+
+
+def autotune_enable(on=True):
+    """Time candidate tilings on the first launch of each shape class."""
+    load().hmat_autotune_enable(int(bool(on)))
+
+
+def autotune_entries():
+    return load().hmat_autotune_entries()
 
 
 def profile_enable(on=True):

@@ -111,6 +111,13 @@ void hmat_options_reference(hmat_options* opts, double tol);
  *   "sketch_gen" / "sketch_dense"  FP64 flops of the block sketch GEMMs. */
 void hmat_profile_enable(int32_t on);
 void hmat_profile_reset(void);
+/* Tile autotuner (no reference counterpart): when on, the first sketch
+ * launch of each shape class times its candidate tilings (rows per
+ * workgroup, split-K target) and the fastest is kept for the process.
+ * Off by default; tuned launches may round differently from the default
+ * tiling.  entries() = number of shape classes decided so far. */
+void hmat_autotune_enable(int32_t on);
+int64_t hmat_autotune_entries(void);
 /* returns 1 and fills the totals when `name` was recorded, else 0 */
 int32_t hmat_profile_query(const char* name, int64_t* launches, double* ms, double* work);
 

@@ -119,3 +119,22 @@ def test_batch_save_load(tmp_path):
     g = hmat_amd.HODLRMatrixOMP(1e-3)
     g.load(f)
     assert np.array_equal(g.to_dense(), h.to_dense())
+
+
+def test_autotuned_batch_build_matches_default():
+    """With the tile autotuner on, the batch operator equals the default
+    build up to rounding (the tuned tilings only regroup the same sums)."""
+    sizes, sigmas, nrhs, pts = _batch(24, 1024, 4096, 12)
+    hd, _ = _build(sizes, sigmas, pts, 1e-2)
+    a0 = hd.to_dense()
+    hmat_amd.autotune_enable(True)
+    try:
+        before = hmat_amd.autotune_entries()
+        h1, _ = _build(sizes, sigmas, pts, 1e-2)   # tunes new shape classes
+        assert hmat_amd.autotune_entries() > before
+        h2, _ = _build(sizes, sigmas, pts, 1e-2)   # reuses the table
+        a1, a2 = h1.to_dense(), h2.to_dense()
+    finally:
+        hmat_amd.autotune_enable(False)
+    assert rel(a1, a0) < 1e-12
+    assert np.array_equal(a1, a2)
