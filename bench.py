@@ -40,6 +40,9 @@ COPY_PROBE_BYTES = 256 << 20
 # profiles/pmc_traffic_hodlr_cauchy_n262144_mixed.txt
 PMC_TRAFFIC = {"sketch_lp": 1.121e9, "matvec": 1.0396e9 + 1.5501e9}
 PMC_SOURCE = "profiles/pmc_traffic_hodlr_cauchy_n262144_mixed.txt"
+KERNEL_LABEL = {"sketch_gen": "k_sketch (FP64 MFMA, generated entries)",
+                "sketch_lp": "k_sketch_lp<3> (BF16x3 MFMA, last range pass)",
+                "sketch_lp1": "k_sketch_lp<1> (BF16 MFMA, power-iteration passes)"}
 
 
 def log(msg):
@@ -88,7 +91,7 @@ def run_mode(hmat_amd, torch, mode, a):
             resid.append(r)
     hmat_amd.profile_enable(False)
     prof = {}
-    for name in ("sketch_gen", "sketch_lp", "jacobi_sweeps", "jacobi_mats"):
+    for name in ("sketch_gen", "sketch_lp", "sketch_lp1", "jacobi_sweeps", "jacobi_mats"):
         q = hmat_amd.profile_query(name)
         if q:
             prof[name] = {"launches": q[0], "ms": q[1], "work": q[2]}
@@ -210,7 +213,8 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "bf16x3+f32 factor+f64 IR" if head == "mixed" else "f64",
+        "dtype": ("bf16 power passes, bf16x3 last range pass, f64 projection, f32 factor, f64 IR"
+                  if head == "mixed" else "f64"),
         "data": "synthetic (deterministic Cauchy grid, b ~ N(0,1) seed 0)",
         "config": {"workload": f"HODLR Cauchy 1/(x_i-y_j), N={a.n}, leaf {a.leaf}, rank {a.rank}, "
                                f"p {a.oversample}, q {a.q}, 1 RHS, resid<={a.rtol:g}",
@@ -238,8 +242,7 @@ def main():
         p = prof[name]
         achieved = p["work"] / (p["ms"] * 1e-3) / 1e12
         peak = PEAK_FP64_TFLOPS if name == "sketch_gen" else PEAK_BF16_TFLOPS
-        out["roofline"] = {"kernel": "k_sketch (FP64 MFMA, generated entries)" if name == "sketch_gen"
-                           else "k_sketch_lp (BF16 MFMA)", "bound": "mfma",
+        out["roofline"] = {"kernel": KERNEL_LABEL[name], "bound": "mfma",
                            "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                            "frac": round(achieved / peak, 4),
                            "traffic": PMC_TRAFFIC.get(name) if a.n == 1 << 18 else None,

@@ -290,7 +290,7 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
   kern::fill_identity_segs(idx_row, dnn_, Yb_.get(), N, stream);
 
   // descriptors
-  std::vector<SegSk> sk_fwd, sk_fwd_rsvd, sk_bwd_rsvd, sk_bwd;
+  std::vector<SegSk> sk_fwd, sk_fwd_rsvd, sk_fwd_exact, sk_bwd_rsvd, sk_bwd;
   std::vector<OrthSeg> orow_rsvd, orow, ocol_rsvd, ocol;
   for (int s = 0; s < ns; ++s) {
     const SegSk fwd{st[s], sst[s], 0, sst[s], st[s], (int32_t)ln[s], (int32_t)sln[s], 0, k[s]};
@@ -299,6 +299,7 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
     ocol.push_back(OrthSeg{sst[s], (int32_t)sln[s], k[s]});
     if (mode[s] != BlockMode::ExactWide) {
       sk_fwd.push_back(fwd);
+      if (mode[s] != BlockMode::RSVD) sk_fwd_exact.push_back(fwd);
       orow.push_back(OrthSeg{st[s], (int32_t)ln[s], k[s]});
     }
     if (mode[s] == BlockMode::RSVD) {
@@ -310,21 +311,30 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
   }
 
   // Y = A Omega, power iterations, Q = orth(Y)      (compression.h:78-84)
-  // Mixed precision runs these range-finding passes on BF16 MFMA; the
-  // projection B^T = A^T Q below stays FP64 in both modes.
+  // Mixed precision runs these range-finding passes on BF16 MFMA: the last
+  // pass before Q with kLowPrecSplit terms, the power-iteration passes with
+  // kPowerSplit.  The projection B^T = A^T Q below stays FP64 in both modes.
   const bool lp = (opt.precision == HMAT_PREC_MIXED);
-  auto range_pass = [&](bool trans, const std::vector<SegSk>& sk, const double* X, double* Y) {
+  auto range_pass = [&](bool trans, const std::vector<SegSk>& sk, const double* X, double* Y, bool last) {
     if (lp)
-      kern::sketch_lp(g, trans, kLowPrecSplit, sk, dsk_, X, N, Y, N, kcap, ws_, stream);
+      kern::sketch_lp(g, trans, last ? kLowPrecSplit : kPowerSplit, sk, dsk_, X, N, Y, N, kcap, ws_, stream);
     else
       kern::sketch(g, trans, sk, dsk_, X, N, Y, N, kcap, ws_, stream);
   };
-  range_pass(false, sk_fwd, Xb_.get(), Yb_.get());
-  for (int it = 0; it < opt.power_iters && !sk_fwd_rsvd.empty(); ++it) {
+  const int q = sk_fwd_rsvd.empty() ? 0 : opt.power_iters;
+  if (lp && q > 0) {
+    // the exact-branch blocks get no power iterations: their first pass is
+    // their last
+    range_pass(false, sk_fwd_rsvd, Xb_.get(), Yb_.get(), false);
+    range_pass(false, sk_fwd_exact, Xb_.get(), Yb_.get(), true);
+  } else {
+    range_pass(false, sk_fwd, Xb_.get(), Yb_.get(), true);
+  }
+  for (int it = 0; it < q; ++it) {
     orth_.run(orow_rsvd, Yb_.get(), Tb_.get(), N, kcap, nullptr, stream);
-    range_pass(true, sk_bwd_rsvd, Yb_.get(), Xb_.get());
+    range_pass(true, sk_bwd_rsvd, Yb_.get(), Xb_.get(), false);
     orth_.run(ocol_rsvd, Xb_.get(), Tb_.get(), N, kcap, nullptr, stream);
-    range_pass(false, sk_fwd_rsvd, Xb_.get(), Yb_.get());
+    range_pass(false, sk_fwd_rsvd, Xb_.get(), Yb_.get(), it == q - 1);
   }
   orth_.run(orow, Yb_.get(), Tb_.get(), N, kcap, nullptr, stream);
   // B^T = A^T Q (n x k), then B^T = Qb R                 (compression.h:87)

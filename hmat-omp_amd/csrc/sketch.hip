@@ -599,7 +599,8 @@ void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>
     dim3 grid((unsigned)p.tiles.size(), (unsigned)p.ncg);
     double flops = 0.0;
     for (auto& sg : segs) flops += 2.0 * sg.outlen * (double)sg.redlen * sg.k;
-    ProfScope prof("sketch_lp", s, flops);
+    static const char* const kProfName[3] = {"sketch_lp1", "sketch_lp2", "sketch_lp"};
+    ProfScope prof(kProfName[split - 1], s, flops);
     if (!prof_on) prof.cancel();
     with_gen_lp(g, [&](auto gen) {
       if (trans)
