@@ -28,6 +28,9 @@ SOURCES = ["kernels.hip", "sketch.hip", "matvec.hip", "engine.hip", "hodlr.hip",
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs: no AGPR round trips around the FP32->FP64
+    # flushes and fewer registers in every MFMA kernel
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall", "-Wno-unused-result", "-Wno-unused-function",
     "-I", os.path.join(HERE, "..", "include"),
 ]

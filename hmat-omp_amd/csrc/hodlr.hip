@@ -709,18 +709,29 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
   ir_n_.reserve((size_t)2 * nrhs);
   kern::col_norms2(dB, ldb, N, nrhs, ir_n_.get(), stream);
   auto bn = download(ir_n_.get(), nrhs, stream);
-  double resid = 0.0;
+  double resid = 0.0, prev = 0.0;
   int it = 0;
+  const bool mixed = opt.precision == HMAT_PREC_MIXED;
   for (;; ++it) {
     matvec(dX, ldx, nrhs, ir_y_.get(), N);
     kern::axpby_cols(N, nrhs, 1.0, dB, ldb, -1.0, ir_y_.get(), N, ir_r_.get(), N, stream);
     kern::col_norms2(ir_r_.get(), N, N, nrhs, ir_n_.get(), stream);
     auto rn = download(ir_n_.get(), nrhs, stream);
     resid = 0.0;
-    for (int j = 0; j < nrhs; ++j)
-      resid = std::max(resid, bn[j] > 0.0 ? std::sqrt(rn[j] / bn[j]) : std::sqrt(rn[j]));
-    if (!std::isfinite(resid)) throw Error(HMAT_ERR_NONFINITE, "non-finite residual in refinement");
+    for (int j = 0; j < nrhs; ++j) {
+      const double rj = bn[j] > 0.0 ? std::sqrt(rn[j] / bn[j]) : std::sqrt(rn[j]);
+      // (std::max would drop a NaN.)  In mixed precision a non-finite
+      // residual means the FP32 factorisation cannot solve this system.
+      if (!std::isfinite(rj))
+        throw Error(mixed ? HMAT_ERR_NOT_CONVERGED : HMAT_ERR_NONFINITE,
+                    "non-finite residual after " + std::to_string(it) + " refinement iterations" +
+                        (mixed ? " (the FP32 factorisation is too inaccurate for this system)" : ""));
+      resid = std::max(resid, rj);
+    }
     if (resid <= refine_rtol || it >= refine_max) break;
+    // a correction that did not reduce the residual: the iteration diverges
+    if (it > 0 && resid >= prev) break;
+    prev = resid;
     // correction d = A_H^{-1} r, accumulated into X in FP64
     apply_inverse(ir_r_.get(), N, nrhs, ir_y_.get(), N);
     kern::axpby_cols(N, nrhs, 1.0, dX, ldx, 1.0, ir_y_.get(), N, dX, ldx, stream);
@@ -728,7 +739,7 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
   HIP_CHECK(hipStreamSynchronize(stream));
   last_iters = it;
   last_resid = resid;
-  if (resid > refine_rtol && opt.precision == HMAT_PREC_MIXED)
+  if (resid > refine_rtol && mixed)
     throw Error(HMAT_ERR_NOT_CONVERGED, "refinement stopped at relative residual " +
                                             std::to_string(resid) + " after " + std::to_string(it) +
                                             " iterations");

@@ -187,13 +187,28 @@ __device__ inline void split_bf16_bits(float v, uint32_t (&o)[3]) {
   }
 }
 
+// No K-tile of 64 reduction rows starting at kb meets the excluded interval.
+__device__ inline bool ktile_clean(const SegSk& sg, int kb) {
+  if (sg.excllen <= 0) return true;
+  const int64_t lo = sg.red0 + kb, hi = lo + 64;
+  return hi <= sg.excl0 || lo >= sg.excl0 + sg.excllen;
+}
+
+// Rows past the segment end and reduction rows past the tile end are clamped
+// to the last valid index, so every generated entry is a finite entry of A:
+// the rows are never stored and the X rows are zero in LDS.  Only the
+// excluded interval has to be masked, by selection (a non-finite entry
+// there, such as a diagonal singularity, must not reach the product).
 template <class Gen, bool TRANS, int SPLIT>
 __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __restrict__ segs,
                                                     const SkTile* __restrict__ tiles,
                                                     const __bf16* __restrict__ Xs, int64_t ldx,
                                                     int64_t ps, double* __restrict__ Y, int64_t ldy,
                                                     int64_t split_stride) {
-  constexpr int RF = 2, NT = 4, NC = 64, KT = 64, KP = KT + 8, FLUSH = 8;
+  constexpr int RF = 2, NT = 4, NC = 64, KT = 64, KP = KT + 8;
+  // the single-term pass keeps FP32 sums (its error is that of BF16 entries);
+  // the three-term pass flushes them to FP64 every 8 K-tiles
+  constexpr int FLUSH = SPLIT > 1 ? 8 : 0;
   __shared__ __attribute__((aligned(16))) __bf16 xs[SPLIT][NC][KP];
   __shared__ __attribute__((aligned(16))) float rp[KT];
   __shared__ __attribute__((aligned(16))) float rm[KT];
@@ -202,7 +217,6 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
   const int cbase = blockIdx.y * NC;
   const int kcols = min(NC, sg.k - cbase);
-  const bool rows_full = tl.row0 + 128 <= sg.outlen;
   const bool aligned = ((sg.xoff & 7) == 0) && ((ldx & 7) == 0);
   mfma::v4d acc64[RF][NT];
   mfma::v4f acc32[RF][NT];
@@ -214,13 +228,12 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
       acc32[f][t] = mfma::v4f{0.f, 0.f, 0.f, 0.f};
     }
   int64_t iglob[RF];
-  float opt[RF], rmask[RF];
+  float opt[RF];
 #pragma unroll
   for (int f = 0; f < RF; ++f) {
     const int rloc = tl.row0 + w * 32 + f * 16 + (lane % 16);
     iglob[f] = sg.out0 + min(rloc, max(sg.outlen - 1, 0));
     opt[f] = TRANS ? gen.col_pt(iglob[f]) : gen.row_pt(iglob[f]);
-    rmask[f] = rloc < sg.outlen ? 1.f : 0.f;
   }
   const int kq = lane / 16;
   int nt = 0;
@@ -273,45 +286,51 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
       rm[threadIdx.x] = pok;
     }
   };
-  commit(tl.rb, issue(tl.rb));
-  __syncthreads();
-  for (int kb = tl.rb; kb < tl.re; kb += KT) {
-    const bool kfull = ktile_full(sg, kb, tl.re);
-    const bool has_next = kb + KT < tl.re;
-    const bool vec_next = has_next ? issue(kb + KT) : false;
-    const bool full = rows_full && kfull;
+  // One K-tile: generate the A fragments (two entries per 32-bit word, word
+  // jp of term sp holding entries 2jp, 2jp+1) and multiply.
+  auto tile = [&](int kb, auto masked) {
 #pragma unroll
     for (int ks = 0; ks < KT / 32; ++ks) {
       const int k0 = ks * 32 + 8 * kq;
-      float pts[8], msk[8];
+      float pts[8];
+      bool okk[8];
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) pts[jj] = rp[k0 + jj];
-      if (!full) {
+      if constexpr (decltype(masked)::value) {
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) msk[jj] = rm[k0 + jj];
+        for (int jj = 0; jj < 8; ++jj) okk[jj] = rm[k0 + jj] != 0.f;
       }
-      // generated operand: two entries per 32-bit word (upper halves of
-      // their split terms), word jp of plane sp holds entries 2jp, 2jp+1
       mfma::v8bf a[RF][SPLIT];
 #pragma unroll
       for (int f = 0; f < RF; ++f) {
-        v4u w[SPLIT];
+        v4u wd[SPLIT];
 #pragma unroll
         for (int jp = 0; jp < 4; ++jp) {
-          uint32_t o[2][3];
+          float val[2];
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             const int jj = 2 * jp + h;
             const int64_t jg = sg.red0 + min(kb + k0 + jj, tl.re - 1);
-            float val = TRANS ? gen.eval(pts[jj], opt[f], jg, iglob[f]) : gen.eval(opt[f], pts[jj], iglob[f], jg);
-            if (!full) val *= msk[jj] * rmask[f];
-            split_bf16_bits<SPLIT>(val, o[h]);
+            val[h] = TRANS ? gen.eval(pts[jj], opt[f], jg, iglob[f]) : gen.eval(opt[f], pts[jj], iglob[f], jg);
+            if constexpr (decltype(masked)::value) val[h] = okk[jj] ? val[h] : 0.f;
           }
+          if constexpr (SPLIT == 1) {
+            // the leading term of the exact split (truncation): the upper
+            // halves of both words in one v_perm.  Rounding to nearest has
+            // the same cost and the same error on scattered points, but on
+            // an equispaced Cauchy grid (a Toeplitz matrix) it made the
+            // final basis 6x less accurate.
+            wd[0][jp] = __builtin_amdgcn_perm(__float_as_uint(val[1]), __float_as_uint(val[0]), 0x07060302u);
+          } else {
+            uint32_t o[2][3];
+            split_bf16_bits<SPLIT>(val[0], o[0]);
+            split_bf16_bits<SPLIT>(val[1], o[1]);
 #pragma unroll
-          for (int sp = 0; sp < SPLIT; ++sp) w[sp][jp] = __builtin_amdgcn_perm(o[1][sp], o[0][sp], 0x07060302u);
+            for (int sp = 0; sp < SPLIT; ++sp) wd[sp][jp] = __builtin_amdgcn_perm(o[1][sp], o[0][sp], 0x07060302u);
+          }
         }
 #pragma unroll
-        for (int sp = 0; sp < SPLIT; ++sp) a[f][sp] = __builtin_bit_cast(mfma::v8bf, w[sp]);
+        for (int sp = 0; sp < SPLIT; ++sp) a[f][sp] = __builtin_bit_cast(mfma::v8bf, wd[sp]);
       }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
@@ -327,22 +346,34 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
             for (int f = 0; f < RF; ++f) acc32[f][t] = mfma::bf16_16x16x32(a[f][sa], b[sb], acc32[f][t]);
       }
     }
+  };
+  commit(tl.rb, issue(tl.rb));
+  __syncthreads();
+  for (int kb = tl.rb; kb < tl.re; kb += KT) {
+    const bool has_next = kb + KT < tl.re;
+    const bool vec_next = has_next ? issue(kb + KT) : false;
+    if (ktile_clean(sg, kb))
+      tile(kb, std::false_type{});
+    else
+      tile(kb, std::true_type{});
     __syncthreads();
     if (has_next) {
       commit(kb + KT, vec_next);
       __syncthreads();
     }
-    if (++nt == FLUSH) {
-      nt = 0;
+    if constexpr (FLUSH > 0) {
+      if (++nt == FLUSH) {
+        nt = 0;
 #pragma unroll
-      for (int f = 0; f < RF; ++f)
+        for (int f = 0; f < RF; ++f)
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
+          for (int t = 0; t < NT; ++t)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            acc64[f][t][v] += (double)acc32[f][t][v];
-            acc32[f][t][v] = 0.f;
-          }
+            for (int v = 0; v < 4; ++v) {
+              acc64[f][t][v] += (double)acc32[f][t][v];
+              acc32[f][t][v] = 0.f;
+            }
+      }
     }
   }
   double* ybase = Y + (int64_t)tl.split * split_stride + sg.yoff;
