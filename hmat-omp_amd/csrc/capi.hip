@@ -192,6 +192,7 @@ hmat_status hmat_kernel_matvec_device(const hmat_kernel* k, int64_t N, const dou
       kern::sketch(g, false, seg, ds, dX, ldx, dY, ldy, nc, ws, s);
     }
     HIP_CHECK(hipStreamSynchronize(s));
+    stager_release(s);
     HIP_CHECK(hipStreamDestroy(s));
   });
 }
@@ -359,6 +360,7 @@ hmat_status hmat_hodlr_to_dense(hmat_hodlr h, double* A, int64_t lda) {
 hmat_status hmat_hodlr_set_stream(hmat_hodlr h, void* stream) {
   return guard([&] {
     HMAT_REQUIRE(h, HMAT_ERR_ARG, "NULL handle");
+    if (h->eng.stream) stager_release(h->eng.stream);
     if (h->eng.own_stream && h->eng.stream) HIP_CHECK(hipStreamDestroy(h->eng.stream));
     if (stream) {
       h->eng.stream = (hipStream_t)stream;

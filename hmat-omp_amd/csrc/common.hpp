@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/hmat_amd.h"
@@ -30,6 +31,44 @@ inline void hip_check(hipError_t e, const char* what, const char* file, int line
 #define HMAT_REQUIRE(cond, code, msg) \
   do { if (!(cond)) throw ::hmat::Error((code), (msg)); } while (0)
 
+// Per-stream staging arena for the small descriptor arrays that every
+// level-batched launch reads (segments, tiles, work items): a pinned host
+// ring and a device ring of the same size.  put_bytes() copies the host data
+// into the pinned ring, enqueues the DMA to the device ring on the stream and
+// returns the device address, with no allocation and no host
+// synchronisation.  Regions are handed out in order; when the ring is full
+// the stream is synchronised once and filling restarts at the front, so no
+// pinned byte is rewritten before its DMA has run, and device bytes are only
+// rewritten by DMAs ordered after every kernel enqueued before them.  All
+// users of one arena must enqueue on its stream, and staged addresses must
+// not be recorded into a graph.
+class Stager {
+ public:
+  explicit Stager(hipStream_t s) : s_(s) {}
+  ~Stager();
+  Stager(const Stager&) = delete;
+  Stager& operator=(const Stager&) = delete;
+  void* put_bytes(const void* h, size_t bytes);
+  // DMA `bytes` from `h` (pageable, may be freed on return) to `dst`
+  void copy_to(void* dst, const void* h, size_t bytes);
+
+ private:
+  size_t reserve(size_t need);  // offset of `need` free bytes
+  hipStream_t s_;
+  char* host_ = nullptr;
+  char* dev_ = nullptr;
+  size_t cap_ = 0, off_ = 0;
+  std::vector<std::pair<char*, char*>> retired_;  // outgrown arenas, freed at the next sync
+};
+Stager& stager(hipStream_t s);
+// Drop the stream's arena (after synchronising the stream); called before a
+// stream the engine owns is destroyed or replaced.
+void stager_release(hipStream_t s);
+template <class T>
+inline const T* stage(const std::vector<T>& v, hipStream_t s) {
+  return v.empty() ? nullptr : static_cast<const T*>(stager(s).put_bytes(v.data(), v.size() * sizeof(T)));
+}
+
 // Owning device buffer (typed, 256-byte aligned by hipMalloc).
 template <class T>
 class DevBuf {
@@ -53,9 +92,14 @@ class DevBuf {
   // grow-only resize (keeps the allocation when large enough)
   void reserve(size_t n) { if (n > n_) resize(n); }
   void zero(hipStream_t s) { if (n_) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(T), s)); }
+  // asynchronous through the stream's staging arena: `h` may be freed on
+  // return, and no host synchronisation happens unless the buffer grows
   void upload(const T* h, size_t n, hipStream_t s) {
-    reserve(n);
-    if (n) HIP_CHECK(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    if (n > n_) {
+      if (p_) HIP_CHECK(hipStreamSynchronize(s));
+      resize(n);
+    }
+    if (n) stager(s).copy_to(p_, h, n * sizeof(T));
   }
   void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
   T* get() const { return p_; }

@@ -4,7 +4,9 @@
 
 #include "engine.hpp"
 
+#include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 
@@ -27,6 +29,73 @@ std::map<TuneKey, int>& tune_map() {
   return m;
 }
 }  // namespace
+
+namespace {
+std::mutex g_stage_mu;
+std::map<hipStream_t, std::unique_ptr<Stager>>& stage_map() {
+  static std::map<hipStream_t, std::unique_ptr<Stager>> m;
+  return m;
+}
+constexpr size_t kStageBytes = (size_t)4 << 20;  // grown on demand
+}  // namespace
+
+Stager& stager(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  auto& p = stage_map()[s];
+  if (!p) p.reset(new Stager(s));
+  return *p;
+}
+void stager_release(hipStream_t s) {
+  std::unique_ptr<Stager> p;
+  {
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    auto it = stage_map().find(s);
+    if (it == stage_map().end()) return;
+    p = std::move(it->second);
+    stage_map().erase(it);
+  }
+  (void)hipStreamSynchronize(s);
+}
+
+Stager::~Stager() {
+  (void)hipStreamSynchronize(s_);
+  for (auto& r : retired_) (void)hipHostFree(r.first), (void)hipFree(r.second);
+  if (host_) (void)hipHostFree(host_);
+  if (dev_) (void)hipFree(dev_);
+}
+
+size_t Stager::reserve(size_t need) {
+  if (need > cap_) {
+    // a bigger arena; the old one stays valid for what is already enqueued
+    // or staged for the next launch, and is freed at the next synchronisation
+    if (host_) retired_.push_back({host_, dev_});
+    cap_ = std::max(kStageBytes, 2 * need);
+    HIP_CHECK(hipHostMalloc((void**)&host_, cap_, hipHostMallocDefault));
+    HIP_CHECK(hipMalloc((void**)&dev_, cap_));
+    off_ = 0;
+  } else if (off_ + need > cap_) {
+    HIP_CHECK(hipStreamSynchronize(s_));
+    for (auto& r : retired_) HIP_CHECK(hipHostFree(r.first)), HIP_CHECK(hipFree(r.second));
+    retired_.clear();
+    off_ = 0;
+  }
+  const size_t o = off_;
+  off_ += need;
+  return o;
+}
+
+void* Stager::put_bytes(const void* h, size_t bytes) {
+  const size_t o = reserve((bytes + 255) & ~(size_t)255);
+  std::memcpy(host_ + o, h, bytes);
+  HIP_CHECK(hipMemcpyAsync(dev_ + o, host_ + o, bytes, hipMemcpyHostToDevice, s_));
+  return dev_ + o;
+}
+
+void Stager::copy_to(void* dst, const void* h, size_t bytes) {
+  const size_t o = reserve((bytes + 255) & ~(size_t)255);
+  std::memcpy(host_ + o, h, bytes);
+  HIP_CHECK(hipMemcpyAsync(dst, host_ + o, bytes, hipMemcpyHostToDevice, s_));
+}
 
 bool autotune_enabled() { return g_tune_on; }
 void autotune_enable(bool on) { g_tune_on = on; }
@@ -114,25 +183,67 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
   if (segs.empty()) return;
   const size_t nb = segs.size();
   const int64_t kk = (int64_t)kcap * kcap;
-  std::vector<SegNN> acc(nb);
-  for (size_t b = 0; b < nb; ++b)
-    acc[b] = SegNN{(int64_t)b * kk, (int64_t)b * kk, (int64_t)b * kk, segs[b].k, segs[b].k, segs[b].k, 0};
-  if (Rout) {
-    Racc_.reserve(nb * kk);
-    Rtmp_.reserve(nb * kk);
+  G_.reserve(nb * kk);
+  R_.reserve(nb * kk);
+  Ri_.reserve(nb * kk);
+  Racc_.reserve(nb * kk);
+  if (Rout) Rtmp_.reserve(nb * kk);
+  const size_t info_bytes = (3 * nb * sizeof(int) + 255) & ~(size_t)255;
+  chk_.reserve(info_bytes + 3 * nb * sizeof(double));
+  int* info = reinterpret_cast<int*>(chk_.get());
+  double* gerr = reinterpret_cast<double*>(chk_.get() + info_bytes);
+  std::vector<SegTN> tn(nb);
+  std::vector<SmallMat> sm(nb);
+  std::vector<SegNN> nn(nb), acc(nb);
+  std::vector<double> coef(nb), zero(nb, 0.0);
+  for (size_t b = 0; b < nb; ++b) {
+    const auto& g = segs[b];
+    tn[b] = SegTN{g.off, g.off, (int64_t)b * kk, g.len, g.k, g.k, 0};
+    sm[b] = SmallMat{(int64_t)b * kk, g.k, kcap};
+    coef[b] = 11.0 * ((double)g.len * g.k + (double)g.k * (g.k + 1)) * 1.1102230246251565e-16;
+    nn[b] = SegNN{g.off, (int64_t)b * kk, g.off, g.len, g.k, g.k, 0};
+    acc[b] = SegNN{(int64_t)b * kk, (int64_t)b * kk, (int64_t)b * kk, g.k, g.k, g.k, 0};
+  }
+  // Enqueued back to back, with one download of all three status arrays:
+  // shifted pass Y -> T (R_1 in Racc_), unshifted pass T -> Y (R_2 in R_),
+  // and the Gram matrix of the result with its deviation from I.
+  kern::seg_gemm_tn(tn, dtn_, Y, ld, Y, ld, G_.get(), kcap, 1.0, 0.0, ws_, s);
+  kern::chol_rinv(sm, dsm_, G_.get(), Racc_.get(), nullptr, coef, shift_, info, gerr, s);
+  kern::seg_trsm_upper(nn, dnn_, Y, ld, Racc_.get(), kcap, T, ld, s);
+  kern::seg_gemm_tn(tn, dtn_, T, ld, T, ld, G_.get(), kcap, 1.0, 0.0, ws_, s);
+  kern::chol_rinv(sm, dsm_, G_.get(), R_.get(), nullptr, zero, shift_, info + nb, gerr + nb, s);
+  kern::seg_trsm_upper(nn, dnn_, T, ld, R_.get(), kcap, Y, ld, s);
+  kern::seg_gemm_tn(tn, dtn_, Y, ld, Y, ld, G_.get(), kcap, 1.0, 0.0, ws_, s);
+  kern::chol_rinv(sm, dsm_, G_.get(), Ri_.get(), nullptr, zero, shift_, info + 2 * nb, gerr + 2 * nb, s);
+  const auto h = download(chk_.get(), info_bytes + 3 * nb * sizeof(double), s);
+  const int* hi = reinterpret_cast<const int*>(h.data());
+  const double* hg = reinterpret_cast<const double*>(h.data() + info_bytes);
+  bool breakdown2 = false, again = false;
+  for (size_t b = 0; b < nb; ++b) {
+    if (hi[b] < 0) throw Error(HMAT_ERR_NONFINITE, "orthogonalisation: non-finite Gram matrix");
+    if (hi[b] > 0) throw Error(HMAT_ERR_NONFINITE, "orthogonalisation: shifted Cholesky broke down");
+    if (hi[nb + b] != 0) breakdown2 = true;
+    if (hi[2 * nb + b] != 0 || !(hg[2 * nb + b] < kOrthTol)) again = true;
   }
   double* src = Y;
   double* dst = T;
-  for (int p = 0; p < kOrthMaxPasses; ++p) {
-    if (!pass(segs, src, dst, ld, kcap, p == 0, s)) break;
+  if (breakdown2) {
+    // the unshifted pass broke down: continue from T = Y_in R_1^{-1}
+    // with a shifted pass
+    src = T;
+    dst = Y;
+    again = true;
+  } else if (Rout) {
+    kern::seg_gemm_nn(acc, dnn_, R_.get(), kcap, Racc_.get(), kcap, Rtmp_.get(), kcap, 1.0, 0.0, s);
+    std::swap(Racc_, Rtmp_);
+  }
+  // rare: more passes, each checked, until the Gram matrix is I to kOrthTol
+  for (int p = 2; again && p < kOrthMaxPasses; ++p) {
+    if (!pass(segs, src, dst, ld, kcap, breakdown2 && src == T && p == 2, s)) break;
     if (Rout) {
-      if (p == 0) {
-        HIP_CHECK(hipMemcpyAsync(Racc_.get(), R_.get(), nb * kk * sizeof(double), hipMemcpyDeviceToDevice, s));
-      } else {
-        // Racc <- R_p * Racc (upper triangular products)
-        kern::seg_gemm_nn(acc, dnn_, R_.get(), kcap, Racc_.get(), kcap, Rtmp_.get(), kcap, 1.0, 0.0, s);
-        std::swap(Racc_, Rtmp_);
-      }
+      // Racc <- R_p * Racc (upper triangular products)
+      kern::seg_gemm_nn(acc, dnn_, R_.get(), kcap, Racc_.get(), kcap, Rtmp_.get(), kcap, 1.0, 0.0, s);
+      std::swap(Racc_, Rtmp_);
     }
     std::swap(src, dst);
   }

@@ -126,9 +126,9 @@ inline GenDesc gen_from_kernel(const hmat_kernel& k, const double* dx, const dou
 
 // Segmented shifted CholeskyQR (Fukaya et al., SIAM J. Sci. Comput. 2020):
 // one shifted pass, then unshifted passes until every segment's Gram matrix
-// is the identity to kOrthTol = 1e-12 (two for well-conditioned input; sketches of
-// numerically low-rank blocks, cond ~1e16, take more).  Q overwrites `Y`;
-// optionally R = R_p ... R_2 R_1.
+// is the identity to kOrthTol = 1e-12 (two passes and the check for
+// well-conditioned input; sketches of numerically low-rank blocks, cond
+// ~1e16, take more).  Q overwrites `Y`; optionally R = R_p ... R_2 R_1.
 constexpr double kOrthTol = 1e-12;
 constexpr int kOrthMaxPasses = 8;
 struct OrthSeg {
@@ -147,6 +147,7 @@ class Orthogonalizer {
   kern::Workspace ws_;
   DevBuf<double> G_, R_, Ri_, Racc_, Rtmp_, shift_, gerr_;
   DevBuf<int> info_;
+  DevBuf<uint8_t> chk_;  // both passes' info and Gram deviations, downloaded together
   DevBuf<SegNN> dnn_;
   DevBuf<SegTN> dtn_;
   DevBuf<SmallMat> dsm_;

@@ -26,6 +26,7 @@ HodlrEngine::HodlrEngine(const hmat_options& o) : opt(o) {
 }
 
 HodlrEngine::~HodlrEngine() {
+  if (stream) stager_release(stream);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -90,7 +91,8 @@ void HodlrEngine::build_forest(const GenDesc& g, const std::vector<int64_t>& siz
 }
 
 void HodlrEngine::build_levels(const GenDesc& g, const double* host_omega, int64_t omega_len) {
-  lv.clear();
+  // levels keep their device arrays from the previous build (reallocated
+  // only when a shape changes)
   lv.resize(tree.max_depth + 1);
   par.clear();
 
@@ -137,9 +139,7 @@ void HodlrEngine::build_levels(const GenDesc& g, const double* host_omega, int64
   }
   kern::gen_blocks(g, bd, dbd_, D.get(), stream);
   HIP_CHECK(hipStreamSynchronize(stream));
-  // free build scratch
-  Yb_.release(); Xb_.release(); Tb_.release(); Rb_.release(); Vr_.release();
-  Wk_.release(); TU_.release(); TV_.release();
+  // the build scratch (three N x k arrays) is kept for the next build
   prepare_matvec();
   built = true;
 }
@@ -196,7 +196,6 @@ void HodlrEngine::prepare_matvec() {
   mv_bytes_base_ = bytes;
   mv_part_.reserve(std::max<size_t>((size_t)nl * L * KM * 4, 1));
   mv_t_.reserve(std::max<size_t>(segs.size() * KM * 4, 1));
-  HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 double HodlrEngine::matvec_bytes(int nrhs) const { return mv_bytes_base_ + 16.0 * (double)N * nrhs; }
@@ -367,9 +366,7 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
   L.S.assign((size_t)ns * L.rmax, 0.0);
   for (int s = 0; s < ns; ++s)
     for (int i = 0; i < L.rank[s]; ++i) L.S[(size_t)s * L.rmax + i] = S[(size_t)s * kcap + i];
-  DevBuf<int> drank;
-  drank.upload(L.rank, stream);
-  kern::trunc_prep(sm, dsm_, Rb_.get(), Vr_.get(), Sv_.get(), kcap, drank.get(), L.rmax, TU_.get(),
+  kern::trunc_prep(sm, dsm_, Rb_.get(), Vr_.get(), Sv_.get(), kcap, stage(L.rank, stream), L.rmax, TU_.get(),
                    TV_.get(), 0, stream);
   const int64_t nr = N * (int64_t)L.rmax;
   L.US.resize(nr);
@@ -383,7 +380,6 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
   }
   kern::seg_gemm_nn(nu, dnn_, Yb_.get(), N, TU_.get(), kcap, L.US.get(), N, 1.0, 0.0, stream);
   kern::seg_gemm_nn(nv, dnn_, Xb_.get(), N, TV_.get(), kcap, L.V.get(), N, 1.0, 0.0, stream);
-  HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 // Y = D X over all leaves (leaf blocks stored lmax x lmax).
@@ -399,13 +395,17 @@ void HodlrEngine::leaf_apply_t(const F* Dm, const F* B, int64_t ldb, int ncols, 
 }
 
 void HodlrEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
+  matvec_enqueue(dX, ldx, nrhs, dY, ldy);
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void HodlrEngine::matvec_enqueue(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "matvec before build");
   HMAT_REQUIRE(nrhs > 0 && ldx >= N && ldy >= N, HMAT_ERR_ARG, "bad matvec dimensions");
   if (nrhs <= 4 && kern::mv_lds_bytes(mv_, tree.lmax, 4) <= 64 * 1024) {
     ProfScope prof("matvec", stream, matvec_bytes(nrhs));
     matvec_async(dX, ldx, nrhs, dY, ldy);
     prof.finish();
-    HIP_CHECK(hipStreamSynchronize(stream));
     return;
   }
   leaf_apply_t<double>(D.get(), dX, ldx, nrhs, dY, ldy);
@@ -426,7 +426,6 @@ void HodlrEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, in
     kern::seg_gemm_tn<double>(tn, dtn_, L.V.get(), N, dX, ldx, tvec_.get(), L.rmax, 1.0, 0.0, ws_, stream);
     kern::seg_gemm_nn<double>(nn, dnn_, L.US.get(), N, tvec_.get(), L.rmax, dY, ldy, 1.0, 1.0, stream);
   }
-  HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 // W(rows of internal depth-j nodes, 0:ncols) -= Q C M^{-1} V^T W, with
@@ -480,26 +479,35 @@ void HodlrEngine::factor_t(HFactor<F>& fa) {
   std::vector<SmallMat> sm(nleaf);
   for (int i = 0; i < nleaf; ++i)
     sm[i] = SmallMat{(int64_t)i * lm * lm, (int32_t)tree.nodes[tree.leaves[i]].len, (int32_t)lm};
-  info_.reserve(std::max(nleaf, 1));
+  // pivot status of every leaf and capacitance inversion, checked together
+  // at the end (one synchronisation)
+  int ninfo = nleaf;
+  for (const TNode& nd : tree.nodes) ninfo += nd.leaf ? 0 : 1;
+  info_.reserve(std::max(ninfo, 1));
   kern::gj_inverse<F>(sm, dsm_, fa.Dinv.get(), info_.get(), stream);
-  {
-    auto info = download(info_.get(), nleaf, stream);
-    for (int i = 0; i < nleaf; ++i)
-      if (info[i] != 0)
-        throw Error(HMAT_ERR_SINGULAR, "singular leaf block " + std::to_string(i) + " (pivot " +
-                                           std::to_string(info[i]) + ")");
-  }
+  std::vector<std::pair<int, int>> cap_info;  // (offset, count) per depth j, deepest first
+  int info_off = nleaf;
   const int dmax = tree.max_depth;
   colofs.assign(dmax + 2, 0);
   for (int c = 1; c <= dmax; ++c) colofs[c + 1] = colofs[c] + lv[c].rmax;
   Rtot = colofs[dmax + 1];
   fa.V.assign(dmax + 1, nullptr);
-  fa.Vown.clear();
   fa.Vown.resize(dmax + 1);
-  fa.Minv.clear();
   fa.Minv.resize(std::max(dmax, 1));
   par.assign(std::max(dmax, 1), HParents{});
+  auto check_info = [&]() {
+    const auto info = download(info_.get(), info_off, stream);
+    for (int i = 0; i < nleaf; ++i)
+      if (info[i] != 0)
+        throw Error(HMAT_ERR_SINGULAR, "singular leaf block " + std::to_string(i) + " (pivot " +
+                                           std::to_string(info[i]) + ")");
+    for (int jj = dmax - 1, q = 0; jj >= 0; --jj, ++q)
+      for (int p = 0; p < cap_info[q].second; ++p)
+        if (info[cap_info[q].first + p] != 0)
+          throw Error(HMAT_ERR_SINGULAR, "singular capacitance matrix at depth " + std::to_string(jj));
+  };
   if (dmax == 0) {
+    check_info();
     fa.ready = true;
     return;
   }
@@ -552,17 +560,13 @@ void HodlrEngine::factor_t(HFactor<F>& fa) {
     fa.Minv[j].zero(stream);
     kern::fill_identity_segs<F>(eye, dnn_, fa.Minv[j].get(), mm, stream);
     kern::seg_gemm_tn<F>(kb, dtn_, fa.V[j + 1], N, Q, N, fa.Minv[j].get(), mm, F(1), F(0), ws_, stream);
-    info_.reserve(np);
-    kern::gj_inverse<F>(ms, dsm_, fa.Minv[j].get(), info_.get(), stream);
-    auto info = download(info_.get(), np, stream);
-    for (int p = 0; p < np; ++p)
-      if (info[p] != 0)
-        throw Error(HMAT_ERR_SINGULAR, "singular capacitance matrix at depth " + std::to_string(j));
+    kern::gj_inverse<F>(ms, dsm_, fa.Minv[j].get(), info_.get() + info_off, stream);
+    cap_info.push_back({info_off, np});
+    info_off += np;
     // push the depth-j correction into the bases of shallower depths
     if (colofs[j + 1] > 0) correction_t<F>(fa, j, fa.Wall.get(), N, (int)colofs[j + 1]);
   }
-  HIP_CHECK(hipStreamSynchronize(stream));
-  fa.wtmp.release();
+  check_info();
   fa.ready = true;
 }
 
@@ -713,7 +717,7 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
   int it = 0;
   const bool mixed = opt.precision == HMAT_PREC_MIXED;
   for (;; ++it) {
-    matvec(dX, ldx, nrhs, ir_y_.get(), N);
+    matvec_enqueue(dX, ldx, nrhs, ir_y_.get(), N);
     kern::axpby_cols(N, nrhs, 1.0, dB, ldb, -1.0, ir_y_.get(), N, ir_r_.get(), N, stream);
     kern::col_norms2(ir_r_.get(), N, N, nrhs, ir_n_.get(), stream);
     auto rn = download(ir_n_.get(), nrhs, stream);

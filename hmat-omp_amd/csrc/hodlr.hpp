@@ -116,10 +116,10 @@ struct HFactor {
   DevBuf<F> tvec, coef, wtmp, b, x;
   SolvePlan<F> plan;
   bool ready = false;
+  // invalidate; the device arrays are kept for the next factorisation
   void clear() {
     plan.clear();
-    Dinv.release(); Wall.release(); Vown.clear(); V.clear(); Minv.clear();
-    tvec.release(); coef.release(); wtmp.release(); b.release(); x.release();
+    V.clear();
     ready = false;
   }
 };
@@ -135,6 +135,8 @@ class HodlrEngine {
   // level for all systems at once.
   void build_forest(const GenDesc& g, const std::vector<int64_t>& sizes, int leaf);
   void matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
+  // the same without the final synchronisation
+  void matvec_enqueue(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
   void factor();
   void solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
   void to_dense_host(double* A, int64_t lda);

@@ -33,6 +33,7 @@ HssEngine::HssEngine(const hmat_options& o) : opt(o) {
   own_stream = true;
 }
 HssEngine::~HssEngine() {
+  if (stream) stager_release(stream);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 

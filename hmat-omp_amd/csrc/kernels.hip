@@ -149,54 +149,64 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_nn_mfma(const SegNN* __restric
       }
 }
 
-void plan_nn(const std::vector<SegNN>& segs, NNPlan& p, hipStream_t s) {
-  p.ntiles = 0;
-  p.mode = 0;
-  if (segs.empty()) return;
+// Tiling of a segmented Y = alpha X T + beta Y: the kernel variant (1: MFMA
+// 64 x 64 tiles, else rows per workgroup) and its tiles.
+static int nn_tiles(const std::vector<SegNN>& segs, std::vector<NNTile>& tiles) {
+  tiles.clear();
   int qmax = 0;
   for (auto& g : segs) qmax = std::max(qmax, g.q);
-  if (qmax == 0) return;
+  if (qmax == 0) return 0;
   int pmin = 1 << 30;
   for (auto& g : segs)
     if (g.len > 0 && g.q > 0) pmin = std::min(pmin, g.p);
-  int R, cw;
+  int mode, R, cw;
   if (pmin >= 32 && qmax >= 16) {
-    p.mode = 1;  // MFMA, 64 x 64 tiles
+    mode = 1;  // MFMA, 64 x 64 tiles
     R = 64;
     cw = 64;
   } else {
     R = qmax <= 16 ? 256 : (qmax <= 32 ? 128 : 64);
-    p.mode = R;
+    mode = R;
     cw = (kWG / R) * 16;
   }
-  std::vector<NNTile> tiles;
   for (size_t i = 0; i < segs.size(); ++i) {
     const auto& g = segs[i];
     if (g.len <= 0 || g.q <= 0) continue;
     for (int r0 = 0; r0 < g.len; r0 += R)
       for (int c0 = 0; c0 < g.q; c0 += cw) tiles.push_back({(int32_t)i, r0, c0, 0});
   }
+  return mode;
+}
+
+template <class F>
+static void launch_nn_raw(int mode, int ntiles, const SegNN* ds, const NNTile* dt, const F* X, int64_t ldx,
+                          const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha, F beta, hipStream_t s) {
+  if (!ntiles) return;
+  const unsigned g = (unsigned)ntiles;
+  if (mode == 1)
+    k_seg_gemm_nn_mfma<F><<<g, kWG, 0, s>>>(ds, dt, X, ldx, T, ldt, Y, ldy, alpha, beta);
+  else if (mode == 256)
+    k_seg_gemm_nn<F, 256><<<g, kWG, 0, s>>>(ds, dt, X, ldx, T, ldt, Y, ldy, alpha, beta);
+  else if (mode == 128)
+    k_seg_gemm_nn<F, 128><<<g, kWG, 0, s>>>(ds, dt, X, ldx, T, ldt, Y, ldy, alpha, beta);
+  else
+    k_seg_gemm_nn<F, 64><<<g, kWG, 0, s>>>(ds, dt, X, ldx, T, ldt, Y, ldy, alpha, beta);
+  HIP_CHECK(hipGetLastError());
+}
+
+void plan_nn(const std::vector<SegNN>& segs, NNPlan& p, hipStream_t s) {
+  std::vector<NNTile> tiles;
+  p.mode = segs.empty() ? 0 : nn_tiles(segs, tiles);
   p.ntiles = (int)tiles.size();
   if (!p.ntiles) return;
   p.segs.upload(segs, s);
   p.tiles.upload(tiles, s);
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 template <class F>
 void launch_nn(const NNPlan& p, const F* X, int64_t ldx, const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha,
                F beta, hipStream_t s) {
-  if (!p.ntiles) return;
-  const unsigned g = (unsigned)p.ntiles;
-  if (p.mode == 1)
-    k_seg_gemm_nn_mfma<F><<<g, kWG, 0, s>>>(p.segs.get(), p.tiles.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
-  else if (p.mode == 256)
-    k_seg_gemm_nn<F, 256><<<g, kWG, 0, s>>>(p.segs.get(), p.tiles.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
-  else if (p.mode == 128)
-    k_seg_gemm_nn<F, 128><<<g, kWG, 0, s>>>(p.segs.get(), p.tiles.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
-  else
-    k_seg_gemm_nn<F, 64><<<g, kWG, 0, s>>>(p.segs.get(), p.tiles.get(), X, ldx, T, ldt, Y, ldy, alpha, beta);
-  HIP_CHECK(hipGetLastError());
+  launch_nn_raw<F>(p.mode, p.ntiles, p.segs.get(), p.tiles.get(), X, ldx, T, ldt, Y, ldy, alpha, beta, s);
 }
 template void launch_nn<double>(const NNPlan&, const double*, int64_t, const double*, int64_t, double*, int64_t,
                                 double, double, hipStream_t);
@@ -207,10 +217,12 @@ template <class F>
 void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* X, int64_t ldx,
                  const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha, F beta, hipStream_t s) {
   (void)dsegs;
-  NNPlan p;
-  plan_nn(segs, p, s);
-  launch_nn<F>(p, X, ldx, T, ldt, Y, ldy, alpha, beta, s);
-  HIP_CHECK(hipStreamSynchronize(s));  // the plan's buffers are freed on return
+  if (segs.empty()) return;
+  std::vector<NNTile> tiles;
+  const int mode = nn_tiles(segs, tiles);
+  if (tiles.empty()) return;
+  launch_nn_raw<F>(mode, (int)tiles.size(), stage(segs, s), stage(tiles, s), X, ldx, T, ldt, Y, ldy, alpha,
+                   beta, s);
 }
 template void seg_gemm_nn<double>(const std::vector<SegNN>&, DevBuf<SegNN>&, const double*, int64_t,
                                   const double*, int64_t, double*, int64_t, double, double, hipStream_t);
@@ -335,27 +347,26 @@ void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const 
     if (g.len <= 0 || g.q <= 0) continue;
     for (int r0 = 0; r0 < g.len; r0 += rows) tiles.push_back({(int32_t)i, r0, 0, 0});
   }
-  dsegs.upload(segs, s);
-  DevBuf<NNTile> dt;
-  dt.upload(tiles, s);
+  (void)dsegs;
+  const SegNN* ds = stage(segs, s);
+  const NNTile* dt = stage(tiles, s);
   const unsigned nt = (unsigned)tiles.size();
   if (kmax <= 16) {
-    k_seg_trsm_reg<16><<<nt, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy);
+    k_seg_trsm_reg<16><<<nt, kWG, 0, s>>>(ds, dt, X, ldx, R, ldr, Y, ldy);
   } else if (kmax <= 32) {
-    k_seg_trsm_reg<32><<<nt, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy);
+    k_seg_trsm_reg<32><<<nt, kWG, 0, s>>>(ds, dt, X, ldx, R, ldr, Y, ldy);
   } else if (kmax <= 48) {
-    k_seg_trsm_reg<48><<<nt, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy);
+    k_seg_trsm_reg<48><<<nt, kWG, 0, s>>>(ds, dt, X, ldx, R, ldr, Y, ldy);
   } else if (kmax <= 64) {
-    k_seg_trsm_reg<64><<<nt, kWG, 0, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy);
+    k_seg_trsm_reg<64><<<nt, kWG, 0, s>>>(ds, dt, X, ldx, R, ldr, Y, ldy);
   } else {
     size_t lds = (size_t)kmax * 64 * sizeof(double);
     const size_t rlds = (size_t)kmax * kmax * sizeof(double);
     const int r_in_lds = (lds + rlds <= 96 * 1024) ? 1 : 0;
     if (r_in_lds) lds += rlds;
-    k_seg_trsm_upper<<<nt, 64, lds, s>>>(dsegs.get(), dt.get(), X, ldx, R, ldr, Y, ldy, r_in_lds);
+    k_seg_trsm_upper<<<nt, 64, lds, s>>>(ds, dt, X, ldx, R, ldr, Y, ldy, r_in_lds);
   }
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // --------------------------------------------------------------------------
@@ -364,106 +375,93 @@ void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const 
 // --------------------------------------------------------------------------
 
 template <class F>
-__global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part(const SegTN* __restrict__ segs,
-                                                          const TNItem* __restrict__ items,
-                                                          const F* __restrict__ X, int64_t ldx,
-                                                          const F* __restrict__ Y, int64_t ldy,
-                                                          F* __restrict__ part) {
-  __shared__ F xs[32][65];
-  __shared__ F ys[32][65];
-  const TNItem it = items[blockIdx.x];
-  const SegTN sg = segs[it.seg];
-  const int ti = threadIdx.x % 16, tj = threadIdx.x / 16;
-  F acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = F(0);
-  const int pc0 = it.pt * 64, qc0 = it.qt * 64;
-  for (int rb = it.r0; rb < it.r1; rb += 32) {
-    for (int e = threadIdx.x; e < 32 * 64; e += kWG) {
-      const int rr = e % 32, cc = e / 32, row = rb + rr;
-      const bool rok = row < it.r1;
-      xs[rr][cc] = (rok && pc0 + cc < sg.p) ? X[sg.xoff + row + (int64_t)(pc0 + cc) * ldx] : F(0);
-      ys[rr][cc] = (rok && qc0 + cc < sg.q) ? Y[sg.yoff + row + (int64_t)(qc0 + cc) * ldy] : F(0);
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int rr = 0; rr < 32; ++rr) {
-      F xv[4], yv[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) xv[a] = xs[rr][ti + 16 * a];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) yv[b] = ys[rr][tj + 16 * b];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = fma(xv[a], yv[b], acc[a][b]);
-    }
-    __syncthreads();
-  }
-  F* pp = part + (int64_t)it.slot * 4096;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) pp[(ti + 16 * a) + (tj + 16 * b) * 64] = acc[a][b];
-}
-
-template <class F>
 __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_reduce(const SegTN* __restrict__ segs,
                                                             const TNOut* __restrict__ outs,
                                                             const F* __restrict__ part,
                                                             F* __restrict__ O, int64_t ldo,
                                                             F alpha, F beta) {
-  // blockIdx.y picks 256 of the 4096 entries of the output tile
+  // blockIdx.y picks 64 of the 4096 entries of the output tile; the four
+  // waves sum interleaved quarters of the slots, then wave 0 adds the
+  // quarters in a fixed order (deterministic)
+  __shared__ F red[4][64];
   const TNOut ot = outs[blockIdx.x];
   const SegTN sg = segs[ot.seg];
-  const int e = blockIdx.y * kWG + threadIdx.x;
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int e = blockIdx.y * 64 + lane;
+  F sum = F(0);
+  for (int c = w; c < ot.nslot; c += 4) sum += part[(int64_t)(ot.slot0 + c) * 4096 + e];
+  red[w][lane] = sum;
+  __syncthreads();
+  if (w != 0) return;
   const int i = e % 64, j = e / 64;
   const int gi = ot.pt * 64 + i, gj = ot.qt * 64 + j;
   if (gi >= sg.p || gj >= sg.q) return;
-  F sum = F(0);
-  for (int c = 0; c < ot.nslot; ++c) sum += part[(int64_t)(ot.slot0 + c) * 4096 + e];
   F* o = O + sg.ooff + gi + (int64_t)gj * ldo;
-  F v = alpha * sum;
+  F v = alpha * ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
   if (beta != F(0)) v = fma(beta, *o, v);
   *o = v;
 }
 
-// MFMA variant of the partial products: the same 64 x 64 output tile and
-// 32-row LDS staging; wave w computes output rows 16w..16w+15 (columns of X)
-// against all four 16-column groups of Y with 16x16x4 MFMAs.
+// MFMA partial products: one 64 x 64 output tile over a row chunk, staged
+// through LDS 32 rows at a time with the next 32 rows prefetched into
+// registers; wave w computes output rows 16w..16w+15 (columns of X) against
+// all four 16-column groups of Y with 16x16x4 MFMAs.  When X and Y are the
+// same rows and columns (a Gram matrix) the chunk is read once.
 template <class F>
 __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part_mfma(const SegTN* __restrict__ segs,
                                                                const TNItem* __restrict__ items,
                                                                const F* __restrict__ X, int64_t ldx,
                                                                const F* __restrict__ Y, int64_t ldy,
                                                                F* __restrict__ part) {
-  __shared__ F xs[32][65];
-  __shared__ F ys[32][65];
+  constexpr int KR = 32, NPRE = KR * 64 / kWG;
+  __shared__ F xs[KR][65];
+  __shared__ F ys[KR][65];
   const TNItem it = items[blockIdx.x];
   const SegTN sg = segs[it.seg];
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const bool same = X == Y && ldx == ldy && sg.xoff == sg.yoff && sg.p == sg.q && it.pt == it.qt;
   typedef typename MfmaNN<F>::acc_t acc_t;
   acc_t acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = acc_t{F(0), F(0), F(0), F(0)};
   const int pc0 = it.pt * 64, qc0 = it.qt * 64;
-  for (int rb = it.r0; rb < it.r1; rb += 32) {
-    for (int e = threadIdx.x; e < 32 * 64; e += kWG) {
-      const int rr = e % 32, cc = e / 32, row = rb + rr;
-      const bool rok = row < it.r1;
-      xs[rr][cc] = (rok && pc0 + cc < sg.p) ? X[sg.xoff + row + (int64_t)(pc0 + cc) * ldx] : F(0);
-      ys[rr][cc] = (rok && qc0 + cc < sg.q) ? Y[sg.yoff + row + (int64_t)(qc0 + cc) * ldy] : F(0);
-    }
-    __syncthreads();
+  F px[NPRE], py[NPRE];
+  auto issue = [&](int rb) {
 #pragma unroll
-    for (int kk = 0; kk < 32; kk += 4) {
+    for (int i = 0; i < NPRE; ++i) {
+      const int e = threadIdx.x + i * kWG;
+      const int rr = e % KR, cc = e / KR, row = rb + rr;
+      const bool rok = row < it.r1;
+      px[i] = (rok && pc0 + cc < sg.p) ? X[sg.xoff + row + (int64_t)(pc0 + cc) * ldx] : F(0);
+      if (!same) py[i] = (rok && qc0 + cc < sg.q) ? Y[sg.yoff + row + (int64_t)(qc0 + cc) * ldy] : F(0);
+    }
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int i = 0; i < NPRE; ++i) {
+      const int e = threadIdx.x + i * kWG;
+      xs[e % KR][e / KR] = px[i];
+      if (!same) ys[e % KR][e / KR] = py[i];
+    }
+  };
+  F(*bs)[65] = same ? xs : ys;
+  issue(it.r0);
+  commit();
+  __syncthreads();
+  for (int rb = it.r0; rb < it.r1; rb += KR) {
+    const bool more = rb + KR < it.r1;
+    if (more) issue(rb + KR);
+#pragma unroll
+    for (int kk = 0; kk < KR; kk += 4) {
       const F a = xs[kk + lane / 16][16 * w + lane % 16];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) acc[t] = MfmaNN<F>::step(a, ys[kk + lane / 16][16 * t + lane % 16], acc[t]);
+      for (int t = 0; t < 4; ++t) acc[t] = MfmaNN<F>::step(a, bs[kk + lane / 16][16 * t + lane % 16], acc[t]);
     }
     __syncthreads();
+    if (more) {
+      commit();
+      __syncthreads();
+    }
   }
   F* pp = part + (int64_t)it.slot * 4096;
 #pragma unroll
@@ -472,15 +470,15 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part_mfma(const SegTN* __re
     for (int v = 0; v < 4; ++v) pp[(16 * w + MfmaNN<F>::row(lane, v)) + (16 * t + (lane & 15)) * 64] = acc[t][v];
 }
 
-void plan_tn(const std::vector<SegTN>& segs, TNPlan& p, hipStream_t s) {
-  p.nitems = p.nouts = p.nslots = 0;
-  if (segs.empty()) return;
+// Work items of a segmented O = alpha X^T Y + beta O: partial 64 x 64
+// products over row chunks (slots of the partial buffer) and their sums.
+static int tn_items(const std::vector<SegTN>& segs, std::vector<TNItem>& items, std::vector<TNOut>& outs) {
+  items.clear();
+  outs.clear();
   int64_t total_rows = 0;
   for (auto& g : segs) total_rows += std::max(g.len, 0);
   // ~1024 partial items in total, chunks a multiple of 32 rows
   int64_t ch = std::max<int64_t>(256, round_up((int)std::min<int64_t>(ceil_div(total_rows, 1024), 1 << 20), 32));
-  std::vector<TNItem> items;
-  std::vector<TNOut> outs;
   int slot = 0;
   for (size_t i = 0; i < segs.size(); ++i) {
     const auto& g = segs[i];
@@ -501,26 +499,40 @@ void plan_tn(const std::vector<SegTN>& segs, TNPlan& p, hipStream_t s) {
         outs.push_back(o);
       }
   }
+  return slot;
+}
+
+template <class F>
+static void launch_tn_raw(int nitems, int nouts, const SegTN* ds, const TNItem* di, const TNOut* dout,
+                          const F* X, int64_t ldx, const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta,
+                          F* part, hipStream_t s) {
+  if (!nouts) return;
+  k_seg_gemm_tn_part_mfma<F><<<(unsigned)nitems, kWG, 0, s>>>(ds, di, X, ldx, Y, ldy, part);
+  HIP_CHECK(hipGetLastError());
+  k_seg_gemm_tn_reduce<F><<<dim3((unsigned)nouts, 4096 / 64), kWG, 0, s>>>(ds, dout, part, O, ldo, alpha, beta);
+  HIP_CHECK(hipGetLastError());
+}
+
+void plan_tn(const std::vector<SegTN>& segs, TNPlan& p, hipStream_t s) {
+  p.nitems = p.nouts = p.nslots = 0;
+  if (segs.empty()) return;
+  std::vector<TNItem> items;
+  std::vector<TNOut> outs;
+  const int nslots = tn_items(segs, items, outs);
   if (outs.empty()) return;
   p.nitems = (int)items.size();
   p.nouts = (int)outs.size();
-  p.nslots = slot;
+  p.nslots = nslots;
   p.segs.upload(segs, s);
   p.items.upload(items, s);
   p.outs.upload(outs, s);
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 template <class F>
 void launch_tn(const TNPlan& p, const F* X, int64_t ldx, const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha,
                F beta, F* part, hipStream_t s) {
-  if (!p.nouts) return;
-  k_seg_gemm_tn_part_mfma<F><<<(unsigned)p.nitems, kWG, 0, s>>>(p.segs.get(), p.items.get(), X, ldx, Y, ldy,
-                                                                 part);
-  HIP_CHECK(hipGetLastError());
-  k_seg_gemm_tn_reduce<F><<<dim3((unsigned)p.nouts, 4096 / kWG), kWG, 0, s>>>(p.segs.get(), p.outs.get(), part, O,
-                                                                             ldo, alpha, beta);
-  HIP_CHECK(hipGetLastError());
+  launch_tn_raw<F>(p.nitems, p.nouts, p.segs.get(), p.items.get(), p.outs.get(), X, ldx, Y, ldy, O, ldo, alpha,
+                   beta, part, s);
 }
 template void launch_tn<double>(const TNPlan&, const double*, int64_t, const double*, int64_t, double*, int64_t,
                                 double, double, double*, hipStream_t);
@@ -532,13 +544,15 @@ void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* 
                  const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta, Workspace& ws,
                  hipStream_t s) {
   (void)dsegs;
-  TNPlan p;
-  plan_tn(segs, p, s);
-  if (!p.nouts) return;
+  if (segs.empty()) return;
+  std::vector<TNItem> items;
+  std::vector<TNOut> outs;
+  const int nslots = tn_items(segs, items, outs);
+  if (outs.empty()) return;
   DevBuf<F>& part = ws.parts<F>();
-  part.reserve((size_t)p.nslots * 4096);
-  launch_tn<F>(p, X, ldx, Y, ldy, O, ldo, alpha, beta, part.get(), s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  part.reserve((size_t)nslots * 4096);
+  launch_tn_raw<F>((int)items.size(), (int)outs.size(), stage(segs, s), stage(items, s), stage(outs, s), X, ldx,
+                   Y, ldy, O, ldo, alpha, beta, part.get(), s);
 }
 template void seg_gemm_tn<double>(const std::vector<SegTN>&, DevBuf<SegTN>&, const double*, int64_t,
                                   const double*, int64_t, double*, int64_t, double, double,
@@ -1356,7 +1370,6 @@ void fill_identity_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, F*
   dsegs.upload(segs, s);
   k_fill_identity<F><<<dim3(64, (unsigned)segs.size()), kWG, 0, s>>>(dsegs.get(), Y, ldy);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 template void fill_identity_segs<double>(const std::vector<SegNN>&, DevBuf<SegNN>&, double*, int64_t, hipStream_t);
 template void fill_identity_segs<float>(const std::vector<SegNN>&, DevBuf<SegNN>&, float*, int64_t, hipStream_t);
@@ -1434,11 +1447,8 @@ __global__ void k_transpose_blocks(double* __restrict__ base, int64_t ld, const 
 
 void transpose_blocks(double* base, int64_t ld, const std::vector<int32_t>& lens, hipStream_t s) {
   if (lens.empty()) return;
-  DevBuf<int32_t> dl;
-  dl.upload(lens, s);
-  k_transpose_blocks<<<(unsigned)lens.size(), kWG, 0, s>>>(base, ld, dl.get());
+  k_transpose_blocks<<<(unsigned)lens.size(), kWG, 0, s>>>(base, ld, stage(lens, s));
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void copy_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X, int64_t ldx,
@@ -1447,7 +1457,6 @@ void copy_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const doubl
   dsegs.upload(segs, s);
   k_copy_segs<<<dim3(64, (unsigned)segs.size()), kWG, 0, s>>>(dsegs.get(), X, ldx, Y, ldy);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 __global__ void k_trunc_prep(const SmallMat* __restrict__ mats, const double* __restrict__ Ur,
@@ -1497,7 +1506,6 @@ void bgemm(const std::vector<BGemm>& items, DevBuf<BGemm>& d, hipStream_t s) {
   d.upload(items, s);
   k_bgemm<<<(unsigned)items.size(), kWG, 0, s>>>(d.get());
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 __global__ void k_rect_axpby(const Rect* __restrict__ items) {
@@ -1520,14 +1528,12 @@ void rect_axpby(const std::vector<Rect>& items, DevBuf<Rect>& d, hipStream_t s) 
   d.upload(items, s);
   k_rect_axpby<<<dim3(16, (unsigned)items.size()), kWG, 0, s>>>(d.get());
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 void rect_add_identity(const std::vector<Rect>& items, DevBuf<Rect>& d, hipStream_t s) {
   if (items.empty()) return;
   d.upload(items, s);
   k_rect_add_identity<<<dim3(4, (unsigned)items.size()), kWG, 0, s>>>(d.get());
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 __global__ void k_zero_rects(const ZeroRect* __restrict__ items) {
@@ -1542,7 +1548,6 @@ void zero_rects(const std::vector<ZeroRect>& items, DevBuf<ZeroRect>& d, hipStre
   d.upload(items, s);
   k_zero_rects<<<dim3(32, (unsigned)items.size()), kWG, 0, s>>>(d.get());
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 __device__ inline uint64_t splitmix64(uint64_t z) {
@@ -1581,7 +1586,6 @@ void fill_random_segs(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, doub
   dsegs.upload(segs, s);
   k_fill_random<<<dim3(64, (unsigned)segs.size()), kWG, 0, s>>>(dsegs.get(), Y, ldy, seed, dist);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 template <class Gen>
@@ -1604,7 +1608,6 @@ void gen_blocks(const GenDesc& g, const std::vector<BlockDesc>& blocks, DevBuf<B
     k_gen_blocks<<<dim3(64, (unsigned)blocks.size()), kWG, 0, s>>>(gen, db.get(), out);
   });
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 __global__ void k_axpby_cols(int64_t n, double alpha, const double* __restrict__ X, int64_t ldx,

@@ -454,11 +454,8 @@ static void sum_splits(const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs, con
   std::vector<SumTile> st;
   for (size_t i = 0; i < segs.size(); ++i)
     for (int r0 = 0; r0 < segs[i].outlen; r0 += 64) st.push_back({(int32_t)i, r0});
-  DevBuf<SumTile> dst;
-  dst.upload(st, s);
-  k_sketch_sum<<<(unsigned)st.size(), kWGs, 0, s>>>(dsegs.get(), dst.get(), part, p.nsplit, p.span, Y, ldy);
+  k_sketch_sum<<<(unsigned)st.size(), kWGs, 0, s>>>(dsegs.get(), stage(st, s), part, p.nsplit, p.span, Y, ldy);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 template <class Gen, bool TRANS>
@@ -494,8 +491,7 @@ static void run_sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& s
   SketchPlan p = plan_sketch(segs, kcap, NT * 16, rows_per_wg, ldy, want);
   if (p.tiles.empty()) return;
   dsegs.upload(segs, s);
-  DevBuf<SkTile> dt;
-  dt.upload(p.tiles, s);
+  const SkTile* dt = stage(p.tiles, s);
   double* out = Y;
   if (p.nsplit > 1) {
     ws.part.reserve((size_t)(p.span * p.nsplit));
@@ -508,14 +504,13 @@ static void run_sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& s
   if (!prof_on) prof.cancel();
   with_gen(g, [&](auto gen) {
     if (trans)
-      launch_sketch<decltype(gen), true>(gen, NT, RF, grid, dsegs.get(), dt.get(), X, ldx, out, ldy, p.span, s);
+      launch_sketch<decltype(gen), true>(gen, NT, RF, grid, dsegs.get(), dt, X, ldx, out, ldy, p.span, s);
     else
-      launch_sketch<decltype(gen), false>(gen, NT, RF, grid, dsegs.get(), dt.get(), X, ldx, out, ldy, p.span, s);
+      launch_sketch<decltype(gen), false>(gen, NT, RF, grid, dsegs.get(), dt, X, ldx, out, ldy, p.span, s);
   });
   HIP_CHECK(hipGetLastError());
   prof.finish();
   if (p.nsplit > 1) sum_splits(segs, dsegs, p, ws.part.get(), Y, ldy, s);
-  else HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // Shape class of a sketch launch for the autotuner.
@@ -620,8 +615,7 @@ void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>
     SketchPlan p = plan_sketch(segs, kcap, 64, 128, ldy, want);
     if (p.tiles.empty()) return;
     dsegs.upload(segs, s);
-    DevBuf<SkTile> dt;
-    dt.upload(p.tiles, s);
+    const SkTile* dt = stage(p.tiles, s);
     double* out = Y;
     if (p.nsplit > 1) {
       ws.part.reserve((size_t)(p.span * p.nsplit));
@@ -635,16 +629,15 @@ void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>
     if (!prof_on) prof.cancel();
     with_gen_lp(g, [&](auto gen) {
       if (trans)
-        launch_sketch_lp<decltype(gen), true>(gen, split, grid, dsegs.get(), dt.get(), ws.bf.get(), ldx, ps, out,
+        launch_sketch_lp<decltype(gen), true>(gen, split, grid, dsegs.get(), dt, ws.bf.get(), ldx, ps, out,
                                               ldy, p.span, s);
       else
-        launch_sketch_lp<decltype(gen), false>(gen, split, grid, dsegs.get(), dt.get(), ws.bf.get(), ldx, ps,
+        launch_sketch_lp<decltype(gen), false>(gen, split, grid, dsegs.get(), dt, ws.bf.get(), ldx, ps,
                                                out, ldy, p.span, s);
     });
     HIP_CHECK(hipGetLastError());
     prof.finish();
     if (p.nsplit > 1) sum_splits(segs, dsegs, p, ws.part.get(), Y, ldy, s);
-    else HIP_CHECK(hipStreamSynchronize(s));
   };
   int64_t want = 1024;
   if (autotune_enabled()) {
