@@ -342,7 +342,11 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
   // R = Ur S Vr^T  =>  A_blk ~ (Q Vr) S (Qb Ur)^T        (compression.h:90-98)
   std::vector<SmallMat> sm(ns);
   for (int s = 0; s < ns; ++s) sm[s] = SmallMat{(int64_t)s * kk, k[s], kcap};
-  kern::jacobi_svd(sm, dsm_, Rb_.get(), Vr_.get(), Sv_.get(), kcap, Wk_.get(), stream);
+  // the mixed-precision basis resolves the block to about 2^-23 of its
+  // norm (FP32 entries in the last range pass): columns below that are not
+  // resolved against each other
+  kern::jacobi_svd(sm, dsm_, Rb_.get(), Vr_.get(), Sv_.get(), kcap, Wk_.get(), stream,
+                   lp ? kMixedJacobiFloor : 1e-30);
   auto S = download(Sv_.get(), (size_t)ns * kcap, stream);
 
   // rank per block (compression.h:42-46 or fixed)
@@ -590,9 +594,18 @@ void HodlrEngine::factor() {
 }
 
 // One application of the factored inverse, FP64 in and out.
+// Whether this apply-inverse replays the recorded graph (recording it first
+// when needed): only once kEagerSolves eager applies of the column count ran.
+template <class F>
+static bool graph_turn(SolvePlan<F>& sp, bool enabled, int ncols) {
+  if (!enabled) return false;
+  if (sp.ncols == ncols && sp.graph.e) return true;
+  return ++sp.uses[ncols] > kEagerSolves;
+}
+
 void HodlrEngine::apply_inverse(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx) {
   if (opt.precision != HMAT_PREC_MIXED) {
-    if (!solve_graph) {
+    if (!graph_turn(f64_.plan, solve_graph, ncols)) {
       apply_inverse_t<double>(f64_, B, ldb, ncols, X, ldx);
       return;
     }
@@ -602,11 +615,12 @@ void HodlrEngine::apply_inverse(const double* B, int64_t ldb, int ncols, double*
     kern::convert_cols<double, double>(N, ncols, f64_.x.get(), N, X, ldx, stream);
     return;
   }
-  if (solve_graph) plan_solve<float>(f32_, ncols);
+  const bool graph = graph_turn(f32_.plan, solve_graph, ncols);
+  if (graph) plan_solve<float>(f32_, ncols);
   f32_.b.reserve((size_t)N * ncols);
   f32_.x.reserve((size_t)N * ncols);
   kern::convert_cols<float, double>(N, ncols, B, ldb, f32_.b.get(), N, stream);
-  if (solve_graph)
+  if (graph)
     run_solve_graph<float>(f32_, ncols);
   else
     apply_inverse_t<float>(f32_, f32_.b.get(), N, ncols, f32_.x.get(), N);

@@ -1,6 +1,7 @@
 // HODLR engine: the MI355X counterpart of HODLRMatrixOMP
 // (include/hodlr_routines_omp.h:9-241).
 #pragma once
+#include <map>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,8 @@ enum class BlockMode { RSVD, ExactWide, ExactTall };
 // passes BF16x3 (1.7e-4 with all passes single BF16).
 constexpr int kLowPrecSplit = 3;
 constexpr int kPowerSplit = 1;
+// Jacobi rotation floor of the mixed path, (2^-23)^2 of ||R||_F^2.
+constexpr double kMixedJacobiFloor = 1.4210854715202004e-14;
 
 // Does an m x n block take the randomised branch?  compression.h:72-75 in
 // tolerance mode; in fixed-rank mode also exact when k >= min(m, n).
@@ -83,12 +86,19 @@ struct GraphExec {
   }
 };
 
+// Solves of one column count launch eagerly kEagerSolves times after a
+// factorisation before they are recorded into a graph: recording costs
+// milliseconds, which one or two solves per factorisation never win back.
+constexpr int kEagerSolves = 2;
+
 // The apply-inverse of one factorisation for a fixed column count: every
 // launch planned (descriptors on the device, buffers sized) and recorded
 // into one HIP graph that reads fa.b and writes fa.x.
+
 template <class F>
 struct SolvePlan {
   int ncols = 0;
+  std::map<int, int> uses;  // eager applies per column count since the factorisation
   kern::NNPlan leaf;
   std::vector<kern::TNPlan> tn;
   std::vector<kern::NNPlan> mn, up;
@@ -119,6 +129,7 @@ struct HFactor {
   // invalidate; the device arrays are kept for the next factorisation
   void clear() {
     plan.clear();
+    plan.uses.clear();
     V.clear();
     ready = false;
   }

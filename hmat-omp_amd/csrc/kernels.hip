@@ -818,7 +818,7 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
                                                     double* __restrict__ A, double* __restrict__ V,
                                                     double* __restrict__ S, int64_t sstride,
                                                     double* __restrict__ work, int use_lds,
-                                                    int* __restrict__ sweeps_out) {
+                                                    int* __restrict__ sweeps_out, double floor_rel) {
   extern __shared__ double sm[];
   __shared__ int rotated;
   __shared__ double sig[512];
@@ -844,7 +844,7 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
   // 20 eps^2) are rounding noise of the rotations and are left alone, so
   // columns at the noise level do not keep the sweep loop alive.
   fro = block_sum(fro, sig);
-  const double floor2 = 1e-30 * fro;
+  const double floor2 = floor_rel * fro;
   __syncthreads();
   const int np = n + (n & 1);  // even player count; column n is a phantom
   const int circ = np - 1;
@@ -954,7 +954,7 @@ constexpr int kJWG = 1024;
 __global__ __launch_bounds__(kJWG) void k_jacobi_svd64(const SmallMat* __restrict__ mats,
                                                        double* __restrict__ A, double* __restrict__ V,
                                                        double* __restrict__ S, int64_t sstride,
-                                                       int* __restrict__ sweeps_out) {
+                                                       int* __restrict__ sweeps_out, double floor_rel) {
   extern __shared__ double sm[];
   __shared__ double sig[64];
   __shared__ double red[kJWG / 64];
@@ -983,7 +983,7 @@ __global__ __launch_bounds__(kJWG) void k_jacobi_svd64(const SmallMat* __restric
   __syncthreads();
   fro = 0.0;
   for (int i = 0; i < kJWG / 64; ++i) fro += red[i];
-  const double floor2 = 1e-30 * fro;
+  const double floor2 = floor_rel * fro;
   const int np = n + (n & 1);
   const int circ = np - 1;
   int sweep = 0;
@@ -1063,7 +1063,7 @@ __global__ __launch_bounds__(kJWG) void k_jacobi_svd64(const SmallMat* __restric
 }
 
 void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* A, double* V,
-                double* S, int64_t sstride, double* work, hipStream_t s) {
+                double* S, int64_t sstride, double* work, hipStream_t s, double floor_rel) {
   if (mats.empty()) return;
   for (auto& mm : mats)
     HMAT_REQUIRE(mm.n <= 512, HMAT_ERR_ARG, "jacobi_svd: n > 512");
@@ -1079,10 +1079,11 @@ void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double*
   if (count) sw.resize(mats.size());
   if (nmax <= 62)
     k_jacobi_svd64<<<(unsigned)mats.size(), kJWG, (size_t)2 * nmax * (nmax + 1) * sizeof(double), s>>>(
-        dm.get(), A, V, S, sstride, count ? sw.get() : nullptr);
+        dm.get(), A, V, S, sstride, count ? sw.get() : nullptr, floor_rel);
   else
     k_jacobi_svd<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), A, V, S, sstride, work,
-                                                                       use_lds, count ? sw.get() : nullptr);
+                                                                       use_lds, count ? sw.get() : nullptr,
+                                                                       floor_rel);
   HIP_CHECK(hipGetLastError());
   if (count) {
     auto h = download(sw.get(), mats.size(), s);

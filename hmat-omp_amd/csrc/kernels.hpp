@@ -150,9 +150,15 @@ void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* 
                int* info, double* gerr, hipStream_t s);
 
 // One-sided Jacobi SVD of n x n matrices A = U diag(S) V^T (A overwritten
-// by U).  S sorted descending, U/V columns permuted accordingly.
+// by U).  S sorted descending, U/V columns permuted accordingly.  A pair of
+// columns is rotated while |a_p . a_q| > 1e-15 |a_p| |a_q| and
+// |a_p . a_q| > floor_rel ||A||_F^2: the default floor only guards against
+// underflow; a larger one stops resolving columns whose norms are below
+// sqrt(floor_rel) ||A||_F against each other (singular values that the
+// input does not determine to better than that anyway).  A V = U diag(S)
+// holds in either case.
 void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* A, double* V,
-                double* S, int64_t sstride, double* work, hipStream_t s);
+                double* S, int64_t sstride, double* work, hipStream_t s, double floor_rel = 1e-30);
 
 // In-place inverse by Gauss-Jordan with partial pivoting.
 template <class F>
