@@ -178,43 +178,87 @@ bool Orthogonalizer::pass(const std::vector<OrthSeg>& segs, const double* in, do
   return true;
 }
 
+void Orthogonalizer::make_plan(const std::vector<OrthSeg>& segs, int kcap, bool with_acc, hipStream_t s) {
+  const size_t nb = segs.size();
+  const int64_t kk = (int64_t)kcap * kcap;
+  std::vector<SegTN> tn(nb);
+  std::vector<SmallMat> sm(nb);
+  std::vector<SegNN> nn(nb), acc(nb);
+  std::vector<double> coef(nb), zero(nb, 0.0);
+  plan_.nmax = 0;
+  for (size_t b = 0; b < nb; ++b) {
+    const auto& g = segs[b];
+    tn[b] = SegTN{g.off, g.off, (int64_t)b * kk, g.len, g.k, g.k, 0};
+    sm[b] = SmallMat{(int64_t)b * kk, g.k, kcap};
+    // shift 11 (m n + n (n+1)) u ||X||^2 with ||X||_F^2 = trace(G) >= ||X||_2^2
+    coef[b] = 11.0 * ((double)g.len * g.k + (double)g.k * (g.k + 1)) * 1.1102230246251565e-16;
+    nn[b] = SegNN{g.off, (int64_t)b * kk, g.off, g.len, g.k, g.k, 0};
+    acc[b] = SegNN{(int64_t)b * kk, (int64_t)b * kk, (int64_t)b * kk, g.k, g.k, g.k, 0};
+    plan_.nmax = std::max(plan_.nmax, g.k);
+  }
+  kern::plan_tn(tn, plan_.gram, s);
+  plan_.mats.upload(sm, s);
+  plan_.shift_on.upload(coef, s);
+  plan_.shift_off.upload(zero, s);
+  kern::plan_trsm(nn, plan_.trsm, s);
+  if (with_acc) kern::plan_nn(acc, plan_.acc, s);
+  G_.reserve(nb * kk);
+  R_.reserve(nb * kk);
+  Ri_.reserve(nb * kk);
+  Racc_.reserve(nb * kk);
+  if (with_acc) Rtmp_.reserve(nb * kk);
+  ws_.part.reserve((size_t)std::max(plan_.gram.nslots, 1) * 4096);
+}
+
+void Orthogonalizer::gram(const double* Y, int64_t ld, int kcap, hipStream_t s) {
+  kern::launch_tn<double>(plan_.gram, Y, ld, Y, ld, G_.get(), kcap, 1.0, 0.0, ws_.part.get(), s);
+}
+
+void Orthogonalizer::run_once(const std::vector<OrthSeg>& segs, double* Y, double* T, int64_t ld, int kcap,
+                              hipStream_t s) {
+  if (segs.empty()) return;
+  const size_t nb = segs.size();
+  make_plan(segs, kcap, false, s);
+  info_.reserve(nb);
+  gerr_.reserve(nb);
+  gram(Y, ld, kcap, s);
+  kern::launch_chol_rinv(plan_.mats.get(), (int)nb, plan_.nmax, G_.get(), R_.get(), plan_.shift_on.get(),
+                         info_.get(), gerr_.get(), s);
+  if (plan_.trsm.kmax <= 64) {
+    kern::launch_trsm(plan_.trsm, Y, ld, R_.get(), kcap, Y, ld, s);  // in place
+  } else {
+    kern::launch_trsm(plan_.trsm, Y, ld, R_.get(), kcap, T, ld, s);
+    std::vector<SegNN> cp(nb);
+    for (size_t b = 0; b < nb; ++b) cp[b] = SegNN{segs[b].off, 0, segs[b].off, segs[b].len, 0, segs[b].k, 0};
+    kern::copy_segs(cp, dnn_, T, ld, Y, ld, s);
+  }
+}
+
 void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T, int64_t ld,
                          int kcap, double* Rout, hipStream_t s) {
   if (segs.empty()) return;
   const size_t nb = segs.size();
   const int64_t kk = (int64_t)kcap * kcap;
-  G_.reserve(nb * kk);
-  R_.reserve(nb * kk);
-  Ri_.reserve(nb * kk);
-  Racc_.reserve(nb * kk);
-  if (Rout) Rtmp_.reserve(nb * kk);
+  make_plan(segs, kcap, Rout != nullptr, s);
   const size_t info_bytes = (3 * nb * sizeof(int) + 255) & ~(size_t)255;
   chk_.reserve(info_bytes + 3 * nb * sizeof(double));
   int* info = reinterpret_cast<int*>(chk_.get());
   double* gerr = reinterpret_cast<double*>(chk_.get() + info_bytes);
-  std::vector<SegTN> tn(nb);
-  std::vector<SmallMat> sm(nb);
-  std::vector<SegNN> nn(nb), acc(nb);
-  std::vector<double> coef(nb), zero(nb, 0.0);
-  for (size_t b = 0; b < nb; ++b) {
-    const auto& g = segs[b];
-    tn[b] = SegTN{g.off, g.off, (int64_t)b * kk, g.len, g.k, g.k, 0};
-    sm[b] = SmallMat{(int64_t)b * kk, g.k, kcap};
-    coef[b] = 11.0 * ((double)g.len * g.k + (double)g.k * (g.k + 1)) * 1.1102230246251565e-16;
-    nn[b] = SegNN{g.off, (int64_t)b * kk, g.off, g.len, g.k, g.k, 0};
-    acc[b] = SegNN{(int64_t)b * kk, (int64_t)b * kk, (int64_t)b * kk, g.k, g.k, g.k, 0};
-  }
+  const int nm = (int)nb;
   // Enqueued back to back, with one download of all three status arrays:
   // shifted pass Y -> T (R_1 in Racc_), unshifted pass T -> Y (R_2 in R_),
   // and the Gram matrix of the result with its deviation from I.
-  kern::seg_gemm_tn(tn, dtn_, Y, ld, Y, ld, G_.get(), kcap, 1.0, 0.0, ws_, s);
-  kern::chol_rinv(sm, dsm_, G_.get(), Racc_.get(), nullptr, coef, shift_, info, gerr, s);
-  kern::seg_trsm_upper(nn, dnn_, Y, ld, Racc_.get(), kcap, T, ld, s);
-  kern::seg_gemm_tn(tn, dtn_, T, ld, T, ld, G_.get(), kcap, 1.0, 0.0, ws_, s);
-  kern::chol_rinv(sm, dsm_, G_.get(), R_.get(), nullptr, zero, shift_, info + nb, gerr + nb, s);
-  kern::seg_trsm_upper(nn, dnn_, T, ld, R_.get(), kcap, Y, ld, s);
-  kern::seg_gemm_tn(tn, dtn_, Y, ld, Y, ld, G_.get(), kcap, 1.0, 0.0, ws_, s);
-  kern::chol_rinv(sm, dsm_, G_.get(), Ri_.get(), nullptr, zero, shift_, info + 2 * nb, gerr + 2 * nb, s);
+  gram(Y, ld, kcap, s);
+  kern::launch_chol_rinv(plan_.mats.get(), nm, plan_.nmax, G_.get(), Racc_.get(), plan_.shift_on.get(), info,
+                         gerr, s);
+  kern::launch_trsm(plan_.trsm, Y, ld, Racc_.get(), kcap, T, ld, s);
+  gram(T, ld, kcap, s);
+  kern::launch_chol_rinv(plan_.mats.get(), nm, plan_.nmax, G_.get(), R_.get(), plan_.shift_off.get(), info + nb,
+                         gerr + nb, s);
+  kern::launch_trsm(plan_.trsm, T, ld, R_.get(), kcap, Y, ld, s);
+  gram(Y, ld, kcap, s);
+  kern::launch_chol_rinv(plan_.mats.get(), nm, plan_.nmax, G_.get(), Ri_.get(), plan_.shift_off.get(),
+                         info + 2 * nb, gerr + 2 * nb, s);
   const auto h = download(chk_.get(), info_bytes + 3 * nb * sizeof(double), s);
   const int* hi = reinterpret_cast<const int*>(h.data());
   const double* hg = reinterpret_cast<const double*>(h.data() + info_bytes);
@@ -234,7 +278,7 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
     dst = Y;
     again = true;
   } else if (Rout) {
-    kern::seg_gemm_nn(acc, dnn_, R_.get(), kcap, Racc_.get(), kcap, Rtmp_.get(), kcap, 1.0, 0.0, s);
+    kern::launch_nn<double>(plan_.acc, R_.get(), kcap, Racc_.get(), kcap, Rtmp_.get(), kcap, 1.0, 0.0, s);
     std::swap(Racc_, Rtmp_);
   }
   // rare: more passes, each checked, until the Gram matrix is I to kOrthTol
@@ -242,7 +286,7 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
     if (!pass(segs, src, dst, ld, kcap, breakdown2 && src == T && p == 2, s)) break;
     if (Rout) {
       // Racc <- R_p * Racc (upper triangular products)
-      kern::seg_gemm_nn(acc, dnn_, R_.get(), kcap, Racc_.get(), kcap, Rtmp_.get(), kcap, 1.0, 0.0, s);
+      kern::launch_nn<double>(plan_.acc, R_.get(), kcap, Racc_.get(), kcap, Rtmp_.get(), kcap, 1.0, 0.0, s);
       std::swap(Racc_, Rtmp_);
     }
     std::swap(src, dst);

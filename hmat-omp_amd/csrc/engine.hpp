@@ -142,8 +142,26 @@ class Orthogonalizer {
   // Rout (optional): nseg matrices of kcap x kcap (ld kcap).
   void run(const std::vector<OrthSeg>& segs, double* Y, double* T, int64_t ld, int kcap,
            double* Rout, hipStream_t s);
+  // One shifted pass and nothing else (no check, no synchronisation): a
+  // basis of the same span with orthogonality defect ~u cond(Y)^2, for
+  // intermediate bases whose input is well conditioned (cond(Y)^2 << 1/u).
+  // Non-finite input is reported by the next run().
+  void run_once(const std::vector<OrthSeg>& segs, double* Y, double* T, int64_t ld, int kcap,
+                hipStream_t s);
 
  private:
+  // the launches of one call, planned once: Gram products, Cholesky
+  // descriptors and shifts, triangular solves, R products
+  struct Plan {
+    kern::TNPlan gram;
+    DevBuf<SmallMat> mats;
+    DevBuf<double> shift_on, shift_off;
+    kern::TrsmPlan trsm;
+    kern::NNPlan acc;
+    int nmax = 0;
+  } plan_;
+  void make_plan(const std::vector<OrthSeg>& segs, int kcap, bool with_acc, hipStream_t s);
+  void gram(const double* Y, int64_t ld, int kcap, hipStream_t s);
   kern::Workspace ws_;
   DevBuf<double> G_, R_, Ri_, Racc_, Rtmp_, shift_, gerr_;
   DevBuf<int> info_;

@@ -329,10 +329,19 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
   } else {
     range_pass(false, sk_fwd, Xb_.get(), Yb_.get(), true);
   }
+  // Intermediate bases of the mixed path come from single-BF16 passes, whose
+  // rounding keeps them well conditioned (cond ~ 1e3): one shifted
+  // CholeskyQR pass orthonormalises them to ~1e-10, far below their own error.
+  auto orth_mid = [&](const std::vector<OrthSeg>& os, double* Y) {
+    if (lp)
+      orth_.run_once(os, Y, Tb_.get(), N, kcap, stream);
+    else
+      orth_.run(os, Y, Tb_.get(), N, kcap, nullptr, stream);
+  };
   for (int it = 0; it < q; ++it) {
-    orth_.run(orow_rsvd, Yb_.get(), Tb_.get(), N, kcap, nullptr, stream);
+    orth_mid(orow_rsvd, Yb_.get());
     range_pass(true, sk_bwd_rsvd, Yb_.get(), Xb_.get(), false);
-    orth_.run(ocol_rsvd, Xb_.get(), Tb_.get(), N, kcap, nullptr, stream);
+    orth_mid(ocol_rsvd, Xb_.get());
     range_pass(false, sk_fwd_rsvd, Xb_.get(), Yb_.get(), it == q - 1);
   }
   orth_.run(orow, Yb_.get(), Tb_.get(), N, kcap, nullptr, stream);

@@ -332,25 +332,26 @@ __global__ __launch_bounds__(64) void k_seg_trsm_upper(const SegNN* __restrict__
   for (int j = 0; j < k; ++j) Y[sg.yoff + row + (int64_t)j * ldy] = q[j * 64 + t];
 }
 
-void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X,
-                    int64_t ldx, const double* R, int64_t ldr, double* Y, int64_t ldy, hipStream_t s) {
-  if (segs.empty()) return;
+static int trsm_tiles(const std::vector<SegNN>& segs, std::vector<NNTile>& tiles) {
+  tiles.clear();
   int kmax = 0;
   for (const auto& g : segs)
     if (g.len > 0 && g.q > 0) kmax = std::max(kmax, g.q);
-  if (kmax == 0) return;
+  if (kmax == 0) return 0;
   HMAT_REQUIRE(kmax <= 300, HMAT_ERR_ARG, "seg_trsm_upper: k > 300");
   const int rows = kmax <= 64 ? kWG : 64;
-  std::vector<NNTile> tiles;
   for (size_t i = 0; i < segs.size(); ++i) {
     const auto& g = segs[i];
     if (g.len <= 0 || g.q <= 0) continue;
     for (int r0 = 0; r0 < g.len; r0 += rows) tiles.push_back({(int32_t)i, r0, 0, 0});
   }
-  (void)dsegs;
-  const SegNN* ds = stage(segs, s);
-  const NNTile* dt = stage(tiles, s);
-  const unsigned nt = (unsigned)tiles.size();
+  return kmax;
+}
+
+static void launch_trsm_raw(int kmax, int ntiles, const SegNN* ds, const NNTile* dt, const double* X, int64_t ldx,
+                            const double* R, int64_t ldr, double* Y, int64_t ldy, hipStream_t s) {
+  if (!ntiles) return;
+  const unsigned nt = (unsigned)ntiles;
   if (kmax <= 16) {
     k_seg_trsm_reg<16><<<nt, kWG, 0, s>>>(ds, dt, X, ldx, R, ldr, Y, ldy);
   } else if (kmax <= 32) {
@@ -367,6 +368,30 @@ void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const 
     k_seg_trsm_upper<<<nt, 64, lds, s>>>(ds, dt, X, ldx, R, ldr, Y, ldy, r_in_lds);
   }
   HIP_CHECK(hipGetLastError());
+}
+
+void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X,
+                    int64_t ldx, const double* R, int64_t ldr, double* Y, int64_t ldy, hipStream_t s) {
+  (void)dsegs;
+  if (segs.empty()) return;
+  std::vector<NNTile> tiles;
+  const int kmax = trsm_tiles(segs, tiles);
+  if (!kmax) return;
+  launch_trsm_raw(kmax, (int)tiles.size(), stage(segs, s), stage(tiles, s), X, ldx, R, ldr, Y, ldy, s);
+}
+
+void plan_trsm(const std::vector<SegNN>& segs, TrsmPlan& p, hipStream_t s) {
+  std::vector<NNTile> tiles;
+  p.kmax = segs.empty() ? 0 : trsm_tiles(segs, tiles);
+  p.ntiles = (int)tiles.size();
+  if (!p.ntiles) return;
+  p.segs.upload(segs, s);
+  p.tiles.upload(tiles, s);
+}
+
+void launch_trsm(const TrsmPlan& p, const double* X, int64_t ldx, const double* R, int64_t ldr, double* Y,
+                 int64_t ldy, hipStream_t s) {
+  launch_trsm_raw(p.kmax, p.ntiles, p.segs.get(), p.tiles.get(), X, ldx, R, ldr, Y, ldy, s);
 }
 
 // --------------------------------------------------------------------------
@@ -792,6 +817,18 @@ __global__ __launch_bounds__(64) void k_chol64(const SmallMat* __restrict__ mats
     for (int i = 0; i < n; ++i) rg[i + (int64_t)c * ld] = r[i + c * lr];
 }
 
+void launch_chol_rinv(const SmallMat* dmats, int nmats, int nmax, double* G, double* R, const double* dshift,
+                      int* info, double* gerr, hipStream_t s) {
+  if (!nmats) return;
+  const size_t lds = (size_t)nmax * (nmax + 1) * sizeof(double);
+  const int use_lds = lds <= 64 * 1024 ? 1 : 0;
+  if (nmax <= 64)
+    k_chol64r<<<(unsigned)nmats, 64, 0, s>>>(dmats, G, R, dshift, info, gerr);
+  else
+    k_chol<<<(unsigned)nmats, kWG, use_lds ? lds : 0, s>>>(dmats, G, R, dshift, info, gerr, use_lds);
+  HIP_CHECK(hipGetLastError());
+}
+
 void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* G, double* R,
                double* Rinv, const std::vector<double>& shift_coef, DevBuf<double>& dshift,
                int* info, double* gerr, hipStream_t s) {
@@ -799,16 +836,9 @@ void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* 
   if (mats.empty()) return;
   int nmax = 0;
   for (auto& mm : mats) nmax = std::max(nmax, mm.n);
-  const size_t lds = (size_t)nmax * (nmax + 1) * sizeof(double);
-  const int use_lds = lds <= 64 * 1024 ? 1 : 0;
   dm.upload(mats, s);
   dshift.upload(shift_coef, s);
-  if (nmax <= 64)
-    k_chol64r<<<(unsigned)mats.size(), 64, 0, s>>>(dm.get(), G, R, dshift.get(), info, gerr);
-  else
-    k_chol<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), G, R, dshift.get(), info, gerr,
-                                                                  use_lds);
-  HIP_CHECK(hipGetLastError());
+  launch_chol_rinv(dm.get(), (int)mats.size(), nmax, G, R, dshift.get(), info, gerr, s);
 }
 
 // One-sided Jacobi (Hestenes) with a round-robin pair schedule; the four

@@ -123,6 +123,17 @@ void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* 
 // grows with cond(R)).  Uses xoff/toff/yoff/len/q of each SegNN.
 void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X,
                     int64_t ldx, const double* R, int64_t ldr, double* Y, int64_t ldy, hipStream_t s);
+// The same with its descriptors uploaded once and launched any number of
+// times.  For k <= 64 each row is read completely before it is written, so
+// X == Y (in place) is allowed there.
+struct TrsmPlan {
+  DevBuf<SegNN> segs;
+  DevBuf<NNTile> tiles;
+  int ntiles = 0, kmax = 0;
+};
+void plan_trsm(const std::vector<SegNN>& segs, TrsmPlan& p, hipStream_t s);
+void launch_trsm(const TrsmPlan& p, const double* X, int64_t ldx, const double* R, int64_t ldr, double* Y,
+                 int64_t ldy, hipStream_t s);
 
 template <class F>
 void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* X, int64_t ldx,
@@ -148,6 +159,9 @@ void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>
 void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* G, double* R,
                double* Rinv, const std::vector<double>& shift_coef, DevBuf<double>& dshift,
                int* info, double* gerr, hipStream_t s);
+// The same on descriptors and shift coefficients already on the device.
+void launch_chol_rinv(const SmallMat* dmats, int nmats, int nmax, double* G, double* R, const double* dshift,
+                      int* info, double* gerr, hipStream_t s);
 
 // One-sided Jacobi SVD of n x n matrices A = U diag(S) V^T (A overwritten
 // by U).  S sorted descending, U/V columns permuted accordingly.  A pair of
