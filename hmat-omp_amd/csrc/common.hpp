@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -49,6 +50,9 @@ class Stager {
   Stager(const Stager&) = delete;
   Stager& operator=(const Stager&) = delete;
   void* put_bytes(const void* h, size_t bytes);
+  // several host arrays in one staged region with one DMA; off[i] receives
+  // the byte offset of array i from the returned device address
+  char* put_gather(const std::pair<const void*, size_t>* items, int n, size_t* off);
   // DMA `bytes` from `h` (pageable, may be freed on return) to `dst`
   void copy_to(void* dst, const void* h, size_t bytes);
 
@@ -68,6 +72,45 @@ template <class T>
 inline const T* stage(const std::vector<T>& v, hipStream_t s) {
   return v.empty() ? nullptr : static_cast<const T*>(stager(s).put_bytes(v.data(), v.size() * sizeof(T)));
 }
+
+// The descriptor arrays of one launch (or one plan), staged together.
+class StageGroup {
+ public:
+  template <class T>
+  int add(const std::vector<T>& v) {
+    items_.push_back({v.data(), v.size() * sizeof(T)});
+    return (int)items_.size() - 1;
+  }
+  // through the stream's staging arena (valid for work enqueued next)
+  void put(hipStream_t s) {
+    off_.assign(items_.size(), 0);
+    base_ = items_.empty() ? nullptr : stager(s).put_gather(items_.data(), (int)items_.size(), off_.data());
+  }
+  // into a device buffer owned by the caller (persistent plans), one DMA
+  template <class Blob>
+  void put_into(Blob& blob, hipStream_t s) {
+    off_.assign(items_.size(), 0);
+    size_t total = 0;
+    for (size_t i = 0; i < items_.size(); ++i) {
+      off_[i] = total;
+      total += (items_[i].second + 255) & ~(size_t)255;
+    }
+    std::vector<uint8_t> h(total);
+    for (size_t i = 0; i < items_.size(); ++i)
+      if (items_[i].second) std::memcpy(h.data() + off_[i], items_[i].first, items_[i].second);
+    blob.upload(h.data(), total, s);
+    base_ = reinterpret_cast<char*>(blob.get());
+  }
+  template <class T>
+  const T* get(int i) const {
+    return items_[i].second ? reinterpret_cast<const T*>(base_ + off_[i]) : nullptr;
+  }
+
+ private:
+  std::vector<std::pair<const void*, size_t>> items_;
+  std::vector<size_t> off_;
+  char* base_ = nullptr;
+};
 
 // Owning device buffer (typed, 256-byte aligned by hipMalloc).
 template <class T>

@@ -91,6 +91,20 @@ void* Stager::put_bytes(const void* h, size_t bytes) {
   return dev_ + o;
 }
 
+char* Stager::put_gather(const std::pair<const void*, size_t>* items, int n, size_t* off) {
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) {
+    off[i] = total;
+    total += (items[i].second + 255) & ~(size_t)255;
+  }
+  if (total == 0) return nullptr;
+  const size_t o = reserve(total);
+  for (int i = 0; i < n; ++i)
+    if (items[i].second) std::memcpy(host_ + o + off[i], items[i].first, items[i].second);
+  HIP_CHECK(hipMemcpyAsync(dev_ + o, host_ + o, total, hipMemcpyHostToDevice, s_));
+  return dev_ + o;
+}
+
 void Stager::copy_to(void* dst, const void* h, size_t bytes) {
   const size_t o = reserve((bytes + 255) & ~(size_t)255);
   std::memcpy(host_ + o, h, bytes);
@@ -183,7 +197,7 @@ void Orthogonalizer::make_plan(const std::vector<OrthSeg>& segs, int kcap, bool 
   const int64_t kk = (int64_t)kcap * kcap;
   std::vector<SegTN> tn(nb);
   std::vector<SmallMat> sm(nb);
-  std::vector<SegNN> nn(nb), acc(nb);
+  std::vector<SegNN> nn(nb), acc;
   std::vector<double> coef(nb), zero(nb, 0.0);
   plan_.nmax = 0;
   for (size_t b = 0; b < nb; ++b) {
@@ -193,25 +207,55 @@ void Orthogonalizer::make_plan(const std::vector<OrthSeg>& segs, int kcap, bool 
     // shift 11 (m n + n (n+1)) u ||X||^2 with ||X||_F^2 = trace(G) >= ||X||_2^2
     coef[b] = 11.0 * ((double)g.len * g.k + (double)g.k * (g.k + 1)) * 1.1102230246251565e-16;
     nn[b] = SegNN{g.off, (int64_t)b * kk, g.off, g.len, g.k, g.k, 0};
-    acc[b] = SegNN{(int64_t)b * kk, (int64_t)b * kk, (int64_t)b * kk, g.k, g.k, g.k, 0};
+    if (with_acc) acc.push_back(SegNN{(int64_t)b * kk, (int64_t)b * kk, (int64_t)b * kk, g.k, g.k, g.k, 0});
     plan_.nmax = std::max(plan_.nmax, g.k);
   }
-  kern::plan_tn(tn, plan_.gram, s);
-  plan_.mats.upload(sm, s);
-  plan_.shift_on.upload(coef, s);
-  plan_.shift_off.upload(zero, s);
-  kern::plan_trsm(nn, plan_.trsm, s);
-  if (with_acc) kern::plan_nn(acc, plan_.acc, s);
+  std::vector<kern::TNItem> items;
+  std::vector<kern::TNOut> outs;
+  plan_.nslots = kern::tn_items(tn, items, outs);
+  plan_.nitems = (int)items.size();
+  plan_.nouts = (int)outs.size();
+  std::vector<kern::NNTile> trt, acct;
+  plan_.kmax = kern::trsm_tiles(nn, trt);
+  plan_.tr_ntiles = (int)trt.size();
+  plan_.acc_mode = with_acc ? kern::nn_tiles(acc, acct) : 0;
+  plan_.acc_ntiles = (int)acct.size();
+  StageGroup g;
+  const int i_tn = g.add(tn), i_it = g.add(items), i_out = g.add(outs), i_sm = g.add(sm), i_on = g.add(coef),
+            i_off = g.add(zero), i_tr = g.add(nn), i_trt = g.add(trt), i_acc = g.add(acc), i_acct = g.add(acct);
+  g.put_into(plan_.blob, s);
+  plan_.tn_segs = g.get<SegTN>(i_tn);
+  plan_.tn_items = g.get<kern::TNItem>(i_it);
+  plan_.tn_outs = g.get<kern::TNOut>(i_out);
+  plan_.mats = g.get<SmallMat>(i_sm);
+  plan_.shift_on = g.get<double>(i_on);
+  plan_.shift_off = g.get<double>(i_off);
+  plan_.tr_segs = g.get<SegNN>(i_tr);
+  plan_.tr_tiles = g.get<kern::NNTile>(i_trt);
+  plan_.acc_segs = g.get<SegNN>(i_acc);
+  plan_.acc_tiles = g.get<kern::NNTile>(i_acct);
   G_.reserve(nb * kk);
   R_.reserve(nb * kk);
   Ri_.reserve(nb * kk);
   Racc_.reserve(nb * kk);
   if (with_acc) Rtmp_.reserve(nb * kk);
-  ws_.part.reserve((size_t)std::max(plan_.gram.nslots, 1) * 4096);
+  ws_.part.reserve((size_t)std::max(plan_.nslots, 1) * 4096);
 }
 
 void Orthogonalizer::gram(const double* Y, int64_t ld, int kcap, hipStream_t s) {
-  kern::launch_tn<double>(plan_.gram, Y, ld, Y, ld, G_.get(), kcap, 1.0, 0.0, ws_.part.get(), s);
+  kern::launch_tn_raw<double>(plan_.nitems, plan_.nouts, plan_.tn_segs, plan_.tn_items, plan_.tn_outs, Y, ld, Y,
+                              ld, G_.get(), kcap, 1.0, 0.0, ws_.part.get(), s);
+}
+
+void Orthogonalizer::trsm(const double* X, int64_t ld, const double* R, int kcap, double* Y, hipStream_t s) {
+  kern::launch_trsm_raw(plan_.kmax, plan_.tr_ntiles, plan_.tr_segs, plan_.tr_tiles, X, ld, R, kcap, Y, ld, s);
+}
+
+// Racc <- R_ * Racc (upper triangular products)
+void Orthogonalizer::acc_product(int kcap, hipStream_t s) {
+  kern::launch_nn_raw<double>(plan_.acc_mode, plan_.acc_ntiles, plan_.acc_segs, plan_.acc_tiles, R_.get(), kcap,
+                              Racc_.get(), kcap, Rtmp_.get(), kcap, 1.0, 0.0, s);
+  std::swap(Racc_, Rtmp_);
 }
 
 void Orthogonalizer::run_once(const std::vector<OrthSeg>& segs, double* Y, double* T, int64_t ld, int kcap,
@@ -222,12 +266,12 @@ void Orthogonalizer::run_once(const std::vector<OrthSeg>& segs, double* Y, doubl
   info_.reserve(nb);
   gerr_.reserve(nb);
   gram(Y, ld, kcap, s);
-  kern::launch_chol_rinv(plan_.mats.get(), (int)nb, plan_.nmax, G_.get(), R_.get(), plan_.shift_on.get(),
-                         info_.get(), gerr_.get(), s);
-  if (plan_.trsm.kmax <= 64) {
-    kern::launch_trsm(plan_.trsm, Y, ld, R_.get(), kcap, Y, ld, s);  // in place
+  kern::launch_chol_rinv(plan_.mats, (int)nb, plan_.nmax, G_.get(), R_.get(), plan_.shift_on, info_.get(),
+                         gerr_.get(), s);
+  if (plan_.kmax <= 64) {
+    trsm(Y, ld, R_.get(), kcap, Y, s);  // in place
   } else {
-    kern::launch_trsm(plan_.trsm, Y, ld, R_.get(), kcap, T, ld, s);
+    trsm(Y, ld, R_.get(), kcap, T, s);
     std::vector<SegNN> cp(nb);
     for (size_t b = 0; b < nb; ++b) cp[b] = SegNN{segs[b].off, 0, segs[b].off, segs[b].len, 0, segs[b].k, 0};
     kern::copy_segs(cp, dnn_, T, ld, Y, ld, s);
@@ -249,16 +293,14 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
   // shifted pass Y -> T (R_1 in Racc_), unshifted pass T -> Y (R_2 in R_),
   // and the Gram matrix of the result with its deviation from I.
   gram(Y, ld, kcap, s);
-  kern::launch_chol_rinv(plan_.mats.get(), nm, plan_.nmax, G_.get(), Racc_.get(), plan_.shift_on.get(), info,
-                         gerr, s);
-  kern::launch_trsm(plan_.trsm, Y, ld, Racc_.get(), kcap, T, ld, s);
+  kern::launch_chol_rinv(plan_.mats, nm, plan_.nmax, G_.get(), Racc_.get(), plan_.shift_on, info, gerr, s);
+  trsm(Y, ld, Racc_.get(), kcap, T, s);
   gram(T, ld, kcap, s);
-  kern::launch_chol_rinv(plan_.mats.get(), nm, plan_.nmax, G_.get(), R_.get(), plan_.shift_off.get(), info + nb,
-                         gerr + nb, s);
-  kern::launch_trsm(plan_.trsm, T, ld, R_.get(), kcap, Y, ld, s);
+  kern::launch_chol_rinv(plan_.mats, nm, plan_.nmax, G_.get(), R_.get(), plan_.shift_off, info + nb, gerr + nb, s);
+  trsm(T, ld, R_.get(), kcap, Y, s);
   gram(Y, ld, kcap, s);
-  kern::launch_chol_rinv(plan_.mats.get(), nm, plan_.nmax, G_.get(), Ri_.get(), plan_.shift_off.get(),
-                         info + 2 * nb, gerr + 2 * nb, s);
+  kern::launch_chol_rinv(plan_.mats, nm, plan_.nmax, G_.get(), Ri_.get(), plan_.shift_off, info + 2 * nb,
+                         gerr + 2 * nb, s);
   const auto h = download(chk_.get(), info_bytes + 3 * nb * sizeof(double), s);
   const int* hi = reinterpret_cast<const int*>(h.data());
   const double* hg = reinterpret_cast<const double*>(h.data() + info_bytes);
@@ -278,16 +320,13 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
     dst = Y;
     again = true;
   } else if (Rout) {
-    kern::launch_nn<double>(plan_.acc, R_.get(), kcap, Racc_.get(), kcap, Rtmp_.get(), kcap, 1.0, 0.0, s);
-    std::swap(Racc_, Rtmp_);
+    acc_product(kcap, s);
   }
   // rare: more passes, each checked, until the Gram matrix is I to kOrthTol
   for (int p = 2; again && p < kOrthMaxPasses; ++p) {
     if (!pass(segs, src, dst, ld, kcap, breakdown2 && src == T && p == 2, s)) break;
     if (Rout) {
-      // Racc <- R_p * Racc (upper triangular products)
-      kern::launch_nn<double>(plan_.acc, R_.get(), kcap, Racc_.get(), kcap, Rtmp_.get(), kcap, 1.0, 0.0, s);
-      std::swap(Racc_, Rtmp_);
+      acc_product(kcap, s);
     }
     std::swap(src, dst);
   }

@@ -153,15 +153,26 @@ class Orthogonalizer {
   // the launches of one call, planned once: Gram products, Cholesky
   // descriptors and shifts, triangular solves, R products
   struct Plan {
-    kern::TNPlan gram;
-    DevBuf<SmallMat> mats;
-    DevBuf<double> shift_on, shift_off;
-    kern::TrsmPlan trsm;
-    kern::NNPlan acc;
+    DevBuf<uint8_t> blob;  // every descriptor array below, uploaded with one copy
+    const SegTN* tn_segs = nullptr;
+    const kern::TNItem* tn_items = nullptr;
+    const kern::TNOut* tn_outs = nullptr;
+    int nitems = 0, nouts = 0, nslots = 0;
+    const SmallMat* mats = nullptr;
+    const double* shift_on = nullptr;
+    const double* shift_off = nullptr;
+    const SegNN* tr_segs = nullptr;
+    const kern::NNTile* tr_tiles = nullptr;
+    int tr_ntiles = 0, kmax = 0;
+    const SegNN* acc_segs = nullptr;
+    const kern::NNTile* acc_tiles = nullptr;
+    int acc_ntiles = 0, acc_mode = 0;
     int nmax = 0;
   } plan_;
   void make_plan(const std::vector<OrthSeg>& segs, int kcap, bool with_acc, hipStream_t s);
   void gram(const double* Y, int64_t ld, int kcap, hipStream_t s);
+  void trsm(const double* X, int64_t ld, const double* R, int kcap, double* Y, hipStream_t s);
+  void acc_product(int kcap, hipStream_t s);
   kern::Workspace ws_;
   DevBuf<double> G_, R_, Ri_, Racc_, Rtmp_, shift_, gerr_;
   DevBuf<int> info_;

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_nn_mfma(const SegNN* __restric
 
 // Tiling of a segmented Y = alpha X T + beta Y: the kernel variant (1: MFMA
 // 64 x 64 tiles, else rows per workgroup) and its tiles.
-static int nn_tiles(const std::vector<SegNN>& segs, std::vector<NNTile>& tiles) {
+int nn_tiles(const std::vector<SegNN>& segs, std::vector<NNTile>& tiles) {
   tiles.clear();
   int qmax = 0;
   for (auto& g : segs) qmax = std::max(qmax, g.q);
@@ -179,7 +179,7 @@ static int nn_tiles(const std::vector<SegNN>& segs, std::vector<NNTile>& tiles) 
 }
 
 template <class F>
-static void launch_nn_raw(int mode, int ntiles, const SegNN* ds, const NNTile* dt, const F* X, int64_t ldx,
+void launch_nn_raw(int mode, int ntiles, const SegNN* ds, const NNTile* dt, const F* X, int64_t ldx,
                           const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha, F beta, hipStream_t s) {
   if (!ntiles) return;
   const unsigned g = (unsigned)ntiles;
@@ -208,6 +208,10 @@ void launch_nn(const NNPlan& p, const F* X, int64_t ldx, const F* T, int64_t ldt
                F beta, hipStream_t s) {
   launch_nn_raw<F>(p.mode, p.ntiles, p.segs.get(), p.tiles.get(), X, ldx, T, ldt, Y, ldy, alpha, beta, s);
 }
+template void launch_nn_raw<double>(int, int, const SegNN*, const NNTile*, const double*, int64_t, const double*,
+                                    int64_t, double*, int64_t, double, double, hipStream_t);
+template void launch_nn_raw<float>(int, int, const SegNN*, const NNTile*, const float*, int64_t, const float*,
+                                   int64_t, float*, int64_t, float, float, hipStream_t);
 template void launch_nn<double>(const NNPlan&, const double*, int64_t, const double*, int64_t, double*, int64_t,
                                 double, double, hipStream_t);
 template void launch_nn<float>(const NNPlan&, const float*, int64_t, const float*, int64_t, float*, int64_t, float,
@@ -221,7 +225,10 @@ void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* 
   std::vector<NNTile> tiles;
   const int mode = nn_tiles(segs, tiles);
   if (tiles.empty()) return;
-  launch_nn_raw<F>(mode, (int)tiles.size(), stage(segs, s), stage(tiles, s), X, ldx, T, ldt, Y, ldy, alpha,
+  StageGroup g;
+  const int is = g.add(segs), it = g.add(tiles);
+  g.put(s);
+  launch_nn_raw<F>(mode, (int)tiles.size(), g.get<SegNN>(is), g.get<NNTile>(it), X, ldx, T, ldt, Y, ldy, alpha,
                    beta, s);
 }
 template void seg_gemm_nn<double>(const std::vector<SegNN>&, DevBuf<SegNN>&, const double*, int64_t,
@@ -332,7 +339,7 @@ __global__ __launch_bounds__(64) void k_seg_trsm_upper(const SegNN* __restrict__
   for (int j = 0; j < k; ++j) Y[sg.yoff + row + (int64_t)j * ldy] = q[j * 64 + t];
 }
 
-static int trsm_tiles(const std::vector<SegNN>& segs, std::vector<NNTile>& tiles) {
+int trsm_tiles(const std::vector<SegNN>& segs, std::vector<NNTile>& tiles) {
   tiles.clear();
   int kmax = 0;
   for (const auto& g : segs)
@@ -348,7 +355,7 @@ static int trsm_tiles(const std::vector<SegNN>& segs, std::vector<NNTile>& tiles
   return kmax;
 }
 
-static void launch_trsm_raw(int kmax, int ntiles, const SegNN* ds, const NNTile* dt, const double* X, int64_t ldx,
+void launch_trsm_raw(int kmax, int ntiles, const SegNN* ds, const NNTile* dt, const double* X, int64_t ldx,
                             const double* R, int64_t ldr, double* Y, int64_t ldy, hipStream_t s) {
   if (!ntiles) return;
   const unsigned nt = (unsigned)ntiles;
@@ -377,7 +384,10 @@ void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const 
   std::vector<NNTile> tiles;
   const int kmax = trsm_tiles(segs, tiles);
   if (!kmax) return;
-  launch_trsm_raw(kmax, (int)tiles.size(), stage(segs, s), stage(tiles, s), X, ldx, R, ldr, Y, ldy, s);
+  StageGroup g;
+  const int is = g.add(segs), it = g.add(tiles);
+  g.put(s);
+  launch_trsm_raw(kmax, (int)tiles.size(), g.get<SegNN>(is), g.get<NNTile>(it), X, ldx, R, ldr, Y, ldy, s);
 }
 
 void plan_trsm(const std::vector<SegNN>& segs, TrsmPlan& p, hipStream_t s) {
@@ -497,7 +507,7 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part_mfma(const SegTN* __re
 
 // Work items of a segmented O = alpha X^T Y + beta O: partial 64 x 64
 // products over row chunks (slots of the partial buffer) and their sums.
-static int tn_items(const std::vector<SegTN>& segs, std::vector<TNItem>& items, std::vector<TNOut>& outs) {
+int tn_items(const std::vector<SegTN>& segs, std::vector<TNItem>& items, std::vector<TNOut>& outs) {
   items.clear();
   outs.clear();
   int64_t total_rows = 0;
@@ -528,7 +538,7 @@ static int tn_items(const std::vector<SegTN>& segs, std::vector<TNItem>& items, 
 }
 
 template <class F>
-static void launch_tn_raw(int nitems, int nouts, const SegTN* ds, const TNItem* di, const TNOut* dout,
+void launch_tn_raw(int nitems, int nouts, const SegTN* ds, const TNItem* di, const TNOut* dout,
                           const F* X, int64_t ldx, const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta,
                           F* part, hipStream_t s) {
   if (!nouts) return;
@@ -559,6 +569,10 @@ void launch_tn(const TNPlan& p, const F* X, int64_t ldx, const F* Y, int64_t ldy
   launch_tn_raw<F>(p.nitems, p.nouts, p.segs.get(), p.items.get(), p.outs.get(), X, ldx, Y, ldy, O, ldo, alpha,
                    beta, part, s);
 }
+template void launch_tn_raw<double>(int, int, const SegTN*, const TNItem*, const TNOut*, const double*, int64_t,
+                                    const double*, int64_t, double*, int64_t, double, double, double*, hipStream_t);
+template void launch_tn_raw<float>(int, int, const SegTN*, const TNItem*, const TNOut*, const float*, int64_t,
+                                   const float*, int64_t, float*, int64_t, float, float, float*, hipStream_t);
 template void launch_tn<double>(const TNPlan&, const double*, int64_t, const double*, int64_t, double*, int64_t,
                                 double, double, double*, hipStream_t);
 template void launch_tn<float>(const TNPlan&, const float*, int64_t, const float*, int64_t, float*, int64_t,
@@ -576,8 +590,11 @@ void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* 
   if (outs.empty()) return;
   DevBuf<F>& part = ws.parts<F>();
   part.reserve((size_t)nslots * 4096);
-  launch_tn_raw<F>((int)items.size(), (int)outs.size(), stage(segs, s), stage(items, s), stage(outs, s), X, ldx,
-                   Y, ldy, O, ldo, alpha, beta, part.get(), s);
+  StageGroup g;
+  const int is = g.add(segs), ii = g.add(items), io = g.add(outs);
+  g.put(s);
+  launch_tn_raw<F>((int)items.size(), (int)outs.size(), g.get<SegTN>(is), g.get<TNItem>(ii), g.get<TNOut>(io), X,
+                   ldx, Y, ldy, O, ldo, alpha, beta, part.get(), s);
 }
 template void seg_gemm_tn<double>(const std::vector<SegTN>&, DevBuf<SegTN>&, const double*, int64_t,
                                   const double*, int64_t, double*, int64_t, double, double,
@@ -836,9 +853,12 @@ void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* 
   if (mats.empty()) return;
   int nmax = 0;
   for (auto& mm : mats) nmax = std::max(nmax, mm.n);
-  dm.upload(mats, s);
-  dshift.upload(shift_coef, s);
-  launch_chol_rinv(dm.get(), (int)mats.size(), nmax, G, R, dshift.get(), info, gerr, s);
+  (void)dm;
+  (void)dshift;
+  StageGroup g;
+  const int im = g.add(mats), ic = g.add(shift_coef);
+  g.put(s);
+  launch_chol_rinv(g.get<SmallMat>(im), (int)mats.size(), nmax, G, R, g.get<double>(ic), info, gerr, s);
 }
 
 // One-sided Jacobi (Hestenes) with a round-robin pair schedule; the four

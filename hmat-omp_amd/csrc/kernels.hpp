@@ -103,6 +103,22 @@ struct TNPlan {
   DevBuf<TNOut> outs;
   int nitems = 0, nouts = 0, nslots = 0;  // launch_tn needs nslots * 4096 partials
 };
+// Tilings and launches on descriptors already on the device (callers that
+// upload several launches' descriptors together).  nn_tiles returns the
+// kernel variant, trsm_tiles the largest k, tn_items the partial slots.
+int nn_tiles(const std::vector<SegNN>& segs, std::vector<NNTile>& tiles);
+template <class F>
+void launch_nn_raw(int mode, int ntiles, const SegNN* ds, const NNTile* dt, const F* X, int64_t ldx, const F* T,
+                   int64_t ldt, F* Y, int64_t ldy, F alpha, F beta, hipStream_t s);
+int trsm_tiles(const std::vector<SegNN>& segs, std::vector<NNTile>& tiles);
+void launch_trsm_raw(int kmax, int ntiles, const SegNN* ds, const NNTile* dt, const double* X, int64_t ldx,
+                     const double* R, int64_t ldr, double* Y, int64_t ldy, hipStream_t s);
+int tn_items(const std::vector<SegTN>& segs, std::vector<TNItem>& items, std::vector<TNOut>& outs);
+template <class F>
+void launch_tn_raw(int nitems, int nouts, const SegTN* ds, const TNItem* di, const TNOut* dout, const F* X,
+                   int64_t ldx, const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta, F* part,
+                   hipStream_t s);
+
 void plan_nn(const std::vector<SegNN>& segs, NNPlan& p, hipStream_t s);
 template <class F>
 void launch_nn(const NNPlan& p, const F* X, int64_t ldx, const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha,

@@ -449,13 +449,12 @@ static SketchPlan plan_sketch(const std::vector<SegSk>& segs, int kcap, int ncol
   return p;
 }
 
-static void sum_splits(const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs, const SketchPlan& p,
-                       const double* part, double* Y, int64_t ldy, hipStream_t s) {
+// Tiles of the split-K sum (64 rows of one segment each).
+static std::vector<SumTile> sum_tiles(const std::vector<SegSk>& segs) {
   std::vector<SumTile> st;
   for (size_t i = 0; i < segs.size(); ++i)
     for (int r0 = 0; r0 < segs[i].outlen; r0 += 64) st.push_back({(int32_t)i, r0});
-  k_sketch_sum<<<(unsigned)st.size(), kWGs, 0, s>>>(dsegs.get(), stage(st, s), part, p.nsplit, p.span, Y, ldy);
-  HIP_CHECK(hipGetLastError());
+  return st;
 }
 
 template <class Gen, bool TRANS>
@@ -490,8 +489,14 @@ static void run_sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& s
   const int rows_per_wg = 64 * RF;
   SketchPlan p = plan_sketch(segs, kcap, NT * 16, rows_per_wg, ldy, want);
   if (p.tiles.empty()) return;
-  dsegs.upload(segs, s);
-  const SkTile* dt = stage(p.tiles, s);
+  (void)dsegs;
+  // segments, tiles and (split-K) sum tiles staged together
+  const std::vector<SumTile> st = p.nsplit > 1 ? sum_tiles(segs) : std::vector<SumTile>();
+  StageGroup sgp;
+  const int is = sgp.add(segs), it = sgp.add(p.tiles), isum = sgp.add(st);
+  sgp.put(s);
+  const SegSk* ds = sgp.get<SegSk>(is);
+  const SkTile* dt = sgp.get<SkTile>(it);
   double* out = Y;
   if (p.nsplit > 1) {
     ws.part.reserve((size_t)(p.span * p.nsplit));
@@ -504,13 +509,17 @@ static void run_sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& s
   if (!prof_on) prof.cancel();
   with_gen(g, [&](auto gen) {
     if (trans)
-      launch_sketch<decltype(gen), true>(gen, NT, RF, grid, dsegs.get(), dt, X, ldx, out, ldy, p.span, s);
+      launch_sketch<decltype(gen), true>(gen, NT, RF, grid, ds, dt, X, ldx, out, ldy, p.span, s);
     else
-      launch_sketch<decltype(gen), false>(gen, NT, RF, grid, dsegs.get(), dt, X, ldx, out, ldy, p.span, s);
+      launch_sketch<decltype(gen), false>(gen, NT, RF, grid, ds, dt, X, ldx, out, ldy, p.span, s);
   });
   HIP_CHECK(hipGetLastError());
   prof.finish();
-  if (p.nsplit > 1) sum_splits(segs, dsegs, p, ws.part.get(), Y, ldy, s);
+  if (p.nsplit > 1) {
+    k_sketch_sum<<<(unsigned)st.size(), kWGs, 0, s>>>(ds, sgp.get<SumTile>(isum), ws.part.get(), p.nsplit, p.span,
+                                                     Y, ldy);
+    HIP_CHECK(hipGetLastError());
+  }
 }
 
 // Shape class of a sketch launch for the autotuner.
@@ -614,8 +623,12 @@ void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>
   auto run = [&](int64_t want, bool prof_on) {
     SketchPlan p = plan_sketch(segs, kcap, 64, 128, ldy, want);
     if (p.tiles.empty()) return;
-    dsegs.upload(segs, s);
-    const SkTile* dt = stage(p.tiles, s);
+    const std::vector<SumTile> st = p.nsplit > 1 ? sum_tiles(segs) : std::vector<SumTile>();
+    StageGroup sgp;
+    const int is = sgp.add(segs), it = sgp.add(p.tiles), isum = sgp.add(st);
+    sgp.put(s);
+    const SegSk* ds = sgp.get<SegSk>(is);
+    const SkTile* dt = sgp.get<SkTile>(it);
     double* out = Y;
     if (p.nsplit > 1) {
       ws.part.reserve((size_t)(p.span * p.nsplit));
@@ -629,15 +642,19 @@ void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>
     if (!prof_on) prof.cancel();
     with_gen_lp(g, [&](auto gen) {
       if (trans)
-        launch_sketch_lp<decltype(gen), true>(gen, split, grid, dsegs.get(), dt, ws.bf.get(), ldx, ps, out,
+        launch_sketch_lp<decltype(gen), true>(gen, split, grid, ds, dt, ws.bf.get(), ldx, ps, out,
                                               ldy, p.span, s);
       else
-        launch_sketch_lp<decltype(gen), false>(gen, split, grid, dsegs.get(), dt, ws.bf.get(), ldx, ps,
+        launch_sketch_lp<decltype(gen), false>(gen, split, grid, ds, dt, ws.bf.get(), ldx, ps,
                                                out, ldy, p.span, s);
     });
     HIP_CHECK(hipGetLastError());
     prof.finish();
-    if (p.nsplit > 1) sum_splits(segs, dsegs, p, ws.part.get(), Y, ldy, s);
+    if (p.nsplit > 1) {
+      k_sketch_sum<<<(unsigned)st.size(), kWGs, 0, s>>>(ds, sgp.get<SumTile>(isum), ws.part.get(), p.nsplit,
+                                                       p.span, Y, ldy);
+      HIP_CHECK(hipGetLastError());
+    }
   };
   int64_t want = 1024;
   if (autotune_enabled()) {
