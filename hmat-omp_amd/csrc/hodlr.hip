@@ -734,7 +734,8 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
   ir_y_.reserve((size_t)N * nrhs);
   ir_r_.reserve((size_t)N * nrhs);
   ir_n_.reserve((size_t)2 * nrhs);
-  kern::col_norms2(dB, ldb, N, nrhs, ir_n_.get(), stream);
+  ir_p_.reserve((size_t)64 * nrhs);
+  kern::col_norms2(dB, ldb, N, nrhs, ir_n_.get(), ir_p_.get(), stream);
   auto bn = download(ir_n_.get(), nrhs, stream);
   double resid = 0.0, prev = 0.0;
   int it = 0;
@@ -742,7 +743,7 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
   for (;; ++it) {
     matvec_enqueue(dX, ldx, nrhs, ir_y_.get(), N);
     kern::axpby_cols(N, nrhs, 1.0, dB, ldb, -1.0, ir_y_.get(), N, ir_r_.get(), N, stream);
-    kern::col_norms2(ir_r_.get(), N, N, nrhs, ir_n_.get(), stream);
+    kern::col_norms2(ir_r_.get(), N, N, nrhs, ir_n_.get(), ir_p_.get(), stream);
     auto rn = download(ir_n_.get(), nrhs, stream);
     resid = 0.0;
     for (int j = 0; j < nrhs; ++j) {

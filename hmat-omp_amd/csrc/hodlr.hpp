@@ -209,7 +209,7 @@ class HodlrEngine {
 
   HFactor<double> f64_;
   HFactor<float> f32_;
-  DevBuf<double> ir_y_, ir_r_, ir_n_;
+  DevBuf<double> ir_y_, ir_r_, ir_n_, ir_p_;
 
   kern::Workspace ws_;
   Orthogonalizer orth_;

@@ -734,15 +734,20 @@ __global__ __launch_bounds__(64) void k_chol64r(const SmallMat* __restrict__ mat
                                                 const double* __restrict__ G, double* __restrict__ R,
                                                 const double* __restrict__ shift_coef,
                                                 int* __restrict__ info, double* __restrict__ gerr) {
+  // G staged through LDS with coalesced loads (column c of G in gs[c][*])
+  __shared__ double gs[64][65];
   const SmallMat m = mats[blockIdx.x];
   const int n = m.n, ld = m.ld;
   const double* g = G + m.off;
   double* rg = R + m.off;
   const int c = threadIdx.x;
+  for (int j = 0; j < n; ++j)
+    if (c < n) gs[j][c] = g[c + (int64_t)j * ld];
+  __syncthreads();
   double tr = 0.0, dev = 0.0;
   if (c < n) {
-    tr = g[c + (int64_t)c * ld];
-    for (int i = 0; i < n; ++i) dev = fmax(dev, fabs(g[i + (int64_t)c * ld] - (i == c ? 1.0 : 0.0)));
+    tr = gs[c][c];
+    for (int i = 0; i < n; ++i) dev = fmax(dev, fabs(gs[c][i] - (i == c ? 1.0 : 0.0)));
   }
   for (int o = 32; o > 0; o >>= 1) {
     tr += __shfl_xor(tr, o);
@@ -755,8 +760,7 @@ __global__ __launch_bounds__(64) void k_chol64r(const SmallMat* __restrict__ mat
   const double shift = shift_coef[blockIdx.x] * tr;
   double col[64];
 #pragma unroll
-  for (int i = 0; i < 64; ++i)
-    col[i] = (c < n && i <= c) ? g[i + (int64_t)c * ld] + (i == c ? shift : 0.0) : 0.0;
+  for (int i = 0; i < 64; ++i) col[i] = (c < n && i <= c) ? gs[c][i] + (i == c ? shift : 0.0) : 0.0;
   if (!(tr > 0.0)) {
     if (!(tr == 0.0) && c == 0) info[blockIdx.x] = -1;  // NaN
     if (c < n)
@@ -1677,19 +1681,33 @@ void axpby_cols(int64_t n, int ncols, double alpha, const double* X, int64_t ldx
   HIP_CHECK(hipGetLastError());
 }
 
-__global__ void k_col_norms2(const double* __restrict__ X, int64_t ldx, int64_t n,
-                             double* __restrict__ out) {
+// Squared column norms in two deterministic stages: kNormParts workgroups
+// per column sum contiguous chunks, then one workgroup per column adds the
+// partial sums in order.
+constexpr int kNormParts = 64;
+__global__ void k_col_norms2_part(const double* __restrict__ X, int64_t ldx, int64_t n,
+                                  double* __restrict__ part) {
   __shared__ double red[kWG / 64];
-  const double* x = X + (int64_t)blockIdx.x * ldx;
+  const double* x = X + (int64_t)blockIdx.y * ldx;
+  const int64_t chunk = (n + kNormParts - 1) / kNormParts;
+  const int64_t b = blockIdx.x * chunk, e = min(n, b + chunk);
   double s = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kWG) s += x[i] * x[i];
+  for (int64_t i = b + threadIdx.x; i < e; i += kWG) s += x[i] * x[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) part[(int64_t)blockIdx.y * kNormParts + blockIdx.x] = s;
+}
+__global__ void k_col_norms2_sum(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double red[kWG / 64];
+  double s = threadIdx.x < kNormParts ? part[(int64_t)blockIdx.x * kNormParts + threadIdx.x] : 0.0;
   s = block_sum(s, red);
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
-void col_norms2(const double* X, int64_t ldx, int64_t n, int ncols, double* out, hipStream_t s) {
+void col_norms2(const double* X, int64_t ldx, int64_t n, int ncols, double* out, double* part, hipStream_t s) {
   if (ncols <= 0) return;
-  k_col_norms2<<<(unsigned)ncols, kWG, 0, s>>>(X, ldx, n, out);
+  k_col_norms2_part<<<dim3(kNormParts, (unsigned)ncols), kWG, 0, s>>>(X, ldx, n, part);
+  HIP_CHECK(hipGetLastError());
+  k_col_norms2_sum<<<(unsigned)ncols, kWG, 0, s>>>(part, out);
   HIP_CHECK(hipGetLastError());
 }
 

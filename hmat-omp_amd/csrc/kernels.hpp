@@ -269,7 +269,8 @@ void add_f32_cols(int64_t n, int ncols, const float* D, int64_t ldd, double* X, 
                   hipStream_t s);
 
 // Per-column 2-norms squared of an n-row, ncols column-major array.
-void col_norms2(const double* X, int64_t ldx, int64_t n, int ncols, double* out, hipStream_t s);
+// part: scratch of 64 * ncols doubles
+void col_norms2(const double* X, int64_t ldx, int64_t n, int ncols, double* out, double* part, hipStream_t s);
 
 }  // namespace kern
 }  // namespace hmat
