@@ -126,7 +126,25 @@ void HodlrEngine::build_levels(const GenDesc& g, const double* host_omega, int64
                  "omega_len " + std::to_string(omega_len) + " != expected " + std::to_string(pos));
   }
 
-  for (int c = 1; c <= tree.max_depth; ++c) build_level(c, g, omega_off, host_omega);
+  // Depths are built together in groups: every launch of a group covers
+  // all its blocks.  A group's scratch (three N x k arrays per depth) is
+  // kept within kGroupScratchBytes, and its BF16 planes below 2^30 entries.
+  {
+    int c0 = 1;
+    while (c0 <= tree.max_depth) {
+      int c1 = c0;
+      int kc = group_kcap(c0, c0);
+      while (c1 + 1 <= tree.max_depth) {
+        const int kn = std::max(kc, group_kcap(c1 + 1, c1 + 1));
+        const double per = (double)N * kn * (c1 + 2 - c0);
+        if (per * 24.0 > kGroupScratchBytes || per * 1.0 >= (double)(1 << 29)) break;
+        kc = kn;
+        ++c1;
+      }
+      build_group(c0, c1, g, omega_off, host_omega);
+      c0 = c1 + 1;
+    }
+  }
 
   // leaf diagonal blocks D (hodlr_routines_omp.h:38-40)
   const int64_t lm = tree.lmax;
@@ -213,71 +231,99 @@ void HodlrEngine::matvec_async(const double* dX, int64_t ldx, int nrhs, double* 
   }
 }
 
-void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t>& omega_off,
-                              const double* host_omega) {
-  HLevel& L = lv[c];
-  L.ids = tree.by_depth[c];
-  const int ns = (int)L.ids.size();
-  L.sib.assign(ns, 0);
-  std::vector<BlockMode> mode(ns);
-  std::vector<int> k(ns);
-  std::vector<int64_t> st(ns), ln(ns), sst(ns), sln(ns);
-  int kcap = 0;
-  for (int s = 0; s < ns; ++s) {
-    const TNode& nd = tree.nodes[L.ids[s]];
-    const TNode& p = tree.nodes[nd.parent];
-    const int sib_id = (p.left == L.ids[s]) ? p.right : p.left;
-    L.sib[s] = tree.nodes[sib_id].pos;
-    st[s] = nd.start;
-    ln[s] = nd.len;
-    sst[s] = tree.nodes[sib_id].start;
-    sln[s] = tree.nodes[sib_id].len;
-    const int64_t m = ln[s], n = sln[s];
-    if (uses_rsvd(m, n, opt)) {
-      mode[s] = BlockMode::RSVD;
-      k[s] = sketch_width(m, n, opt);
-    } else if (m <= n) {
-      mode[s] = BlockMode::ExactWide;
-      k[s] = (int)m;
-    } else {
-      mode[s] = BlockMode::ExactTall;
-      k[s] = (int)n;
+// Largest sketch width of the blocks of depths c0..c1.
+int HodlrEngine::group_kcap(int c0, int c1) const {
+  int kc = 0;
+  for (int c = c0; c <= c1; ++c)
+    for (int id : tree.by_depth[c]) {
+      const TNode& nd = tree.nodes[id];
+      const TNode& p = tree.nodes[nd.parent];
+      const TNode& sb = tree.nodes[(p.left == id) ? p.right : p.left];
+      const int64_t m = nd.len, n = sb.len;
+      kc = std::max(kc, uses_rsvd(m, n, opt) ? sketch_width(m, n, opt) : (int)std::min(m, n));
     }
-    kcap = std::max(kcap, k[s]);
+  return kc;
+}
+
+void HodlrEngine::build_group(int c0, int c1, const GenDesc& g, const std::vector<int64_t>& omega_off,
+                              const double* host_omega) {
+  // the segments of depths c0..c1, depth by depth; segment t of the group
+  // uses column block (lvl[t] - c0) of the N x (kcap * ncl) scratch arrays
+  const int ncl = c1 - c0 + 1;
+  std::vector<BlockMode> mode;
+  std::vector<int> k, lvl, pos;
+  std::vector<int64_t> st, ln, sst, sln;
+  std::vector<int> base(ncl + 1, 0);
+  for (int c = c0; c <= c1; ++c) {
+    HLevel& L = lv[c];
+    L.ids = tree.by_depth[c];
+    const int ns = (int)L.ids.size();
+    L.sib.assign(ns, 0);
+    base[c - c0] = (int)k.size();
+    for (int s = 0; s < ns; ++s) {
+      const TNode& nd = tree.nodes[L.ids[s]];
+      const TNode& p = tree.nodes[nd.parent];
+      const int sib_id = (p.left == L.ids[s]) ? p.right : p.left;
+      L.sib[s] = tree.nodes[sib_id].pos;
+      st.push_back(nd.start);
+      ln.push_back(nd.len);
+      sst.push_back(tree.nodes[sib_id].start);
+      sln.push_back(tree.nodes[sib_id].len);
+      lvl.push_back(c);
+      pos.push_back(s);
+      const int64_t m = nd.len, n = tree.nodes[sib_id].len;
+      if (uses_rsvd(m, n, opt)) {
+        mode.push_back(BlockMode::RSVD);
+        k.push_back(sketch_width(m, n, opt));
+      } else if (m <= n) {
+        mode.push_back(BlockMode::ExactWide);
+        k.push_back((int)m);
+      } else {
+        mode.push_back(BlockMode::ExactTall);
+        k.push_back((int)n);
+      }
+    }
   }
+  base[ncl] = (int)k.size();
+  const int nt = (int)k.size();
+  int kcap = 0;
+  for (int kv : k) kcap = std::max(kcap, kv);
   HMAT_REQUIRE(kcap <= 300, HMAT_ERR_ARG, "block sketch width above 300 is not supported");
-  const int64_t nk = N * (int64_t)kcap;
+  const int64_t lvs = N * (int64_t)kcap;  // elements of one depth's column block
+  auto cb = [&](int t) { return (int64_t)(lvl[t] - c0) * lvs; };
+  const int64_t nk = lvs * ncl;
   Yb_.reserve(nk); Xb_.reserve(nk); Tb_.reserve(nk);
   HIP_CHECK(hipMemsetAsync(Yb_.get(), 0, nk * sizeof(double), stream));
   HIP_CHECK(hipMemsetAsync(Xb_.get(), 0, nk * sizeof(double), stream));
   const int64_t kk = (int64_t)kcap * kcap;
-  Rb_.reserve(ns * kk); Vr_.reserve(ns * kk); Wk_.reserve(2 * ns * kk);
-  TU_.reserve(ns * kk); TV_.reserve(ns * kk);
-  Sv_.reserve((size_t)ns * kcap);
-  info_.reserve(ns);
+  Rb_.reserve(nt * kk); Vr_.reserve(nt * kk); Wk_.reserve(2 * nt * kk);
+  TU_.reserve(nt * kk); TV_.reserve(nt * kk);
+  Sv_.reserve((size_t)nt * kcap);
+  info_.reserve(nt);
 
-  // Omega (RSVD) / identity (exact) operands
+  // Omega (RSVD) / identity (exact) operands; a block's device Omega stream
+  // is keyed by (depth, position), as when depths were built one at a time
   std::vector<SegNN> rnd, idx_col, idx_row;
-  for (int s = 0; s < ns; ++s) {
-    if (mode[s] == BlockMode::RSVD && !host_omega)
-      rnd.push_back(SegNN{sst[s], (int64_t)c * (1 << 24) + s, sst[s], (int32_t)sln[s], 0, k[s], 0});
-    if (mode[s] == BlockMode::ExactTall)
-      idx_col.push_back(SegNN{0, 0, sst[s], (int32_t)sln[s], 0, k[s], 0});
-    if (mode[s] == BlockMode::ExactWide)
-      idx_row.push_back(SegNN{0, 0, st[s], (int32_t)ln[s], 0, k[s], 0});
+  for (int t = 0; t < nt; ++t) {
+    if (mode[t] == BlockMode::RSVD && !host_omega)
+      rnd.push_back(SegNN{sst[t], (int64_t)lvl[t] * (1 << 24) + pos[t], sst[t] + cb(t), (int32_t)sln[t], 0, k[t], 0});
+    if (mode[t] == BlockMode::ExactTall)
+      idx_col.push_back(SegNN{0, 0, sst[t] + cb(t), (int32_t)sln[t], 0, k[t], 0});
+    if (mode[t] == BlockMode::ExactWide)
+      idx_row.push_back(SegNN{0, 0, st[t] + cb(t), (int32_t)ln[t], 0, k[t], 0});
   }
   if (host_omega) {
     bool any = false;
-    for (int s = 0; s < ns; ++s) any |= (mode[s] == BlockMode::RSVD);
+    for (int t = 0; t < nt; ++t) any |= (mode[t] == BlockMode::RSVD);
     if (any) {
       std::vector<double> h((size_t)nk, 0.0);
-      for (int s = 0; s < ns; ++s) {
-        if (mode[s] != BlockMode::RSVD) continue;
-        const int64_t o = omega_off[L.ids[s]];
+      for (int t = 0; t < nt; ++t) {
+        if (mode[t] != BlockMode::RSVD) continue;
+        const int64_t o = omega_off[lv[lvl[t]].ids[pos[t]]];
         HMAT_REQUIRE(o >= 0, HMAT_ERR_STATE, "omega schedule mismatch");
-        for (int j = 0; j < k[s]; ++j)
-          std::memcpy(&h[sst[s] + (int64_t)j * N], host_omega + o + (int64_t)j * sln[s],
-                      sizeof(double) * sln[s]);
+        for (int j = 0; j < k[t]; ++j)
+          std::memcpy(&h[cb(t) + sst[t] + (int64_t)j * N], host_omega + o + (int64_t)j * sln[t],
+                      sizeof(double) * sln[t]);
       }
       HIP_CHECK(hipMemcpyAsync(Xb_.get(), h.data(), nk * sizeof(double), hipMemcpyHostToDevice, stream));
       HIP_CHECK(hipStreamSynchronize(stream));
@@ -291,21 +337,22 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
   // descriptors
   std::vector<SegSk> sk_fwd, sk_fwd_rsvd, sk_fwd_exact, sk_bwd_rsvd, sk_bwd;
   std::vector<OrthSeg> orow_rsvd, orow, ocol_rsvd, ocol;
-  for (int s = 0; s < ns; ++s) {
-    const SegSk fwd{st[s], sst[s], 0, sst[s], st[s], (int32_t)ln[s], (int32_t)sln[s], 0, k[s]};
-    const SegSk bwd{sst[s], st[s], 0, st[s], sst[s], (int32_t)sln[s], (int32_t)ln[s], 0, k[s]};
+  for (int t = 0; t < nt; ++t) {
+    const int64_t o = cb(t);
+    const SegSk fwd{st[t], sst[t], 0, sst[t] + o, st[t] + o, (int32_t)ln[t], (int32_t)sln[t], 0, k[t]};
+    const SegSk bwd{sst[t], st[t], 0, st[t] + o, sst[t] + o, (int32_t)sln[t], (int32_t)ln[t], 0, k[t]};
     sk_bwd.push_back(bwd);
-    ocol.push_back(OrthSeg{sst[s], (int32_t)sln[s], k[s]});
-    if (mode[s] != BlockMode::ExactWide) {
+    ocol.push_back(OrthSeg{sst[t] + o, (int32_t)sln[t], k[t]});
+    if (mode[t] != BlockMode::ExactWide) {
       sk_fwd.push_back(fwd);
-      if (mode[s] != BlockMode::RSVD) sk_fwd_exact.push_back(fwd);
-      orow.push_back(OrthSeg{st[s], (int32_t)ln[s], k[s]});
+      if (mode[t] != BlockMode::RSVD) sk_fwd_exact.push_back(fwd);
+      orow.push_back(OrthSeg{st[t] + o, (int32_t)ln[t], k[t]});
     }
-    if (mode[s] == BlockMode::RSVD) {
+    if (mode[t] == BlockMode::RSVD) {
       sk_fwd_rsvd.push_back(fwd);
       sk_bwd_rsvd.push_back(bwd);
-      orow_rsvd.push_back(OrthSeg{st[s], (int32_t)ln[s], k[s]});
-      ocol_rsvd.push_back(OrthSeg{sst[s], (int32_t)sln[s], k[s]});
+      orow_rsvd.push_back(OrthSeg{st[t] + o, (int32_t)ln[t], k[t]});
+      ocol_rsvd.push_back(OrthSeg{sst[t] + o, (int32_t)sln[t], k[t]});
     }
   }
 
@@ -349,50 +396,58 @@ void HodlrEngine::build_level(int c, const GenDesc& g, const std::vector<int64_t
   kern::sketch(g, true, sk_bwd, dsk_, Yb_.get(), N, Xb_.get(), N, kcap, ws_, stream);
   orth_.run(ocol, Xb_.get(), Tb_.get(), N, kcap, Rb_.get(), stream);
   // R = Ur S Vr^T  =>  A_blk ~ (Q Vr) S (Qb Ur)^T        (compression.h:90-98)
-  std::vector<SmallMat> sm(ns);
-  for (int s = 0; s < ns; ++s) sm[s] = SmallMat{(int64_t)s * kk, k[s], kcap};
+  std::vector<SmallMat> sm(nt);
+  for (int t = 0; t < nt; ++t) sm[t] = SmallMat{(int64_t)t * kk, k[t], kcap};
   // the mixed-precision basis resolves the block to about 2^-23 of its
   // norm (FP32 entries in the last range pass): columns below that are not
   // resolved against each other
   kern::jacobi_svd(sm, dsm_, Rb_.get(), Vr_.get(), Sv_.get(), kcap, Wk_.get(), stream,
                    lp ? kMixedJacobiFloor : 1e-30);
-  auto S = download(Sv_.get(), (size_t)ns * kcap, stream);
+  auto S = download(Sv_.get(), (size_t)nt * kcap, stream);
 
   // rank per block (compression.h:42-46 or fixed)
-  L.rank.assign(ns, 0);
-  L.rmax = 0;
-  for (int s = 0; s < ns; ++s) {
-    const double* sv = &S[(size_t)s * kcap];
+  std::vector<int> rank(nt, 0);
+  int rmax_all = 0;
+  for (int t = 0; t < nt; ++t) {
+    const double* sv = &S[(size_t)t * kcap];
     int r = 0;
     if (opt.fixed_rank) {
-      r = std::min(opt.rank, k[s]);
+      r = std::min(opt.rank, k[t]);
     } else {
-      if (k[s] > 0 && sv[0] > 1e-15)
-        for (int i = 0; i < k[s]; ++i) r += (sv[i] > opt.tol * sv[0]) ? 1 : 0;
+      if (k[t] > 0 && sv[0] > 1e-15)
+        for (int i = 0; i < k[t]; ++i) r += (sv[i] > opt.tol * sv[0]) ? 1 : 0;
       if (r == 0) r = 1;
     }
-    for (int i = 0; i < k[s]; ++i)
+    for (int i = 0; i < k[t]; ++i)
       if (!std::isfinite(sv[i])) throw Error(HMAT_ERR_NONFINITE, "non-finite singular value in build");
-    L.rank[s] = r;
-    L.rmax = std::max(L.rmax, r);
+    rank[t] = r;
+    rmax_all = std::max(rmax_all, r);
   }
-  L.S.assign((size_t)ns * L.rmax, 0.0);
-  for (int s = 0; s < ns; ++s)
-    for (int i = 0; i < L.rank[s]; ++i) L.S[(size_t)s * L.rmax + i] = S[(size_t)s * kcap + i];
-  kern::trunc_prep(sm, dsm_, Rb_.get(), Vr_.get(), Sv_.get(), kcap, stage(L.rank, stream), L.rmax, TU_.get(),
+  kern::trunc_prep(sm, dsm_, Rb_.get(), Vr_.get(), Sv_.get(), kcap, stage(rank, stream), rmax_all, TU_.get(),
                    TV_.get(), 0, stream);
-  const int64_t nr = N * (int64_t)L.rmax;
-  L.US.resize(nr);
-  L.V.resize(nr);
-  L.US.zero(stream);
-  L.V.zero(stream);
-  std::vector<SegNN> nu(ns), nv(ns);
-  for (int s = 0; s < ns; ++s) {
-    nu[s] = SegNN{st[s], (int64_t)s * kk, st[s], (int32_t)ln[s], k[s], L.rmax, 0};
-    nv[s] = SegNN{sst[s], (int64_t)s * kk, sst[s], (int32_t)sln[s], k[s], L.rmax, 0};
+  for (int c = c0; c <= c1; ++c) {
+    HLevel& L = lv[c];
+    const int b0 = base[c - c0], ns = base[c - c0 + 1] - b0;
+    L.rank.assign(rank.begin() + b0, rank.begin() + b0 + ns);
+    L.rmax = 0;
+    for (int r : L.rank) L.rmax = std::max(L.rmax, r);
+    L.S.assign((size_t)ns * L.rmax, 0.0);
+    for (int s = 0; s < ns; ++s)
+      for (int i = 0; i < L.rank[s]; ++i) L.S[(size_t)s * L.rmax + i] = S[(size_t)(b0 + s) * kcap + i];
+    const int64_t nr = N * (int64_t)L.rmax;
+    L.US.resize(nr);
+    L.V.resize(nr);
+    L.US.zero(stream);
+    L.V.zero(stream);
+    std::vector<SegNN> nu(ns), nv(ns);
+    for (int s = 0; s < ns; ++s) {
+      const int t = b0 + s;
+      nu[s] = SegNN{st[t] + cb(t), (int64_t)t * kk, st[t], (int32_t)ln[t], k[t], L.rmax, 0};
+      nv[s] = SegNN{sst[t] + cb(t), (int64_t)t * kk, sst[t], (int32_t)sln[t], k[t], L.rmax, 0};
+    }
+    kern::seg_gemm_nn(nu, dnn_, Yb_.get(), N, TU_.get(), kcap, L.US.get(), N, 1.0, 0.0, stream);
+    kern::seg_gemm_nn(nv, dnn_, Xb_.get(), N, TV_.get(), kcap, L.V.get(), N, 1.0, 0.0, stream);
   }
-  kern::seg_gemm_nn(nu, dnn_, Yb_.get(), N, TU_.get(), kcap, L.US.get(), N, 1.0, 0.0, stream);
-  kern::seg_gemm_nn(nv, dnn_, Xb_.get(), N, TV_.get(), kcap, L.V.get(), N, 1.0, 0.0, stream);
 }
 
 // Y = D X over all leaves (leaf blocks stored lmax x lmax).

@@ -24,6 +24,9 @@ constexpr int kLowPrecSplit = 3;
 constexpr int kPowerSplit = 1;
 // Jacobi rotation floor of the mixed path, (2^-23)^2 of ||R||_F^2.
 constexpr double kMixedJacobiFloor = 1.4210854715202004e-14;
+// Scratch budget of one group of depths built together (three N x k
+// arrays per depth).
+constexpr double kGroupScratchBytes = 8e9;
 
 // Does an m x n block take the randomised branch?  compression.h:72-75 in
 // tolerance mode; in fixed-rank mode also exact when k >= min(m, n).
@@ -186,8 +189,9 @@ class HodlrEngine {
  private:
   void reset_for_build(int leaf);
   void build_levels(const GenDesc& g, const double* host_omega, int64_t omega_len);
-  void build_level(int c, const GenDesc& g, const std::vector<int64_t>& omega_off,
+  void build_group(int c0, int c1, const GenDesc& g, const std::vector<int64_t>& omega_off,
                    const double* host_omega);
+  int group_kcap(int c0, int c1) const;
   template <class F> void factor_t(HFactor<F>& fa);
   template <class F> void correction_t(HFactor<F>& fa, int j, F* W, int64_t ldw, int ncols);
   template <class F> void apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int ncols, F* X,

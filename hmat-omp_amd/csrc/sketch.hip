@@ -431,6 +431,11 @@ static SketchPlan plan_sketch(const std::vector<SegSk>& segs, int kcap, int ncol
     p.nsplit = (int)std::min<int64_t>(ceil_div(want, std::max<int64_t>(nrow_tiles * p.ncg, 1)),
                                       std::max<int64_t>(1, redmax / 256));
     p.nsplit = std::max(1, std::min(p.nsplit, 64));
+    // the partial sums span the output's extent once per split: at most
+    // 2^28 doubles
+    int64_t span = 0;
+    for (auto& sg : segs) span = std::max<int64_t>(span, sg.yoff + sg.outlen + (int64_t)(sg.k - 1) * ldy + 1);
+    p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(p.nsplit, ((int64_t)1 << 28) / std::max<int64_t>(span, 1)));
   }
   for (size_t i = 0; i < segs.size(); ++i) {
     const auto& sg = segs[i];
