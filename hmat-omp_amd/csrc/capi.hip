@@ -335,6 +335,13 @@ hmat_status hmat_hodlr_set_solve_graph(hmat_hodlr h, int32_t on) {
   });
 }
 
+hmat_status hmat_hodlr_set_group_bytes(hmat_hodlr h, double bytes) {
+  return guard([&] {
+    HMAT_REQUIRE(h && bytes >= 0.0, HMAT_ERR_ARG, "bad group scratch budget");
+    h->eng.group_bytes = bytes > 0.0 ? bytes : kGroupScratchBytes;
+  });
+}
+
 int64_t hmat_hodlr_size(hmat_hodlr h) { return h ? h->eng.N : -1; }
 int32_t hmat_hodlr_levels(hmat_hodlr h) { return h && h->eng.built ? h->eng.tree.max_depth : -1; }
 

@@ -128,7 +128,8 @@ void HodlrEngine::build_levels(const GenDesc& g, const double* host_omega, int64
 
   // Depths are built together in groups: every launch of a group covers
   // all its blocks.  A group's scratch (three N x k arrays per depth) is
-  // kept within kGroupScratchBytes, and its BF16 planes below 2^30 entries.
+  // kept within group_bytes (at least one depth per group), and its BF16
+  // planes below 2^30 entries.
   {
     int c0 = 1;
     while (c0 <= tree.max_depth) {
@@ -137,7 +138,7 @@ void HodlrEngine::build_levels(const GenDesc& g, const double* host_omega, int64
       while (c1 + 1 <= tree.max_depth) {
         const int kn = std::max(kc, group_kcap(c1 + 1, c1 + 1));
         const double per = (double)N * kn * (c1 + 2 - c0);
-        if (per * 24.0 > kGroupScratchBytes || per * 1.0 >= (double)(1 << 29)) break;
+        if (per * 24.0 > group_bytes || per * 1.0 >= (double)(1 << 29)) break;
         kc = kn;
         ++c1;
       }

@@ -183,6 +183,7 @@ class HodlrEngine {
   double refine_rtol = 0.0;
   int refine_max = 30;
   bool solve_graph = true;  // replay the apply-inverse from a captured HIP graph
+  double group_bytes = kGroupScratchBytes;  // scratch budget of one group of depths
   int last_iters = 0;
   double last_resid = -1.0;
 

@@ -88,6 +88,7 @@ def _proto(lib):
         "hmat_hodlr_set_stream": (st, [vp, vp]),
         "hmat_hodlr_save": (st, [vp, C.c_char_p]),
         "hmat_hodlr_set_solve_graph": (st, [vp, i32]),
+        "hmat_hodlr_set_group_bytes": (st, [vp, C.c_double]),
         "hmat_hodlr_load": (st, [vp, C.c_char_p]),
         "hmat_hss_create": (st, [C.POINTER(vp), C.POINTER(Options)]),
         "hmat_hss_destroy": (st, [vp]),
@@ -281,6 +282,10 @@ class HODLRMatrixOMP(_Base):
     def set_solve_graph(self, on):
         """Replay solves from a captured HIP graph (default) or launch eagerly."""
         _check(self._lib.hmat_hodlr_set_solve_graph(self._h, int(bool(on))))
+
+    def set_group_bytes(self, nbytes):
+        """Scratch budget of one group of depths built together (0: default)."""
+        _check(self._lib.hmat_hodlr_set_group_bytes(self._h, float(nbytes)))
 
     def save(self, path):
         """Write the built operator to `path` (see hmat_hodlr_save)."""

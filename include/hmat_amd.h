@@ -233,6 +233,12 @@ hmat_status hmat_hodlr_load(hmat_hodlr h, const char* path);
  * factorisation and column count (default on); off launches the same
  * kernels one by one.  No reference counterpart. */
 hmat_status hmat_hodlr_set_solve_graph(hmat_hodlr h, int32_t on);
+/* Scratch budget in bytes of one group of tree depths built together (the
+ * build stacks a group's sketch bases: three N x k arrays per depth; every
+ * launch covers all blocks of the group).  Groups hold at least one depth;
+ * 0 restores the default (8e9).  Results agree with any grouping to
+ * rounding.  No reference counterpart (the reference recurses per node). */
+hmat_status hmat_hodlr_set_group_bytes(hmat_hodlr h, double bytes);
 /* Set the HIP stream (hipStream_t) used by the handle; NULL = own stream. */
 hmat_status hmat_hodlr_set_stream(hmat_hodlr h, void* stream);
 

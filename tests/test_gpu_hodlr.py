@@ -313,6 +313,36 @@ def test_mixed_precision_ill_conditioned_reports_not_converged():
 
 
 @pytest.mark.parametrize("mixed", [False, True])
+def test_depth_groups_agree(mixed):
+    """A scratch budget that forces one tree depth per build group gives the
+    operator of the default build (all depths in one group) to rounding:
+    the same Omega streams, ranks and solution."""
+    n = 4096
+    xc, yc = kernels.cauchy_grid_points(n)
+    b = np.random.default_rng(4).standard_normal((n, 1))
+
+    def build(budget):
+        o = hmat_amd.make_options(1e-10, rank=32, fixed_rank=True, oversample=10, power_iters=1,
+                                  exact_below=0,
+                                  precision=hmat_amd.PREC_MIXED if mixed else hmat_amd.PREC_FP64,
+                                  omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=11)
+        h = hmat_amd.HODLRMatrixOMP(opts=o)
+        h.set_group_bytes(budget)
+        h.build_kernel(hmat_amd.make_kernel(hmat_amd.KERNEL_CAUCHY, xc, yc), n, 128)
+        return h
+
+    one, per_depth = build(0), build(1)
+    assert one.levels() == per_depth.levels() == 5
+    for d in range(1, one.levels() + 1):
+        assert list(one.level_ranks(d)) == list(per_depth.level_ranks(d))
+    # FP64: summation order only; mixed: BF16 sketches of differently tiled
+    # launches, at the mixed path's own accuracy
+    bar = 1e-6 if mixed else 1e-12
+    assert rel(per_depth.to_dense(), one.to_dense()) < bar
+    assert rel(per_depth.solve(b), one.solve(b)) < bar
+
+
+@pytest.mark.parametrize("mixed", [False, True])
 def test_solve_graph_replays_match_eager(mixed):
     """Solves replayed from the captured graph equal the eagerly launched
     solve bit for bit, over several replays with eager work in between."""
