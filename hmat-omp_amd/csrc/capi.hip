@@ -462,6 +462,20 @@ hmat_status hmat_hss_solve(hmat_hss h, const double* B, int64_t ldb, int32_t nrh
 int64_t hmat_hss_size(hmat_hss h) { return h ? h->eng.N : -1; }
 int32_t hmat_hss_root_rank(hmat_hss h) { return h && h->eng.built ? h->eng.root_rank() : -1; }
 
+hmat_status hmat_hss_save(hmat_hss h, const char* path) {
+  return guard([&] {
+    HMAT_REQUIRE(h && path, HMAT_ERR_ARG, "bad save arguments");
+    h->eng.save(path);
+  });
+}
+
+hmat_status hmat_hss_load(hmat_hss h, const char* path) {
+  return guard([&] {
+    HMAT_REQUIRE(h && path, HMAT_ERR_ARG, "bad load arguments");
+    h->eng.load(path);
+  });
+}
+
 hmat_status hmat_hss_to_dense(hmat_hss h, double* A, int64_t lda) {
   return guard([&] {
     HMAT_REQUIRE(h && A && lda >= h->eng.N, HMAT_ERR_ARG, "bad to_dense arguments");

@@ -17,6 +17,7 @@
 //     306-343) restated on nested bases, Q_p = A_p^{-1} U_p = diag(Q_a, Q_b) G_p,
 //     factored once bottom-up and applied by one up-sweep and one down-sweep.
 #include <algorithm>
+#include <fstream>
 #include <cmath>
 #include <cstring>
 
@@ -512,6 +513,15 @@ void HssEngine::build(const GenDesc& g, int64_t n, int leaf_size, bool nested_, 
   kern::gen_blocks(g, bd, dbd_, D.get(), stream);
   leafT[0].release();
   leafT[1].release();
+  pool_built = pool_used;
+  set_ldv();
+  HIP_CHECK(hipStreamSynchronize(stream));
+  built = true;
+}
+
+// Leading dimension of the per-node vectors of the sweeps: wide enough for a
+// node's [rU; rV] and for both children's stacked ranks.
+void HssEngine::set_ldv() {
   int lv = 1;
   for (auto& x : sn) lv = std::max(lv, std::max(std::max(x.rU, x.rV), x.rU + x.rV));
   for (size_t id = 0; id < tree.nodes.size(); ++id)
@@ -520,8 +530,6 @@ void HssEngine::build(const GenDesc& g, int64_t n, int leaf_size, bool nested_, 
       lv = std::max(lv, sn[nd.left].rU + sn[nd.right].rU + sn[nd.left].rV + sn[nd.right].rV);
     }
   ldv = lv;
-  HIP_CHECK(hipStreamSynchronize(stream));
-  built = true;
 }
 
 // ------------------------------------------------------------------ matvec
@@ -598,6 +606,8 @@ void HssEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int6
 //          G = (I - C M^{-1} K) R,   K_p = W^T K G   (Q_p = diag(Q_a, Q_b) G)
 void HssEngine::factor() {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "factor before build");
+  factored = false;
+  pool_used = pool_built;  // a refactorisation reuses the factor's part of the pool
   const int64_t lm = tree.lmax;
   const int nl = (int)tree.leaves.size();
   Dinv.resize((size_t)nl * lm * lm);
@@ -872,6 +882,151 @@ void HssEngine::to_dense_host(double* A, int64_t lda) {
         }
     }
   }
+}
+
+}  // namespace hmat
+
+namespace hmat {
+
+// ------------------------------------------------------------ operator file
+// Version 1 (native byte order and struct layout):
+//   char magic[8] "HMATHSS"; int32 version
+//   hmat_options opt; int64 N; int32 leaf; int32 nested
+//   int32 nnodes; int32 rUmax; int32 rVmax
+//   per tree node (pre-order ids): int32 rU, rV, ldR, ldW, ldB;
+//                                  int64 oR, oW, oBab, oBba
+//   int64 npool; double pool[npool]   (transfers R/W and couplings B)
+//   double Ul[N * rUmax], Vl[N * rVmax]   (leaf bases, column-major, ld N)
+//   double D[nleaf * lmax * lmax]
+// The tree is rebuilt from (N, leaf); the factorisation is not stored.
+static const char kHssMagic[8] = {'H', 'M', 'A', 'T', 'H', 'S', 'S', 0};
+
+void HssEngine::save(const std::string& path) {
+  HMAT_REQUIRE(built, HMAT_ERR_STATE, "save before build");
+  std::ofstream f(path, std::ios::binary);
+  HMAT_REQUIRE(f.good(), HMAT_ERR_ARG, "cannot open " + path + " for writing");
+  auto put = [&](const void* p, size_t n) { f.write(reinterpret_cast<const char*>(p), (std::streamsize)n); };
+  const int32_t version = 1, lf = leaf, ne = nested ? 1 : 0, nn = (int32_t)tree.nodes.size();
+  put(kHssMagic, sizeof(kHssMagic));
+  put(&version, 4);
+  put(&opt, sizeof(opt));
+  put(&N, 8);
+  put(&lf, 4);
+  put(&ne, 4);
+  put(&nn, 4);
+  put(&rUmax, 4);
+  put(&rVmax, 4);
+  for (const SNode& x : sn) {
+    const int32_t iv[5] = {x.rU, x.rV, x.ldR, x.ldW, x.ldB};
+    const int64_t ov[4] = {x.oR, x.oW, x.oBab, x.oBba};
+    put(iv, sizeof(iv));
+    put(ov, sizeof(ov));
+  }
+  put(&pool_built, 8);
+  auto dump = [&](const DevBuf<double>& b, size_t n) {
+    if (n == 0) return;
+    auto h = download(b.get(), n, stream);
+    put(h.data(), 8 * n);
+  };
+  const bool tiny = tree.max_depth == 0;
+  dump(pool, tiny ? 0 : (size_t)pool_built);
+  dump(Ul, tiny ? 0 : (size_t)N * rUmax);
+  dump(Vl, tiny ? 0 : (size_t)N * rVmax);
+  dump(D, (size_t)tree.leaves.size() * tree.lmax * tree.lmax);
+  HMAT_REQUIRE(f.good(), HMAT_ERR_ARG, "write to " + path + " failed");
+}
+
+void HssEngine::load(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  HMAT_REQUIRE(f.good(), HMAT_ERR_ARG, "cannot open " + path);
+  auto get = [&](void* p, size_t n) {
+    f.read(reinterpret_cast<char*>(p), (std::streamsize)n);
+    HMAT_REQUIRE(f.good(), HMAT_ERR_ARG, path + ": truncated operator file");
+  };
+  char magic[8];
+  int32_t version = 0;
+  get(magic, 8);
+  get(&version, 4);
+  HMAT_REQUIRE(std::memcmp(magic, kHssMagic, 8) == 0 && version == 1, HMAT_ERR_ARG,
+               path + ": not a version-1 HSS operator file");
+  hmat_options o;
+  int64_t n = 0;
+  int32_t lf = 0, ne = 0, nn = 0, ru = 0, rv = 0;
+  get(&o, sizeof(o));
+  get(&n, 8);
+  get(&lf, 4);
+  get(&ne, 4);
+  get(&nn, 4);
+  get(&ru, 4);
+  get(&rv, 4);
+  HMAT_REQUIRE(n > 0 && lf > 0 && (ne == 0 || ne == 1) && ru >= 0 && rv >= 0 && o.precision == HMAT_PREC_FP64,
+               HMAT_ERR_ARG, path + ": bad header");
+  Tree t;
+  t.build(n, lf);
+  HMAT_REQUIRE(nn == (int32_t)t.nodes.size(), HMAT_ERR_ARG, path + ": tree does not match the header");
+  std::vector<SNode> s(nn);
+  for (SNode& x : s) {
+    int32_t iv[5];
+    int64_t ov[4];
+    get(iv, sizeof(iv));
+    get(ov, sizeof(ov));
+    x.rU = iv[0], x.rV = iv[1], x.ldR = iv[2], x.ldW = iv[3], x.ldB = iv[4];
+    x.oR = ov[0], x.oW = ov[1], x.oBab = ov[2], x.oBba = ov[3];
+  }
+  int64_t np = 0;
+  get(&np, 8);
+  HMAT_REQUIRE(np >= 0, HMAT_ERR_ARG, path + ": bad pool size");
+  // every generator must lie inside the pool / the leaf bases
+  auto inside = [&](int64_t off, int64_t ld, int64_t rows, int64_t cols) {
+    if (rows == 0 || cols == 0) return true;
+    return off >= 0 && ld >= rows && off + ld * (cols - 1) + rows <= np;
+  };
+  for (int id = 0; id < nn; ++id) {
+    const TNode& nd = t.nodes[id];
+    const SNode& x = s[id];
+    bool ok = x.rU >= 0 && x.rV >= 0;
+    if (nd.leaf) {
+      ok = ok && x.rU <= ru && x.rV <= rv && x.rU <= nd.len && x.rV <= nd.len;
+    } else if (t.max_depth > 0) {
+      const SNode& a = s[nd.left];
+      const SNode& b = s[nd.right];
+      ok = ok && x.rU <= a.rU + b.rU && x.rV <= a.rV + b.rV;
+      if (id != 0)  // the root's transfers are never applied
+        ok = ok && inside(x.oR, x.ldR, a.rU + b.rU, x.rU) && inside(x.oW, x.ldW, a.rV + b.rV, x.rV);
+      ok = ok && inside(x.oBab, x.ldB, a.rU, b.rV) && inside(x.oBba, x.ldB, b.rU, a.rV);
+    }
+    HMAT_REQUIRE(ok, HMAT_ERR_ARG, path + ": bad generator layout at node " + std::to_string(id));
+  }
+  const bool tiny = t.max_depth == 0;
+  HMAT_REQUIRE(!tiny || np == 0, HMAT_ERR_ARG, path + ": bad pool size");
+  std::vector<double> hp(tiny ? 0 : (size_t)np), hu(tiny ? 0 : (size_t)n * ru), hv(tiny ? 0 : (size_t)n * rv),
+      hd((size_t)t.leaves.size() * t.lmax * t.lmax);
+  get(hp.data(), 8 * hp.size());
+  get(hu.data(), 8 * hu.size());
+  get(hv.data(), 8 * hv.size());
+  get(hd.data(), 8 * hd.size());
+  // the file is whole: replace the handle's operator
+  built = factored = false;
+  opt = o;
+  N = n;
+  leaf = lf;
+  nested = ne != 0;
+  tree = std::move(t);
+  sn = std::move(s);
+  rUmax = ru;
+  rVmax = rv;
+  pool_used = pool_built = np;
+  auto put_dev = [&](DevBuf<double>& b, const std::vector<double>& h) {
+    b.resize(std::max<size_t>(h.size(), 1));
+    if (!h.empty()) b.upload(h, stream);
+  };
+  put_dev(pool, hp);
+  put_dev(Ul, hu);
+  put_dev(Vl, hv);
+  put_dev(D, hd);
+  HIP_CHECK(hipStreamSynchronize(stream));
+  set_ldv();
+  built = true;
 }
 
 }  // namespace hmat

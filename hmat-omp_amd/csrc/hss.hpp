@@ -13,6 +13,7 @@
 // (tolerance or fixed rank), so ranks stay bounded and the operator costs
 // O(N r) to apply.
 #pragma once
+#include <string>
 #include <vector>
 
 #include "hodlr.hpp"
@@ -41,6 +42,10 @@ class HssEngine {
   void factor();
   void solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
   void to_dense_host(double* A, int64_t lda);
+  // operator file: options, tree, per-node ranks and generator offsets,
+  // transfers/couplings, leaf bases and leaf blocks (no factorisation)
+  void save(const std::string& path);
+  void load(const std::string& path);
   int root_rank() const;
   static int64_t omega_length(const hmat_options& o, int64_t N, int leaf);
 
@@ -61,6 +66,7 @@ class HssEngine {
                        const std::vector<int64_t>& omega_off);
   void build_internal();
   int64_t pool_alloc(int64_t n);
+  void set_ldv();
   void leaf_apply(const double* Dm, const double* B, int64_t ldb, int ncols, double* Y, int64_t ldy);
 
   // leaf generators
@@ -72,6 +78,7 @@ class HssEngine {
   // small matrices (transfers, couplings, factors) in one device pool
   DevBuf<double> pool;
   int64_t pool_used = 0;
+  int64_t pool_built = 0;  // end of the build's part (the factor's follows)
   int ldv = 1;  // leading dimension of every per-node vector in the sweeps
   DevBuf<double> va_, vb_, vc_, vd_, ve_;
 

@@ -101,6 +101,8 @@ def _proto(lib):
         "hmat_hss_size": (i64, [vp]),
         "hmat_hss_root_rank": (i32, [vp]),
         "hmat_hss_to_dense": (st, [vp, _D, i64]),
+        "hmat_hss_save": (st, [vp, C.c_char_p]),
+        "hmat_hss_load": (st, [vp, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -381,3 +383,12 @@ class HSSMatrixOMP(_Base):
 
     def root_rank(self):
         return self._lib.hmat_hss_root_rank(self._h)
+
+    def save(self, path):
+        """Write the built operator to `path` (see hmat_hss_save)."""
+        _check(self._lib.hmat_hss_save(self._h, str(path).encode()))
+
+    def load(self, path):
+        """Replace this handle's operator and options with the file's."""
+        _check(self._lib.hmat_hss_load(self._h, str(path).encode()))
+        self.N = self._lib.hmat_hss_size(self._h)

@@ -270,6 +270,15 @@ int64_t hmat_hss_size(hmat_hss h);
 /* rank_U of the root's children summed (reference: root->rank_U). */
 int32_t hmat_hss_root_rank(hmat_hss h);
 hmat_status hmat_hss_to_dense(hmat_hss h, double* A, int64_t lda);
+/* Operator files (no reference counterpart: the reference has no
+ * serialisation).  save writes the built operator (options, N, leaf size,
+ * nested flag, per-node ranks, transfers, couplings, leaf bases and leaf
+ * blocks) to `path`; load replaces the handle's operator and options with
+ * the file's after checking that every generator lies inside the stored
+ * arrays.  The factorisation is recomputed on the first solve after a
+ * load. */
+hmat_status hmat_hss_save(hmat_hss h, const char* path);
+hmat_status hmat_hss_load(hmat_hss h, const char* path);
 
 #ifdef __cplusplus
 }

@@ -121,3 +121,79 @@ def test_trivial_tree():
     assert h.root_rank() == 0
     assert rel(h.multiply(b), a @ b) < 1e-15
     assert rel(h.solve(b), np.linalg.solve(a, b)) < 1e-12
+
+
+@pytest.mark.parametrize("nested", [False, True])
+def test_save_load_round_trip(tmp_path, nested):
+    """The operator file reproduces the operator bit for bit (generators,
+    matvec); solves after a load (which refactors) equal solves on the
+    original handle.  The file is written after a solve, so the factor's
+    part of the small-matrix pool exists and must not be stored."""
+    n, leaf = 3000, 100
+    x = kernels.sorted_uniform_points(n, 17)
+    a = kernels.gaussian_points(x, 0.1, 1.0)
+    o = hmat_amd.make_options(1e-10, rank=48, oversample=10, power_iters=1, exact_below=0,
+                              omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=5)
+    h = hmat_amd.HSSMatrixOMP(opts=o, nested=nested)
+    h.build_from_dense(a, leaf)
+    b = np.random.default_rng(3).standard_normal((n, 2))
+    xs = h.solve(b)
+    f = tmp_path / "op.hss"
+    h.save(f)
+    g = hmat_amd.HSSMatrixOMP(1e-3)  # options and nesting come from the file
+    g.load(f)
+    assert g.N == n and g.root_rank() == h.root_rank()
+    assert np.array_equal(g.to_dense(), h.to_dense())
+    assert np.array_equal(g.multiply(b), h.multiply(b))
+    assert rel(g.solve(b), xs) < 1e-14
+    # a second factorisation reuses the factor's part of the pool
+    g.factor()
+    assert rel(g.solve(b), xs) < 1e-14
+
+
+def test_save_load_trivial_tree(tmp_path):
+    n = 40
+    a = kernels.gaussian_points(np.linspace(0, 1, n), 0.2, 1.0)
+    h = hmat_amd.HSSMatrixOMP(1e-8)
+    h.build_from_dense(a, 64)
+    f = tmp_path / "leaf.hss"
+    h.save(f)
+    g = hmat_amd.HSSMatrixOMP(1e-3)
+    g.load(f)
+    b = np.arange(n, dtype=float).reshape(-1, 1)
+    assert np.array_equal(g.to_dense(), h.to_dense())
+    assert rel(g.solve(b), np.linalg.solve(a, b)) < 1e-12
+
+
+def test_load_rejects_bad_files(tmp_path):
+    import ctypes
+    n, leaf = 1024, 128
+    a = kernels.generate_matrix(kernels.CAUCHY, n)
+    h = hmat_amd.HSSMatrixOMP(1e-8)
+    h.build_from_dense(a, leaf)
+    good = tmp_path / "ok.hss"
+    h.save(good)
+    raw = good.read_bytes()
+    g = hmat_amd.HSSMatrixOMP(1e-8)
+    # a HODLR operator file is not an HSS one
+    hd = hmat_amd.HODLRMatrixOMP(1e-8)
+    hd.build_from_dense(a, leaf)
+    hodlr_file = tmp_path / "op.hodlr"
+    hd.save(hodlr_file)
+    # node records follow: magic, version, options, N, leaf, nested,
+    # node count, rUmax, rVmax; each record is 5 int32 + 4 int64, and the
+    # root's coupling offset oBab is its 7th field
+    head = 8 + 4 + ctypes.sizeof(hmat_amd.Options) + 8 + 4 * 5
+    bad_off = bytearray(raw)
+    bad_off[head + 20 + 16:head + 20 + 24] = np.int64(1 << 40).tobytes()
+    cases = {"junk.hss": b"not an operator", "trunc.hss": raw[:len(raw) // 2],
+             "off.hss": bytes(bad_off)}
+    for name, data in cases.items():
+        (tmp_path / name).write_bytes(data)
+    for p in [hodlr_file] + [tmp_path / c for c in cases]:
+        with pytest.raises(hmat_amd.HmatError) as e:
+            g.load(p)
+        assert e.value.status == 1, p.name
+    # a failed load leaves the handle unbuilt-or-unchanged; a good file still loads
+    g.load(good)
+    assert np.array_equal(g.to_dense(), h.to_dense())
