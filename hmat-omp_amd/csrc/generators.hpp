@@ -2,9 +2,12 @@
 //
 // row_pt / col_pt fetch the point of a row / column index once, eval forms
 // the entry from the two points (and the global indices, for the diagonal
-// terms).  The FP64 reciprocal is v_rcp_f64 refined by two Newton steps
-// (2^-26 -> 2^-52); entries can differ from a correctly rounded division by
-// one unit in the last place.
+// terms).  The FP64 reciprocal is v_rcp_f64 (relative error up to about
+// 2^-24) refined by one Newton step: the error after it is about the square,
+// 2^-48, i.e. within 16 units in the last place of the correctly rounded
+// quotient (at most 11 measured over 4M inputs across 2^-40 .. 2^20).  A
+// second step would round correctly but costs 2 of the 7 VALU operations of
+// an entry in the FP64 sketch, which is VALU + MFMA bound.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -15,9 +18,7 @@ namespace kern {
 
 __device__ inline double rcp64(double d) {
   double r = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-d, r, 1.0);
+  const double e = fma(-d, r, 1.0);
   return fma(r, e, r);
 }
 

@@ -75,6 +75,68 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project(const MvLeaf* __restrict__
   const MvLeaf lf = leaves[blockIdx.x];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   gdouble* Xg = (gdouble*)X;
+  const MvEntry* le = ent + (int64_t)blockIdx.x * L;
+  auto flush = [&](int c, int vcols) {  // the four waves' sums of depth c -> partials
+    __syncthreads();
+    for (int i = threadIdx.x; i < vcols * NR; i += kMvWG) {
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < kMvWG / 64; ++j) t += red[j * KMP * NR + i];
+      part[((int64_t)blockIdx.x * L + c) * KM * NR + i] = t;
+    }
+    __syncthreads();
+  };
+  if (lf.len <= kMvWG) {
+    // One row per thread.  The steps (depth c, column group kb) run as a
+    // pipeline: the next step's loads are issued before this step's
+    // reduction, so every wave keeps loads in flight across its shuffles.
+    const int r = threadIdx.x;
+    const bool live = r < lf.len;
+    const int64_t row = lf.start + min(r, lf.len - 1);
+    double xr[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) xr[q] = live ? Xg[row + (int64_t)q * ldx] : 0.0;
+    auto load = [&](double (&a)[KG], int c, int vcols, int kb) {
+      gdouble* V = (gdouble*)lvls[c].V;
+      const int64_t N = lvls[c].ld;
+#pragma unroll
+      for (int kk = 0; kk < KG; ++kk) a[kk] = V[row + (int64_t)min(kb + kk, vcols - 1) * N];
+    };
+    int c = 0;
+    while (c < L && le[c].vcols <= 0) ++c;
+    if (c == L) return;
+    int vc = le[c].vcols, kb = 0;
+    double a[KG];
+    load(a, c, vc, kb);
+    while (true) {
+      int nc = c, nkb = kb + KG, nvc = vc;
+      if (nkb >= vc) {
+        nkb = 0;
+        ++nc;
+        while (nc < L && le[nc].vcols <= 0) ++nc;
+        if (nc < L) nvc = le[nc].vcols;
+      }
+      double an[KG];
+      if (nc < L) load(an, nc, nvc, nkb);
+      double v[16];
+#pragma unroll
+      for (int kk = 0; kk < KG; ++kk) {
+        const double ak = (live && kb + kk < vc) ? a[kk] : 0.0;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) v[kk * NR + q] = ak * xr[q];
+      }
+      const double sum = wave_reduce_scatter16(v);  // lane & 15 = kk * NR + q
+      if (lane < 16) red[w * KMP * NR + kb * NR + lane] = sum;
+      if (nkb == 0) flush(c, vc);
+      if (nc >= L) break;
+      c = nc;
+      kb = nkb;
+      vc = nvc;
+#pragma unroll
+      for (int kk = 0; kk < KG; ++kk) a[kk] = an[kk];
+    }
+    return;
+  }
   for (int c = 0; c < L; ++c) {
     const MvEntry e = ent[(int64_t)blockIdx.x * L + c];
     if (e.vcols <= 0) continue;
@@ -102,14 +164,7 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project(const MvLeaf* __restrict__
       const double s = wave_reduce_scatter16(v);  // lane & 15 = kk * NR + q
       if (lane < 16) red[w * KMP * NR + kb * NR + lane] = s;
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < e.vcols * NR; i += kMvWG) {
-      double t = 0.0;
-#pragma unroll
-      for (int j = 0; j < kMvWG / 64; ++j) t += red[j * KMP * NR + i];
-      part[((int64_t)blockIdx.x * L + c) * KM * NR + i] = t;
-    }
-    __syncthreads();
+    flush(c, e.vcols);
   }
 }
 
@@ -122,12 +177,25 @@ __global__ __launch_bounds__(kMvWG) void k_mv_reduce(const MvSeg* __restrict__ s
   const MvSeg sg = segs[blockIdx.x];
   const int P = sg.vcols * NR;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int WS = kMvWG / 64;  // wave w sums leaves w, w + WS, w + 2 WS, ...
+  const int64_t lstride = (int64_t)L * KM * NR;
+  const int nlf = sg.leaf1 - sg.leaf0;
   for (int i0 = 0; i0 < P; i0 += 64) {
     const int i = i0 + lane;
     double acc = 0.0;
-    if (i < P)
-      for (int leaf = sg.leaf0 + w; leaf < sg.leaf1; leaf += kMvWG / 64)
-        acc += part[((int64_t)leaf * L + sg.level) * KM * NR + i];
+    if (i < P) {
+      const double* pp = part + ((int64_t)sg.leaf0 * L + sg.level) * KM * NR + i;
+      int l = w;
+      // eight independent loads in flight; the additions keep their order
+      for (; l + 7 * WS < nlf; l += 8 * WS) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = pp[(int64_t)(l + u * WS) * lstride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u];
+      }
+      for (; l < nlf; l += WS) acc += pp[(int64_t)l * lstride];
+    }
     red[w][lane] = acc;
     __syncthreads();
     if (threadIdx.x < 64 && i < P) {
@@ -168,42 +236,40 @@ __global__ __launch_bounds__(kMvWG) void k_mv_expand(const MvLeaf* __restrict__ 
     double acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = 0.0;
-    int j = 0;
-    for (; j + 8 <= lf.len; j += 8) {
-      double d[8];
+    // acc += M(row, j0 .. j0+n) v(j0 .. j0+n): batches of 16 and 8
+    // independent loads, then single ones; M column j at M + j * ldm
+    auto dot = [&](gdouble* M, int64_t ldm, const double* v, int n) {
+      int j = 0;
+      for (; j + 16 <= n; j += 16) {
+        double m[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) d[u] = Dl[r + (int64_t)(j + u) * lm];
+        for (int u = 0; u < 16; ++u) m[u] = M[(int64_t)(j + u) * ldm];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < 16; ++u)
 #pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] = fma(d[u], xs[(j + u) * NR + q], acc[q]);
-    }
-    for (; j < lf.len; ++j) {
-      const double d = Dl[r + (int64_t)j * lm];
+          for (int q = 0; q < NR; ++q) acc[q] = fma(m[u], v[(j + u) * NR + q], acc[q]);
+      }
+      if (j + 8 <= n) {
+        double m[8];
 #pragma unroll
-      for (int q = 0; q < NR; ++q) acc[q] = fma(d, xs[j * NR + q], acc[q]);
-    }
+        for (int u = 0; u < 8; ++u) m[u] = M[(int64_t)(j + u) * ldm];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int q = 0; q < NR; ++q) acc[q] = fma(m[u], v[(j + u) * NR + q], acc[q]);
+        j += 8;
+      }
+      for (; j < n; ++j) {
+        const double m = M[(int64_t)j * ldm];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = fma(m, v[j * NR + q], acc[q]);
+      }
+    };
+    dot(Dl + r, lm, xs, lf.len);
     const int64_t row = lf.start + r;
     for (int c = 0; c < L; ++c) {
       const MvEntry en = ent[(int64_t)blockIdx.x * L + c];
-      gdouble* U = (gdouble*)lvls[c].US;
-      const int64_t N = lvls[c].ld;
-      const double* tc = ts + (int64_t)c * KM * NR;
-      int k = 0;
-      for (; k + 8 <= en.ucols; k += 8) {
-        double u[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) u[i] = U[row + (int64_t)(k + i) * N];
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-#pragma unroll
-          for (int q = 0; q < NR; ++q) acc[q] = fma(u[i], tc[(k + i) * NR + q], acc[q]);
-      }
-      for (; k < en.ucols; ++k) {
-        const double u = U[row + (int64_t)k * N];
-#pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] = fma(u, tc[k * NR + q], acc[q]);
-      }
+      dot((gdouble*)lvls[c].US + row, lvls[c].ld, ts + (int64_t)c * KM * NR, en.ucols);
     }
 #pragma unroll
     for (int q = 0; q < NR; ++q) Y[row + (int64_t)q * ldy] = acc[q];
