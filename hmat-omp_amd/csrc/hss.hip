@@ -105,12 +105,18 @@ void HssEngine::compress_leaves(const GenDesc& g, bool col, const double* host_o
   }
   HMAT_REQUIRE(kcap <= 300, HMAT_ERR_ARG, "leaf sketch width above 300 is not supported");
   const int64_t nk = N * (int64_t)kcap;
-  DevBuf<double> Yb(nk), Tb(nk), Bw((size_t)nk * nl), Tw2((size_t)nk * nl);
+  LeafScratch& sc = ls_;
+  DevBuf<double>&Yb = sc.Yb, &Tb = sc.Tb, &Bw = sc.Bw, &Tw2 = sc.Tw2;
+  Yb.reserve(nk);
+  Tb.reserve(nk);
+  Bw.reserve((size_t)nk * nl);
+  Tw2.reserve((size_t)nk * nl);
   Yb.zero(stream);
   Bw.zero(stream);
   // Omega: per leaf (host, reference draw order) or one shared device draw
   const bool per_leaf = host_omega != nullptr;
-  DevBuf<double> X(per_leaf ? (size_t)nk * nl : (size_t)nk);
+  DevBuf<double>& X = sc.X;
+  X.reserve(per_leaf ? (size_t)nk * nl : (size_t)nk);
   X.zero(stream);
   if (per_leaf) {
     std::vector<double> h((size_t)nk * nl, 0.0);
@@ -167,8 +173,13 @@ void HssEngine::compress_leaves(const GenDesc& g, bool col, const double* host_o
   kern::sketch(g, !tr_fwd, bwd, dsk_, Yb.get(), N, Bw.get(), N, kcap, ws_, stream);
   kern::zero_rects(zr, dzr_, stream);
   const int64_t kk = (int64_t)kcap * kcap;
-  DevBuf<double> Rb((size_t)nl * kk), Vr((size_t)nl * kk), Sv((size_t)nl * kcap), Wk((size_t)2 * nl * kk),
-      TU((size_t)nl * kk), TV((size_t)nl * kk);
+  DevBuf<double>&Rb = sc.Rb, &Vr = sc.Vr, &Sv = sc.Sv, &Wk = sc.Wk, &TU = sc.TU, &TV = sc.TV;
+  Rb.reserve((size_t)nl * kk);
+  Vr.reserve((size_t)nl * kk);
+  Sv.reserve((size_t)nl * kcap);
+  Wk.reserve((size_t)2 * nl * kk);
+  TU.reserve((size_t)nl * kk);
+  TV.reserve((size_t)nl * kk);
   orth_.run(ob, Bw.get(), Tw2.get(), N, kcap, Rb.get(), stream);
   std::vector<SmallMat> sm(nl);
   for (int i = 0; i < nl; ++i) sm[i] = SmallMat{(int64_t)i * kk, k[i], kcap};
@@ -207,7 +218,7 @@ void HssEngine::compress_leaves(const GenDesc& g, bool col, const double* host_o
     tsum += rank[i];
   }
   DevBuf<double>& Tleaf = leafT[col ? 1 : 0];
-  Tleaf.resize((size_t)N * std::max<int64_t>(tsum, 1));
+  Tleaf.reserve((size_t)N * std::max<int64_t>(tsum, 1));
   std::vector<SegNN> nb, nt;
   for (int i = 0; i < nl; ++i) {
     const TNode& nd = tree.nodes[L[i]];
@@ -231,7 +242,11 @@ void HssEngine::build_internal() {
   const int dmax = tree.max_depth;
   const int nn = (int)tree.nodes.size();
   std::vector<int64_t> tcol(nn, 0), scol(nn, 0);
-  DevBuf<double> Tc, Sc, VFc;  // child depth: wide T^T, S^T and full V
+  // child depth: wide T^T, S^T and full V; the current depth's go to the
+  // other buffer of each pair and the pairs swap after every depth
+  NodeScratch& sc = ns_;
+  DevBuf<double>*Tc = &sc.T[0], *Sc = &sc.S[0], *VFc = &sc.VF[0];
+  DevBuf<double>*Tn = &sc.T[1], *Sn = &sc.S[1], *VFn = &sc.VF[1];
 
   // wide projection arrays of one depth: leaves from the leaf arrays,
   // internal nodes from (Tin, Sin) at columns (icT, icS)
@@ -245,8 +260,8 @@ void HssEngine::build_internal() {
       scol[id] = ss;
       ss += sn[id].rV;
     }
-    Tw.resize((size_t)N * std::max<int64_t>(ts, 1));
-    Sw.resize((size_t)N * std::max<int64_t>(ss, 1));
+    Tw.reserve((size_t)N * std::max<int64_t>(ts, 1));
+    Sw.reserve((size_t)N * std::max<int64_t>(ss, 1));
     std::vector<SegNN> lt, ls, it, is;
     for (int id : ids) {
       const TNode& nd = tree.nodes[id];
@@ -272,7 +287,7 @@ void HssEngine::build_internal() {
       rv = std::max(rv, sn[id].rV);
       if (!tree.nodes[id].leaf) ldw = sn[id].ldW;
     }
-    VF.resize((size_t)N * rv);
+    VF.reserve((size_t)N * rv);
     VF.zero(stream);
     std::vector<SegNN> cp, mm;
     for (int id : ids) {
@@ -288,12 +303,12 @@ void HssEngine::build_internal() {
       }
     }
     kern::copy_segs(cp, dnn_, Vl.get(), N, VF.get(), N, stream);
-    kern::seg_gemm_nn(mm, dnn_, VFc.get(), N, pool.get(), ldw, VF.get(), N, 1.0, 0.0, stream);
+    kern::seg_gemm_nn(mm, dnn_, VFc->get(), N, pool.get(), ldw, VF.get(), N, 1.0, 0.0, stream);
   };
 
   const std::vector<int64_t> none(nn, 0);
-  make_wide(tree.by_depth[dmax], nullptr, nullptr, none, none, Tc, Sc);
-  make_vf(tree.by_depth[dmax], VFc);
+  make_wide(tree.by_depth[dmax], nullptr, nullptr, none, none, *Tc, *Sc);
+  make_vf(tree.by_depth[dmax], *VFc);
 
   for (int d = dmax - 1; d >= 0; --d) {
     const auto& ids = tree.by_depth[d];
@@ -322,7 +337,7 @@ void HssEngine::build_internal() {
         tb.push_back(SegTN{tcol[nd.right] * N + a.start, a.start, sn[id].oBba, (int32_t)a.len, sn[nd.right].rU,
                            sn[nd.left].rV, 0});
       }
-      kern::seg_gemm_tn(tb, dtn_, Tc.get(), N, VFc.get(), N, pool.get(), ldb, 1.0, 0.0, ws_, stream);
+      kern::seg_gemm_tn(tb, dtn_, Tc->get(), N, VFc->get(), N, pool.get(), ldb, 1.0, 0.0, ws_, stream);
     }
     if (d == 0) break;  // the root has no transfer and no projection
     // ---- stacked child projections P^T = [T_a^T | T_b^T], own rows zeroed
@@ -342,7 +357,9 @@ void HssEngine::build_internal() {
       qcol[id] = qsum;
       qsum += kW[id];
     }
-    DevBuf<double> Pt((size_t)N * std::max<int64_t>(psum, 1)), Ps((size_t)N * std::max<int64_t>(qsum, 1));
+    DevBuf<double>&Pt = sc.Pt, &Ps = sc.Ps;
+    Pt.reserve((size_t)N * std::max<int64_t>(psum, 1));
+    Ps.reserve((size_t)N * std::max<int64_t>(qsum, 1));
     {
       std::vector<SegNN> ct, cs;
       std::vector<ZeroRect> zr;
@@ -356,11 +373,10 @@ void HssEngine::build_internal() {
         zr.push_back(ZeroRect{Pt.get() + pcol[id] * N, N, (int32_t)nd.start, (int32_t)nd.len, 0, kR[id]});
         zr.push_back(ZeroRect{Ps.get() + qcol[id] * N, N, (int32_t)nd.start, (int32_t)nd.len, 0, kW[id]});
       }
-      kern::copy_segs(ct, dnn_, Tc.get(), N, Pt.get(), N, stream);
-      kern::copy_segs(cs, dnn_, Sc.get(), N, Ps.get(), N, stream);
+      kern::copy_segs(ct, dnn_, Tc->get(), N, Pt.get(), N, stream);
+      kern::copy_segs(cs, dnn_, Sc->get(), N, Ps.get(), N, stream);
       kern::zero_rects(zr, dzr_, stream);
     }
-    DevBuf<double> Tn, Sn;
     std::vector<int64_t> ncT(nn, 0), ncS(nn, 0);
     if (!nested) {
       // identity transfers, ranks add up (hss_routines_omp.h:110-119)
@@ -383,21 +399,26 @@ void HssEngine::build_internal() {
         ncT[id] = pcol[id];
         ncS[id] = qcol[id];
       }
-      make_wide(ids, Pt.get(), Ps.get(), ncT, ncS, Tn, Sn);
+      make_wide(ids, Pt.get(), Ps.get(), ncT, ncS, *Tn, *Sn);
     } else {
       // compress P^T (N x k) per node: P^T = Qp R, R = Ur S Vr^T ->
       // transfer Vr(:, :r), new projection^T = Qp Ur(:, :r) S(:r)
       const int ni = (int)inner.size();
       const int64_t kk = (int64_t)kcap * kcap;
-      DevBuf<double> Pn[2];
+      DevBuf<double>* Pn = sc.Pn;
       for (int side = 0; side < 2; ++side) {
         DevBuf<double>& P = side ? Ps : Pt;
         const std::vector<int64_t>& pc = side ? qcol : pcol;
         const std::vector<int>& kx = side ? kW : kR;
         const int ldx = side ? ldW : ldR;
-        DevBuf<double> Tmp(P.size());
-        DevBuf<double> Rb((size_t)ni * kk), Vr((size_t)ni * kk), Sv((size_t)ni * kcap),
-            Wk((size_t)2 * ni * kk), TU((size_t)ni * kk), TV((size_t)ni * kk);
+        DevBuf<double>&Tmp = sc.Tmp, &Rb = sc.Rb, &Vr = sc.Vr, &Sv = sc.Sv, &Wk = sc.Wk, &TU = sc.TU, &TV = sc.TV;
+        Tmp.reserve(P.size());
+        Rb.reserve((size_t)ni * kk);
+        Vr.reserve((size_t)ni * kk);
+        Sv.reserve((size_t)ni * kcap);
+        Wk.reserve((size_t)2 * ni * kk);
+        TU.reserve((size_t)ni * kk);
+        TV.reserve((size_t)ni * kk);
         std::vector<OrthSeg> os;
         std::vector<SmallMat> sm;
         for (int i = 0; i < ni; ++i) {
@@ -450,17 +471,16 @@ void HssEngine::build_internal() {
           sum += rk[i];
         }
         kern::rect_axpby(cp, drc_, stream);
-        Pn[side].resize((size_t)N * std::max<int64_t>(sum, 1));
+        Pn[side].reserve((size_t)N * std::max<int64_t>(sum, 1));
         kern::seg_gemm_nn(nt, dnn_, P.get(), N, TV.get(), kcap, Pn[side].get(), N, 1.0, 0.0, stream);
         for (int i = 0; i < ni; ++i) (side ? sn[inner[i]].rV : sn[inner[i]].rU) = rk[i];
       }
-      make_wide(ids, Pn[0].get(), Pn[1].get(), ncT, ncS, Tn, Sn);
+      make_wide(ids, Pn[0].get(), Pn[1].get(), ncT, ncS, *Tn, *Sn);
     }
-    DevBuf<double> VFn;
-    make_vf(ids, VFn);
-    Tc = std::move(Tn);
-    Sc = std::move(Sn);
-    VFc = std::move(VFn);
+    make_vf(ids, *VFn);
+    std::swap(Tc, Tn);
+    std::swap(Sc, Sn);
+    std::swap(VFc, VFn);
   }
   HIP_CHECK(hipStreamSynchronize(stream));
 }
@@ -511,8 +531,6 @@ void HssEngine::build(const GenDesc& g, int64_t n, int leaf_size, bool nested_, 
     bd.push_back({nd.start, nd.start, (int64_t)i * lm * lm, (int32_t)nd.len, (int32_t)nd.len, (int32_t)lm, 0});
   }
   kern::gen_blocks(g, bd, dbd_, D.get(), stream);
-  leafT[0].release();
-  leafT[1].release();
   pool_built = pool_used;
   set_ldv();
   HIP_CHECK(hipStreamSynchronize(stream));

@@ -81,6 +81,15 @@ class HssEngine {
   int64_t pool_built = 0;  // end of the build's part (the factor's follows)
   int ldv = 1;  // leading dimension of every per-node vector in the sweeps
   DevBuf<double> va_, vb_, vc_, vd_, ve_;
+  // Build scratch, kept for the next build and grown only when a build
+  // needs more (freeing and reallocating multi-GB arrays per build stalls
+  // the host for up to a second per array).
+  struct LeafScratch {  // compress_leaves
+    DevBuf<double> Yb, Tb, Bw, Tw2, X, Rb, Vr, Sv, Wk, TU, TV;
+  } ls_;
+  struct NodeScratch {  // build_internal: wide arrays of the child and the current depth
+    DevBuf<double> T[2], S[2], VF[2], Pt, Ps, Tmp, Pn[2], Rb, Vr, Sv, Wk, TU, TV;
+  } ns_;
 
   kern::Workspace ws_;
   Orthogonalizer orth_;
