@@ -8,9 +8,10 @@
 //                products whose term indices sum to < SPLIT, FP32
 //                accumulation flushed to FP64 every 512 reduction indices.
 //
-// Both stage a K-tile of 64 reduction rows of X (and their points) in LDS;
-// a tile whose reduction rows are all valid and not excluded and whose
-// output rows are all live takes a mask-free path (uniform branch).
+// Both stage a K-tile of 64 reduction rows of X (and their points) in LDS
+// and zero the X rows of reduction indices past the end or inside the
+// excluded interval as they stage them, so their inner loops carry no
+// masks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,29 +48,30 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
                                                  const double* __restrict__ X, int64_t ldx,
                                                  double* __restrict__ Y, int64_t ldy,
                                                  int64_t split_stride) {
-  constexpr int KT = 64, NC = NT * 16, ROWS = 4 * 16 * RF;
+  constexpr int KT = 64, NC = NT * 16;
   __shared__ double xs[KT][NC + 1];
   __shared__ double rp[KT];
-  __shared__ double rm[KT];
   const SkTile tl = tiles[blockIdx.x];
   const SegSk sg = segs[tl.seg];
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
   const int cbase = blockIdx.y * NC;
   const int kcols = min(NC, sg.k - cbase);
-  const bool rows_full = tl.row0 + ROWS <= sg.outlen;
+  if (kcols <= 0) return;  // a column group past this segment's width: nothing to store
   mfma::v4d acc[RF][NT];
 #pragma unroll
   for (int f = 0; f < RF; ++f)
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[f][t] = mfma::v4d{0.0, 0.0, 0.0, 0.0};
+  // Rows past the block's end take the last row's point (finite entries)
+  // and are never stored.  Reduction indices past the end or inside the
+  // excluded interval take the last index's point and meet zero rows of X.
   int64_t iglob[RF];
-  double opt[RF], rmask[RF];
+  double opt[RF];
 #pragma unroll
   for (int f = 0; f < RF; ++f) {
     const int rloc = tl.row0 + w * 16 * RF + f * 16 + (lane % 16);
     iglob[f] = sg.out0 + min(rloc, max(sg.outlen - 1, 0));
     opt[f] = TRANS ? gen.col_pt(iglob[f]) : gen.row_pt(iglob[f]);
-    rmask[f] = rloc < sg.outlen ? 1.0 : 0.0;
   }
   const int kq = lane / 16;
   // Software pipeline: the next K-tile of X and its points are loaded into
@@ -77,21 +79,20 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
   constexpr int NPRE = KT * NC / kWGs;
   static_assert(KT * NC % kWGs == 0, "prefetch layout");
   double pre[NPRE];
-  double ppt = 0.0, pok = 0.0;
+  double ppt = 0.0;
   auto issue = [&](int kb) {
 #pragma unroll
     for (int i = 0; i < NPRE; ++i) {
       const int e = threadIdx.x + i * kWGs;
       const int kk = e % KT, cc = e / KT, j = kb + kk;
-      pre[i] = (j < tl.re && cc < kcols) ? X[sg.xoff + j + (int64_t)(cbase + cc) * ldx] : 0.0;
+      const bool excl = (uint64_t)(sg.red0 + j - sg.excl0) < (uint64_t)sg.excllen;
+      // every load in bounds (clamped index), masked by a select: no branches
+      const double v = X[sg.xoff + min(j, tl.re - 1) + (int64_t)(cbase + min(cc, kcols - 1)) * ldx];
+      pre[i] = (j < tl.re && cc < kcols && !excl) ? v : 0.0;
     }
     if (threadIdx.x < KT) {
       const int j = kb + threadIdx.x;
-      const bool inr = j < tl.re;
-      const int64_t jg = sg.red0 + (inr ? j : tl.re - 1);
-      const bool ok = inr && !((uint64_t)(jg - sg.excl0) < (uint64_t)sg.excllen);
-      ppt = TRANS ? gen.row_pt(jg) : gen.col_pt(jg);
-      pok = ok ? 1.0 : 0.0;
+      ppt = TRANS ? gen.row_pt(sg.red0 + min(j, tl.re - 1)) : gen.col_pt(sg.red0 + min(j, tl.re - 1));
     }
   };
   auto commit = [&]() {
@@ -100,10 +101,7 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
       const int e = threadIdx.x + i * kWGs;
       xs[e % KT][e / KT] = pre[i];
     }
-    if (threadIdx.x < KT) {
-      rp[threadIdx.x] = ppt;
-      rm[threadIdx.x] = pok;
-    }
+    if (threadIdx.x < KT) rp[threadIdx.x] = ppt;
   };
   // Dense input reads A from memory in the inner loop already; there the
   // next tile is fetched after the compute, keeping its registers free.
@@ -114,7 +112,6 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
   for (int kb = tl.rb; kb < tl.re; kb += KT) {
     const bool has_next = kb + KT < tl.re;
     if (PF && has_next) issue(kb + KT);
-    const bool full = rows_full && ktile_full(sg, kb, tl.re);
 #pragma unroll 4
     for (int ks = 0; ks < KT / 4; ++ks) {
       const int kk = ks * 4 + kq;
@@ -123,10 +120,8 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
       const double rpt = rp[kk];
       double a[RF];
 #pragma unroll
-      for (int f = 0; f < RF; ++f) {
+      for (int f = 0; f < RF; ++f)
         a[f] = TRANS ? gen.eval(rpt, opt[f], jg, iglob[f]) : gen.eval(opt[f], rpt, iglob[f], jg);
-        if (!full) a[f] *= rm[kk] * rmask[f];
-      }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const double b = xs[kk][t * 16 + (lane % 16)];
@@ -188,17 +183,12 @@ __device__ inline void split_bf16_bits(float v, uint32_t (&o)[3]) {
 }
 
 // No K-tile of 64 reduction rows starting at kb meets the excluded interval.
-__device__ inline bool ktile_clean(const SegSk& sg, int kb) {
-  if (sg.excllen <= 0) return true;
-  const int64_t lo = sg.red0 + kb, hi = lo + 64;
-  return hi <= sg.excl0 || lo >= sg.excl0 + sg.excllen;
-}
-
 // Rows past the segment end and reduction rows past the tile end are clamped
-// to the last valid index, so every generated entry is a finite entry of A:
-// the rows are never stored and the X rows are zero in LDS.  Only the
-// excluded interval has to be masked, by selection (a non-finite entry
-// there, such as a diagonal singularity, must not reach the product).
+// to the last valid index, so every generated entry is an entry of A: the
+// rows are never stored and the X rows are zero in LDS.  X rows inside the
+// excluded interval are zero too (tiles that touch it are staged element by
+// element); that interval is the node's own diagonal block of A, so its
+// entries are finite whenever A's are.
 template <class Gen, bool TRANS, int SPLIT>
 __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __restrict__ segs,
                                                     const SkTile* __restrict__ tiles,
@@ -211,12 +201,12 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
   constexpr int FLUSH = SPLIT > 1 ? 8 : 0;
   __shared__ __attribute__((aligned(16))) __bf16 xs[SPLIT][NC][KP];
   __shared__ __attribute__((aligned(16))) float rp[KT];
-  __shared__ __attribute__((aligned(16))) float rm[KT];
   const SkTile tl = tiles[blockIdx.x];
   const SegSk sg = segs[tl.seg];
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
   const int cbase = blockIdx.y * NC;
   const int kcols = min(NC, sg.k - cbase);
+  if (kcols <= 0) return;  // a column group past this segment's width: nothing to store
   const bool aligned = ((sg.xoff & 7) == 0) && ((ldx & 7) == 0);
   mfma::v4d acc64[RF][NT];
   mfma::v4f acc32[RF][NT];
@@ -243,7 +233,7 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
   constexpr int NPRE = SPLIT * NC * (KT / 8) / kWGs;
   static_assert(SPLIT * NC * (KT / 8) % kWGs == 0, "prefetch layout");
   uint4 pre[NPRE];
-  float ppt = 0.f, pok = 0.f;
+  float ppt = 0.f;
   auto issue = [&](int kb) -> bool {
     const bool vec = aligned && ktile_full(sg, kb, tl.re);
     if (vec) {
@@ -251,18 +241,15 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
       for (int i = 0; i < NPRE; ++i) {
         const int e = threadIdx.x + i * kWGs;
         const int ch = e % (KT / 8), cc = (e / (KT / 8)) % NC, sp = e / ((KT / 8) * NC);
-        pre[i] = cc < kcols ? *reinterpret_cast<const uint4*>(Xs + sp * ps + sg.xoff + kb + 8 * ch +
-                                                              (int64_t)(cbase + cc) * ldx)
-                            : make_uint4(0, 0, 0, 0);
+        // in bounds for every lane (clamped column), masked by a select
+        const uint4 v = *reinterpret_cast<const uint4*>(Xs + sp * ps + sg.xoff + kb + 8 * ch +
+                                                        (int64_t)(cbase + min(cc, kcols - 1)) * ldx);
+        pre[i] = cc < kcols ? v : make_uint4(0, 0, 0, 0);
       }
     }
     if (threadIdx.x < KT) {
-      const int j = kb + threadIdx.x;
-      const bool inr = j < tl.re;
-      const int64_t jg = sg.red0 + (inr ? j : tl.re - 1);
-      const bool ok = inr && !((uint64_t)(jg - sg.excl0) < (uint64_t)sg.excllen);
+      const int64_t jg = sg.red0 + min(kb + (int)threadIdx.x, tl.re - 1);
       ppt = TRANS ? gen.row_pt(jg) : gen.col_pt(jg);
-      pok = ok ? 1.f : 0.f;
     }
     return vec;
   };
@@ -277,29 +264,23 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
     } else {
       for (int e = threadIdx.x; e < SPLIT * NC * KT; e += kWGs) {
         const int kk = e % KT, cc = (e / KT) % NC, sp = e / (KT * NC), j = kb + kk;
-        xs[sp][cc][kk] = (j < tl.re && cc < kcols) ? Xs[sp * ps + sg.xoff + j + (int64_t)(cbase + cc) * ldx]
-                                                   : (__bf16)0.f;
+        const bool excl = (uint64_t)(sg.red0 + j - sg.excl0) < (uint64_t)sg.excllen;
+        xs[sp][cc][kk] = (j < tl.re && cc < kcols && !excl)
+                             ? Xs[sp * ps + sg.xoff + j + (int64_t)(cbase + cc) * ldx]
+                             : (__bf16)0.f;
       }
     }
-    if (threadIdx.x < KT) {
-      rp[threadIdx.x] = ppt;
-      rm[threadIdx.x] = pok;
-    }
+    if (threadIdx.x < KT) rp[threadIdx.x] = ppt;
   };
   // One K-tile: generate the A fragments (two entries per 32-bit word, word
   // jp of term sp holding entries 2jp, 2jp+1) and multiply.
-  auto tile = [&](int kb, auto masked) {
+  auto tile = [&](int kb) {
 #pragma unroll
     for (int ks = 0; ks < KT / 32; ++ks) {
       const int k0 = ks * 32 + 8 * kq;
       float pts[8];
-      bool okk[8];
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) pts[jj] = rp[k0 + jj];
-      if constexpr (decltype(masked)::value) {
-#pragma unroll
-        for (int jj = 0; jj < 8; ++jj) okk[jj] = rm[k0 + jj] != 0.f;
-      }
       mfma::v8bf a[RF][SPLIT];
 #pragma unroll
       for (int f = 0; f < RF; ++f) {
@@ -312,7 +293,6 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
             const int jj = 2 * jp + h;
             const int64_t jg = sg.red0 + min(kb + k0 + jj, tl.re - 1);
             val[h] = TRANS ? gen.eval(pts[jj], opt[f], jg, iglob[f]) : gen.eval(opt[f], pts[jj], iglob[f], jg);
-            if constexpr (decltype(masked)::value) val[h] = okk[jj] ? val[h] : 0.f;
           }
           if constexpr (SPLIT == 1) {
             // the leading term of the exact split (truncation): the upper
@@ -352,10 +332,7 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
   for (int kb = tl.rb; kb < tl.re; kb += KT) {
     const bool has_next = kb + KT < tl.re;
     const bool vec_next = has_next ? issue(kb + KT) : false;
-    if (ktile_clean(sg, kb))
-      tile(kb, std::false_type{});
-    else
-      tile(kb, std::true_type{});
+    tile(kb);
     __syncthreads();
     if (has_next) {
       commit(kb + KT, vec_next);
