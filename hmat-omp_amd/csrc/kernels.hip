@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_nn(const SegNN* __restrict__ s
   }
 }
 
-// MFMA variant for wide segments (p >= 32 and q >= 16, e.g. the leaf
+// MFMA variant for wide segments (p >= 16 and q >= 16, e.g. the leaf
 // blocks D_i (len x len) applied to many columns): 64 x 64 output tile per
 // workgroup, each wave a 32 x 32 quarter as 2 x 2 MFMA 16x16 tiles; K-tiles
 // of 16 staged in LDS.  FP64: v_mfma_f64_16x16x4f64, FP32:
@@ -160,8 +160,8 @@ int nn_tiles(const std::vector<SegNN>& segs, std::vector<NNTile>& tiles) {
   for (auto& g : segs)
     if (g.len > 0 && g.q > 0) pmin = std::min(pmin, g.p);
   int mode, R, cw;
-  if (pmin >= 32 && qmax >= 16) {
-    mode = 1;  // MFMA, 64 x 64 tiles
+  if (pmin >= 16 && qmax >= 16) {
+    mode = 1;  // MFMA, 64 x 64 tiles (K-tiles of 16, the last zero-padded)
     R = 64;
     cw = 64;
   } else {
@@ -441,13 +441,16 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_reduce(const SegTN* __restr
 // through LDS 32 rows at a time with the next 32 rows prefetched into
 // registers; wave w computes output rows 16w..16w+15 (columns of X) against
 // all four 16-column groups of Y with 16x16x4 MFMAs.  When X and Y are the
-// same rows and columns (a Gram matrix) the chunk is read once.
+// same rows and columns (a Gram matrix) the chunk is read once.  An output
+// tile covered by one chunk (slot < 0) is written to O directly, exactly as
+// the sum stage would write it; the others go to their partial slot.
 template <class F>
 __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part_mfma(const SegTN* __restrict__ segs,
                                                                const TNItem* __restrict__ items,
                                                                const F* __restrict__ X, int64_t ldx,
                                                                const F* __restrict__ Y, int64_t ldy,
-                                                               F* __restrict__ part) {
+                                                               F* __restrict__ part, F* __restrict__ O,
+                                                               int64_t ldo, F alpha, F beta) {
   constexpr int KR = 32, NPRE = KR * 64 / kWG;
   __shared__ F xs[KR][65];
   __shared__ F ys[KR][65];
@@ -498,6 +501,21 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part_mfma(const SegTN* __re
       __syncthreads();
     }
   }
+  if (it.slot < 0) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gi = pc0 + 16 * w + MfmaNN<F>::row(lane, v), gj = qc0 + 16 * t + (lane & 15);
+        if (gi < sg.p && gj < sg.q) {
+          F* o = O + sg.ooff + gi + (int64_t)gj * ldo;
+          F val = alpha * acc[t][v];
+          if (beta != F(0)) val = fma(beta, *o, val);
+          *o = val;
+        }
+      }
+    return;
+  }
   F* pp = part + (int64_t)it.slot * 4096;
 #pragma unroll
   for (int t = 0; t < 4; ++t)
@@ -507,6 +525,7 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_tn_part_mfma(const SegTN* __re
 
 // Work items of a segmented O = alpha X^T Y + beta O: partial 64 x 64
 // products over row chunks (slots of the partial buffer) and their sums.
+// An output tile of one chunk needs no sum: its item writes O (slot -1).
 int tn_items(const std::vector<SegTN>& segs, std::vector<TNItem>& items, std::vector<TNOut>& outs) {
   items.clear();
   outs.clear();
@@ -521,15 +540,14 @@ int tn_items(const std::vector<SegTN>& segs, std::vector<TNItem>& items, std::ve
     const int npt = (int)ceil_div(g.p, 64), nqt = (int)ceil_div(g.q, 64);
     for (int pt = 0; pt < npt; ++pt)
       for (int qt = 0; qt < nqt; ++qt) {
+        if (g.len <= ch) {
+          items.push_back({(int32_t)i, 0, (int32_t)std::max<int64_t>(g.len, 0), pt, qt, -1});
+          continue;
+        }
         TNOut o{(int32_t)i, pt, qt, slot, 0, 0};
-        if (g.len <= 0) {
-          items.push_back({(int32_t)i, 0, 0, pt, qt, slot++});
-          o.nslot = 1;
-        } else {
-          for (int64_t r0 = 0; r0 < g.len; r0 += ch) {
-            items.push_back({(int32_t)i, (int32_t)r0, (int32_t)std::min<int64_t>(r0 + ch, g.len), pt, qt, slot++});
-            o.nslot++;
-          }
+        for (int64_t r0 = 0; r0 < g.len; r0 += ch) {
+          items.push_back({(int32_t)i, (int32_t)r0, (int32_t)std::min<int64_t>(r0 + ch, g.len), pt, qt, slot++});
+          o.nslot++;
         }
         outs.push_back(o);
       }
@@ -541,9 +559,10 @@ template <class F>
 void launch_tn_raw(int nitems, int nouts, const SegTN* ds, const TNItem* di, const TNOut* dout,
                           const F* X, int64_t ldx, const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta,
                           F* part, hipStream_t s) {
-  if (!nouts) return;
-  k_seg_gemm_tn_part_mfma<F><<<(unsigned)nitems, kWG, 0, s>>>(ds, di, X, ldx, Y, ldy, part);
+  if (!nitems) return;
+  k_seg_gemm_tn_part_mfma<F><<<(unsigned)nitems, kWG, 0, s>>>(ds, di, X, ldx, Y, ldy, part, O, ldo, alpha, beta);
   HIP_CHECK(hipGetLastError());
+  if (!nouts) return;
   k_seg_gemm_tn_reduce<F><<<dim3((unsigned)nouts, 4096 / 64), kWG, 0, s>>>(ds, dout, part, O, ldo, alpha, beta);
   HIP_CHECK(hipGetLastError());
 }
@@ -554,7 +573,7 @@ void plan_tn(const std::vector<SegTN>& segs, TNPlan& p, hipStream_t s) {
   std::vector<TNItem> items;
   std::vector<TNOut> outs;
   const int nslots = tn_items(segs, items, outs);
-  if (outs.empty()) return;
+  if (items.empty()) return;
   p.nitems = (int)items.size();
   p.nouts = (int)outs.size();
   p.nslots = nslots;
@@ -587,9 +606,9 @@ void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* 
   std::vector<TNItem> items;
   std::vector<TNOut> outs;
   const int nslots = tn_items(segs, items, outs);
-  if (outs.empty()) return;
+  if (items.empty()) return;
   DevBuf<F>& part = ws.parts<F>();
-  part.reserve((size_t)nslots * 4096);
+  part.reserve((size_t)std::max(nslots, 1) * 4096);
   StageGroup g;
   const int is = g.add(segs), ii = g.add(items), io = g.add(outs);
   g.put(s);
@@ -733,21 +752,23 @@ __device__ __forceinline__ int chol_step(double (&col)[64], int n, int c) {
 __global__ __launch_bounds__(64) void k_chol64r(const SmallMat* __restrict__ mats,
                                                 const double* __restrict__ G, double* __restrict__ R,
                                                 const double* __restrict__ shift_coef,
-                                                int* __restrict__ info, double* __restrict__ gerr) {
-  // G staged through LDS with coalesced loads (column c of G in gs[c][*])
-  __shared__ double gs[64][65];
+                                                int* __restrict__ info, double* __restrict__ gerr, int lg) {
+  // G staged through LDS with coalesced loads, column j of G at gs[j * lg];
+  // lg = the batch's largest n + 1, so small matrices leave room for more
+  // workgroups per CU
+  extern __shared__ double gs[];
   const SmallMat m = mats[blockIdx.x];
   const int n = m.n, ld = m.ld;
   const double* g = G + m.off;
   double* rg = R + m.off;
   const int c = threadIdx.x;
   for (int j = 0; j < n; ++j)
-    if (c < n) gs[j][c] = g[c + (int64_t)j * ld];
+    if (c < n) gs[j * lg + c] = g[c + (int64_t)j * ld];
   __syncthreads();
   double tr = 0.0, dev = 0.0;
   if (c < n) {
-    tr = gs[c][c];
-    for (int i = 0; i < n; ++i) dev = fmax(dev, fabs(gs[c][i] - (i == c ? 1.0 : 0.0)));
+    tr = gs[c * lg + c];
+    for (int i = 0; i < n; ++i) dev = fmax(dev, fabs(gs[c * lg + i] - (i == c ? 1.0 : 0.0)));
   }
   for (int o = 32; o > 0; o >>= 1) {
     tr += __shfl_xor(tr, o);
@@ -760,7 +781,7 @@ __global__ __launch_bounds__(64) void k_chol64r(const SmallMat* __restrict__ mat
   const double shift = shift_coef[blockIdx.x] * tr;
   double col[64];
 #pragma unroll
-  for (int i = 0; i < 64; ++i) col[i] = (c < n && i <= c) ? gs[c][i] + (i == c ? shift : 0.0) : 0.0;
+  for (int i = 0; i < 64; ++i) col[i] = (c < n && i <= c) ? gs[c * lg + i] + (i == c ? shift : 0.0) : 0.0;
   if (!(tr > 0.0)) {
     if (!(tr == 0.0) && c == 0) info[blockIdx.x] = -1;  // NaN
     if (c < n)
@@ -844,7 +865,8 @@ void launch_chol_rinv(const SmallMat* dmats, int nmats, int nmax, double* G, dou
   const size_t lds = (size_t)nmax * (nmax + 1) * sizeof(double);
   const int use_lds = lds <= 64 * 1024 ? 1 : 0;
   if (nmax <= 64)
-    k_chol64r<<<(unsigned)nmats, 64, 0, s>>>(dmats, G, R, dshift, info, gerr);
+    k_chol64r<<<(unsigned)nmats, 64, (size_t)nmax * (nmax + 1) * sizeof(double), s>>>(dmats, G, R, dshift,
+                                                                                     info, gerr, nmax + 1);
   else
     k_chol<<<(unsigned)nmats, kWG, use_lds ? lds : 0, s>>>(dmats, G, R, dshift, info, gerr, use_lds);
   HIP_CHECK(hipGetLastError());
