@@ -1022,8 +1022,10 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
   for (int j = threadIdx.x; j < n; j += kWG) sv[j] = sig[perm[j]];
 }
 
-// n <= 62: 1024 threads, one column pair per half-wave (two rows per lane),
-// so the n/2 disjoint pairs of a round-robin step rotate at the same time.
+// n <= 62: one column pair per half-wave (two rows per lane), so the n/2
+// disjoint pairs of a round-robin step rotate at the same time; the
+// workgroup has 64 * ceil(pairs / 2) threads (up to 1024), so small
+// matrices leave room for more workgroups per CU.
 // A and V live in LDS (dynamic, 2 n (n+1) doubles, under 64 KB).  Same
 // rotation and stopping rule as k_jacobi_svd.
 constexpr int kJWG = 1024;
@@ -1044,10 +1046,12 @@ __global__ __launch_bounds__(kJWG) void k_jacobi_svd64(const SmallMat* __restric
   const int LA = n + 1;
   double* a = sm;
   double* v = sm + n * LA;
+  // blockDim.x = 64 * ceil(pairs / 2): one half-wave per column pair
+  const int nthr = blockDim.x, nhw = nthr / 32;
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
   const int hl = lane & 31, half = lane >> 5;
   double fro = 0.0;
-  for (int e = threadIdx.x; e < n * n; e += kJWG) {
+  for (int e = threadIdx.x; e < n * n; e += nthr) {
     const int i = e % n, j = e / n;
     const double x = ag[i + (int64_t)j * ld];
     fro += x * x;
@@ -1058,7 +1062,7 @@ __global__ __launch_bounds__(kJWG) void k_jacobi_svd64(const SmallMat* __restric
   if (lane == 0) red[w] = fro;
   __syncthreads();
   fro = 0.0;
-  for (int i = 0; i < kJWG / 64; ++i) fro += red[i];
+  for (int i = 0; i < nthr / 64; ++i) fro += red[i];
   const double floor2 = floor_rel * fro;
   const int np = n + (n & 1);
   const int circ = np - 1;
@@ -1067,7 +1071,7 @@ __global__ __launch_bounds__(kJWG) void k_jacobi_svd64(const SmallMat* __restric
     if (threadIdx.x == 0) rotated = 0;
     __syncthreads();
     for (int step = 0; step < circ; ++step) {
-      for (int pi = w * 2 + half; pi < np / 2; pi += kJWG / 32) {
+      for (int pi = w * 2 + half; pi < np / 2; pi += nhw) {
         int p, q;
         if (pi == 0) {
           p = 0;
@@ -1114,7 +1118,7 @@ __global__ __launch_bounds__(kJWG) void k_jacobi_svd64(const SmallMat* __restric
     __syncthreads();
   }
   if (sweeps_out && threadIdx.x == 0) sweeps_out[blockIdx.x] = sweep;
-  for (int j = w * 2 + half; j < n; j += kJWG / 32) {
+  for (int j = w * 2 + half; j < n; j += nhw) {
     double s2 = 0.0;
     for (int i = hl; i < n; i += 32) s2 += a[i + j * LA] * a[i + j * LA];
     for (int o = 16; o > 0; o >>= 1) s2 += __shfl_xor(s2, o);
@@ -1128,14 +1132,14 @@ __global__ __launch_bounds__(kJWG) void k_jacobi_svd64(const SmallMat* __restric
     perm[r] = j;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < n * n; e += kJWG) {
+  for (int e = threadIdx.x; e < n * n; e += nthr) {
     const int i = e % n, j = e / n;
     const int pj = perm[j];
     const double sj = sig[pj];
     ag[i + (int64_t)j * ld] = sj > 0.0 ? a[i + pj * LA] / sj : 0.0;
     vg[i + (int64_t)j * ld] = v[i + pj * LA];
   }
-  for (int j = threadIdx.x; j < n; j += kJWG) sv[j] = sig[perm[j]];
+  for (int j = threadIdx.x; j < n; j += nthr) sv[j] = sig[perm[j]];
 }
 
 void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* A, double* V,
@@ -1153,9 +1157,13 @@ void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double*
   DevBuf<int> sw;
   const bool count = prof_enabled();
   if (count) sw.resize(mats.size());
-  if (nmax <= 62)
-    k_jacobi_svd64<<<(unsigned)mats.size(), kJWG, (size_t)2 * nmax * (nmax + 1) * sizeof(double), s>>>(
+  if (nmax <= 62) {
+    // one half-wave per column pair of the round-robin ordering
+    const int pairs = (nmax + (nmax & 1)) / 2;
+    const unsigned nthr = (unsigned)std::max(64, 64 * ((pairs + 1) / 2));
+    k_jacobi_svd64<<<(unsigned)mats.size(), nthr, (size_t)2 * nmax * (nmax + 1) * sizeof(double), s>>>(
         dm.get(), A, V, S, sstride, count ? sw.get() : nullptr, floor_rel);
+  }
   else
     k_jacobi_svd<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), A, V, S, sstride, work,
                                                                        use_lds, count ? sw.get() : nullptr,
