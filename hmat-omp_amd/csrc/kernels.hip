@@ -45,9 +45,24 @@ __global__ __launch_bounds__(kWG) void k_seg_gemm_nn(const SegNN* __restrict__ s
   F acc[CPT];
 #pragma unroll
   for (int i = 0; i < CPT; ++i) acc[i] = F(0);
-  const F* xr = X + sg.xoff + row;
+  const F* xr = X + sg.xoff + (live ? row : 0);
   const F* tb = T + sg.toff + (int64_t)(tl.c0 + g) * ldt;
-  for (int i = 0; i < sg.p; ++i) {
+  // eight independent X loads in flight per step (a one-load loop waits on
+  // every load); the accumulation order over i is unchanged
+  int i = 0;
+  for (; i + 8 <= sg.p; i += 8) {
+    F xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = xr[(int64_t)(i + u) * ldx];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (!live) xv[u] = F(0);
+#pragma unroll
+      for (int cc = 0; cc < CPT; ++cc)
+        if (g + cc * G < ncols) acc[cc] = fma(xv[u], tb[i + u + (int64_t)(cc * G) * ldt], acc[cc]);
+    }
+  }
+  for (; i < sg.p; ++i) {
     const F xv = live ? xr[(int64_t)i * ldx] : F(0);
 #pragma unroll
     for (int cc = 0; cc < CPT; ++cc) {
