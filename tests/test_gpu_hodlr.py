@@ -232,10 +232,11 @@ def test_state_and_argument_errors():
 
 
 @pytest.mark.parametrize("n,leaf,nrhs", [(4096, 128, 1), (3000, 100, 2), (2500, 64, 3),
-                                         (1000, 60, 4), (5000, 256, 5)])
+                                         (1000, 60, 4), (5000, 256, 5), (4000, 512, 2)])
 def test_fused_matvec_matches_dense(n, leaf, nrhs):
     """The fused leaf-organised matvec (1, 2 or 4 columns per pass) equals the
-    dense reconstruction of the operator, uneven trees included."""
+    dense reconstruction of the operator, uneven trees included; leaves above
+    256 rows take the projection's several-rows-per-thread loop."""
     x = kernels.sorted_uniform_points(n, 9)
     a = kernels.gaussian_points(x, 0.1, 1.0)
     o = hmat_amd.make_options(1e-10, rank=24, oversample=10, power_iters=1, exact_below=0,
