@@ -1037,12 +1037,13 @@ __global__ __launch_bounds__(kWG) void k_jacobi_svd(const SmallMat* __restrict__
   for (int j = threadIdx.x; j < n; j += kWG) sv[j] = sig[perm[j]];
 }
 
-// n <= 62: one column pair per half-wave (two rows per lane), so the n/2
+// n <= 64: one column pair per half-wave (two rows per lane), so the n/2
 // disjoint pairs of a round-robin step rotate at the same time; the
 // workgroup has 64 * ceil(pairs / 2) threads (up to 1024), so small
 // matrices leave room for more workgroups per CU.
-// A and V live in LDS (dynamic, 2 n (n+1) doubles, under 64 KB).  Same
-// rotation and stopping rule as k_jacobi_svd.
+// A and V live in LDS (dynamic, 2 n (n+1) doubles, at most 66.5 KB of the
+// 160 KB a gfx950 workgroup may use).  Same rotation and stopping rule as
+// k_jacobi_svd.
 constexpr int kJWG = 1024;
 __global__ __launch_bounds__(kJWG) void k_jacobi_svd64(const SmallMat* __restrict__ mats,
                                                        double* __restrict__ A, double* __restrict__ V,
@@ -1172,7 +1173,7 @@ void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double*
   DevBuf<int> sw;
   const bool count = prof_enabled();
   if (count) sw.resize(mats.size());
-  if (nmax <= 62) {
+  if (nmax <= 64) {
     // one half-wave per column pair of the round-robin ordering
     const int pairs = (nmax + (nmax & 1)) / 2;
     const unsigned nthr = (unsigned)std::max(64, 64 * ((pairs + 1) / 2));

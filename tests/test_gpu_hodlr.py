@@ -370,3 +370,21 @@ def test_solve_graph_replays_match_eager(mixed):
     for rep in range(2):
         assert np.array_equal(h.solve_transpose(bs[1]), te)
         assert np.array_equal(h.solve(bs[1]), eager[1])
+
+
+def test_sketch_width_64():
+    """k = 64 (rank 54 + oversampling 10): the one-wave-per-pair Jacobi SVD at
+    its largest size (66.5 KB of LDS).  Fixed rank 54 is above the numerical
+    rank of the sigma = 0.05 Gaussian blocks, so the operator reproduces A and
+    the solve equals dense LAPACK on the reconstruction."""
+    n = 2048
+    a = kernels.gaussian_points(kernels.sorted_uniform_points(n, 3), 0.05, 1.0)
+    o = hmat_amd.make_options(0.0, rank=54, fixed_rank=True, oversample=10, power_iters=1, exact_below=0,
+                              omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=2)
+    h = hmat_amd.HODLRMatrixOMP(opts=o)
+    h.build_from_dense(a, 128)
+    assert all(r == 54 for d in range(1, h.levels() + 1) for r in h.level_ranks(d))
+    ad = h.to_dense()
+    assert rel(ad, a) < 1e-10
+    b = np.random.default_rng(8).standard_normal((n, 2))
+    assert rel(h.solve(b), np.linalg.solve(ad, b)) < 1e-10
