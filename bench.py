@@ -38,8 +38,8 @@ COPY_PROBE_BYTES = 256 << 20
 # HBM bytes per dispatch from the two calibrated rocprofv3 --pmc passes
 # (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, from a 256 MiB copy) recorded in
 # profiles/pmc_traffic_hodlr_cauchy_n262144_mixed.txt
-PMC_TRAFFIC = {"sketch_gen": 9.977e9, "sketch_lp": 10.919e9, "sketch_lp1": 4.120e9,
-               "matvec": 1.0396e9 + 0.0050e9 + 1.5501e9}
+PMC_TRAFFIC = {"sketch_gen": 9.980e9, "sketch_lp": 10.970e9, "sketch_lp1": 4.364e9,
+               "matvec": 1.0141e9 + 0.0050e9 + 1.5501e9}
 PMC_SOURCE = "profiles/pmc_traffic_hodlr_cauchy_n262144_mixed.txt"
 KERNEL_LABEL = {"sketch_gen": "k_sketch (FP64 MFMA, generated entries)",
                 "sketch_lp": "k_sketch_lp<3> (BF16x3 MFMA, last range pass)",
