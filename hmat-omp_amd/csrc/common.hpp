@@ -177,6 +177,11 @@ bool prof_enabled();
 void prof_set(bool on);
 void prof_reset();
 void prof_add(const char* name, double ms, double work);
+// a launch timed by events a..b on its stream, read when queried
+void prof_defer(const char* name, hipEvent_t a, hipEvent_t b, double work);
+// per-item counts in a device buffer, summed into sum_name (n items into
+// n_name) when queried, after the work enqueued on s so far
+void prof_defer_count(const char* sum_name, const char* n_name, DevBuf<int>&& buf, size_t n, hipStream_t s);
 bool prof_get(const char* name, ProfEntry* out);
 
 // RAII timer: records start at construction, stop + accumulate at finish().
@@ -190,15 +195,12 @@ class ProfScope {
       HIP_CHECK(hipEventRecord(a_, s_));
     }
   }
+  // records the stop event; the registry reads the time when queried, so
+  // a profiled launch costs no host wait
   void finish() {
     if (!on_) return;
     HIP_CHECK(hipEventRecord(b_, s_));
-    HIP_CHECK(hipEventSynchronize(b_));
-    float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, a_, b_));
-    prof_add(name_, ms, work_);
-    (void)hipEventDestroy(a_);
-    (void)hipEventDestroy(b_);
+    prof_defer(name_, a_, b_, work_);
     on_ = false;
   }
   void cancel() {  // drop this scope's record (untimed work such as autotuning)

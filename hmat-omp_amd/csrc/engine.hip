@@ -9,6 +9,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <vector>
 
 namespace hmat {
 
@@ -18,6 +19,55 @@ bool g_prof_on = false;
 std::map<std::string, ProfEntry>& prof_map() {
   static std::map<std::string, ProfEntry> m;
   return m;
+}
+// Timed launches and counter buffers whose results are read only when the
+// statistics are queried, so profiling adds no host wait to the work.
+struct ProfPending {
+  std::string name;
+  hipEvent_t a, b;
+  double work;
+};
+struct CountPending {
+  std::string sum_name, n_name;
+  hipEvent_t done;
+  DevBuf<int> buf;
+  size_t n;
+};
+std::vector<ProfPending>& prof_pending() {
+  static std::vector<ProfPending> v;
+  return v;
+}
+std::vector<CountPending>& count_pending() {
+  static std::vector<CountPending> v;
+  return v;
+}
+void prof_add_locked(const std::string& name, double ms, double work) {
+  auto& e = prof_map()[name];
+  e.launches += 1;
+  e.ms += ms;
+  e.work += work;
+}
+void prof_resolve_locked() {
+  for (auto& p : prof_pending()) {
+    float ms = 0.f;
+    HIP_CHECK(hipEventSynchronize(p.b));
+    HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    prof_add_locked(p.name, ms, p.work);
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  prof_pending().clear();
+  for (auto& c : count_pending()) {
+    HIP_CHECK(hipEventSynchronize(c.done));
+    std::vector<int> h(c.n);
+    if (c.n) HIP_CHECK(hipMemcpy(h.data(), c.buf.get(), c.n * sizeof(int), hipMemcpyDeviceToHost));
+    double tot = 0.0;
+    for (int v : h) tot += v;
+    prof_add_locked(c.sum_name, 0.0, tot);
+    prof_add_locked(c.n_name, 0.0, (double)c.n);
+    (void)hipEventDestroy(c.done);
+  }
+  count_pending().clear();
 }
 }  // namespace
 
@@ -133,17 +183,29 @@ bool prof_enabled() { return g_prof_on; }
 void prof_set(bool on) { g_prof_on = on; }
 void prof_reset() {
   std::lock_guard<std::mutex> lk(g_prof_mu);
+  prof_resolve_locked();
   prof_map().clear();
 }
 void prof_add(const char* name, double ms, double work) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
-  auto& e = prof_map()[name];
-  e.launches += 1;
-  e.ms += ms;
-  e.work += work;
+  prof_add_locked(name, ms, work);
+}
+void prof_defer(const char* name, hipEvent_t a, hipEvent_t b, double work) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  // a long profiling session without queries stays bounded
+  if (prof_pending().size() >= 4096) prof_resolve_locked();
+  prof_pending().push_back(ProfPending{name, a, b, work});
+}
+void prof_defer_count(const char* sum_name, const char* n_name, DevBuf<int>&& buf, size_t n, hipStream_t s) {
+  hipEvent_t e;
+  HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIP_CHECK(hipEventRecord(e, s));
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  count_pending().push_back(CountPending{sum_name, n_name, e, std::move(buf), n});
 }
 bool prof_get(const char* name, ProfEntry* out) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
+  prof_resolve_locked();
   auto it = prof_map().find(name);
   if (it == prof_map().end()) return false;
   *out = it->second;

@@ -1179,19 +1179,13 @@ void jacobi_svd(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double*
     const unsigned nthr = (unsigned)std::max(64, 64 * ((pairs + 1) / 2));
     k_jacobi_svd64<<<(unsigned)mats.size(), nthr, (size_t)2 * nmax * (nmax + 1) * sizeof(double), s>>>(
         dm.get(), A, V, S, sstride, count ? sw.get() : nullptr, floor_rel);
-  }
-  else
+  } else {
     k_jacobi_svd<<<(unsigned)mats.size(), kWG, use_lds ? lds : 0, s>>>(dm.get(), A, V, S, sstride, work,
                                                                        use_lds, count ? sw.get() : nullptr,
                                                                        floor_rel);
-  HIP_CHECK(hipGetLastError());
-  if (count) {
-    auto h = download(sw.get(), mats.size(), s);
-    double tot = 0.0;
-    for (int v : h) tot += v;
-    prof_add("jacobi_sweeps", 0.0, tot);
-    prof_add("jacobi_mats", 0.0, (double)mats.size());
   }
+  HIP_CHECK(hipGetLastError());
+  if (count) prof_defer_count("jacobi_sweeps", "jacobi_mats", std::move(sw), mats.size(), s);
 }
 
 // Gauss-Jordan inversion with partial pivoting, in place.
