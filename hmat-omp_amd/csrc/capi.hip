@@ -462,6 +462,22 @@ hmat_status hmat_hss_solve(hmat_hss h, const double* B, int64_t ldb, int32_t nrh
 int64_t hmat_hss_size(hmat_hss h) { return h ? h->eng.N : -1; }
 int32_t hmat_hss_root_rank(hmat_hss h) { return h && h->eng.built ? h->eng.root_rank() : -1; }
 
+hmat_status hmat_hss_set_refinement(hmat_hss h, double rtol, int32_t max_iters) {
+  return guard([&] {
+    HMAT_REQUIRE(h && rtol >= 0 && max_iters >= 0, HMAT_ERR_ARG, "bad refinement arguments");
+    h->eng.refine_rtol = rtol;
+    h->eng.refine_max = max_iters;
+  });
+}
+
+hmat_status hmat_hss_last_refinement(hmat_hss h, int32_t* iters, double* resid) {
+  return guard([&] {
+    HMAT_REQUIRE(h, HMAT_ERR_ARG, "NULL handle");
+    if (iters) *iters = h->eng.last_iters;
+    if (resid) *resid = h->eng.last_resid;
+  });
+}
+
 hmat_status hmat_hss_save(hmat_hss h, const char* path) {
   return guard([&] {
     HMAT_REQUIRE(h && path, HMAT_ERR_ARG, "bad save arguments");

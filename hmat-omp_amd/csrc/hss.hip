@@ -734,10 +734,53 @@ void HssEngine::factor() {
 // x_i = D_i^{-1} b_i, h_i = V_i^T x_i; up: u = M^{-1} [h_a; h_b],
 // s = C u, h_p = W^T ([h_a; h_b] - K s); down: tau_children = s + G tau_p;
 // leaves: x_i -= Q_i tau_i.
+// X = A^{-1} B with FP64 iterative refinement through the HSS matvec when
+// refine_rtol > 0: r = B - A X, X += A^{-1} r until max_j ||r_j|| / ||b_j||
+// <= refine_rtol, refine_max corrections, or a correction that does not
+// reduce the residual (the push-through solve amplifies rounding through
+// its capacitance systems; refinement takes the residual to the rounding
+// floor of the FP64 product).
 void HssEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "solve before build");
   HMAT_REQUIRE(nrhs > 0 && ldb >= N && ldx >= N, HMAT_ERR_ARG, "bad solve dimensions");
   if (!factored) factor();
+  apply_inverse(dB, ldb, nrhs, dX, ldx);
+  last_iters = 0;
+  last_resid = -1.0;
+  if (!(refine_rtol > 0.0)) return;
+  ir_y_.reserve((size_t)N * nrhs);
+  ir_r_.reserve((size_t)N * nrhs);
+  ir_n_.reserve((size_t)2 * nrhs);
+  ir_p_.reserve((size_t)64 * nrhs);
+  kern::col_norms2(dB, ldb, N, nrhs, ir_n_.get(), ir_p_.get(), stream);
+  const auto bn = download(ir_n_.get(), nrhs, stream);
+  double resid = 0.0, prev = 0.0;
+  int it = 0;
+  for (;; ++it) {
+    matvec(dX, ldx, nrhs, ir_y_.get(), N);
+    kern::axpby_cols(N, nrhs, 1.0, dB, ldb, -1.0, ir_y_.get(), N, ir_r_.get(), N, stream);
+    kern::col_norms2(ir_r_.get(), N, N, nrhs, ir_n_.get(), ir_p_.get(), stream);
+    const auto rn = download(ir_n_.get(), nrhs, stream);
+    resid = 0.0;
+    for (int j = 0; j < nrhs; ++j) {
+      const double rj = bn[j] > 0.0 ? std::sqrt(rn[j] / bn[j]) : std::sqrt(rn[j]);
+      if (!std::isfinite(rj))  // (std::max would drop a NaN)
+        throw Error(HMAT_ERR_NONFINITE, "non-finite residual after " + std::to_string(it) +
+                                            " refinement iterations");
+      resid = std::max(resid, rj);
+    }
+    if (resid <= refine_rtol || it >= refine_max) break;
+    if (it > 0 && resid >= prev) break;
+    prev = resid;
+    apply_inverse(ir_r_.get(), N, nrhs, ir_y_.get(), N);
+    kern::axpby_cols(N, nrhs, 1.0, dX, ldx, 1.0, ir_y_.get(), N, dX, ldx, stream);
+  }
+  HIP_CHECK(hipStreamSynchronize(stream));
+  last_iters = it;
+  last_resid = resid;
+}
+
+void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
   leaf_apply(Dinv.get(), dB, ldb, nrhs, dX, ldx);
   if (tree.max_depth == 0) {
     HIP_CHECK(hipStreamSynchronize(stream));

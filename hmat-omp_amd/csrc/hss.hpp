@@ -56,6 +56,12 @@ class HssEngine {
   int leaf = 0;
   bool nested = false;
   bool built = false, factored = false;
+  // FP64 iterative refinement of solves through the HSS matvec (off by
+  // default: rtol 0); iterations and max relative residual of the last solve
+  double refine_rtol = 0.0;
+  int refine_max = 30;
+  int last_iters = 0;
+  double last_resid = -1.0;
   DevBuf<double> px, py;  // kernel points
   Tree tree;
   std::vector<SNode> sn;  // per tree node
@@ -68,6 +74,7 @@ class HssEngine {
   int64_t pool_alloc(int64_t n);
   void set_ldv();
   void leaf_apply(const double* Dm, const double* B, int64_t ldb, int ncols, double* Y, int64_t ldy);
+  void apply_inverse(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
 
   // leaf generators
   DevBuf<double> D, Dinv, Ul, Vl, Ql;  // D/Dinv: nleaf x lmax^2; Ul, Vl, Ql: N x rmax
@@ -81,6 +88,7 @@ class HssEngine {
   int64_t pool_built = 0;  // end of the build's part (the factor's follows)
   int ldv = 1;  // leading dimension of every per-node vector in the sweeps
   DevBuf<double> va_, vb_, vc_, vd_, ve_;
+  DevBuf<double> ir_y_, ir_r_, ir_n_, ir_p_;  // refinement: A x, residual, norms, norm partials
   // Build scratch, kept for the next build and grown only when a build
   // needs more (freeing and reallocating multi-GB arrays per build stalls
   // the host for up to a second per array).

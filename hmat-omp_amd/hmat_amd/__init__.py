@@ -101,6 +101,8 @@ def _proto(lib):
         "hmat_hss_size": (i64, [vp]),
         "hmat_hss_root_rank": (i32, [vp]),
         "hmat_hss_to_dense": (st, [vp, _D, i64]),
+        "hmat_hss_set_refinement": (st, [vp, C.c_double, i32]),
+        "hmat_hss_last_refinement": (st, [vp, C.POINTER(i32), C.POINTER(C.c_double)]),
         "hmat_hss_save": (st, [vp, C.c_char_p]),
         "hmat_hss_load": (st, [vp, C.c_char_p]),
     }
@@ -383,6 +385,15 @@ class HSSMatrixOMP(_Base):
 
     def root_rank(self):
         return self._lib.hmat_hss_root_rank(self._h)
+
+    def set_refinement(self, rtol, max_iters=30):
+        """FP64 refinement of solves to max relative residual rtol (0: off)."""
+        _check(self._lib.hmat_hss_set_refinement(self._h, rtol, max_iters))
+
+    def last_refinement(self):
+        it, res = C.c_int32(), C.c_double()
+        _check(self._lib.hmat_hss_last_refinement(self._h, C.byref(it), C.byref(res)))
+        return it.value, res.value
 
     def save(self, path):
         """Write the built operator to `path` (see hmat_hss_save)."""

@@ -270,6 +270,15 @@ int64_t hmat_hss_size(hmat_hss h);
 /* rank_U of the root's children summed (reference: root->rank_U). */
 int32_t hmat_hss_root_rank(hmat_hss h);
 hmat_status hmat_hss_to_dense(hmat_hss h, double* A, int64_t lda);
+/* FP64 iterative refinement of HSS solves through the HSS matvec (no
+ * reference counterpart; the reference solves once): with rtol > 0 every
+ * solve iterates X += A^{-1}(B - A X) until max_j ||b_j - A x_j|| / ||b_j||
+ * <= rtol, max_iters corrections, or a correction that does not reduce the
+ * residual.  rtol = 0 (the default) turns it off.  last_refinement reports
+ * the iterations and the final max relative residual of the last solve
+ * (-1 when refinement was off). */
+hmat_status hmat_hss_set_refinement(hmat_hss h, double rtol, int32_t max_iters);
+hmat_status hmat_hss_last_refinement(hmat_hss h, int32_t* iters, double* resid);
 /* Operator files (no reference counterpart: the reference has no
  * serialisation).  save writes the built operator (options, N, leaf size,
  * nested flag, per-node ranks, transfers, couplings, leaf bases and leaf

@@ -197,3 +197,27 @@ def test_load_rejects_bad_files(tmp_path):
     # a failed load leaves the handle unbuilt-or-unchanged; a good file still loads
     g.load(good)
     assert np.array_equal(g.to_dense(), h.to_dense())
+
+
+def test_refinement_reaches_dense_solution():
+    """FP64 refinement through the HSS matvec: the refined solution equals
+    dense LAPACK on the reconstruction, the residual meets the target, and
+    rtol = 0 switches it off again (last_refinement reports -1)."""
+    n, leaf = 3000, 100
+    x = kernels.sorted_uniform_points(n, 23)
+    a = kernels.gaussian_points(x, 0.05, 1e-2)  # kappa ~ 4e4
+    o = hmat_amd.make_options(1e-10, rank=32, oversample=10, power_iters=2, exact_below=0,
+                              omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=4)
+    h = hmat_amd.HSSMatrixOMP(opts=o, nested=True)
+    h.build_from_dense(a, leaf)
+    ad = h.to_dense()
+    b = np.random.default_rng(6).standard_normal((n, 2))
+    h.set_refinement(1e-12, 10)
+    xs = h.solve(b)
+    it, res = h.last_refinement()
+    assert res <= 1e-12 and it <= 10
+    assert rel(ad @ xs, b) < 1e-12
+    assert rel(xs, np.linalg.solve(ad, b)) < 1e-10
+    h.set_refinement(0.0)
+    h.solve(b)
+    assert h.last_refinement() == (0, -1.0)

@@ -13,10 +13,10 @@ the checks are size-independent:
     lam), so a backward-stable solve would reach eps kappa = 2e-10; the
     recursive Woodbury solve amplifies rounding through its capacitance
     systems (2.5e-8 .. 4.7e-8 measured for both formats, bar 1e-6);
-  * FP64 refinement through the compressed operator (HODLR) reaches the
-    residual floor of the FP64 product, eps kappa (bar 1e-9), and the
-    compression is accurate enough that the true operator's residual is
-    as small;
+  * FP64 refinement through the compressed operator (HODLR and HSS)
+    reaches the residual floor of the FP64 product, eps kappa (bar 1e-9),
+    and the compression is accurate enough that the true operator's
+    residual is as small;
   * multi-RHS products and solves equal column-by-column ones.
 """
 import numpy as np
@@ -99,6 +99,13 @@ def test_hss_full_size(work):
     for j in (0, NRHS - 1):
         assert rel(y[:, j:j + 1], h.multiply(b[:, j])) < 1e-14
         assert rel(x[:, j:j + 1], h.solve(b[:, j])) < 1e-12
+    h.set_refinement(1e-10, 5)
+    xr = h.solve(b)
+    it, res = h.last_refinement()
+    assert 1 <= it <= 5
+    assert rel(h.multiply(xr), b) < 1e-9
+    assert rel(work["exact"](xr), b) < 1e-9
+    h.set_refinement(0.0)
     # a rebuild on the same handle reuses the build scratch and gives the same operator
     h.build_kernel(work["kernel"], N, LEAF)
     assert np.array_equal(h.multiply(b), y)
