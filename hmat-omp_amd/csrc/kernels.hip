@@ -1327,11 +1327,13 @@ __device__ __forceinline__ bool gj_panel_col(F (&pan)[16], int kb, int nbk, int 
       for (int c = 0; c < NB; ++c) pan[c] = xk[c];
   }
   if (i == k) {
-    const F pv = pan[J];
+    // one division, then products: 16 serial divisions on this one thread
+    // would hold every other thread at the next barrier
+    const F rpv = F(1) / pan[J];
     pan[J] = F(1);
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
-      pan[c] /= pv;
+      pan[c] *= rpv;
       rowk[c] = pan[c];
     }
   }
