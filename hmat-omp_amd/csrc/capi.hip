@@ -189,7 +189,7 @@ hmat_status hmat_kernel_matvec_device(const hmat_kernel* k, int64_t N, const dou
     for (int c0 = 0; c0 < nrhs; c0 += 64) {
       const int nc = std::min(64, nrhs - c0);
       std::vector<SegSk> seg{SegSk{0, 0, 0, (int64_t)c0 * ldx, (int64_t)c0 * ldy, (int32_t)N, (int32_t)N, 0, nc}};
-      kern::sketch(g, false, seg, ds, dX, ldx, dY, ldy, nc, ws, s);
+      kern::sketch(g, false, seg, dX, ldx, dY, ldy, nc, ws, s);
     }
     HIP_CHECK(hipStreamSynchronize(s));
     stager_release(s);

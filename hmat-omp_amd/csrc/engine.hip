@@ -222,7 +222,7 @@ bool Orthogonalizer::pass(const std::vector<OrthSeg>& segs, const double* in, do
   const int64_t kk = (int64_t)kcap * kcap;
   G_.reserve(nb * kk);
   R_.reserve(nb * kk);
-  Ri_.reserve(nb * kk);
+  Rchk_.reserve(nb * kk);
   info_.reserve(nb);
   gerr_.reserve(nb);
   std::vector<SegTN> tn(nb);
@@ -238,8 +238,8 @@ bool Orthogonalizer::pass(const std::vector<OrthSeg>& segs, const double* in, do
                       : 0.0;
     nn[b] = SegNN{g.off, (int64_t)b * kk, g.off, g.len, g.k, g.k, 0};
   }
-  kern::seg_gemm_tn(tn, dtn_, in, ld, in, ld, G_.get(), kcap, 1.0, 0.0, ws_, s);
-  kern::chol_rinv(sm, dsm_, G_.get(), R_.get(), Ri_.get(), coef, shift_, info_.get(), gerr_.get(), s);
+  kern::seg_gemm_tn(tn, in, ld, in, ld, G_.get(), kcap, 1.0, 0.0, ws_, s);
+  kern::chol_shifted(sm, G_.get(), R_.get(), coef, info_.get(), gerr_.get(), s);
   auto info = download(info_.get(), nb, s);
   auto gerr = download(gerr_.get(), nb, s);
   bool breakdown = false;
@@ -254,7 +254,7 @@ bool Orthogonalizer::pass(const std::vector<OrthSeg>& segs, const double* in, do
     if (shifted) throw Error(HMAT_ERR_NONFINITE, "orthogonalisation: shifted Cholesky broke down");
     return pass(segs, in, out, ld, kcap, true, s);
   }
-  kern::seg_trsm_upper(nn, dnn_, in, ld, R_.get(), kcap, out, ld, s);
+  kern::seg_trsm_upper(nn, in, ld, R_.get(), kcap, out, ld, s);
   return true;
 }
 
@@ -302,7 +302,7 @@ void Orthogonalizer::make_plan(const std::vector<OrthSeg>& segs, int kcap, bool 
   plan_.acc_tiles = g.get<kern::NNTile>(i_acct);
   G_.reserve(nb * kk);
   R_.reserve(nb * kk);
-  Ri_.reserve(nb * kk);
+  Rchk_.reserve(nb * kk);
   Racc_.reserve(nb * kk);
   if (with_acc) Rtmp_.reserve(nb * kk);
   ws_.part.reserve((size_t)std::max(plan_.nslots, 1) * 4096);
@@ -332,7 +332,7 @@ void Orthogonalizer::run_once(const std::vector<OrthSeg>& segs, double* Y, doubl
   info_.reserve(nb);
   gerr_.reserve(nb);
   gram(Y, ld, kcap, s);
-  kern::launch_chol_rinv(plan_.mats, (int)nb, plan_.nmax, G_.get(), R_.get(), plan_.shift_on, info_.get(),
+  kern::launch_chol_shifted(plan_.mats, (int)nb, plan_.nmax, G_.get(), R_.get(), plan_.shift_on, info_.get(),
                          gerr_.get(), s);
   if (plan_.kmax <= 64) {
     trsm(Y, ld, R_.get(), kcap, Y, s);  // in place
@@ -359,13 +359,13 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
   // shifted pass Y -> T (R_1 in Racc_), unshifted pass T -> Y (R_2 in R_),
   // and the Gram matrix of the result with its deviation from I.
   gram(Y, ld, kcap, s);
-  kern::launch_chol_rinv(plan_.mats, nm, plan_.nmax, G_.get(), Racc_.get(), plan_.shift_on, info, gerr, s);
+  kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), Racc_.get(), plan_.shift_on, info, gerr, s);
   trsm(Y, ld, Racc_.get(), kcap, T, s);
   gram(T, ld, kcap, s);
-  kern::launch_chol_rinv(plan_.mats, nm, plan_.nmax, G_.get(), R_.get(), plan_.shift_off, info + nb, gerr + nb, s);
+  kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), R_.get(), plan_.shift_off, info + nb, gerr + nb, s);
   trsm(T, ld, R_.get(), kcap, Y, s);
   gram(Y, ld, kcap, s);
-  kern::launch_chol_rinv(plan_.mats, nm, plan_.nmax, G_.get(), Ri_.get(), plan_.shift_off, info + 2 * nb,
+  kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), Rchk_.get(), plan_.shift_off, info + 2 * nb,
                          gerr + 2 * nb, s);
   const auto h = download(chk_.get(), info_bytes + 3 * nb * sizeof(double), s);
   const int* hi = reinterpret_cast<const int*>(h.data());

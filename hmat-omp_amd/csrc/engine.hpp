@@ -174,11 +174,10 @@ class Orthogonalizer {
   void trsm(const double* X, int64_t ld, const double* R, int kcap, double* Y, hipStream_t s);
   void acc_product(int kcap, hipStream_t s);
   kern::Workspace ws_;
-  DevBuf<double> G_, R_, Ri_, Racc_, Rtmp_, shift_, gerr_;
+  DevBuf<double> G_, R_, Rchk_, Racc_, Rtmp_, gerr_;  // Rchk_: the check pass's R (discarded)
   DevBuf<int> info_;
   DevBuf<uint8_t> chk_;  // both passes' info and Gram deviations, downloaded together
   DevBuf<SegNN> dnn_;
-  DevBuf<SegTN> dtn_;
   DevBuf<SmallMat> dsm_;
   // One CholeskyQR pass in -> out.  Returns false (and leaves out alone)
   // when `in` is already orthonormal to kOrthTol and `shifted` is false.

@@ -364,9 +364,9 @@ void HodlrEngine::build_group(int c0, int c1, const GenDesc& g, const std::vecto
   const bool lp = (opt.precision == HMAT_PREC_MIXED);
   auto range_pass = [&](bool trans, const std::vector<SegSk>& sk, const double* X, double* Y, bool last) {
     if (lp)
-      kern::sketch_lp(g, trans, last ? kLowPrecSplit : kPowerSplit, sk, dsk_, X, N, Y, N, kcap, ws_, stream);
+      kern::sketch_lp(g, trans, last ? kLowPrecSplit : kPowerSplit, sk, X, N, Y, N, kcap, ws_, stream);
     else
-      kern::sketch(g, trans, sk, dsk_, X, N, Y, N, kcap, ws_, stream);
+      kern::sketch(g, trans, sk, X, N, Y, N, kcap, ws_, stream);
   };
   const int q = sk_fwd_rsvd.empty() ? 0 : opt.power_iters;
   if (lp && q > 0) {
@@ -394,7 +394,7 @@ void HodlrEngine::build_group(int c0, int c1, const GenDesc& g, const std::vecto
   }
   orth_.run(orow, Yb_.get(), Tb_.get(), N, kcap, nullptr, stream);
   // B^T = A^T Q (n x k), then B^T = Qb R                 (compression.h:87)
-  kern::sketch(g, true, sk_bwd, dsk_, Yb_.get(), N, Xb_.get(), N, kcap, ws_, stream);
+  kern::sketch(g, true, sk_bwd, Yb_.get(), N, Xb_.get(), N, kcap, ws_, stream);
   orth_.run(ocol, Xb_.get(), Tb_.get(), N, kcap, Rb_.get(), stream);
   // R = Ur S Vr^T  =>  A_blk ~ (Q Vr) S (Qb Ur)^T        (compression.h:90-98)
   std::vector<SmallMat> sm(nt);
@@ -446,8 +446,8 @@ void HodlrEngine::build_group(int c0, int c1, const GenDesc& g, const std::vecto
       nu[s] = SegNN{st[t] + cb(t), (int64_t)t * kk, st[t], (int32_t)ln[t], k[t], L.rmax, 0};
       nv[s] = SegNN{sst[t] + cb(t), (int64_t)t * kk, sst[t], (int32_t)sln[t], k[t], L.rmax, 0};
     }
-    kern::seg_gemm_nn(nu, dnn_, Yb_.get(), N, TU_.get(), kcap, L.US.get(), N, 1.0, 0.0, stream);
-    kern::seg_gemm_nn(nv, dnn_, Xb_.get(), N, TV_.get(), kcap, L.V.get(), N, 1.0, 0.0, stream);
+    kern::seg_gemm_nn(nu, Yb_.get(), N, TU_.get(), kcap, L.US.get(), N, 1.0, 0.0, stream);
+    kern::seg_gemm_nn(nv, Xb_.get(), N, TV_.get(), kcap, L.V.get(), N, 1.0, 0.0, stream);
   }
 }
 
@@ -460,7 +460,7 @@ void HodlrEngine::leaf_apply_t(const F* Dm, const F* B, int64_t ldb, int ncols, 
     const TNode& nd = tree.nodes[tree.leaves[i]];
     segs.push_back(SegNN{(int64_t)i * lm * lm, nd.start, nd.start, (int32_t)nd.len, (int32_t)nd.len, ncols, 0});
   }
-  kern::seg_gemm_nn<F>(segs, dnn_, Dm, lm, B, ldb, Y, ldy, F(1), F(0), stream);
+  kern::seg_gemm_nn<F>(segs, Dm, lm, B, ldb, Y, ldy, F(1), F(0), stream);
 }
 
 void HodlrEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
@@ -492,8 +492,8 @@ void HodlrEngine::matvec_enqueue(const double* dX, int64_t ldx, int nrhs, double
       // y_s += (U S)_s t_sib
       nn[s] = SegNN{nd.start, (int64_t)L.sib[s] * tstride, nd.start, (int32_t)nd.len, L.rank[s], nrhs, 0};
     }
-    kern::seg_gemm_tn<double>(tn, dtn_, L.V.get(), N, dX, ldx, tvec_.get(), L.rmax, 1.0, 0.0, ws_, stream);
-    kern::seg_gemm_nn<double>(nn, dnn_, L.US.get(), N, tvec_.get(), L.rmax, dY, ldy, 1.0, 1.0, stream);
+    kern::seg_gemm_tn<double>(tn, L.V.get(), N, dX, ldx, tvec_.get(), L.rmax, 1.0, 0.0, ws_, stream);
+    kern::seg_gemm_nn<double>(nn, L.US.get(), N, tvec_.get(), L.rmax, dY, ldy, 1.0, 1.0, stream);
   }
 }
 
@@ -522,9 +522,9 @@ void HodlrEngine::correction_t(HFactor<F>& fa, int j, F* W, int64_t ldw, int nco
     up.push_back(SegNN{l.start, (int64_t)p * tst + rr, l.start, (int32_t)l.len, rl, ncols, 0});
     up.push_back(SegNN{r.start, (int64_t)p * tst, r.start, (int32_t)r.len, rr, ncols, 0});
   }
-  kern::seg_gemm_tn<F>(tn, dtn_, fa.V[j + 1], N, W, ldw, fa.tvec.get(), mm, F(1), F(0), ws_, stream);
-  kern::seg_gemm_nn<F>(mn, dnn_, fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(0), stream);
-  kern::seg_gemm_nn<F>(up, dnn_, Q, N, fa.coef.get(), mm, W, ldw, F(-1), F(1), stream);
+  kern::seg_gemm_tn<F>(tn, fa.V[j + 1], N, W, ldw, fa.tvec.get(), mm, F(1), F(0), ws_, stream);
+  kern::seg_gemm_nn<F>(mn, fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(0), stream);
+  kern::seg_gemm_nn<F>(up, Q, N, fa.coef.get(), mm, W, ldw, F(-1), F(1), stream);
 }
 
 template <class F>
@@ -628,7 +628,7 @@ void HodlrEngine::factor_t(HFactor<F>& fa) {
     }
     fa.Minv[j].zero(stream);
     kern::fill_identity_segs<F>(eye, dnn_, fa.Minv[j].get(), mm, stream);
-    kern::seg_gemm_tn<F>(kb, dtn_, fa.V[j + 1], N, Q, N, fa.Minv[j].get(), mm, F(1), F(0), ws_, stream);
+    kern::seg_gemm_tn<F>(kb, fa.V[j + 1], N, Q, N, fa.Minv[j].get(), mm, F(1), F(0), ws_, stream);
     kern::gj_inverse<F>(ms, dsm_, fa.Minv[j].get(), info_.get() + info_off, stream);
     cap_info.push_back({info_off, np});
     info_off += np;

@@ -219,8 +219,6 @@ class HodlrEngine {
   kern::Workspace ws_;
   Orthogonalizer orth_;
   DevBuf<SegNN> dnn_;
-  DevBuf<SegTN> dtn_;
-  DevBuf<SegSk> dsk_;
   DevBuf<SmallMat> dsm_;
   DevBuf<BlockDesc> dbd_;
   DevBuf<double> Yb_, Xb_, Tb_, Rb_, Vr_, Sv_, Wk_, TU_, TV_;

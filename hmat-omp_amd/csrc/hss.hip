@@ -78,7 +78,7 @@ void HssEngine::leaf_apply(const double* Dm, const double* B, int64_t ldb, int n
     const TNode& nd = tree.nodes[tree.leaves[i]];
     segs.push_back(SegNN{(int64_t)i * lm * lm, nd.start, nd.start, (int32_t)nd.len, (int32_t)nd.len, ncols, 0});
   }
-  kern::seg_gemm_nn<double>(segs, dnn_, Dm, lm, B, ldb, Y, ldy, 1.0, 0.0, stream);
+  kern::seg_gemm_nn<double>(segs, Dm, lm, B, ldb, Y, ldy, 1.0, 0.0, stream);
 }
 
 // Leaf bases of one side.  Row side: range of A(I, I^c) (len x N with the
@@ -161,16 +161,16 @@ void HssEngine::compress_leaves(const GenDesc& g, bool col, const double* host_o
   }
   const bool tr_fwd = col;  // row side reads A(I, red); column side A(red, I)^T
   kern::fill_identity_segs(eye, dnn_, Yb.get(), N, stream);
-  kern::sketch(g, tr_fwd, fwd, dsk_, X.get(), N, Yb.get(), N, kcap, ws_, stream);
+  kern::sketch(g, tr_fwd, fwd, X.get(), N, Yb.get(), N, kcap, ws_, stream);
   for (int it = 0; it < opt.power_iters && !fwd_r.empty(); ++it) {
     orth_.run(oy, Yb.get(), Tb.get(), N, kcap, nullptr, stream);
-    kern::sketch(g, !tr_fwd, bwd_r, dsk_, Yb.get(), N, Bw.get(), N, kcap, ws_, stream);
+    kern::sketch(g, !tr_fwd, bwd_r, Yb.get(), N, Bw.get(), N, kcap, ws_, stream);
     kern::zero_rects(zr, dzr_, stream);
     orth_.run(ob_r, Bw.get(), Tw2.get(), N, kcap, nullptr, stream);
-    kern::sketch(g, tr_fwd, fwd_r, dsk_, Bw.get(), N, Yb.get(), N, kcap, ws_, stream);
+    kern::sketch(g, tr_fwd, fwd_r, Bw.get(), N, Yb.get(), N, kcap, ws_, stream);
   }
   orth_.run(oy, Yb.get(), Tb.get(), N, kcap, nullptr, stream);
-  kern::sketch(g, !tr_fwd, bwd, dsk_, Yb.get(), N, Bw.get(), N, kcap, ws_, stream);
+  kern::sketch(g, !tr_fwd, bwd, Yb.get(), N, Bw.get(), N, kcap, ws_, stream);
   kern::zero_rects(zr, dzr_, stream);
   const int64_t kk = (int64_t)kcap * kcap;
   DevBuf<double>&Rb = sc.Rb, &Vr = sc.Vr, &Sv = sc.Sv, &Wk = sc.Wk, &TU = sc.TU, &TV = sc.TV;
@@ -225,8 +225,8 @@ void HssEngine::compress_leaves(const GenDesc& g, bool col, const double* host_o
     nb.push_back(SegNN{nd.start, (int64_t)i * kk, nd.start, (int32_t)nd.len, k[i], rank[i], 0});
     nt.push_back(SegNN{(int64_t)i * nk, (int64_t)i * kk, tcol[i] * N, (int32_t)N, k[i], rank[i], 0});
   }
-  kern::seg_gemm_nn(nb, dnn_, Yb.get(), N, TU.get(), kcap, basis.get(), N, 1.0, 0.0, stream);
-  kern::seg_gemm_nn(nt, dnn_, Bw.get(), N, TV.get(), kcap, Tleaf.get(), N, 1.0, 0.0, stream);
+  kern::seg_gemm_nn(nb, Yb.get(), N, TU.get(), kcap, basis.get(), N, 1.0, 0.0, stream);
+  kern::seg_gemm_nn(nt, Bw.get(), N, TV.get(), kcap, Tleaf.get(), N, 1.0, 0.0, stream);
   HIP_CHECK(hipStreamSynchronize(stream));
   for (int i = 0; i < nl; ++i) {
     if (col) sn[L[i]].rV = rank[i];
@@ -303,7 +303,7 @@ void HssEngine::build_internal() {
       }
     }
     kern::copy_segs(cp, dnn_, Vl.get(), N, VF.get(), N, stream);
-    kern::seg_gemm_nn(mm, dnn_, VFc->get(), N, pool.get(), ldw, VF.get(), N, 1.0, 0.0, stream);
+    kern::seg_gemm_nn(mm, VFc->get(), N, pool.get(), ldw, VF.get(), N, 1.0, 0.0, stream);
   };
 
   const std::vector<int64_t> none(nn, 0);
@@ -337,7 +337,7 @@ void HssEngine::build_internal() {
         tb.push_back(SegTN{tcol[nd.right] * N + a.start, a.start, sn[id].oBba, (int32_t)a.len, sn[nd.right].rU,
                            sn[nd.left].rV, 0});
       }
-      kern::seg_gemm_tn(tb, dtn_, Tc->get(), N, VFc->get(), N, pool.get(), ldb, 1.0, 0.0, ws_, stream);
+      kern::seg_gemm_tn(tb, Tc->get(), N, VFc->get(), N, pool.get(), ldb, 1.0, 0.0, ws_, stream);
     }
     if (d == 0) break;  // the root has no transfer and no projection
     // ---- stacked child projections P^T = [T_a^T | T_b^T], own rows zeroed
@@ -472,7 +472,7 @@ void HssEngine::build_internal() {
         }
         kern::rect_axpby(cp, drc_, stream);
         Pn[side].reserve((size_t)N * std::max<int64_t>(sum, 1));
-        kern::seg_gemm_nn(nt, dnn_, P.get(), N, TV.get(), kcap, Pn[side].get(), N, 1.0, 0.0, stream);
+        kern::seg_gemm_nn(nt, P.get(), N, TV.get(), kcap, Pn[side].get(), N, 1.0, 0.0, stream);
         for (int i = 0; i < ni; ++i) (side ? sn[inner[i]].rV : sn[inner[i]].rU) = rk[i];
       }
       make_wide(ids, Pn[0].get(), Pn[1].get(), ncT, ncS, *Tn, *Sn);
@@ -575,7 +575,7 @@ void HssEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int6
       const TNode& nd = tree.nodes[id];
       tn.push_back(SegTN{nd.start, nd.start, (int64_t)id * vs, (int32_t)nd.len, sn[id].rV, nrhs, 0});
     }
-    kern::seg_gemm_tn(tn, dtn_, Vl.get(), N, dX, ldx, G, ldv, 1.0, 0.0, ws_, stream);
+    kern::seg_gemm_tn(tn, Vl.get(), N, dX, ldx, G, ldv, 1.0, 0.0, ws_, stream);
   }
   const double* P = pool.get();
   for (int d = tree.max_depth - 1; d >= 1; --d) {
@@ -612,7 +612,7 @@ void HssEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int6
       const TNode& nd = tree.nodes[id];
       nn2.push_back(SegNN{nd.start, (int64_t)id * vs, nd.start, (int32_t)nd.len, sn[id].rU, nrhs, 0});
     }
-    kern::seg_gemm_nn(nn2, dnn_, Ul.get(), N, F, ldv, dY, ldy, 1.0, 1.0, stream);
+    kern::seg_gemm_nn(nn2, Ul.get(), N, F, ldv, dY, ldy, 1.0, 1.0, stream);
   }
   HIP_CHECK(hipStreamSynchronize(stream));
 }
@@ -652,7 +652,7 @@ void HssEngine::factor() {
       const TNode& nd = tree.nodes[id];
       tn.push_back(SegTN{nd.start, nd.start, sn[id].oK, (int32_t)nd.len, sn[id].rV, sn[id].rU, 0});
     }
-    kern::seg_gemm_tn(tn, dtn_, Vl.get(), N, Ql.get(), N, pool.get(), ldv, 1.0, 0.0, ws_, stream);
+    kern::seg_gemm_tn(tn, Vl.get(), N, Ql.get(), N, pool.get(), ldv, 1.0, 0.0, ws_, stream);
   }
   for (int d = tree.max_depth - 1; d >= 0; --d) {
     std::vector<int> inner;
@@ -801,7 +801,7 @@ void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* d
       const TNode& nd = tree.nodes[id];
       tn.push_back(SegTN{nd.start, nd.start, (int64_t)id * vs, (int32_t)nd.len, sn[id].rV, nrhs, 0});
     }
-    kern::seg_gemm_tn(tn, dtn_, Vl.get(), N, dX, ldx, H, ldv, 1.0, 0.0, ws_, stream);
+    kern::seg_gemm_tn(tn, Vl.get(), N, dX, ldx, H, ldv, 1.0, 0.0, ws_, stream);
   }
   const double* P = pool.get();
   for (int d = tree.max_depth - 1; d >= 0; --d) {
@@ -860,7 +860,7 @@ void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* d
       const TNode& nd = tree.nodes[id];
       up.push_back(SegNN{nd.start, (int64_t)id * vs, nd.start, (int32_t)nd.len, sn[id].rU, nrhs, 0});
     }
-    kern::seg_gemm_nn(up, dnn_, Ql.get(), N, T, ldv, dX, ldx, -1.0, 1.0, stream);
+    kern::seg_gemm_nn(up, Ql.get(), N, T, ldv, dX, ldx, -1.0, 1.0, stream);
   }
   HIP_CHECK(hipStreamSynchronize(stream));
 }

@@ -102,8 +102,6 @@ class HssEngine {
   kern::Workspace ws_;
   Orthogonalizer orth_;
   DevBuf<SegNN> dnn_;
-  DevBuf<SegTN> dtn_;
-  DevBuf<SegSk> dsk_;
   DevBuf<SmallMat> dsm_;
   DevBuf<BlockDesc> dbd_;
   DevBuf<kern::BGemm> dbg_;

@@ -233,9 +233,8 @@ template void launch_nn<float>(const NNPlan&, const float*, int64_t, const float
                                float, hipStream_t);
 
 template <class F>
-void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* X, int64_t ldx,
+void seg_gemm_nn(const std::vector<SegNN>& segs, const F* X, int64_t ldx,
                  const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha, F beta, hipStream_t s) {
-  (void)dsegs;
   if (segs.empty()) return;
   std::vector<NNTile> tiles;
   const int mode = nn_tiles(segs, tiles);
@@ -246,9 +245,9 @@ void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* 
   launch_nn_raw<F>(mode, (int)tiles.size(), g.get<SegNN>(is), g.get<NNTile>(it), X, ldx, T, ldt, Y, ldy, alpha,
                    beta, s);
 }
-template void seg_gemm_nn<double>(const std::vector<SegNN>&, DevBuf<SegNN>&, const double*, int64_t,
+template void seg_gemm_nn<double>(const std::vector<SegNN>&, const double*, int64_t,
                                   const double*, int64_t, double*, int64_t, double, double, hipStream_t);
-template void seg_gemm_nn<float>(const std::vector<SegNN>&, DevBuf<SegNN>&, const float*, int64_t,
+template void seg_gemm_nn<float>(const std::vector<SegNN>&, const float*, int64_t,
                                  const float*, int64_t, float*, int64_t, float, float, hipStream_t);
 
 // --------------------------------------------------------------------------
@@ -392,9 +391,8 @@ void launch_trsm_raw(int kmax, int ntiles, const SegNN* ds, const NNTile* dt, co
   HIP_CHECK(hipGetLastError());
 }
 
-void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X,
+void seg_trsm_upper(const std::vector<SegNN>& segs, const double* X,
                     int64_t ldx, const double* R, int64_t ldr, double* Y, int64_t ldy, hipStream_t s) {
-  (void)dsegs;
   if (segs.empty()) return;
   std::vector<NNTile> tiles;
   const int kmax = trsm_tiles(segs, tiles);
@@ -613,10 +611,9 @@ template void launch_tn<float>(const TNPlan&, const float*, int64_t, const float
                                float, float, float*, hipStream_t);
 
 template <class F>
-void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* X, int64_t ldx,
+void seg_gemm_tn(const std::vector<SegTN>& segs, const F* X, int64_t ldx,
                  const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta, Workspace& ws,
                  hipStream_t s) {
-  (void)dsegs;
   if (segs.empty()) return;
   std::vector<TNItem> items;
   std::vector<TNOut> outs;
@@ -630,10 +627,10 @@ void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* 
   launch_tn_raw<F>((int)items.size(), (int)outs.size(), g.get<SegTN>(is), g.get<TNItem>(ii), g.get<TNOut>(io), X,
                    ldx, Y, ldy, O, ldo, alpha, beta, part.get(), s);
 }
-template void seg_gemm_tn<double>(const std::vector<SegTN>&, DevBuf<SegTN>&, const double*, int64_t,
+template void seg_gemm_tn<double>(const std::vector<SegTN>&, const double*, int64_t,
                                   const double*, int64_t, double*, int64_t, double, double,
                                   Workspace&, hipStream_t);
-template void seg_gemm_tn<float>(const std::vector<SegTN>&, DevBuf<SegTN>&, const float*, int64_t,
+template void seg_gemm_tn<float>(const std::vector<SegTN>&, const float*, int64_t,
                                  const float*, int64_t, float*, int64_t, float, float, Workspace&,
                                  hipStream_t);
 
@@ -874,7 +871,7 @@ __global__ __launch_bounds__(64) void k_chol64(const SmallMat* __restrict__ mats
     for (int i = 0; i < n; ++i) rg[i + (int64_t)c * ld] = r[i + c * lr];
 }
 
-void launch_chol_rinv(const SmallMat* dmats, int nmats, int nmax, double* G, double* R, const double* dshift,
+void launch_chol_shifted(const SmallMat* dmats, int nmats, int nmax, double* G, double* R, const double* dshift,
                       int* info, double* gerr, hipStream_t s) {
   if (!nmats) return;
   const size_t lds = (size_t)nmax * (nmax + 1) * sizeof(double);
@@ -887,19 +884,15 @@ void launch_chol_rinv(const SmallMat* dmats, int nmats, int nmax, double* G, dou
   HIP_CHECK(hipGetLastError());
 }
 
-void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* G, double* R,
-               double* Rinv, const std::vector<double>& shift_coef, DevBuf<double>& dshift,
-               int* info, double* gerr, hipStream_t s) {
-  (void)Rinv;
+void chol_shifted(const std::vector<SmallMat>& mats, double* G, double* R, const std::vector<double>& shift_coef,
+                  int* info, double* gerr, hipStream_t s) {
   if (mats.empty()) return;
   int nmax = 0;
   for (auto& mm : mats) nmax = std::max(nmax, mm.n);
-  (void)dm;
-  (void)dshift;
   StageGroup g;
   const int im = g.add(mats), ic = g.add(shift_coef);
   g.put(s);
-  launch_chol_rinv(g.get<SmallMat>(im), (int)mats.size(), nmax, G, R, g.get<double>(ic), info, gerr, s);
+  launch_chol_shifted(g.get<SmallMat>(im), (int)mats.size(), nmax, G, R, g.get<double>(ic), info, gerr, s);
 }
 
 // One-sided Jacobi (Hestenes) with a round-robin pair schedule; the four

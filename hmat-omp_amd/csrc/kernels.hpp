@@ -130,14 +130,14 @@ void launch_tn(const TNPlan& p, const F* X, int64_t ldx, const F* Y, int64_t ldy
 
 // F = double or float (FP32 factorisation of the mixed-precision path)
 template <class F>
-void seg_gemm_nn(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const F* X, int64_t ldx,
+void seg_gemm_nn(const std::vector<SegNN>& segs, const F* X, int64_t ldx,
                  const F* T, int64_t ldt, F* Y, int64_t ldy, F alpha, F beta, hipStream_t s);
 
 // Y(seg rows, 0:q) = X(seg rows, 0:q) R_s^{-1} for upper-triangular R_s
 // (q x q at toff, leading dimension ldt): row-wise forward substitution,
 // backward stable (unlike multiplying by an explicit inverse, whose error
 // grows with cond(R)).  Uses xoff/toff/yoff/len/q of each SegNN.
-void seg_trsm_upper(const std::vector<SegNN>& segs, DevBuf<SegNN>& dsegs, const double* X,
+void seg_trsm_upper(const std::vector<SegNN>& segs, const double* X,
                     int64_t ldx, const double* R, int64_t ldr, double* Y, int64_t ldy, hipStream_t s);
 // The same with its descriptors uploaded once and launched any number of
 // times.  For k <= 64 each row is read completely before it is written, so
@@ -152,32 +152,30 @@ void launch_trsm(const TrsmPlan& p, const double* X, int64_t ldx, const double* 
                  int64_t ldy, hipStream_t s);
 
 template <class F>
-void seg_gemm_tn(const std::vector<SegTN>& segs, DevBuf<SegTN>& dsegs, const F* X, int64_t ldx,
+void seg_gemm_tn(const std::vector<SegTN>& segs, const F* X, int64_t ldx,
                  const F* Y, int64_t ldy, F* O, int64_t ldo, F alpha, F beta, Workspace& ws,
                  hipStream_t s);
 
-void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs,
-            const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap, Workspace& ws,
-            hipStream_t s);
+void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, const double* X, int64_t ldx,
+            double* Y, int64_t ldy, int kcap, Workspace& ws, hipStream_t s);
 
 // Same contraction on BF16 MFMA: entries generated in FP32 and split into
 // `split` (1..3) BF16 terms, the tall operand likewise; products of terms
 // whose split indices sum to < split are accumulated in FP32 and flushed to
 // FP64 every 512 reduction indices.  Y is FP64.
 void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>& segs,
-               DevBuf<SegSk>& dsegs, const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap,
+               const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap,
                Workspace& ws, hipStream_t s);
 
-// Gram G (n x n per entry) -> upper R with R^T R = G + shift_s I and
-// Rinv = R^{-1}.  shift_s = shift_coef * trace(G) when shift_coef > 0.
-// info[b] != 0 on breakdown; gerr[b] = max |G - I| (0 for an all-zero G),
-// the orthogonality defect of the columns the Gram came from.
-void chol_rinv(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, double* G, double* R,
-               double* Rinv, const std::vector<double>& shift_coef, DevBuf<double>& dshift,
-               int* info, double* gerr, hipStream_t s);
+// Gram G (n x n per entry) -> upper R with R^T R = G + shift_s I,
+// shift_s = shift_coef * trace(G) when shift_coef > 0.  info[b] != 0 on
+// breakdown; gerr[b] = max |G - I| (0 for an all-zero G), the
+// orthogonality defect of the columns the Gram came from.
+void chol_shifted(const std::vector<SmallMat>& mats, double* G, double* R, const std::vector<double>& shift_coef,
+                  int* info, double* gerr, hipStream_t s);
 // The same on descriptors and shift coefficients already on the device.
-void launch_chol_rinv(const SmallMat* dmats, int nmats, int nmax, double* G, double* R, const double* dshift,
-                      int* info, double* gerr, hipStream_t s);
+void launch_chol_shifted(const SmallMat* dmats, int nmats, int nmax, double* G, double* R, const double* dshift,
+                         int* info, double* gerr, hipStream_t s);
 
 // One-sided Jacobi SVD of n x n matrices A = U diag(S) V^T (A overwritten
 // by U).  S sorted descending, U/V columns permuted accordingly.  A pair of

@@ -463,15 +463,14 @@ static void launch_sketch(const Gen& gen, int NT, int RF, dim3 grid, const SegSk
 }
 
 // One sketch with a given tiling: rows per workgroup 64 * RF, split-K target.
-static void run_sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs,
-                       const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap, Workspace& ws, int RF,
-                       int64_t want, bool prof_on, hipStream_t s) {
+static void run_sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, const double* X,
+                       int64_t ldx, double* Y, int64_t ldy, int kcap, Workspace& ws, int RF, int64_t want,
+                       bool prof_on, hipStream_t s) {
   const int kpad = round_up(kcap, 16);
   const int NT = std::min(4, kpad / 16);
   const int rows_per_wg = 64 * RF;
   SketchPlan p = plan_sketch(segs, kcap, NT * 16, rows_per_wg, ldy, want);
   if (p.tiles.empty()) return;
-  (void)dsegs;
   // segments, tiles and (split-K) sum tiles staged together
   const std::vector<SumTile> st = p.nsplit > 1 ? sum_tiles(segs) : std::vector<SumTile>();
   StageGroup sgp;
@@ -543,9 +542,8 @@ static int time_candidates(int ncand, Run&& run, hipStream_t s) {
   return best;
 }
 
-void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf<SegSk>& dsegs,
-            const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap, Workspace& ws,
-            hipStream_t s) {
+void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, const double* X, int64_t ldx,
+            double* Y, int64_t ldy, int kcap, Workspace& ws, hipStream_t s) {
   if (segs.empty() || kcap <= 0) return;
   int64_t rows = 0;
   for (auto& sg : segs) rows += sg.outlen;
@@ -560,7 +558,7 @@ void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf
     int choice = -1;
     if (!tune_lookup(key, choice)) {
       choice = time_candidates(6, [&](int c) {
-        run_sketch(g, trans, segs, dsegs, X, ldx, Y, ldy, kcap, ws, kRF[c / 3], kWant[c % 3], false, s);
+        run_sketch(g, trans, segs, X, ldx, Y, ldy, kcap, ws, kRF[c / 3], kWant[c % 3], false, s);
       }, s);
       tune_store(key, choice);
       if (choice == 5) return;  // the last candidate's result is in Y
@@ -568,7 +566,7 @@ void sketch(const GenDesc& g, bool trans, const std::vector<SegSk>& segs, DevBuf
     RF = kRF[choice / 3];
     want = kWant[choice % 3];
   }
-  run_sketch(g, trans, segs, dsegs, X, ldx, Y, ldy, kcap, ws, RF, want, true, s);
+  run_sketch(g, trans, segs, X, ldx, Y, ldy, kcap, ws, RF, want, true, s);
 }
 
 template <class Gen, bool TRANS>
@@ -583,7 +581,7 @@ static void launch_sketch_lp(const Gen& gen, int split, dim3 grid, const SegSk* 
 }
 
 void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>& segs,
-               DevBuf<SegSk>& dsegs, const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap,
+               const double* X, int64_t ldx, double* Y, int64_t ldy, int kcap,
                Workspace& ws, hipStream_t s) {
   if (segs.empty() || kcap <= 0) return;
   split = std::max(1, std::min(split, 3));
