@@ -11,7 +11,9 @@
 // Both stage a K-tile of 64 reduction rows of X (and their points) in LDS
 // and zero the X rows of reduction indices past the end or inside the
 // excluded interval as they stage them, so their inner loops carry no
-// masks.
+// masks.  The next tile is loaded into registers during the current one's
+// products and, in k_sketch and the multi-term k_sketch_lp, written to a
+// second LDS buffer, so a tile costs one barrier.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,8 +51,11 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
                                                  double* __restrict__ Y, int64_t ldy,
                                                  int64_t split_stride) {
   constexpr int KT = 64, NC = NT * 16;
-  __shared__ double xs[KT][NC + 1];
-  __shared__ double rp[KT];
+  // two K-tile buffers: the next tile is committed into the other one, so
+  // one barrier per tile separates the readers of a buffer from its next
+  // writers
+  __shared__ double xs[2][KT][NC + 1];
+  __shared__ double rp[2][KT];
   const SkTile tl = tiles[blockIdx.x];
   const SegSk sg = segs[tl.seg];
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
@@ -95,20 +100,21 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
       ppt = TRANS ? gen.row_pt(sg.red0 + min(j, tl.re - 1)) : gen.col_pt(sg.red0 + min(j, tl.re - 1));
     }
   };
-  auto commit = [&]() {
+  auto commit = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < NPRE; ++i) {
       const int e = threadIdx.x + i * kWGs;
-      xs[e % KT][e / KT] = pre[i];
+      xs[buf][e % KT][e / KT] = pre[i];
     }
-    if (threadIdx.x < KT) rp[threadIdx.x] = ppt;
+    if (threadIdx.x < KT) rp[buf][threadIdx.x] = ppt;
   };
   // Dense input reads A from memory in the inner loop already; there the
   // next tile is fetched after the compute, keeping its registers free.
   constexpr bool PF = !std::is_same<Gen, GDense>::value;
   issue(tl.rb);
-  commit();
+  commit(0);
   __syncthreads();
+  int cur = 0;
   for (int kb = tl.rb; kb < tl.re; kb += KT) {
     const bool has_next = kb + KT < tl.re;
     if (PF && has_next) issue(kb + KT);
@@ -117,24 +123,24 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
       const int kk = ks * 4 + kq;
       const int jl = min(kb + kk, tl.re - 1);
       const int64_t jg = sg.red0 + jl;
-      const double rpt = rp[kk];
+      const double rpt = rp[cur][kk];
       double a[RF];
 #pragma unroll
       for (int f = 0; f < RF; ++f)
         a[f] = TRANS ? gen.eval(rpt, opt[f], jg, iglob[f]) : gen.eval(opt[f], rpt, iglob[f], jg);
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const double b = xs[kk][t * 16 + (lane % 16)];
+        const double b = xs[cur][kk][t * 16 + (lane % 16)];
 #pragma unroll
         for (int f = 0; f < RF; ++f) acc[f][t] = mfma::f64_16x16x4(a[f], b, acc[f][t]);
       }
     }
-    __syncthreads();
     if (has_next) {
       if (!PF) issue(kb + KT);
-      commit();
-      __syncthreads();
+      commit(cur ^ 1);
     }
+    __syncthreads();
+    cur ^= 1;
   }
   double* ybase = Y + (int64_t)tl.split * split_stride + sg.yoff;
 #pragma unroll
@@ -199,8 +205,12 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
   // the single-term pass keeps FP32 sums (its error is that of BF16 entries);
   // the three-term pass flushes them to FP64 every 8 K-tiles
   constexpr int FLUSH = SPLIT > 1 ? 8 : 0;
-  __shared__ __attribute__((aligned(16))) __bf16 xs[SPLIT][NC][KP];
-  __shared__ __attribute__((aligned(16))) float rp[KT];
+  // SPLIT > 1: two K-tile buffers, as in k_sketch (one barrier per tile).
+  // The single-term pass keeps one buffer and two barriers per tile: the
+  // second buffer's address costs it a register and a wave per SIMD.
+  constexpr int NB = SPLIT > 1 ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) __bf16 xs[NB][SPLIT][NC][KP];
+  __shared__ __attribute__((aligned(16))) float rp[NB][KT];
   const SkTile tl = tiles[blockIdx.x];
   const SegSk sg = segs[tl.seg];
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
@@ -253,34 +263,34 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
     }
     return vec;
   };
-  auto commit = [&](int kb, bool vec) {
+  auto commit = [&](int buf, int kb, bool vec) {
     if (vec) {
 #pragma unroll
       for (int i = 0; i < NPRE; ++i) {
         const int e = threadIdx.x + i * kWGs;
         const int ch = e % (KT / 8), cc = (e / (KT / 8)) % NC, sp = e / ((KT / 8) * NC);
-        *reinterpret_cast<uint4*>(&xs[sp][cc][8 * ch]) = pre[i];
+        *reinterpret_cast<uint4*>(&xs[buf][sp][cc][8 * ch]) = pre[i];
       }
     } else {
       for (int e = threadIdx.x; e < SPLIT * NC * KT; e += kWGs) {
         const int kk = e % KT, cc = (e / KT) % NC, sp = e / (KT * NC), j = kb + kk;
         const bool excl = (uint64_t)(sg.red0 + j - sg.excl0) < (uint64_t)sg.excllen;
-        xs[sp][cc][kk] = (j < tl.re && cc < kcols && !excl)
+        xs[buf][sp][cc][kk] = (j < tl.re && cc < kcols && !excl)
                              ? Xs[sp * ps + sg.xoff + j + (int64_t)(cbase + cc) * ldx]
                              : (__bf16)0.f;
       }
     }
-    if (threadIdx.x < KT) rp[threadIdx.x] = ppt;
+    if (threadIdx.x < KT) rp[buf][threadIdx.x] = ppt;
   };
   // One K-tile: generate the A fragments (two entries per 32-bit word, word
   // jp of term sp holding entries 2jp, 2jp+1) and multiply.
-  auto tile = [&](int kb) {
+  auto tile = [&](int buf, int kb) {
 #pragma unroll
     for (int ks = 0; ks < KT / 32; ++ks) {
       const int k0 = ks * 32 + 8 * kq;
       float pts[8];
 #pragma unroll
-      for (int jj = 0; jj < 8; ++jj) pts[jj] = rp[k0 + jj];
+      for (int jj = 0; jj < 8; ++jj) pts[jj] = rp[buf][k0 + jj];
       mfma::v8bf a[RF][SPLIT];
 #pragma unroll
       for (int f = 0; f < RF; ++f) {
@@ -317,7 +327,7 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
         mfma::v8bf b[SPLIT];
 #pragma unroll
         for (int sp = 0; sp < SPLIT; ++sp)
-          b[sp] = *reinterpret_cast<const mfma::v8bf*>(&xs[sp][t * 16 + (lane % 16)][k0]);
+          b[sp] = *reinterpret_cast<const mfma::v8bf*>(&xs[buf][sp][t * 16 + (lane % 16)][k0]);
 #pragma unroll
         for (int sa = 0; sa < SPLIT; ++sa)
 #pragma unroll
@@ -327,16 +337,24 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
       }
     }
   };
-  commit(tl.rb, issue(tl.rb));
+  commit(0, tl.rb, issue(tl.rb));
   __syncthreads();
+  int cur = 0;
   for (int kb = tl.rb; kb < tl.re; kb += KT) {
     const bool has_next = kb + KT < tl.re;
     const bool vec_next = has_next ? issue(kb + KT) : false;
-    tile(kb);
-    __syncthreads();
-    if (has_next) {
-      commit(kb + KT, vec_next);
+    if constexpr (NB == 2) {
+      tile(cur, kb);
+      if (has_next) commit(cur ^ 1, kb + KT, vec_next);
       __syncthreads();
+      cur ^= 1;
+    } else {
+      tile(0, kb);
+      __syncthreads();
+      if (has_next) {
+        commit(0, kb + KT, vec_next);
+        __syncthreads();
+      }
     }
     if constexpr (FLUSH > 0) {
       if (++nt == FLUSH) {
