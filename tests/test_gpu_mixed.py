@@ -72,3 +72,33 @@ def test_mixed_reference_tolerance_mode():
     assert res <= 1e-10
     assert true_res < 1e-7
     assert rel(x, np.ones((n, 1))) < 1e-3
+
+
+def test_mixed_ragged_tree():
+    """Odd N and leaf sizes: rows of the BF16 planes start unaligned and
+    reduction ranges end inside a K-tile, so the low-precision sketches stage
+    their tiles element-wise, several per workgroup.  The operator stays
+    within the mixed path's accuracy of the FP64 one and the refined solve
+    reaches 1e-10."""
+    n = 3001
+    xc, yc = kernels.cauchy_grid_points(n)
+    a = kernels.cauchy_points(xc, yc)
+
+    def build(prec):
+        o = hmat_amd.make_options(0.0, rank=48, fixed_rank=True, oversample=10, power_iters=2,
+                                  exact_below=0, precision=prec, omega_dist=hmat_amd.OMEGA_GAUSSIAN,
+                                  seed=0x5EED)
+        h = hmat_amd.HODLRMatrixOMP(opts=o)
+        h.build_kernel(hmat_amd.make_kernel(hmat_amd.KERNEL_CAUCHY, xc, yc), n, 100)
+        return h
+
+    hm, hf = build(hmat_amd.PREC_MIXED), build(hmat_amd.PREC_FP64)
+    Am, Af = hm.to_dense(), hf.to_dense()
+    print(f"\nN={n}, leaf 100: mixed vs A {rel(Am, a):.2e}, fp64 vs A {rel(Af, a):.2e}")
+    assert rel(Af, a) < 1e-12
+    assert rel(Am, a) < 1e-6
+    b = np.random.default_rng(2).standard_normal((n, 1))
+    x = hm.solve(b)
+    it, res = hm.last_refinement()
+    assert res <= 1e-10 and 1 <= it <= 10
+    assert rel(Am @ x, b) <= 1.5e-10
