@@ -12,8 +12,8 @@
 // and zero the X rows of reduction indices past the end or inside the
 // excluded interval as they stage them, so their inner loops carry no
 // masks.  The next tile is loaded into registers during the current one's
-// products and, in k_sketch and the multi-term k_sketch_lp, written to a
-// second LDS buffer, so a tile costs one barrier.
+// products; the multi-term k_sketch_lp writes it to a second LDS buffer, so
+// a tile costs it one barrier.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,11 +51,8 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
                                                  double* __restrict__ Y, int64_t ldy,
                                                  int64_t split_stride) {
   constexpr int KT = 64, NC = NT * 16;
-  // two K-tile buffers: the next tile is committed into the other one, so
-  // one barrier per tile separates the readers of a buffer from its next
-  // writers
-  __shared__ double xs[2][KT][NC + 1];
-  __shared__ double rp[2][KT];
+  __shared__ double xs[KT][NC + 1];
+  __shared__ double rp[KT];
   const SkTile tl = tiles[blockIdx.x];
   const SegSk sg = segs[tl.seg];
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
@@ -100,21 +97,20 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
       ppt = TRANS ? gen.row_pt(sg.red0 + min(j, tl.re - 1)) : gen.col_pt(sg.red0 + min(j, tl.re - 1));
     }
   };
-  auto commit = [&](int buf) {
+  auto commit = [&]() {
 #pragma unroll
     for (int i = 0; i < NPRE; ++i) {
       const int e = threadIdx.x + i * kWGs;
-      xs[buf][e % KT][e / KT] = pre[i];
+      xs[e % KT][e / KT] = pre[i];
     }
-    if (threadIdx.x < KT) rp[buf][threadIdx.x] = ppt;
+    if (threadIdx.x < KT) rp[threadIdx.x] = ppt;
   };
   // Dense input reads A from memory in the inner loop already; there the
   // next tile is fetched after the compute, keeping its registers free.
   constexpr bool PF = !std::is_same<Gen, GDense>::value;
   issue(tl.rb);
-  commit(0);
+  commit();
   __syncthreads();
-  int cur = 0;
   for (int kb = tl.rb; kb < tl.re; kb += KT) {
     const bool has_next = kb + KT < tl.re;
     if (PF && has_next) issue(kb + KT);
@@ -123,24 +119,24 @@ __global__ __launch_bounds__(kWGs) void k_sketch(Gen gen, const SegSk* __restric
       const int kk = ks * 4 + kq;
       const int jl = min(kb + kk, tl.re - 1);
       const int64_t jg = sg.red0 + jl;
-      const double rpt = rp[cur][kk];
+      const double rpt = rp[kk];
       double a[RF];
 #pragma unroll
       for (int f = 0; f < RF; ++f)
         a[f] = TRANS ? gen.eval(rpt, opt[f], jg, iglob[f]) : gen.eval(opt[f], rpt, iglob[f], jg);
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const double b = xs[cur][kk][t * 16 + (lane % 16)];
+        const double b = xs[kk][t * 16 + (lane % 16)];
 #pragma unroll
         for (int f = 0; f < RF; ++f) acc[f][t] = mfma::f64_16x16x4(a[f], b, acc[f][t]);
       }
     }
+    __syncthreads();
     if (has_next) {
       if (!PF) issue(kb + KT);
-      commit(cur ^ 1);
+      commit();
+      __syncthreads();
     }
-    __syncthreads();
-    cur ^= 1;
   }
   double* ybase = Y + (int64_t)tl.split * split_stride + sg.yoff;
 #pragma unroll
