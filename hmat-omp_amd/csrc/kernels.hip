@@ -1134,7 +1134,7 @@ __global__ __launch_bounds__(kJWG) void k_jacobi_svd64(const SmallMat* __restric
     if (hl == 0) sig[j] = sqrt(s2);
   }
   __syncthreads();
-  if (threadIdx.x < n) {  // rank of column j in descending order (ties by index)
+  if ((int)threadIdx.x < n) {  // rank of column j in descending order (ties by index)
     const int j = threadIdx.x;
     int r = 0;
     for (int t = 0; t < n; ++t) r += (sig[t] > sig[j] || (sig[t] == sig[j] && t < j)) ? 1 : 0;
@@ -1204,7 +1204,7 @@ __global__ __launch_bounds__(kWG) void k_gj_inverse(const SmallMat* __restrict__
     bi[threadIdx.x] = bidx;
     __syncthreads();
     for (int o = kWG / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) {
+      if ((int)threadIdx.x < o) {
         const F v2 = bv[threadIdx.x + o];
         const int i2 = bi[threadIdx.x + o];
         if (v2 > bv[threadIdx.x] || (v2 == bv[threadIdx.x] && i2 < bi[threadIdx.x])) {

@@ -132,7 +132,6 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project(const MvLeaf* __restrict__
       c = nc;
       kb = nkb;
       vc = nvc;
-#pragma unroll
       for (int kk = 0; kk < KG; ++kk) a[kk] = an[kk];
     }
     return;
