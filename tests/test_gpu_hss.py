@@ -221,3 +221,48 @@ def test_refinement_reaches_dense_solution():
     h.set_refinement(0.0)
     h.solve(b)
     assert h.last_refinement() == (0, -1.0)
+
+
+def test_state_and_argument_errors():
+    """Calls before a build report HMAT_ERR_STATE; a zero leaf size, mixed
+    precision (HSS is FP64 only) and negative refinement settings report
+    HMAT_ERR_ARG."""
+    h = hmat_amd.HSSMatrixOMP(1e-9)
+    for call in (lambda: h.solve(np.ones(4)), lambda: h.multiply(np.ones(4))):
+        with pytest.raises(hmat_amd.HmatError) as e:
+            call()
+        assert e.value.status == 6  # HMAT_ERR_STATE
+    with pytest.raises(hmat_amd.HmatError) as e:
+        h.build_from_dense(np.eye(8), 0)
+    assert e.value.status == 1  # HMAT_ERR_ARG
+    with pytest.raises(hmat_amd.HmatError) as e:
+        h.set_refinement(-1.0)
+    assert e.value.status == 1
+    hm = hmat_amd.HSSMatrixOMP(opts=hmat_amd.make_options(1e-9, precision=hmat_amd.PREC_MIXED))
+    with pytest.raises(hmat_amd.HmatError) as e:
+        hm.build_from_dense(np.eye(64), 16)
+    assert e.value.status == 1
+
+
+def test_nonfinite_input_reported():
+    n = 600
+    a = kernels.generate_matrix(kernels.CAUCHY, n)
+    a[400, 10] = np.nan
+    h = hmat_amd.HSSMatrixOMP(1e-9)
+    with pytest.raises(hmat_amd.HmatError) as e:
+        h.build_from_dense(a, 100)
+    assert e.value.status == 5  # HMAT_ERR_NONFINITE
+
+
+def test_singular_leaf_reported():
+    """A zero row and column make one leaf block singular: the solve reports
+    HMAT_ERR_SINGULAR instead of returning a non-finite x."""
+    n = 256
+    a = kernels.gaussian_points(np.linspace(0, 1, n), 0.1, 1.0)
+    a[5, :] = 0.0
+    a[:, 5] = 0.0
+    h = hmat_amd.HSSMatrixOMP(1e-9)
+    h.build_from_dense(a, 64)
+    with pytest.raises(hmat_amd.HmatError) as e:
+        h.solve(np.ones(n))
+    assert e.value.status == 4  # HMAT_ERR_SINGULAR
