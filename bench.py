@@ -12,7 +12,9 @@ points and b are resident in HBM before the timed region.
 Prints one JSON line.  `value` is the mixed-precision step time (seconds,
 lower is better); the plain FP64 step time and the ratio are reported next
 to it.  `roofline` is for the dominant kernel of the mixed step, from HIP
-events recorded around its launches on its own stream.  `cpu_baseline` is
+events recorded around its launches on its own stream.  `hbm_probe` is what
+plain device kernels (torch's sum and copy over 2 GiB) reach on the same
+GPU, next to the matvec's fraction of the 8 TB/s peak.  `cpu_baseline` is
 the CPU oracle (numpy/OpenBLAS restatement of the reference algorithm) on a
 bounded sample, scaled to the full size by off-diagonal area.
 """
@@ -125,6 +127,31 @@ def true_residual(hmat_amd, torch, res, n):
     hmat_amd.kernel_matvec_device(k, n, res["x"].data_ptr(), n, 1, y.data_ptr(), n)
     torch.cuda.synchronize()
     return float(torch.linalg.norm(res["b"] - y) / torch.linalg.norm(res["b"]))
+
+
+def hbm_probe(torch, nbytes=1 << 31, reps=5):
+    """What plain device kernels reach on this GPU's HBM: torch's sum (a pure
+    read of `nbytes`) and copy (`nbytes` read + `nbytes` written), timed with
+    events on torch's stream after one untimed call each."""
+    src = torch.rand(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+
+    def ms(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    t_read, t_copy = ms(lambda: src.sum()), ms(lambda: dst.copy_(src))
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"bytes": nbytes, "read_GBs": round(nbytes / (t_read * 1e-3) / 1e9, 1),
+            "copy_GBs": round(2 * nbytes / (t_copy * 1e-3) / 1e9, 1)}
 
 
 def cpu_baseline(a):
@@ -264,6 +291,9 @@ def main():
                                   "traffic_source": PMC_SOURCE,
                                   "bytes_per_launch": mv["work"] / mv["launches"],
                                   "avg_launch_ms": round(avg, 4), "launches": mv["launches"]}
+        probe = hbm_probe(torch)
+        out["hbm_probe"] = probe
+        out["matvec_roofline"]["frac_of_probe_read"] = round(gbs / probe["read_GBs"], 4)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a)
     print(json.dumps(out), flush=True)
