@@ -227,19 +227,22 @@ bool Orthogonalizer::pass(const std::vector<OrthSeg>& segs, const double* in, do
   gerr_.reserve(nb);
   std::vector<SegTN> tn(nb);
   std::vector<SmallMat> sm(nb);
-  std::vector<double> coef(nb);
+  std::vector<double> coef(nb), shift(nb);
   std::vector<SegNN> nn(nb);
   for (size_t b = 0; b < nb; ++b) {
     const auto& g = segs[b];
     tn[b] = SegTN{g.off, g.off, (int64_t)b * kk, g.len, g.k, g.k, 0};
     sm[b] = SmallMat{(int64_t)b * kk, g.k, kcap};
     // shift 11 (m n + n (n+1)) u ||X||^2 with ||X||_F^2 = trace(G) >= ||X||_2^2
-    coef[b] = shifted ? 11.0 * ((double)g.len * g.k + (double)g.k * (g.k + 1)) * 1.1102230246251565e-16
-                      : 0.0;
+    shift[b] = 11.0 * ((double)g.len * g.k + (double)g.k * (g.k + 1)) * 1.1102230246251565e-16;
+    coef[b] = shifted ? shift[b] : 0.0;
     nn[b] = SegNN{g.off, (int64_t)b * kk, g.off, g.len, g.k, g.k, 0};
   }
   kern::seg_gemm_tn(tn, in, ld, in, ld, G_.get(), kcap, 1.0, 0.0, ws_, s);
-  kern::chol_shifted(sm, G_.get(), R_.get(), coef, info_.get(), gerr_.get(), s);
+  // an unshifted pass whose Cholesky breaks down (columns numerically
+  // dependent) takes the shifted factor in the same launch
+  kern::chol_shifted(sm, G_.get(), R_.get(), coef, info_.get(), gerr_.get(), s,
+                     shifted ? std::vector<double>() : shift);
   auto info = download(info_.get(), nb, s);
   auto gerr = download(gerr_.get(), nb, s);
   bool breakdown = false;
@@ -356,13 +359,15 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
   double* gerr = reinterpret_cast<double*>(chk_.get() + info_bytes);
   const int nm = (int)nb;
   // Enqueued back to back, with one download of all three status arrays:
-  // shifted pass Y -> T (R_1 in Racc_), unshifted pass T -> Y (R_2 in R_),
+  // shifted pass Y -> T (R_1 in Racc_), unshifted pass T -> Y (R_2 in R_;
+  // shifted instead for an entry whose unshifted Cholesky breaks down),
   // and the Gram matrix of the result with its deviation from I.
   gram(Y, ld, kcap, s);
   kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), Racc_.get(), plan_.shift_on, info, gerr, s);
   trsm(Y, ld, Racc_.get(), kcap, T, s);
   gram(T, ld, kcap, s);
-  kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), R_.get(), plan_.shift_off, info + nb, gerr + nb, s);
+  kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), R_.get(), plan_.shift_off, info + nb, gerr + nb, s,
+                            plan_.shift_on);
   trsm(T, ld, R_.get(), kcap, Y, s);
   gram(Y, ld, kcap, s);
   kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), Rchk_.get(), plan_.shift_off, info + 2 * nb,

@@ -657,7 +657,7 @@ __global__ __launch_bounds__(kWG) void k_chol(const SmallMat* __restrict__ mats,
                                               const double* __restrict__ G, double* __restrict__ R,
                                               const double* __restrict__ shift_coef,
                                               int* __restrict__ info, double* __restrict__ gerr,
-                                              int use_lds) {
+                                              int use_lds, const double* __restrict__ fallback_coef) {
   extern __shared__ double sm[];
   __shared__ double red[kWG / 64];
   __shared__ double dred[kWG / 64];
@@ -683,34 +683,47 @@ __global__ __launch_bounds__(kWG) void k_chol(const SmallMat* __restrict__ mats,
     gerr[blockIdx.x] = (tr == 0.0) ? 0.0 : d;
     info[blockIdx.x] = 0;
   }
-  const double shift = shift_coef[blockIdx.x] * tr;
-  for (int e = threadIdx.x; e < n * n; e += kWG) {
-    const int i = e % n, j = e / n;
-    r[i + (int64_t)j * lr] = (i <= j) ? g[i + (int64_t)j * ld] + (i == j ? shift : 0.0) : 0.0;
-  }
-  __syncthreads();
   if (!(tr > 0.0)) {
     if (!(tr == 0.0) && threadIdx.x == 0) info[blockIdx.x] = -1;  // NaN
     for (int e = threadIdx.x; e < n * n; e += kWG) rg[e % n + (int64_t)(e / n) * ld] = 0.0;
     return;
   }
-  for (int j = 0; j < n; ++j) {
-    const double d = r[j + (int64_t)j * lr];
-    if (!(d > 0.0) || !isfinite(d)) {
-      if (threadIdx.x == 0) info[blockIdx.x] = j + 1;
+  double shift = shift_coef[blockIdx.x] * tr;
+  // an unshifted factorisation that breaks down is redone with the
+  // fallback shift (when one is given), in this launch
+  const double fb = (fallback_coef && shift == 0.0) ? fallback_coef[blockIdx.x] * tr : 0.0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    for (int e = threadIdx.x; e < n * n; e += kWG) {
+      const int i = e % n, j = e / n;
+      r[i + (int64_t)j * lr] = (i <= j) ? g[i + (int64_t)j * ld] + (i == j ? shift : 0.0) : 0.0;
+    }
+    __syncthreads();
+    int bad = 0;
+    for (int j = 0; j < n; ++j) {
+      const double d = r[j + (int64_t)j * lr];
+      if (!(d > 0.0) || !isfinite(d)) {
+        bad = j + 1;
+        break;
+      }
+      const double rjj = sqrt(d);
+      __syncthreads();
+      for (int c = j + 1 + threadIdx.x; c < n; c += kWG) r[j + (int64_t)c * lr] /= rjj;
+      if (threadIdx.x == 0) r[j + (int64_t)j * lr] = rjj;
+      __syncthreads();
+      const int rem = n - j - 1;
+      for (int e = threadIdx.x; e < rem * rem; e += kWG) {
+        const int a2 = j + 1 + e % rem, c = j + 1 + e / rem;
+        if (a2 <= c) r[a2 + (int64_t)c * lr] -= r[j + (int64_t)a2 * lr] * r[j + (int64_t)c * lr];
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+    if (!bad) break;
+    if (attempt == 1 || !(fb > 0.0)) {
+      if (threadIdx.x == 0) info[blockIdx.x] = bad;
       break;
     }
-    const double rjj = sqrt(d);
-    __syncthreads();
-    for (int c = j + 1 + threadIdx.x; c < n; c += kWG) r[j + (int64_t)c * lr] /= rjj;
-    if (threadIdx.x == 0) r[j + (int64_t)j * lr] = rjj;
-    __syncthreads();
-    const int rem = n - j - 1;
-    for (int e = threadIdx.x; e < rem * rem; e += kWG) {
-      const int a2 = j + 1 + e % rem, c = j + 1 + e / rem;
-      if (a2 <= c) r[a2 + (int64_t)c * lr] -= r[j + (int64_t)a2 * lr] * r[j + (int64_t)c * lr];
-    }
-    __syncthreads();
+    shift = fb;
   }
   __syncthreads();
   if (use_lds)
@@ -764,7 +777,8 @@ __device__ __forceinline__ int chol_step(double (&col)[64], int n, int c) {
 __global__ __launch_bounds__(64) void k_chol64r(const SmallMat* __restrict__ mats,
                                                 const double* __restrict__ G, double* __restrict__ R,
                                                 const double* __restrict__ shift_coef,
-                                                int* __restrict__ info, double* __restrict__ gerr, int lg) {
+                                                int* __restrict__ info, double* __restrict__ gerr, int lg,
+                                                const double* __restrict__ fallback_coef) {
   // G staged through LDS with coalesced loads, column j of G at gs[j * lg];
   // lg = the batch's largest n + 1, so small matrices leave room for more
   // workgroups per CU
@@ -790,17 +804,26 @@ __global__ __launch_bounds__(64) void k_chol64r(const SmallMat* __restrict__ mat
     gerr[blockIdx.x] = (tr == 0.0) ? 0.0 : dev;
     info[blockIdx.x] = 0;
   }
-  const double shift = shift_coef[blockIdx.x] * tr;
+  double shift = shift_coef[blockIdx.x] * tr;
+  // an unshifted factorisation that breaks down is redone with the
+  // fallback shift (when one is given), in this launch
+  const double fb = (fallback_coef && shift == 0.0) ? fallback_coef[blockIdx.x] * tr : 0.0;
   double col[64];
-#pragma unroll
-  for (int i = 0; i < 64; ++i) col[i] = (c < n && i <= c) ? gs[c * lg + i] + (i == c ? shift : 0.0) : 0.0;
   if (!(tr > 0.0)) {
     if (!(tr == 0.0) && c == 0) info[blockIdx.x] = -1;  // NaN
     if (c < n)
       for (int i = 0; i < n; ++i) rg[i + (int64_t)c * ld] = 0.0;
     return;
   }
-  const int bad = chol_step<0>(col, n, c);
+  int bad = 0;
+#pragma nounroll
+  for (int attempt = 0; attempt < 2; ++attempt) {
+#pragma unroll
+    for (int i = 0; i < 64; ++i) col[i] = (c < n && i <= c) ? gs[c * lg + i] + (i == c ? shift : 0.0) : 0.0;
+    bad = chol_step<0>(col, n, c);
+    if (!bad || !(fb > 0.0)) break;
+    shift = fb;
+  }
   if (bad) {
     if (c == 0) info[blockIdx.x] = bad;
     return;
@@ -872,27 +895,30 @@ __global__ __launch_bounds__(64) void k_chol64(const SmallMat* __restrict__ mats
 }
 
 void launch_chol_shifted(const SmallMat* dmats, int nmats, int nmax, double* G, double* R, const double* dshift,
-                      int* info, double* gerr, hipStream_t s) {
+                         int* info, double* gerr, hipStream_t s, const double* dfallback) {
   if (!nmats) return;
   const size_t lds = (size_t)nmax * (nmax + 1) * sizeof(double);
   const int use_lds = lds <= 64 * 1024 ? 1 : 0;
   if (nmax <= 64)
-    k_chol64r<<<(unsigned)nmats, 64, (size_t)nmax * (nmax + 1) * sizeof(double), s>>>(dmats, G, R, dshift,
-                                                                                     info, gerr, nmax + 1);
+    k_chol64r<<<(unsigned)nmats, 64, (size_t)nmax * (nmax + 1) * sizeof(double), s>>>(
+        dmats, G, R, dshift, info, gerr, nmax + 1, dfallback);
   else
-    k_chol<<<(unsigned)nmats, kWG, use_lds ? lds : 0, s>>>(dmats, G, R, dshift, info, gerr, use_lds);
+    k_chol<<<(unsigned)nmats, kWG, use_lds ? lds : 0, s>>>(dmats, G, R, dshift, info, gerr, use_lds, dfallback);
   HIP_CHECK(hipGetLastError());
 }
 
 void chol_shifted(const std::vector<SmallMat>& mats, double* G, double* R, const std::vector<double>& shift_coef,
-                  int* info, double* gerr, hipStream_t s) {
+                  int* info, double* gerr, hipStream_t s, const std::vector<double>& fallback_coef) {
   if (mats.empty()) return;
+  HMAT_REQUIRE(fallback_coef.empty() || fallback_coef.size() == mats.size(), HMAT_ERR_ARG,
+               "chol_shifted: one fallback coefficient per matrix");
   int nmax = 0;
   for (auto& mm : mats) nmax = std::max(nmax, mm.n);
   StageGroup g;
-  const int im = g.add(mats), ic = g.add(shift_coef);
+  const int im = g.add(mats), ic = g.add(shift_coef), ifb = g.add(fallback_coef);
   g.put(s);
-  launch_chol_shifted(g.get<SmallMat>(im), (int)mats.size(), nmax, G, R, g.get<double>(ic), info, gerr, s);
+  launch_chol_shifted(g.get<SmallMat>(im), (int)mats.size(), nmax, G, R, g.get<double>(ic), info, gerr, s,
+                      g.get<double>(ifb));
 }
 
 // One-sided Jacobi (Hestenes) with a round-robin pair schedule; the four

@@ -168,14 +168,19 @@ void sketch_lp(const GenDesc& g, bool trans, int split, const std::vector<SegSk>
                Workspace& ws, hipStream_t s);
 
 // Gram G (n x n per entry) -> upper R with R^T R = G + shift_s I,
-// shift_s = shift_coef * trace(G) when shift_coef > 0.  info[b] != 0 on
-// breakdown; gerr[b] = max |G - I| (0 for an all-zero G), the
-// orthogonality defect of the columns the Gram came from.
+// shift_s = shift_coef * trace(G) when shift_coef > 0.  An entry with
+// shift_coef = 0 whose factorisation breaks down is factorised again with
+// shift fallback_coef * trace(G) when fallback coefficients are given (one
+// per entry).  info[b] != 0 when the last attempt broke down; gerr[b] =
+// max |G - I| (0 for an all-zero G), the orthogonality defect of the
+// columns the Gram came from.
 void chol_shifted(const std::vector<SmallMat>& mats, double* G, double* R, const std::vector<double>& shift_coef,
-                  int* info, double* gerr, hipStream_t s);
-// The same on descriptors and shift coefficients already on the device.
+                  int* info, double* gerr, hipStream_t s,
+                  const std::vector<double>& fallback_coef = std::vector<double>());
+// The same on descriptors and coefficients already on the device
+// (dfallback may be null).
 void launch_chol_shifted(const SmallMat* dmats, int nmats, int nmax, double* G, double* R, const double* dshift,
-                         int* info, double* gerr, hipStream_t s);
+                         int* info, double* gerr, hipStream_t s, const double* dfallback = nullptr);
 
 // One-sided Jacobi SVD of n x n matrices A = U diag(S) V^T (A overwritten
 // by U).  S sorted descending, U/V columns permuted accordingly.  A pair of
