@@ -361,7 +361,8 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
   // Enqueued back to back, with one download of all three status arrays:
   // shifted pass Y -> T (R_1 in Racc_), unshifted pass T -> Y (R_2 in R_;
   // shifted instead for an entry whose unshifted Cholesky breaks down),
-  // and the Gram matrix of the result with its deviation from I.
+  // and the Gram matrix of the result with its deviation from I and its
+  // Cholesky factor (Rchk_, the third pass's when one is needed).
   gram(Y, ld, kcap, s);
   kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), Racc_.get(), plan_.shift_on, info, gerr, s);
   trsm(Y, ld, Racc_.get(), kcap, T, s);
@@ -371,7 +372,7 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
   trsm(T, ld, R_.get(), kcap, Y, s);
   gram(Y, ld, kcap, s);
   kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), Rchk_.get(), plan_.shift_off, info + 2 * nb,
-                         gerr + 2 * nb, s);
+                            gerr + 2 * nb, s, plan_.shift_on);
   const auto h = download(chk_.get(), info_bytes + 3 * nb * sizeof(double), s);
   const int* hi = reinterpret_cast<const int*>(h.data());
   const double* hg = reinterpret_cast<const double*>(h.data() + info_bytes);
@@ -393,8 +394,24 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
   } else if (Rout) {
     acc_product(kcap, s);
   }
+  int p = 2;
+  if (again && !breakdown2) {
+    bool factored = true;
+    for (size_t b = 0; b < nb; ++b)
+      if (hi[2 * nb + b] != 0) factored = false;
+    if (factored) {
+      // the check's Cholesky factor of Y (shifted only where the unshifted
+      // one broke down) is the next pass's factor: apply it without
+      // forming the same Gram matrix again
+      std::swap(R_, Rchk_);
+      trsm(src, ld, R_.get(), kcap, dst, s);
+      if (Rout) acc_product(kcap, s);
+      std::swap(src, dst);
+      p = 3;
+    }
+  }
   // rare: more passes, each checked, until the Gram matrix is I to kOrthTol
-  for (int p = 2; again && p < kOrthMaxPasses; ++p) {
+  for (; again && p < kOrthMaxPasses; ++p) {
     if (!pass(segs, src, dst, ld, kcap, breakdown2 && src == T && p == 2, s)) break;
     if (Rout) {
       acc_product(kcap, s);
