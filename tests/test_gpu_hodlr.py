@@ -106,6 +106,27 @@ def test_operator_algebra_exact_fixed_rank():
     assert all(r == 16 for d in range(1, h.levels() + 1) for r in h.level_ranks(d))
 
 
+def test_exactly_low_rank_blocks():
+    """Off-diagonal blocks of exact rank 3 under a sketch width of 26: the
+    sketches Y and B^T have numerically dependent columns, so unshifted
+    Cholesky factors of their Gram matrices break down and are redone
+    shifted in the same launch.  The operator still reproduces A to
+    rounding, tolerance truncation finds rank 3, and the solve matches
+    dense LAPACK."""
+    n = 2048
+    rng = np.random.default_rng(7)
+    u, v = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    a = u @ v.T + n * np.eye(n)
+    o = hmat_amd.make_options(1e-10, rank=16, fixed_rank=False, oversample=10, power_iters=1,
+                              exact_below=0, omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=5)
+    h = hmat_amd.HODLRMatrixOMP(opts=o)
+    h.build_from_dense(a, 128)
+    assert rel(h.to_dense(), a) < 1e-12
+    assert all(r == 3 for d in range(1, h.levels() + 1) for r in h.level_ranks(d))
+    b = rng.standard_normal((n, 2))
+    assert rel(h.solve(b), np.linalg.solve(a, b)) < 1e-12
+
+
 def test_cfg1_gaussian_against_dense_lapack():
     """N=4096 sorted U[0,1] points, sigma=0.1, lambda=1, leaf 64, rank 16,
     oversample 10, q=2, b ~ N(0,1): build, matvec and solve vs dense LAPACK."""
