@@ -4,8 +4,7 @@
 
 Each HIP source is compiled to an object in hmat-omp_amd/_build/, then the
 shared library is linked into hmat-omp_amd/lib/, and the C++ drop-in example
-(examples/drop_in_driver.cpp) and the MFMA issue-rate probe
-(microbench/mfma_issue.hip) into hmat-omp_amd/bin/.  Packed FP32
+(examples/drop_in_driver.cpp) into hmat-omp_amd/bin/.  Packed FP32
 instructions are disabled for every device compile.
 """
 import argparse
@@ -22,8 +21,6 @@ LIB = os.path.join(LIB_DIR, "libhmat_amd.so")
 BIN_DIR = os.path.join(HERE, "bin")
 DRIVER_SRC = os.path.join(HERE, "examples", "drop_in_driver.cpp")
 DRIVER = os.path.join(BIN_DIR, "drop_in_driver")
-PROBE_SRC = os.path.join(HERE, "microbench", "mfma_issue.hip")
-PROBE = os.path.join(BIN_DIR, "mfma_issue")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -73,12 +70,6 @@ def build(jobs=None, verbose=False):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("drop-in driver build failed:\n" + r.stderr)
-    # the MFMA issue-rate probe (a standalone program, see its header)
-    cmd = [HIPCC, "-O3", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-mllvm", "-amdgpu-mfma-vgpr-form", PROBE_SRC, "-o", PROBE]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("mfma_issue probe build failed:\n" + r.stderr)
     return LIB
 
 

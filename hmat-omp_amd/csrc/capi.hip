@@ -47,12 +47,7 @@ void check_device() {
 
 void check_opts(const hmat_options* o) {
   HMAT_REQUIRE(o != nullptr, HMAT_ERR_ARG, "options pointer is NULL");
-  HMAT_REQUIRE(o->rank > 0, HMAT_ERR_ARG, "rank must be positive");
-  HMAT_REQUIRE(o->oversample >= 0 && o->power_iters >= 0 && o->exact_below >= 0, HMAT_ERR_ARG,
-               "negative oversample / power_iters / exact_below");
-  HMAT_REQUIRE(o->precision == HMAT_PREC_FP64 || o->precision == HMAT_PREC_MIXED, HMAT_ERR_ARG,
-               "unknown precision");
-  HMAT_REQUIRE(std::isfinite(o->tol), HMAT_ERR_ARG, "tolerance is not finite");
+  check_options(*o);
 }
 
 // Pack a host column-major (ld) block into a dense device N x nc array.

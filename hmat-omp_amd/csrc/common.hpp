@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -243,6 +244,27 @@ void autotune_enable(bool on);
 bool tune_lookup(const TuneKey& k, int& choice);
 void tune_store(const TuneKey& k, int choice);
 size_t tune_entries();
+
+// The option rules of hmat_*_create, also applied to options read from an
+// operator file.
+inline void check_options(const hmat_options& o) {
+  HMAT_REQUIRE(o.rank > 0, HMAT_ERR_ARG, "rank must be positive");
+  HMAT_REQUIRE(o.oversample >= 0 && o.power_iters >= 0 && o.exact_below >= 0, HMAT_ERR_ARG,
+               "negative oversample / power_iters / exact_below");
+  HMAT_REQUIRE(o.precision == HMAT_PREC_FP64 || o.precision == HMAT_PREC_MIXED, HMAT_ERR_ARG,
+               "unknown precision");
+  HMAT_REQUIRE(o.omega_dist == HMAT_OMEGA_UNIFORM || o.omega_dist == HMAT_OMEGA_GAUSSIAN, HMAT_ERR_ARG,
+               "unknown Omega distribution");
+  HMAT_REQUIRE(o.fixed_rank == 0 || o.fixed_rank == 1, HMAT_ERR_ARG, "fixed_rank must be 0 or 1");
+  HMAT_REQUIRE(std::isfinite(o.tol), HMAT_ERR_ARG, "tolerance is not finite");
+}
+
+// %.3e for error messages (std::to_string prints 1e-12 as 0.000000)
+inline std::string sci(double v) {
+  char b[32];
+  std::snprintf(b, sizeof(b), "%.3e", v);
+  return b;
+}
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline int round_up(int a, int b) { return (int)((a + b - 1) / b) * b; }

@@ -332,11 +332,18 @@ void Orthogonalizer::run_once(const std::vector<OrthSeg>& segs, double* Y, doubl
   if (segs.empty()) return;
   const size_t nb = segs.size();
   make_plan(segs, kcap, false, s);
-  info_.reserve(nb);
   gerr_.reserve(nb);
+  // this pass's Cholesky status goes to its own slice of once_info_, read
+  // with the next run()'s status download
+  if (once_off_ + nb > once_info_.size()) {
+    check_once(s);
+    once_info_.reserve(std::max<size_t>(nb, 4096));
+  }
+  int* info = once_info_.get() + once_off_;
+  once_off_ += nb;
   gram(Y, ld, kcap, s);
-  kern::launch_chol_shifted(plan_.mats, (int)nb, plan_.nmax, G_.get(), R_.get(), plan_.shift_on, info_.get(),
-                         gerr_.get(), s);
+  kern::launch_chol_shifted(plan_.mats, (int)nb, plan_.nmax, G_.get(), R_.get(), plan_.shift_on, info,
+                            gerr_.get(), s);
   if (plan_.kmax <= 64) {
     trsm(Y, ld, R_.get(), kcap, Y, s);  // in place
   } else {
@@ -344,6 +351,19 @@ void Orthogonalizer::run_once(const std::vector<OrthSeg>& segs, double* Y, doubl
     std::vector<SegNN> cp(nb);
     for (size_t b = 0; b < nb; ++b) cp[b] = SegNN{segs[b].off, 0, segs[b].off, segs[b].len, 0, segs[b].k, 0};
     kern::copy_segs(cp, dnn_, T, ld, Y, ld, s);
+  }
+}
+
+// Statuses of the run_once passes enqueued since the last check: a shifted
+// Cholesky that broke down left R unwritten (only non-finite input does
+// that, the shift keeps a finite Gram matrix positive definite).
+void Orthogonalizer::check_once(hipStream_t s) {
+  if (once_off_ == 0) return;
+  const auto h = download(once_info_.get(), once_off_, s);
+  once_off_ = 0;
+  for (int v : h) {
+    if (v < 0) throw Error(HMAT_ERR_NONFINITE, "orthogonalisation: non-finite Gram matrix");
+    if (v > 0) throw Error(HMAT_ERR_NONFINITE, "orthogonalisation: shifted Cholesky broke down");
   }
 }
 
@@ -374,6 +394,7 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
   kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), Rchk_.get(), plan_.shift_off, info + 2 * nb,
                             gerr + 2 * nb, s, plan_.shift_on);
   const auto h = download(chk_.get(), info_bytes + 3 * nb * sizeof(double), s);
+  check_once(s);  // the stream is drained: this download costs no extra wait
   const int* hi = reinterpret_cast<const int*>(h.data());
   const double* hg = reinterpret_cast<const double*>(h.data() + info_bytes);
   bool breakdown2 = false, again = false;
@@ -411,13 +432,22 @@ void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T,
     }
   }
   // rare: more passes, each checked, until the Gram matrix is I to kOrthTol
+  bool orthonormal = !again;
   for (; again && p < kOrthMaxPasses; ++p) {
-    if (!pass(segs, src, dst, ld, kcap, breakdown2 && src == T && p == 2, s)) break;
-    if (Rout) {
-      acc_product(kcap, s);
+    if (!pass(segs, src, dst, ld, kcap, breakdown2 && src == T && p == 2, s)) {
+      orthonormal = true;
+      break;
     }
+    if (Rout) acc_product(kcap, s);
     std::swap(src, dst);
   }
+  // CholeskyQR stalls on blocks whose columns span many orders of magnitude
+  // below the shift (Gaussian blocks whose singular values fall to
+  // underflow, an exactly zero row): the columns it cannot resolve have norms
+  // ~1e-12 of the block's and carry nothing measurable into Q R, which stays
+  // an exact factorisation.  The last pass's basis is kept and the event is
+  // counted ("orth_unconverged", calls with a segment left unresolved).
+  if (!orthonormal) prof_add("orth_unconverged", 0.0, 1.0);
   if (src != Y) {
     std::vector<SegNN> cp(nb);
     for (size_t b = 0; b < nb; ++b) cp[b] = SegNN{segs[b].off, 0, segs[b].off, segs[b].len, 0, segs[b].k, 0};

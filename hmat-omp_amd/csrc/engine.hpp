@@ -145,9 +145,12 @@ class Orthogonalizer {
   // One shifted pass and nothing else (no check, no synchronisation): a
   // basis of the same span with orthogonality defect ~u cond(Y)^2, for
   // intermediate bases whose input is well conditioned (cond(Y)^2 << 1/u).
-  // Non-finite input is reported by the next run().
+  // A breakdown of its Cholesky factorisation (non-finite input) is
+  // reported by the next run() or check_once().
   void run_once(const std::vector<OrthSeg>& segs, double* Y, double* T, int64_t ld, int kcap,
                 hipStream_t s);
+  // Download and check the statuses of the run_once passes so far.
+  void check_once(hipStream_t s);
 
  private:
   // the launches of one call, planned once: Gram products, Cholesky
@@ -176,6 +179,8 @@ class Orthogonalizer {
   kern::Workspace ws_;
   DevBuf<double> G_, R_, Rchk_, Racc_, Rtmp_, gerr_;  // Rchk_: the check pass's R (discarded)
   DevBuf<int> info_;
+  DevBuf<int> once_info_;  // run_once statuses not yet checked: [0, once_off_)
+  size_t once_off_ = 0;
   DevBuf<uint8_t> chk_;  // both passes' info and Gram deviations, downloaded together
   DevBuf<SegNN> dnn_;
   DevBuf<SmallMat> dsm_;

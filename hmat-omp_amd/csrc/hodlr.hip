@@ -453,14 +453,26 @@ void HodlrEngine::build_group(int c0, int c1, const GenDesc& g, const std::vecto
 
 // Y = D X over all leaves (leaf blocks stored lmax x lmax).
 template <class F>
-void HodlrEngine::leaf_apply_t(const F* Dm, const F* B, int64_t ldb, int ncols, F* Y, int64_t ldy) {
+void HodlrEngine::leaf_apply_t(const F* Dm, const F* B, int64_t ldb, int ncols, F* Y, int64_t ldy, F alpha,
+                               F beta) {
   const int64_t lm = tree.lmax;
   std::vector<SegNN> segs;
   for (size_t i = 0; i < tree.leaves.size(); ++i) {
     const TNode& nd = tree.nodes[tree.leaves[i]];
     segs.push_back(SegNN{(int64_t)i * lm * lm, nd.start, nd.start, (int32_t)nd.len, (int32_t)nd.len, ncols, 0});
   }
-  kern::seg_gemm_nn<F>(segs, Dm, lm, B, ldb, Y, ldy, F(1), F(0), stream);
+  kern::seg_gemm_nn<F>(segs, Dm, lm, B, ldb, Y, ldy, alpha, beta, stream);
+}
+
+template <class F>
+void HodlrEngine::leaf_solve_t(HFactor<F>& fa, const F* B, int64_t ldb, int ncols, F* Y, int64_t ldy, F* R) {
+  leaf_apply_t<F>(fa.Dinv.get(), B, ldb, ncols, Y, ldy);
+  if constexpr (kRefineBlocks<F>) {
+    // R = B - D Y, Y += D^{-1} R
+    if (R != B) kern::convert_cols<F, F>(N, ncols, B, ldb, R, N, stream);
+    leaf_apply_t<F>(D.get(), Y, ldy, ncols, R, N, F(-1), F(1));
+    leaf_apply_t<F>(fa.Dinv.get(), R, N, ncols, Y, ldy, F(1), F(1));
+  }
 }
 
 void HodlrEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
@@ -524,6 +536,11 @@ void HodlrEngine::correction_t(HFactor<F>& fa, int j, F* W, int64_t ldw, int nco
   }
   kern::seg_gemm_tn<F>(tn, fa.V[j + 1], N, W, ldw, fa.tvec.get(), mm, F(1), F(0), ws_, stream);
   kern::seg_gemm_nn<F>(mn, fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(0), stream);
+  if constexpr (kRefineBlocks<F>) {
+    // tvec <- tvec - M coef, coef += M^{-1} tvec
+    kern::seg_gemm_nn<F>(mn, fa.Mmat[j].get(), mm, fa.coef.get(), mm, fa.tvec.get(), mm, F(-1), F(1), stream);
+    kern::seg_gemm_nn<F>(mn, fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(1), stream);
+  }
   kern::seg_gemm_nn<F>(up, Q, N, fa.coef.get(), mm, W, ldw, F(-1), F(1), stream);
 }
 
@@ -563,6 +580,7 @@ void HodlrEngine::factor_t(HFactor<F>& fa) {
   fa.V.assign(dmax + 1, nullptr);
   fa.Vown.resize(dmax + 1);
   fa.Minv.resize(std::max(dmax, 1));
+  if constexpr (kRefineBlocks<F>) fa.Mmat.resize(std::max(dmax, 1));
   par.assign(std::max(dmax, 1), HParents{});
   auto check_info = [&]() {
     const auto info = download(info_.get(), info_off, stream);
@@ -594,7 +612,7 @@ void HodlrEngine::factor_t(HFactor<F>& fa) {
   fa.wtmp.reserve((size_t)N * Rtot);
   for (int c = 1; c <= dmax; ++c)
     if (lv[c].rmax) copy_or_convert<F>(lv[c].US.get(), fa.wtmp.get() + colofs[c] * N, N * lv[c].rmax, stream);
-  leaf_apply_t<F>(fa.Dinv.get(), fa.wtmp.get(), N, (int)Rtot, fa.Wall.get(), N);
+  leaf_solve_t<F>(fa, fa.wtmp.get(), N, (int)Rtot, fa.Wall.get(), N, fa.wtmp.get());
 
   for (int j = dmax - 1; j >= 0; --j) {
     HParents& P = par[j];
@@ -629,6 +647,11 @@ void HodlrEngine::factor_t(HFactor<F>& fa) {
     fa.Minv[j].zero(stream);
     kern::fill_identity_segs<F>(eye, dnn_, fa.Minv[j].get(), mm, stream);
     kern::seg_gemm_tn<F>(kb, fa.V[j + 1], N, Q, N, fa.Minv[j].get(), mm, F(1), F(0), ws_, stream);
+    if constexpr (kRefineBlocks<F>) {
+      fa.Mmat[j].resize((size_t)np * mm * mm);
+      HIP_CHECK(hipMemcpyAsync(fa.Mmat[j].get(), fa.Minv[j].get(), (size_t)np * mm * mm * sizeof(F),
+                               hipMemcpyDeviceToDevice, stream));
+    }
     kern::gj_inverse<F>(ms, dsm_, fa.Minv[j].get(), info_.get() + info_off, stream);
     cap_info.push_back({info_off, np});
     info_off += np;
@@ -642,7 +665,8 @@ void HodlrEngine::factor_t(HFactor<F>& fa) {
 template <class F>
 void HodlrEngine::apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int ncols, F* X,
                                   int64_t ldx) {
-  leaf_apply_t<F>(fa.Dinv.get(), B, ldb, ncols, X, ldx);
+  if constexpr (kRefineBlocks<F>) fa.rtmp.reserve((size_t)N * ncols);
+  leaf_solve_t<F>(fa, B, ldb, ncols, X, ldx, fa.rtmp.get());
   for (int j = tree.max_depth - 1; j >= 0; --j) correction_t<F>(fa, j, X, ldx, ncols);
 }
 
@@ -692,12 +716,25 @@ void HodlrEngine::apply_inverse(const double* B, int64_t ldb, int ncols, double*
   kern::convert_cols<double, float>(N, ncols, f32_.x.get(), N, X, ldx, stream);
 }
 
+// Every device buffer that the recorded apply-inverse reads or writes.
+template <class F>
+static std::vector<const void*> solve_buffers(const HFactor<F>& fa) {
+  std::vector<const void*> v{fa.Dinv.get(), fa.Wall.get(), fa.tvec.get(), fa.coef.get(),
+                             fa.b.get(), fa.x.get(), fa.plan.part.get(), fa.rtmp.get()};
+  for (const auto& m : fa.Minv) v.push_back(m.get());
+  for (const auto& m : fa.Mmat) v.push_back(m.get());
+  for (const F* p : fa.V) v.push_back(p);
+  return v;
+}
+
 // Plan every launch of the apply-inverse for `ncols` columns and record them
 // into a graph (the same launches, in the same order, as apply_inverse_t).
+// A graph recorded earlier is kept only while every buffer it uses is still
+// at its recorded address.
 template <class F>
 void HodlrEngine::plan_solve(HFactor<F>& fa, int ncols) {
   SolvePlan<F>& sp = fa.plan;
-  if (sp.ncols == ncols && sp.graph.e) return;
+  if (sp.ncols == ncols && sp.graph.e && sp.bufs == solve_buffers(fa)) return;
   sp.clear();
   const int64_t lm = tree.lmax;
   fa.b.reserve((size_t)N * ncols);
@@ -744,11 +781,17 @@ void HodlrEngine::plan_solve(HFactor<F>& fa, int ncols) {
   }
   fa.tvec.reserve(tv);
   fa.coef.reserve(tv);
+  if constexpr (kRefineBlocks<F>) fa.rtmp.reserve((size_t)N * ncols);
   sp.part.reserve(slots * 4096);
   HIP_CHECK(hipStreamSynchronize(stream));
   // record
   HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   kern::launch_nn<F>(sp.leaf, fa.Dinv.get(), lm, fa.b.get(), N, fa.x.get(), N, F(1), F(0), stream);
+  if constexpr (kRefineBlocks<F>) {
+    kern::convert_cols<F, F>(N, ncols, fa.b.get(), N, fa.rtmp.get(), N, stream);
+    kern::launch_nn<F>(sp.leaf, D.get(), lm, fa.x.get(), N, fa.rtmp.get(), N, F(-1), F(1), stream);
+    kern::launch_nn<F>(sp.leaf, fa.Dinv.get(), lm, fa.rtmp.get(), N, fa.x.get(), N, F(1), F(1), stream);
+  }
   for (int j = dmax - 1; j >= 0; --j) {
     if (par[j].ids.empty()) continue;
     const int64_t mm = sp.mm[j];
@@ -756,11 +799,17 @@ void HodlrEngine::plan_solve(HFactor<F>& fa, int ncols) {
     kern::launch_tn<F>(sp.tn[j], fa.V[j + 1], N, fa.x.get(), N, fa.tvec.get(), mm, F(1), F(0), sp.part.get(),
                        stream);
     kern::launch_nn<F>(sp.mn[j], fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(0), stream);
+    if constexpr (kRefineBlocks<F>) {
+      kern::launch_nn<F>(sp.mn[j], fa.Mmat[j].get(), mm, fa.coef.get(), mm, fa.tvec.get(), mm, F(-1), F(1),
+                         stream);
+      kern::launch_nn<F>(sp.mn[j], fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(1), stream);
+    }
     kern::launch_nn<F>(sp.up[j], Q, N, fa.coef.get(), mm, fa.x.get(), N, F(-1), F(1), stream);
   }
   HIP_CHECK(hipStreamEndCapture(stream, &sp.graph.g));
   HIP_CHECK(hipGraphInstantiate(&sp.graph.e, sp.graph.g, nullptr, nullptr, 0));
   sp.ncols = ncols;
+  sp.bufs = solve_buffers(fa);
 }
 
 template <class F>
@@ -796,6 +845,7 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
   double resid = 0.0, prev = 0.0;
   int it = 0;
   const bool mixed = opt.precision == HMAT_PREC_MIXED;
+  ir_best_.reserve((size_t)N * nrhs);
   for (;; ++it) {
     matvec_enqueue(dX, ldx, nrhs, ir_y_.get(), N);
     kern::axpby_cols(N, nrhs, 1.0, dB, ldb, -1.0, ir_y_.get(), N, ir_r_.get(), N, stream);
@@ -812,10 +862,20 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
                         (mixed ? " (the FP32 factorisation is too inaccurate for this system)" : ""));
       resid = std::max(resid, rj);
     }
+    // a correction that did not reduce the residual: the iteration diverges;
+    // X goes back to the previous iterate, the one with the lowest residual
+    if (it > 0 && !(resid < prev)) {
+      HIP_CHECK(hipMemcpy2DAsync(dX, ldx * sizeof(double), ir_best_.get(), N * sizeof(double),
+                                 N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream));
+      resid = prev;
+      --it;
+      break;
+    }
     if (resid <= refine_rtol || it >= refine_max) break;
-    // a correction that did not reduce the residual: the iteration diverges
-    if (it > 0 && resid >= prev) break;
     prev = resid;
+    // the iterate before the correction (every earlier one had a larger residual)
+    HIP_CHECK(hipMemcpy2DAsync(ir_best_.get(), N * sizeof(double), dX, ldx * sizeof(double),
+                               N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream));
     // correction d = A_H^{-1} r, accumulated into X in FP64
     apply_inverse(ir_r_.get(), N, nrhs, ir_y_.get(), N);
     kern::axpby_cols(N, nrhs, 1.0, dX, ldx, 1.0, ir_y_.get(), N, dX, ldx, stream);
@@ -823,10 +883,13 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
   HIP_CHECK(hipStreamSynchronize(stream));
   last_iters = it;
   last_resid = resid;
-  if (resid > refine_rtol && mixed)
+  // the target was asked for (by default in mixed precision, opt-in in FP64)
+  // and missed: X holds the iterate with the lowest residual, and the call
+  // says so
+  if (resid > refine_rtol)
     throw Error(HMAT_ERR_NOT_CONVERGED, "refinement stopped at relative residual " +
-                                            std::to_string(resid) + " after " + std::to_string(it) +
-                                            " iterations");
+                                            sci(resid) + " after " + std::to_string(it) +
+                                            " iterations (target " + sci(refine_rtol) + ")");
 }
 
 void HodlrEngine::to_dense_host(double* A, int64_t lda) {
@@ -961,6 +1024,8 @@ void HodlrEngine::save(const std::string& path) {
 }
 
 void HodlrEngine::load(const std::string& path) {
+  // Everything is read and checked into locals first; the handle's
+  // operator, options and tree change only once the whole file is good.
   std::ifstream f(path, std::ios::binary);
   HMAT_REQUIRE(f.good(), HMAT_ERR_ARG, "cannot open " + path);
   auto get = [&](void* p, size_t n) {
@@ -975,6 +1040,11 @@ void HodlrEngine::load(const std::string& path) {
                path + ": not a version-1 HODLR operator file");
   hmat_options o;
   get(&o, sizeof(o));
+  try {
+    check_options(o);
+  } catch (const Error& e) {
+    throw Error(HMAT_ERR_ARG, path + ": bad options (" + e.what() + ")");
+  }
   int64_t n = 0;
   int32_t lf = 0, nsys = 0;
   get(&n, 8);
@@ -985,27 +1055,62 @@ void HodlrEngine::load(const std::string& path) {
   get(sizes.data(), 8 * (size_t)nsys);
   int64_t tot = 0;
   for (int64_t v : sizes) {
-    HMAT_REQUIRE(v > 0, HMAT_ERR_ARG, path + ": bad system size");
+    HMAT_REQUIRE(v > 0 && v <= n, HMAT_ERR_ARG, path + ": bad system size");
     tot += v;
   }
   HMAT_REQUIRE(tot == n, HMAT_ERR_ARG, path + ": sizes do not add up to N");
-  // the file's options replace the handle's; a mixed-precision operator
-  // refines to the default target unless one was set on this handle
-  if (o.precision == HMAT_PREC_MIXED && !(refine_rtol > 0.0)) refine_rtol = 1e-10;
-  opt = o;
-  reset_for_build(lf);
-  N = n;
+  Tree t;
   if (nsys == 1)
-    tree.build(N, leaf);
+    t.build(n, lf);
   else
-    tree.build_forest(sizes, leaf);
+    t.build_forest(sizes, lf);
   int32_t md = 0, nl = 0;
   int64_t lm = 0;
   get(&md, 4);
   get(&lm, 8);
   get(&nl, 4);
-  HMAT_REQUIRE(md == tree.max_depth && lm == tree.lmax && nl == (int32_t)tree.leaves.size(), HMAT_ERR_ARG,
+  HMAT_REQUIRE(md == t.max_depth && lm == t.lmax && nl == (int32_t)t.leaves.size(), HMAT_ERR_ARG,
                path + ": tree does not match the header");
+  struct LevelData {
+    std::vector<int> rank;
+    int rmax = 0;
+    std::vector<double> us, v;
+  };
+  std::vector<LevelData> ld(t.max_depth + 1);
+  for (int c = 1; c <= t.max_depth; ++c) {
+    const int ns = (int)t.by_depth[c].size();
+    int32_t fns = 0, rm = 0;
+    get(&fns, 4);
+    get(&rm, 4);
+    HMAT_REQUIRE(fns == ns && rm >= 0 && rm <= n, HMAT_ERR_ARG, path + ": level layout does not match the tree");
+    std::vector<int32_t> rk(ns);
+    get(rk.data(), 4 * (size_t)ns);
+    for (int s = 0; s < ns; ++s) {
+      // a block's rank is at most its smaller side
+      const TNode& nd = t.nodes[t.by_depth[c][s]];
+      const TNode& pp = t.nodes[nd.parent];
+      const int64_t sib_len = t.nodes[(pp.left == t.by_depth[c][s]) ? pp.right : pp.left].len;
+      HMAT_REQUIRE(rk[s] >= 0 && rk[s] <= rm && rk[s] <= std::min(nd.len, sib_len), HMAT_ERR_ARG,
+                   path + ": bad rank");
+    }
+    ld[c].rank.assign(rk.begin(), rk.end());
+    ld[c].rmax = rm;
+    ld[c].us.resize((size_t)n * rm);
+    ld[c].v.resize((size_t)n * rm);
+    get(ld[c].us.data(), 8 * ld[c].us.size());
+    get(ld[c].v.data(), 8 * ld[c].v.size());
+  }
+  std::vector<double> d((size_t)nl * lm * lm);
+  get(d.data(), 8 * d.size());
+  HMAT_REQUIRE(f.peek() == std::char_traits<char>::eof(), HMAT_ERR_ARG, path + ": trailing bytes after the operator");
+
+  // commit: the file's options replace the handle's; a mixed-precision
+  // operator refines to the default target unless one was set on this handle
+  if (o.precision == HMAT_PREC_MIXED && !(refine_rtol > 0.0)) refine_rtol = 1e-10;
+  opt = o;
+  reset_for_build(lf);
+  N = n;
+  tree = std::move(t);
   lv.clear();
   lv.resize(tree.max_depth + 1);
   par.clear();
@@ -1020,25 +1125,13 @@ void HodlrEngine::load(const std::string& path) {
       const int sib_id = (par_.left == L.ids[s]) ? par_.right : par_.left;
       L.sib[s] = tree.nodes[sib_id].pos;
     }
-    int32_t fns = 0, rm = 0;
-    get(&fns, 4);
-    get(&rm, 4);
-    HMAT_REQUIRE(fns == ns && rm >= 0, HMAT_ERR_ARG, path + ": level layout does not match the tree");
-    std::vector<int32_t> rk(ns);
-    get(rk.data(), 4 * (size_t)ns);
-    L.rank.assign(rk.begin(), rk.end());
-    L.rmax = rm;
-    for (int r : L.rank) HMAT_REQUIRE(r >= 0 && r <= rm, HMAT_ERR_ARG, path + ": bad rank");
-    std::vector<double> buf((size_t)N * rm);
-    get(buf.data(), 8 * buf.size());
-    L.US.upload(buf, stream);
-    get(buf.data(), 8 * buf.size());
-    L.V.upload(buf, stream);
+    L.rank = std::move(ld[c].rank);
+    L.rmax = ld[c].rmax;
+    L.US.upload(ld[c].us, stream);
+    L.V.upload(ld[c].v, stream);
     HIP_CHECK(hipStreamSynchronize(stream));
-    L.S.assign((size_t)ns * rm, 0.0);
+    L.S.assign((size_t)ns * L.rmax, 0.0);
   }
-  std::vector<double> d((size_t)nl * lm * lm);
-  get(d.data(), 8 * d.size());
   D.upload(d, stream);
   HIP_CHECK(hipStreamSynchronize(stream));
   prepare_matvec();

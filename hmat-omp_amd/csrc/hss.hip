@@ -756,6 +756,7 @@ void HssEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64
   const auto bn = download(ir_n_.get(), nrhs, stream);
   double resid = 0.0, prev = 0.0;
   int it = 0;
+  ir_best_.reserve((size_t)N * nrhs);
   for (;; ++it) {
     matvec(dX, ldx, nrhs, ir_y_.get(), N);
     kern::axpby_cols(N, nrhs, 1.0, dB, ldb, -1.0, ir_y_.get(), N, ir_r_.get(), N, stream);
@@ -769,15 +770,29 @@ void HssEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64
                                             " refinement iterations");
       resid = std::max(resid, rj);
     }
+    // a correction that did not reduce the residual: back to the previous
+    // iterate, the one with the lowest residual
+    if (it > 0 && !(resid < prev)) {
+      HIP_CHECK(hipMemcpy2DAsync(dX, ldx * sizeof(double), ir_best_.get(), N * sizeof(double),
+                                 N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream));
+      resid = prev;
+      --it;
+      break;
+    }
     if (resid <= refine_rtol || it >= refine_max) break;
-    if (it > 0 && resid >= prev) break;
     prev = resid;
+    HIP_CHECK(hipMemcpy2DAsync(ir_best_.get(), N * sizeof(double), dX, ldx * sizeof(double),
+                               N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream));
     apply_inverse(ir_r_.get(), N, nrhs, ir_y_.get(), N);
     kern::axpby_cols(N, nrhs, 1.0, dX, ldx, 1.0, ir_y_.get(), N, dX, ldx, stream);
   }
   HIP_CHECK(hipStreamSynchronize(stream));
   last_iters = it;
   last_resid = resid;
+  if (resid > refine_rtol)
+    throw Error(HMAT_ERR_NOT_CONVERGED, "refinement stopped at relative residual " + sci(resid) +
+                                            " after " + std::to_string(it) + " iterations (target " +
+                                            sci(refine_rtol) + ")");
 }
 
 void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {

@@ -89,6 +89,7 @@ class HssEngine {
   int ldv = 1;  // leading dimension of every per-node vector in the sweeps
   DevBuf<double> va_, vb_, vc_, vd_, ve_;
   DevBuf<double> ir_y_, ir_r_, ir_n_, ir_p_;  // refinement: A x, residual, norms, norm partials
+  DevBuf<double> ir_best_;                     // the iterate before the last correction
   // Build scratch, kept for the next build and grown only when a build
   // needs more (freeing and reallocating multi-GB arrays per build stalls
   // the host for up to a second per array).

@@ -657,7 +657,8 @@ __global__ __launch_bounds__(kWG) void k_chol(const SmallMat* __restrict__ mats,
                                               const double* __restrict__ G, double* __restrict__ R,
                                               const double* __restrict__ shift_coef,
                                               int* __restrict__ info, double* __restrict__ gerr,
-                                              int use_lds, const double* __restrict__ fallback_coef) {
+                                              int use_lds, const double* __restrict__ fallback_coef,
+                                              int* __restrict__ fb_used) {
   extern __shared__ double sm[];
   __shared__ double red[kWG / 64];
   __shared__ double dred[kWG / 64];
@@ -669,9 +670,13 @@ __global__ __launch_bounds__(kWG) void k_chol(const SmallMat* __restrict__ mats,
   const int lr = use_lds ? n + 1 : ld;
   double tr = 0.0, dev = 0.0;
   for (int i = threadIdx.x; i < n; i += kWG) tr += g[i + (int64_t)i * ld];
+  // deviation from I over the nonzero columns: a column that is exactly
+  // zero (G_jj = 0, hence G_ij = 0 for every i) stays zero through every
+  // pass and carries nothing into the projection
   for (int e = threadIdx.x; e < n * n; e += kWG) {
     const int i = e % n, j = e / n;
-    dev = fmax(dev, fabs(g[i + (int64_t)j * ld] - (i == j ? 1.0 : 0.0)));
+    if (g[i + (int64_t)i * ld] != 0.0 && g[j + (int64_t)j * ld] != 0.0)
+      dev = fmax(dev, fabs(g[i + (int64_t)j * ld] - (i == j ? 1.0 : 0.0)));
   }
   tr = block_sum(tr, red);
   for (int o = 32; o > 0; o >>= 1) dev = fmax(dev, __shfl_xor(dev, o));
@@ -682,6 +687,7 @@ __global__ __launch_bounds__(kWG) void k_chol(const SmallMat* __restrict__ mats,
     for (int i = 0; i < kWG / 64; ++i) d = fmax(d, dred[i]);
     gerr[blockIdx.x] = (tr == 0.0) ? 0.0 : d;
     info[blockIdx.x] = 0;
+    if (fb_used) fb_used[blockIdx.x] = 0;
   }
   if (!(tr > 0.0)) {
     if (!(tr == 0.0) && threadIdx.x == 0) info[blockIdx.x] = -1;  // NaN
@@ -724,6 +730,7 @@ __global__ __launch_bounds__(kWG) void k_chol(const SmallMat* __restrict__ mats,
       break;
     }
     shift = fb;
+    if (fb_used && threadIdx.x == 0) fb_used[blockIdx.x] = 1;
   }
   __syncthreads();
   if (use_lds)
@@ -778,7 +785,8 @@ __global__ __launch_bounds__(64) void k_chol64r(const SmallMat* __restrict__ mat
                                                 const double* __restrict__ G, double* __restrict__ R,
                                                 const double* __restrict__ shift_coef,
                                                 int* __restrict__ info, double* __restrict__ gerr, int lg,
-                                                const double* __restrict__ fallback_coef) {
+                                                const double* __restrict__ fallback_coef,
+                                                int* __restrict__ fb_used) {
   // G staged through LDS with coalesced loads, column j of G at gs[j * lg];
   // lg = the batch's largest n + 1, so small matrices leave room for more
   // workgroups per CU
@@ -794,7 +802,10 @@ __global__ __launch_bounds__(64) void k_chol64r(const SmallMat* __restrict__ mat
   double tr = 0.0, dev = 0.0;
   if (c < n) {
     tr = gs[c * lg + c];
-    for (int i = 0; i < n; ++i) dev = fmax(dev, fabs(gs[c * lg + i] - (i == c ? 1.0 : 0.0)));
+    // nonzero columns only (an exactly zero column stays zero, see k_chol)
+    if (tr != 0.0)
+      for (int i = 0; i < n; ++i)
+        if (gs[i * lg + i] != 0.0) dev = fmax(dev, fabs(gs[c * lg + i] - (i == c ? 1.0 : 0.0)));
   }
   for (int o = 32; o > 0; o >>= 1) {
     tr += __shfl_xor(tr, o);
@@ -803,6 +814,7 @@ __global__ __launch_bounds__(64) void k_chol64r(const SmallMat* __restrict__ mat
   if (c == 0) {
     gerr[blockIdx.x] = (tr == 0.0) ? 0.0 : dev;
     info[blockIdx.x] = 0;
+    if (fb_used) fb_used[blockIdx.x] = 0;
   }
   double shift = shift_coef[blockIdx.x] * tr;
   // an unshifted factorisation that breaks down is redone with the
@@ -823,6 +835,7 @@ __global__ __launch_bounds__(64) void k_chol64r(const SmallMat* __restrict__ mat
     bad = chol_step<0>(col, n, c);
     if (!bad || !(fb > 0.0)) break;
     shift = fb;
+    if (fb_used && c == 0) fb_used[blockIdx.x] = 1;
   }
   if (bad) {
     if (c == 0) info[blockIdx.x] = bad;
@@ -851,7 +864,10 @@ __global__ __launch_bounds__(64) void k_chol64(const SmallMat* __restrict__ mats
   double tr = 0.0, dev = 0.0;
   if (c < n) {
     tr = g[c + (int64_t)c * ld];
-    for (int i = 0; i < n; ++i) dev = fmax(dev, fabs(g[i + (int64_t)c * ld] - (i == c ? 1.0 : 0.0)));
+    // nonzero columns only (an exactly zero column stays zero, see k_chol)
+    if (tr != 0.0)
+      for (int i = 0; i < n; ++i)
+        if (g[i + (int64_t)i * ld] != 0.0) dev = fmax(dev, fabs(g[i + (int64_t)c * ld] - (i == c ? 1.0 : 0.0)));
   }
   for (int o = 32; o > 0; o >>= 1) {
     tr += __shfl_xor(tr, o);
@@ -899,12 +915,18 @@ void launch_chol_shifted(const SmallMat* dmats, int nmats, int nmax, double* G, 
   if (!nmats) return;
   const size_t lds = (size_t)nmax * (nmax + 1) * sizeof(double);
   const int use_lds = lds <= 64 * 1024 ? 1 : 0;
+  // with profiling on, every matrix whose factorisation took the fallback
+  // shift is counted ("chol_fallbacks" over "chol_fallback_mats")
+  DevBuf<int> fb;
+  if (dfallback && prof_enabled()) fb.resize(nmats);
   if (nmax <= 64)
     k_chol64r<<<(unsigned)nmats, 64, (size_t)nmax * (nmax + 1) * sizeof(double), s>>>(
-        dmats, G, R, dshift, info, gerr, nmax + 1, dfallback);
+        dmats, G, R, dshift, info, gerr, nmax + 1, dfallback, fb.get());
   else
-    k_chol<<<(unsigned)nmats, kWG, use_lds ? lds : 0, s>>>(dmats, G, R, dshift, info, gerr, use_lds, dfallback);
+    k_chol<<<(unsigned)nmats, kWG, use_lds ? lds : 0, s>>>(dmats, G, R, dshift, info, gerr, use_lds, dfallback,
+                                                           fb.get());
   HIP_CHECK(hipGetLastError());
+  if (fb.get()) prof_defer_count("chol_fallbacks", "chol_fallback_mats", std::move(fb), (size_t)nmats, s);
 }
 
 void chol_shifted(const std::vector<SmallMat>& mats, double* G, double* R, const std::vector<double>& shift_coef,

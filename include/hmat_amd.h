@@ -108,7 +108,16 @@ void hmat_options_reference(hmat_options* opts, double tol);
  * std::chrono Timer in its drivers, src/main_hodlr.cpp:14-24).  When enabled,
  * named launches are bracketed by HIP events on their own stream and their
  * algorithmic work is summed:
- *   "sketch_gen" / "sketch_dense"  FP64 flops of the block sketch GEMMs. */
+ *   "sketch_gen" / "sketch_dense"  FP64 flops of the block sketch GEMMs;
+ *   "matvec"                       bytes of the fused HODLR matvec;
+ * and counters are summed into `work`:
+ *   "chol_fallbacks" / "chol_fallback_mats"  Cholesky factorisations that
+ *       broke down unshifted and were redone shifted in the same launch,
+ *       over those that could;
+ *   "jacobi_sweeps" / "jacobi_mats"  Jacobi SVD sweeps over matrices.
+ * "orth_unconverged" is counted whether or not timing is on: calls of the
+ * CholeskyQR orthogonaliser that left a segment short of Q^T Q = I after
+ * its last pass (a diagnostic; the basis is kept). */
 void hmat_profile_enable(int32_t on);
 void hmat_profile_reset(void);
 /* Tile autotuner (no reference counterpart): when on, the first sketch
@@ -188,7 +197,11 @@ hmat_status hmat_hodlr_factor(hmat_hodlr h);
 /* HODLRMatrixOMP::solve(const Matrix& b)     hodlr_routines_omp.h:137-145
  *   X = A_H^{-1} B, nrhs columns.  In HMAT_PREC_MIXED the factorisation is
  *   FP32 and the answer is refined in FP64 until ||B - A_H X|| <= rtol ||B||
- *   per column (rtol from hmat_hodlr_set_refinement, default 1e-10). */
+ *   per column (rtol from hmat_hodlr_set_refinement, default 1e-10; in FP64
+ *   refinement is off unless set).  Refinement stops early when a correction
+ *   does not reduce the residual; X then holds the iterate with the lowest
+ *   residual.  A missed target returns HMAT_ERR_NOT_CONVERGED (X written,
+ *   last_refinement tells the residual reached). */
 hmat_status hmat_hodlr_solve(hmat_hodlr h, const double* B, int64_t ldb,
                              int32_t nrhs, double* X, int64_t ldx);
 hmat_status hmat_hodlr_solve_device(hmat_hodlr h, const double* dB,
@@ -274,9 +287,11 @@ hmat_status hmat_hss_to_dense(hmat_hss h, double* A, int64_t lda);
  * reference counterpart; the reference solves once): with rtol > 0 every
  * solve iterates X += A^{-1}(B - A X) until max_j ||b_j - A x_j|| / ||b_j||
  * <= rtol, max_iters corrections, or a correction that does not reduce the
- * residual.  rtol = 0 (the default) turns it off.  last_refinement reports
- * the iterations and the final max relative residual of the last solve
- * (-1 when refinement was off). */
+ * residual (X then goes back to the iterate with the lowest residual).  A
+ * missed target returns HMAT_ERR_NOT_CONVERGED with X written.  rtol = 0
+ * (the default) turns it off.  last_refinement reports the iterations kept
+ * and the final max relative residual of the last solve (-1 when
+ * refinement was off). */
 hmat_status hmat_hss_set_refinement(hmat_hss h, double rtol, int32_t max_iters);
 hmat_status hmat_hss_last_refinement(hmat_hss h, int32_t* iters, double* resid);
 /* Operator files (no reference counterpart: the reference has no

@@ -137,8 +137,10 @@ def test_gradients_against_central_differences():
 
 
 def test_cfg5_size_forward_backward():
-    """N = 65536 (the differentiation workload): gradients finite, dlam
-    consistent with a central difference."""
+    """N = 65536 (the differentiation workload): gradients finite; dlam
+    against a second-order central difference and dsigma against a
+    fourth-order one (the compressed operator is smooth in sigma at fixed
+    rank and a fixed Omega seed)."""
     n, sigma, lam = 65536, 0.1, 1e-2
     rng = np.random.default_rng(11)
     dp = torch.tensor(np.sort(rng.random(n)), device="cuda")
@@ -151,10 +153,18 @@ def test_cfg5_size_forward_backward():
     (0.5 * ((x - t) ** 2).sum()).backward()
     assert torch.isfinite(b.grad).all() and np.isfinite(ts.grad.item()) and np.isfinite(tl.grad.item())
 
-    def loss(l):
-        x = had.gaussian_solve(b.detach(), sigma, l, dp, o, leaf=128)
+    def loss(s, l):
+        x = had.gaussian_solve(b.detach(), s, l, dp, o, leaf=128)
         return 0.5 * float(((x - t) ** 2).sum())
 
     hl = 1e-5 * lam
-    fd_l = (loss(lam + hl) - loss(lam - hl)) / (2 * hl)
-    assert abs(tl.grad.item() - fd_l) <= 1e-5 * abs(fd_l)
+    fd_l = (loss(sigma, lam + hl) - loss(sigma, lam - hl)) / (2 * hl)
+    hs = 2e-3 * sigma
+    fd_s = (-loss(sigma + 2 * hs, lam) + 8 * loss(sigma + hs, lam) - 8 * loss(sigma - hs, lam)
+            + loss(sigma - 2 * hs, lam)) / (12 * hs)
+    e_l = abs(tl.grad.item() - fd_l) / abs(fd_l)
+    e_s = abs(ts.grad.item() - fd_s) / abs(fd_s)
+    print(f"\nN={n}: dl/dlam {tl.grad.item():.6e} vs FD {fd_l:.6e} ({e_l:.1e}); "
+          f"dl/dsigma {ts.grad.item():.6e} vs FD {fd_s:.6e} ({e_s:.1e})", flush=True)
+    assert e_l <= 1e-5
+    assert e_s <= 1e-4

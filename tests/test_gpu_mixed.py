@@ -39,7 +39,9 @@ def test_mixed_refines_cauchy_to_1e10():
     print(f"\nN={n}: mixed vs dense {e_mixed:.2e}, fp64 vs dense {e_fp64:.2e}, "
           f"mixed vs fp64 {rel(xm, xf):.2e}, refinement iters {it}")
     assert e_fp64 < 1e-10
-    assert e_mixed < 1e-5
+    # the BF16x3 range finder resolves blocks to ~1e-8 of their norm
+    assert e_mixed < 1e-7
+    assert rel(xm, xf) < 1e-7
 
 
 def test_mixed_operator_close_to_fp64_operator():
@@ -49,7 +51,8 @@ def test_mixed_operator_close_to_fp64_operator():
     Am, Af = hm.to_dense(), hf.to_dense()
     print(f"\noperator: mixed vs A {rel(Am, a):.2e}, fp64 vs A {rel(Af, a):.2e}")
     assert rel(Af, a) < 1e-12
-    assert rel(Am, a) < 1e-6
+    assert rel(Am, a) < 1e-7
+    assert rel(Am, Af) < 1e-7
 
 
 def test_mixed_reference_tolerance_mode():
@@ -96,9 +99,65 @@ def test_mixed_ragged_tree():
     Am, Af = hm.to_dense(), hf.to_dense()
     print(f"\nN={n}, leaf 100: mixed vs A {rel(Am, a):.2e}, fp64 vs A {rel(Af, a):.2e}")
     assert rel(Af, a) < 1e-12
-    assert rel(Am, a) < 1e-6
+    assert rel(Am, a) < 3e-7  # measured 1.1e-7
     b = np.random.default_rng(2).standard_normal((n, 1))
     x = hm.solve(b)
     it, res = hm.last_refinement()
     assert res <= 1e-10 and 1 <= it <= 10
     assert rel(Am @ x, b) <= 1.5e-10
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_cfg3_full_size(mode):
+    """The precision benchmark's own size: N = 2^18 Cauchy system, leaf 256,
+    fixed rank 48, p 10, q 2, one right-hand side b ~ N(0,1), refinement to
+    1e-10 through the FP64 compressed operator.  The true-matrix residual
+    ||b - A x|| / ||b|| uses the exact FP64 product (every entry generated on
+    the device, A never stored; the mixed path is bounded by its range
+    finder's ~1e-8 accuracy), and the mixed solution is compared with the
+    FP64 one."""
+    import torch
+    n = 1 << 18
+    xs = torch.arange(n, dtype=torch.float64, device="cuda") / n
+    ys = (torch.arange(n, dtype=torch.float64, device="cuda") + 0.5) / n
+    b = torch.tensor(np.random.default_rng(0).standard_normal(n), device="cuda")
+    k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_CAUCHY, xs.data_ptr(), ys.data_ptr())
+
+    def run(prec):
+        o = hmat_amd.make_options(0.0, rank=48, fixed_rank=True, oversample=10, power_iters=2,
+                                  exact_below=0, precision=prec, omega_dist=hmat_amd.OMEGA_GAUSSIAN,
+                                  seed=0x5EED)
+        h = hmat_amd.HODLRMatrixOMP(opts=o)
+        h.set_refinement(1e-10, 30)
+        torch.cuda.synchronize()
+        h.build_kernel_device(k, n, 256)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        h.solve_device(b.data_ptr(), n, 1, x.data_ptr(), n)
+        torch.cuda.synchronize()
+        return h, x
+
+    def true_resid(x):
+        y = torch.zeros_like(x)
+        hmat_amd.kernel_matvec_device(k, n, x.data_ptr(), n, 1, y.data_ptr(), n)
+        torch.cuda.synchronize()
+        return float(torch.linalg.norm(b - y) / torch.linalg.norm(b))
+
+    hf, xf = run(hmat_amd.PREC_FP64)
+    rf = true_resid(xf)
+    itf, resf = hf.last_refinement()
+    if mode == "fp64":
+        print(f"\nN=2^18 FP64: resid(A_H) {resf:.1e} ({itf} steps), resid(A) {rf:.1e}", flush=True)
+        assert hf.levels() == 10
+        assert resf <= 1e-10
+        assert rf < 1e-13
+        return
+    del hf
+    hm, xm = run(hmat_amd.PREC_MIXED)
+    itm, resm = hm.last_refinement()
+    rm = true_resid(xm)
+    e = float(torch.linalg.norm(xm - xf) / torch.linalg.norm(xf))
+    print(f"\nN=2^18 mixed: resid(A_H) {resm:.1e} ({itm} steps), resid(A) {rm:.1e}, vs FP64 {e:.1e}",
+          flush=True)
+    assert resm <= 1e-10 and 1 <= itm <= 5
+    assert rm < 1e-7
+    assert e < 1e-7
