@@ -214,6 +214,16 @@ hmat_status hmat_hodlr_matvec_device(hmat_hodlr h, const double* dX, int64_t ldx
   });
 }
 
+hmat_status hmat_hodlr_matvec_device_noalloc(hmat_hodlr h, const double* dX, int64_t ldx, int32_t nrhs,
+                                             double* dY, int64_t ldy) {
+  return guard([&] {
+    HMAT_REQUIRE(h && dX && dY && nrhs > 0, HMAT_ERR_ARG, "bad matvec arguments");
+    hodlr_matvec_noalloc(h->eng, dX, ldx, nrhs, dY, ldy);
+  });
+}
+
+int64_t hmat_hodlr_matvec_noalloc_capacity(void) { return (int64_t)hodlr_matvec_noalloc_capacity(); }
+
 hmat_status hmat_hodlr_factor(hmat_hodlr h) {
   return guard([&] {
     HMAT_REQUIRE(h, HMAT_ERR_ARG, "NULL handle");

@@ -20,6 +20,7 @@
 namespace hmat {
 
 HodlrEngine::HodlrEngine(const hmat_options& o) : opt(o) {
+  HIP_CHECK(hipGetDevice(&device));
   HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   own_stream = true;
   if (opt.precision == HMAT_PREC_MIXED) refine_rtol = 1e-10;
@@ -477,6 +478,36 @@ void HodlrEngine::leaf_solve_t(HFactor<F>& fa, const F* B, int64_t ldb, int ncol
 
 void HodlrEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
   matvec_enqueue(dX, ldx, nrhs, dY, ldy);
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+size_t HodlrEngine::matvec_scratch_need(int nrhs) const {
+  const int nr = nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1);  // widest pass of matvec_async
+  return ((size_t)mv_.nleaf * mv_.L * mv_.KM + (size_t)mv_.nseg * mv_.KM) * nr;
+}
+
+void HodlrEngine::matvec_with_scratch(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy,
+                                      double* scratch, size_t cap) {
+  HMAT_REQUIRE(built, HMAT_ERR_STATE, "matvec before build");
+  HMAT_REQUIRE(nrhs > 0 && ldx >= N && ldy >= N, HMAT_ERR_ARG, "bad matvec dimensions");
+  HMAT_REQUIRE(kern::mv_lds_bytes(mv_, tree.lmax, 4) <= 64 * 1024, HMAT_ERR_ARG,
+               "the operator's leaves are too large for the fused matvec kernels");
+  const size_t need = matvec_scratch_need(nrhs);
+  HMAT_REQUIRE(need <= cap, HMAT_ERR_ARG,
+               "matvec needs " + std::to_string(need) + " doubles of scratch, the fixed scratch holds " +
+                   std::to_string(cap));
+  const int nr = nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1);
+  double* part = scratch;
+  double* t = scratch + (size_t)mv_.nleaf * mv_.L * mv_.KM * nr;
+  const int64_t lm = tree.lmax;
+  int done = 0;
+  while (done < nrhs) {
+    const int rem = nrhs - done;
+    const int w = rem >= 4 ? 4 : (rem >= 2 ? 2 : 1);
+    kern::hodlr_matvec_fused(mv_, D.get(), lm, dX + (int64_t)done * ldx, ldx, w, part, t,
+                             dY + (int64_t)done * ldy, ldy, stream);
+    done += w;
+  }
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 

@@ -175,6 +175,15 @@ class HodlrEngine {
   void matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
   // the same without the final synchronisation
   void matvec_enqueue(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
+  // The fused matvec with its partial products in caller-provided device
+  // scratch of `cap` doubles (groups of up to four columns, one after
+  // another), then a synchronisation: no allocation, copy or descriptor
+  // upload.  matvec_scratch_need(nrhs) doubles are needed; HMAT_ERR_ARG when
+  // cap is smaller or the operator's leaves exceed the fused kernels' LDS.
+  size_t matvec_scratch_need(int nrhs) const;
+  void matvec_with_scratch(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy, double* scratch,
+                           size_t cap);
+  int device = 0;  // the HIP device current when the engine was created
   void factor();
   void solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
   void to_dense_host(double* A, int64_t lda);
@@ -260,5 +269,11 @@ class HodlrEngine {
   DevBuf<double> mv_part_, mv_t_;
   double mv_bytes_base_ = 0.0;  // D + (U S) + V bytes of one pass
 };
+
+// Allocation-free matvec (noalloc.hip): the fused kernels with their
+// partials in a fixed per-device scratch array of
+// hodlr_matvec_noalloc_capacity() doubles.
+void hodlr_matvec_noalloc(HodlrEngine& e, const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
+size_t hodlr_matvec_noalloc_capacity();
 
 }  // namespace hmat

@@ -73,6 +73,8 @@ def _proto(lib):
         "hmat_hodlr_build_kernel_batch_device": (st, [vp, C.POINTER(Kernel), C.POINTER(i64), i32, i32]),
         "hmat_hodlr_matvec": (st, [vp, _D, i64, i32, _D, i64]),
         "hmat_hodlr_matvec_device": (st, [vp, vp, i64, i32, vp, i64]),
+        "hmat_hodlr_matvec_device_noalloc": (st, [vp, vp, i64, i32, vp, i64]),
+        "hmat_hodlr_matvec_noalloc_capacity": (i64, []),
         "hmat_hodlr_factor": (st, [vp]),
         "hmat_hodlr_solve": (st, [vp, _D, i64, i32, _D, i64]),
         "hmat_hodlr_solve_device": (st, [vp, vp, i64, i32, vp, i64]),
@@ -311,6 +313,10 @@ class HODLRMatrixOMP(_Base):
 
     def multiply_device(self, x_ptr, ldx, nrhs, y_ptr, ldy):
         _check(self._lib.hmat_hodlr_matvec_device(self._h, x_ptr, ldx, nrhs, y_ptr, ldy))
+
+    def multiply_device_noalloc(self, x_ptr, ldx, nrhs, y_ptr, ldy):
+        """The same product through the allocation-free entry point."""
+        _check(self._lib.hmat_hodlr_matvec_device_noalloc(self._h, x_ptr, ldx, nrhs, y_ptr, ldy))
 
     def solve_transpose(self, b):
         """X = A_H^{-T} B (adjoint solve; no reference counterpart)."""

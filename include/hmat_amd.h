@@ -191,6 +191,21 @@ hmat_status hmat_hodlr_matvec_device(hmat_hodlr h, const double* dX,
                                      int64_t ldx, int32_t nrhs, double* dY,
                                      int64_t ldy);
 
+/* Allocation-free hmat_hodlr_matvec_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels as hmat_hodlr_matvec_device (1, 2 or 4 columns
+ * per pass; equal results bit for bit up to 4 columns), with their partial
+ * products in one fixed device array per GPU of
+ * hmat_hodlr_matvec_noalloc_capacity() doubles (32 MiB), held until the
+ * process ends, instead of buffers of the handle.  The call allocates
+ * nothing and synchronises the handle's stream.  An operator whose partials
+ * exceed the array returns HMAT_ERR_ARG.  Calls that overlap on one GPU
+ * race for the array (serialise them). */
+hmat_status hmat_hodlr_matvec_device_noalloc(hmat_hodlr h, const double* dX, int64_t ldx, int32_t nrhs,
+                                             double* dY, int64_t ldy);
+int64_t hmat_hodlr_matvec_noalloc_capacity(void);
+
 /* Factor once for HODLRMatrixOMP::solve (hodlr_routines_omp.h:137-240);
  * solve() factors on first use when this was not called. */
 hmat_status hmat_hodlr_factor(hmat_hodlr h);
