@@ -11,12 +11,17 @@ points and b are resident in HBM before the timed region.
 
 Prints one JSON line.  `value` is the mixed-precision step time (seconds,
 lower is better); the plain FP64 step time and the ratio are reported next
-to it.  `roofline` is for the dominant kernel of the mixed step, from HIP
-events recorded around its launches on its own stream.  `hbm_probe` is what
-plain device kernels (torch's sum and copy over 2 GiB) reach on the same
-GPU, next to the matvec's fraction of the 8 TB/s peak.  `cpu_baseline` is
-the CPU oracle (numpy/OpenBLAS restatement of the reference algorithm) on a
-bounded sample, scaled to the full size by off-diagonal area.
+to it, with the true-matrix residual and the error against the FP64
+solution, and the same figures at the other power-iteration counts
+(`q_sweep`, q = 0 being the reference's RSVD).  Timed steps run without
+kernel timing; one further step with HIP-event timing on the engine's
+stream gives `roofline` (the dominant kernel of the mixed step),
+`solve_roofline` (the apply-inverse) and `matvec_roofline`.  `hbm_probe` is
+what plain device kernels (torch's sum and copy over 2 GiB) reach on the
+same GPU.  `cpu_baseline` is the oracle's C++/OpenMP restatement of the
+reference algorithm (oracle/hodlr_omp.cpp, one BLAS thread, OpenMP tasks)
+measured at bounded sizes on the host cores and extrapolated to N = 2^18 by
+the measured growth.
 """
 import argparse
 import json
@@ -52,28 +57,23 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_mode(hmat_amd, torch, mode, a):
-    """Time `a.steps` build+factor+solve steps of one precision mode."""
+def run_mode(hmat_amd, torch, mode, a, q, steps, warmup, dev):
+    """Time `steps` build+factor+solve steps of one precision mode after
+    `warmup` untimed ones (the first of which is the cold first call on a
+    fresh handle), then one step with kernel timing."""
     n = a.n
-    xs, ys = (np.arange(n) / n), ((np.arange(n) + 0.5) / n)
-    dx = torch.tensor(xs, dtype=torch.float64, device="cuda")
-    dy = torch.tensor(ys, dtype=torch.float64, device="cuda")
-    b = torch.tensor(np.random.default_rng(0).standard_normal(n), dtype=torch.float64, device="cuda")
-    x = torch.zeros(n, dtype=torch.float64, device="cuda")
     prec = hmat_amd.PREC_MIXED if mode == "mixed" else hmat_amd.PREC_FP64
     o = hmat_amd.make_options(0.0, rank=a.rank, fixed_rank=True, oversample=a.oversample,
-                              power_iters=a.q, exact_below=0, precision=prec,
+                              power_iters=q, exact_below=0, precision=prec,
                               omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=0x5EED)
     h = hmat_amd.HODLRMatrixOMP(opts=o)
     h.set_refinement(a.rtol, 30)
-    k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_CAUCHY, dx.data_ptr(), dy.data_ptr())
+    k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_CAUCHY, dev["x"].data_ptr(), dev["y"].data_ptr())
+    b = dev["b"]
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
-    times, parts, iters, resid = [], [], [], []
-    for step in range(a.warmup + a.steps):
-        timed = step >= a.warmup
-        if timed and step == a.warmup:
-            hmat_amd.profile_reset()
-            hmat_amd.profile_enable(True)
+
+    def step():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         h.build_kernel_device(k, n, a.leaf)
@@ -83,25 +83,44 @@ def run_mode(hmat_amd, torch, mode, a):
         h.solve_device(b.data_ptr(), n, 1, x.data_ptr(), n)
         torch.cuda.synchronize()
         t3 = time.perf_counter()
+        return t0, t1, t2, t3
+
+    times, parts, first = [], [], None
+    for i in range(warmup + steps):
+        t0, t1, t2, t3 = step()
         it, r = h.last_refinement()
-        log(f"[{mode}] step {step} ({'timed' if timed else 'warmup'}): build {t1 - t0:.3f}s "
+        timed = i >= warmup
+        log(f"[{mode} q={q}] step {i} ({'timed' if timed else 'warmup'}): build {t1 - t0:.3f}s "
             f"factor {t2 - t1:.3f}s solve {t3 - t2:.3f}s total {t3 - t0:.3f}s "
             f"refinement iters {it} resid {r:.2e}")
+        if i == 0:
+            first = t3 - t0
         if timed:
             times.append(t3 - t0)
             parts.append((t1 - t0, t2 - t1, t3 - t2))
-            iters.append(it)
-            resid.append(r)
+    iters, resid = h.last_refinement()
+    out = {"times": times, "parts": parts, "first_call_s": first, "iters": iters, "resid": resid,
+           "x": x.clone(), "ranks": [max(h.level_ranks(d)) for d in range(1, h.levels() + 1)]}
+    if not a.profile:
+        return out, h
+    # one more step with HIP-event timing of the named launches
+    hmat_amd.profile_reset()
+    hmat_amd.profile_enable(True)
+    step()
+    # further applies of the factored inverse for its own average
+    for _ in range(a.solve_reps):
+        h.solve_device(b.data_ptr(), n, 1, x.data_ptr(), n)
+    torch.cuda.synchronize()
     hmat_amd.profile_enable(False)
     prof = {}
-    for name in ("sketch_gen", "sketch_lp", "sketch_lp1", "jacobi_sweeps", "jacobi_mats"):
-        q = hmat_amd.profile_query(name)
-        if q:
-            prof[name] = {"launches": q[0], "ms": q[1], "work": q[2]}
-    sweeps = prof.pop("jacobi_sweeps", None)
-    mats = prof.pop("jacobi_mats", None)
-    jac = round(sweeps["work"] / mats["work"], 2) if sweeps and mats and mats["work"] else None
-    ranks = [max(h.level_ranks(d)) for d in range(1, h.levels() + 1)]
+    for name in ("sketch_gen", "sketch_lp", "sketch_lp1", "jacobi_sweeps", "jacobi_mats", "solve_apply"):
+        pq = hmat_amd.profile_query(name)
+        if pq:
+            prof[name] = {"launches": pq[0], "ms": pq[1], "work": pq[2]}
+    sweeps, mats = prof.pop("jacobi_sweeps", None), prof.pop("jacobi_mats", None)
+    out["jacobi_avg_sweeps"] = round(sweeps["work"] / mats["work"], 2) if sweeps and mats and mats["work"] else None
+    out["solve"] = prof.pop("solve_apply", None)
+    out["prof"] = prof
     # matvec y = A_H x of the built operator (FP64 generators), 1 RHS
     y = torch.empty(n, dtype=torch.float64, device="cuda")
     for _ in range(3):
@@ -112,21 +131,19 @@ def run_mode(hmat_amd, torch, mode, a):
     for _ in range(a.matvec_reps):
         h.multiply_device(b.data_ptr(), n, 1, y.data_ptr(), n)
     hmat_amd.profile_enable(False)
-    q = hmat_amd.profile_query("matvec")
-    mv = {"launches": q[0], "ms": q[1], "work": q[2]} if q else None
-    return {"times": times, "parts": parts, "iters": iters, "resid": resid, "prof": prof,
-            "x": x.clone(), "b": b, "dx": dx, "dy": dy, "ranks": ranks, "h": h, "matvec": mv,
-            "jacobi_avg_sweeps": jac}
+    mq = hmat_amd.profile_query("matvec")
+    out["matvec"] = {"launches": mq[0], "ms": mq[1], "work": mq[2]} if mq else None
+    return out, h
 
 
-def true_residual(hmat_amd, torch, res, n):
+def true_residual(hmat_amd, torch, x, dev, n):
     """||b - A x|| / ||b|| with the exact generated A (FP64, never stored)."""
-    k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_CAUCHY, res["dx"].data_ptr(), res["dy"].data_ptr())
+    k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_CAUCHY, dev["x"].data_ptr(), dev["y"].data_ptr())
     y = torch.zeros(n, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
-    hmat_amd.kernel_matvec_device(k, n, res["x"].data_ptr(), n, 1, y.data_ptr(), n)
+    hmat_amd.kernel_matvec_device(k, n, x.data_ptr(), n, 1, y.data_ptr(), n)
     torch.cuda.synchronize()
-    return float(torch.linalg.norm(res["b"] - y) / torch.linalg.norm(res["b"]))
+    return float(torch.linalg.norm(dev["b"] - y) / torch.linalg.norm(dev["b"]))
 
 
 def hbm_probe(torch, nbytes=1 << 31, reps=5):
@@ -154,35 +171,74 @@ def hbm_probe(torch, nbytes=1 << 31, reps=5):
             "copy_GBs": round(2 * nbytes / (t_copy * 1e-3) / 1e9, 1)}
 
 
-def cpu_baseline(a):
-    """Oracle build+solve of the same workload at a bounded size."""
-    from oracle import hodlr, kernels, lowrank, rng
+def _lapack_env():
+    """Environment for the restatement: MKL's single dynamic library when the
+    image has it, else scipy's bundled OpenBLAS (symbols with a scipy_
+    prefix, which the restatement also accepts)."""
+    import glob
+    env = dict(os.environ)
+    if not os.path.exists("/opt/conda/lib/libmkl_rt.so.1") and "HMAT_LAPACK" not in env:
+        import scipy
+        libs = sorted(glob.glob(os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs",
+                                             "libscipy_openblas*.so")))
+        if libs:
+            env["HMAT_LAPACK"] = libs[0]
+    return env
+
+
+def cpu_model():
     try:
-        from threadpoolctl import threadpool_info
-        cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    except Exception:
-        cores = os.cpu_count() or 1
-    ns = a.cpu_n
-    xc, yc = kernels.cauchy_grid_points(ns)
-    A = kernels.cauchy_points(xc, yc)
-    o = lowrank.Options(tol=0.0, rank=a.rank, fixed_rank=True, oversample=a.oversample,
-                        power_iters=a.q, exact_below=0)
-    h = hodlr.HODLRMatrix(opts=o)
-    b = np.random.default_rng(0).standard_normal((ns, 1))
-    t0 = time.perf_counter()
-    h.build_from_dense(A, a.leaf, rng.RandomStream(1))
-    x = h.solve(b)
-    t = time.perf_counter() - t0
-    r = float(np.linalg.norm(b - h.multiply(x)) / np.linalg.norm(b))
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(a):
+    """The oracle's C++/OpenMP restatement of HODLRMatrixOMP (oracle/hodlr_omp.cpp):
+    build + recursive Woodbury solve of the same Cauchy workload (leaf, rank,
+    oversample, q) at bounded sizes, OpenMP tasks over `threads` host cores,
+    one BLAS thread (run_tests.sh:7-11).  The full size is extrapolated from
+    the two largest measured sizes by their measured growth exponent."""
+    import subprocess
+    from oracle import build_restatement
+    exe = build_restatement.ensure()
+    threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    env = _lapack_env()
+    env["OMP_NUM_THREADS"] = str(threads)
+    meas = []
+    for ns in [int(v) for v in a.cpu_sizes.split(",")]:
+        cmd = [exe, "--kernel", "cauchy", "--n", str(ns), "--leaf", str(a.leaf), "--rank", str(a.rank),
+               "--oversample", str(a.oversample), "--q", str(a.q), "--fixed", "1", "--tol", "0",
+               "--exact-below", "0", "--seed", "1"]
+        r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=900)
+        if r.returncode != 0:
+            log(f"[cpu] restatement failed at N={ns}: {r.stderr.strip()}")
+            return None
+        d = json.loads(r.stdout)
+        meas.append({"n": ns, "build_s": round(d["build_s"], 4), "solve_s": round(d["solve_s"], 4),
+                     "total_s": round(d["build_s"] + d["solve_s"], 4), "resid_AH": d["resid"]})
+        log(f"[cpu] restatement N={ns}: build {d['build_s']:.3f}s solve {d['solve_s']:.3f}s "
+            f"resid {d['resid']:.1e} ({threads} threads, {d['lapack']})")
     lev = lambda n: max(0, int(np.ceil(np.log2(n / a.leaf))))
     area = lambda n: n * n * (1.0 - 2.0 ** (-lev(n)))
-    scale = area(a.n) / area(ns)
-    log(f"[cpu] oracle N={ns}: {t:.2f}s resid {r:.1e}; x{scale:.0f} by off-diagonal area")
-    return {"value": t * scale, "unit": "s", "cores": int(cores), "kind": "port",
-            "sample": f"oracle/hodlr.py (numpy + OpenBLAS, reference algorithm incl. its "
-                      f"recursive Woodbury solve) build+solve of the same Cauchy kernel at "
-                      f"N={ns}, leaf {a.leaf}, rank {a.rank}, p {a.oversample}, q {a.q}: "
-                      f"{t:.2f} s measured, scaled x{scale:.0f} by off-diagonal area to N={a.n}"}
+    m1, m2 = meas[-2], meas[-1]
+    expo = float(np.log(m2["total_s"] / m1["total_s"]) / np.log(m2["n"] / m1["n"]))
+    value = m2["total_s"] * (a.n / m2["n"]) ** expo
+    area_pred = m1["total_s"] * area(m2["n"]) / area(m1["n"])
+    return {"value": round(value, 1), "unit": "s", "cores": int(threads), "kind": "port",
+            "sample": (f"oracle/hodlr_omp.cpp (C++/OpenMP restatement of HODLRMatrixOMP: tasks above 512 rows, "
+                       f"recursive Woodbury solve, one BLAS thread) build+solve of the same Cauchy workload "
+                       f"(leaf {a.leaf}, rank {a.rank}, p {a.oversample}, q {a.q}) measured at N="
+                       f"{','.join(str(m['n']) for m in meas)} on {threads} threads of {cpu_model()}; "
+                       f"value = the N={m2['n']} time x (N/{m2['n']})^{expo:.2f}, the growth exponent measured "
+                       f"between the two largest sizes"),
+            "measured": meas, "growth_exponent": round(expo, 3),
+            "area_scaling_check": {"predicted_s": round(area_pred, 4), "measured_s": m2["total_s"],
+                                   "measured_over_predicted": round(m2["total_s"] / area_pred, 3)},
+            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
 
 
 def main():
@@ -195,11 +251,18 @@ def main():
     ap.add_argument("--rank", type=int, default=48)
     ap.add_argument("--oversample", type=int, default=10)
     ap.add_argument("--q", type=int, default=2)
+    ap.add_argument("--q-sweep", default="0,1",
+                    help="further power-iteration counts measured beside the headline q ('' for none)")
+    ap.add_argument("--sweep-steps", type=int, default=3)
     ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--cpu-n", type=int, default=4096)
+    ap.add_argument("--cpu-sizes", default="4096,8192,16384")
+    ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--matvec-reps", type=int, default=20)
+    ap.add_argument("--solve-reps", type=int, default=10)
+    ap.add_argument("--no-profile", dest="profile", action="store_false",
+                    help="skip the kernel-timed step (rooflines)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tilings (the autotuner runs in the warm-up step otherwise)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -225,8 +288,16 @@ def main():
         del src, dst
     if not a.no_autotune:
         hmat_amd.autotune_enable(True)
+    n = a.n
+    dev = {"x": torch.tensor(np.arange(n) / n, dtype=torch.float64, device="cuda"),
+           "y": torch.tensor((np.arange(n) + 0.5) / n, dtype=torch.float64, device="cuda"),
+           "b": torch.tensor(np.random.default_rng(0).standard_normal(n), dtype=torch.float64, device="cuda")}
     modes = a.modes.split(",")
-    res = {m: run_mode(hmat_amd, torch, m, a) for m in modes}
+    res = {}
+    for m in modes:
+        r, h = run_mode(hmat_amd, torch, m, a, a.q, a.steps, a.warmup, dev)
+        del h
+        res[m] = r
     med = {m: statistics.median(r["times"]) for m, r in res.items()}
     head = "mixed" if "mixed" in res else modes[0]
     r = res[head]
@@ -254,17 +325,45 @@ def main():
         out[f"{m}_s"] = round(med[m], 5)
         bp = [statistics.median(p[i] for p in res[m]["parts"]) for i in range(3)]
         out[f"{m}_build_factor_solve_s"] = [round(v, 5) for v in bp]
-        out[f"{m}_refine_iters"] = res[m]["iters"][-1]
-        out[f"{m}_resid_AH"] = res[m]["resid"][-1]
-        out[f"{m}_resid_trueA"] = true_residual(hmat_amd, torch, res[m], a.n)
+        out[f"{m}_first_call_s"] = round(res[m]["first_call_s"], 4)
+        out[f"{m}_refine_iters"] = res[m]["iters"]
+        out[f"{m}_resid_AH"] = res[m]["resid"]
+        out[f"{m}_resid_trueA"] = true_residual(hmat_amd, torch, res[m]["x"], dev, n)
         out[f"{m}_max_rank_per_level"] = res[m]["ranks"]
-        out[f"{m}_jacobi_avg_sweeps"] = res[m]["jacobi_avg_sweeps"]
+        out[f"{m}_jacobi_avg_sweeps"] = res[m].get("jacobi_avg_sweeps")
     if "fp64" in res and "mixed" in res:
         out["speedup_mixed_vs_fp64"] = round(med["fp64"] / med["mixed"], 3)
         xf, xm = res["fp64"]["x"], res["mixed"]["x"]
         out["err_mixed_vs_fp64"] = float(torch.linalg.norm(xm - xf) / torch.linalg.norm(xf))
+    # the same workload at the other power-iteration counts (q = 0: the
+    # reference's RSVD, compression.h:77-90, has no power iterations)
+    sweep = []
+    for qv in [int(v) for v in a.q_sweep.split(",") if v.strip() != ""]:
+        if qv == a.q:
+            continue
+        ent = {"q": qv}
+        xs = {}
+        for m in modes:
+            rr, h = run_mode(hmat_amd, torch, m, a, qv, a.sweep_steps, 1, dev)
+            del h
+            ent[f"{m}_s"] = round(statistics.median(rr["times"]), 5)
+            ent[f"{m}_first_call_s"] = round(rr["first_call_s"], 4)
+            ent[f"{m}_resid_AH"] = rr["resid"]
+            ent[f"{m}_resid_trueA"] = true_residual(hmat_amd, torch, rr["x"], dev, n)
+            if rr.get("prof", {}).get("sketch_gen"):
+                p = rr["prof"]["sketch_gen"]
+                ent[f"{m}_fp64_sketch_ms"] = round(p["ms"], 2)
+            xs[m] = rr["x"]
+        if "fp64" in xs and "mixed" in xs:
+            ent["speedup_mixed_vs_fp64"] = round(ent["fp64_s"] / ent["mixed_s"], 3)
+            ent["err_mixed_vs_fp64"] = float(torch.linalg.norm(xs["mixed"] - xs["fp64"]) /
+                                             torch.linalg.norm(xs["fp64"]))
+        sweep.append(ent)
+        torch.cuda.empty_cache()
+    if sweep:
+        out["q_sweep"] = sweep
     # roofline of the dominant kernel of the headline step
-    prof = r["prof"]
+    prof = r.get("prof", {})
     name = max(prof, key=lambda k: prof[k]["ms"]) if prof else None
     if name:
         p = prof[name]
@@ -276,11 +375,24 @@ def main():
                            "traffic": PMC_TRAFFIC.get(name) if a.n == 1 << 18 else None,
                            "traffic_source": PMC_SOURCE,
                            "launches": p["launches"], "avg_launch_ms": round(p["ms"] / p["launches"], 4),
-                           "step_share": round(p["ms"] / 1000 / a.steps / med[head], 3)}
+                           "step_share": round(p["ms"] / 1000 / med[head], 3)}
         for k2, p2 in prof.items():
             out[f"kernel_{k2}"] = {"launches": p2["launches"], "ms_total": round(p2["ms"], 2),
                                    "tflops": round(p2["work"] / (p2["ms"] * 1e-3) / 1e12, 2)}
-    mv = res[modes[0]]["matvec"] if "fp64" not in res else res["fp64"]["matvec"]
+    sr = {}
+    for m in modes:
+        sv = res[m].get("solve")
+        if sv:
+            avg = sv["ms"] / sv["launches"]
+            gbs = sv["work"] / sv["launches"] / (avg * 1e-3) / 1e9
+            sr[m] = {"kernel": ("apply-inverse of the FP64 factorisation (refined leaf and capacitance solves)"
+                                if m == "fp64" else "apply-inverse of the FP32 factorisation"),
+                     "bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_launch": sv["work"] / sv["launches"],
+                     "avg_launch_ms": round(avg, 4), "launches": sv["launches"]}
+    if sr:
+        out["solve_roofline"] = sr
+    mv = res["fp64"].get("matvec") if "fp64" in res else res[modes[0]].get("matvec")
     if mv:
         avg = mv["ms"] / mv["launches"]
         gbs = mv["work"] / mv["launches"] / (avg * 1e-3) / 1e9
@@ -295,7 +407,9 @@ def main():
         out["hbm_probe"] = probe
         out["matvec_roofline"]["frac_of_probe_read"] = round(gbs / probe["read_GBs"], 4)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(a)
+        cb = cpu_baseline(a)
+        if cb:
+            out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 

@@ -723,7 +723,24 @@ static bool graph_turn(SolvePlan<F>& sp, bool enabled, int ncols) {
   return ++sp.uses[ncols] > kEagerSolves;
 }
 
+// Bytes one apply-inverse must move at least: the leaf inverses (and, for
+// the refined FP64 leaf solve, the leaf blocks), Q and V of every depth, the
+// right-hand side in and the solution out in FP64.
+double HodlrEngine::solve_bytes(int ncols) const {
+  const bool mixed = opt.precision == HMAT_PREC_MIXED;
+  const double fb = mixed ? 4.0 : 8.0;
+  double leaf2 = 0.0;
+  for (int id : tree.leaves) leaf2 += (double)tree.nodes[id].len * (double)tree.nodes[id].len;
+  return fb * ((mixed ? 1.0 : 2.0) * leaf2 + 2.0 * (double)N * (double)Rtot) + 16.0 * (double)N * ncols;
+}
+
 void HodlrEngine::apply_inverse(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx) {
+  ProfScope prof("solve_apply", stream, solve_bytes(ncols));
+  apply_inverse_impl(B, ldb, ncols, X, ldx);
+  prof.finish();
+}
+
+void HodlrEngine::apply_inverse_impl(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx) {
   if (opt.precision != HMAT_PREC_MIXED) {
     if (!graph_turn(f64_.plan, solve_graph, ncols)) {
       apply_inverse_t<double>(f64_, B, ldb, ncols, X, ldx);

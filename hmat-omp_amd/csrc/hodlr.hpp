@@ -240,6 +240,8 @@ class HodlrEngine {
   template <class F> void plan_solve(HFactor<F>& fa, int ncols);
   template <class F> void run_solve_graph(HFactor<F>& fa, int ncols);
   void apply_inverse(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx);
+  void apply_inverse_impl(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx);
+  double solve_bytes(int ncols) const;
   // fused matvec plan (built once per build) and its launch without a host sync
   void prepare_matvec();
   void flip();  // switch between A_H and A_H^T in place (involution)
