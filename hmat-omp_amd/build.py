@@ -66,7 +66,7 @@ def build(jobs=None, verbose=False):
     os.replace(tmp, LIB)
     # the C++ drop-in example links the library through include/hmat_amd.hpp
     os.makedirs(BIN_DIR, exist_ok=True)
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"), DRIVER_SRC,
+    cmd = ["g++", "-O3", "-fopenmp", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"), DRIVER_SRC,
            "-L", LIB_DIR, "-lhmat_amd", "-Wl,-rpath,$ORIGIN/../lib", "-o", DRIVER]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

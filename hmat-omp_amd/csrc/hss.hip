@@ -71,14 +71,21 @@ int64_t HssEngine::pool_alloc(int64_t n) {
 }
 
 void HssEngine::leaf_apply(const double* Dm, const double* B, int64_t ldb, int ncols, double* Y,
-                           int64_t ldy) {
+                           int64_t ldy, double alpha, double beta) {
   const int64_t lm = tree.lmax;
   std::vector<SegNN> segs;
   for (size_t i = 0; i < tree.leaves.size(); ++i) {
     const TNode& nd = tree.nodes[tree.leaves[i]];
     segs.push_back(SegNN{(int64_t)i * lm * lm, nd.start, nd.start, (int32_t)nd.len, (int32_t)nd.len, ncols, 0});
   }
-  kern::seg_gemm_nn<double>(segs, Dm, lm, B, ldb, Y, ldy, 1.0, 0.0, stream);
+  kern::seg_gemm_nn<double>(segs, Dm, lm, B, ldb, Y, ldy, alpha, beta, stream);
+}
+
+void HssEngine::leaf_solve(const double* B, int64_t ldb, int ncols, double* Y, int64_t ldy, double* R) {
+  leaf_apply(Dinv.get(), B, ldb, ncols, Y, ldy);
+  kern::convert_cols<double, double>(N, ncols, B, ldb, R, N, stream);
+  leaf_apply(D.get(), Y, ldy, ncols, R, N, -1.0, 1.0);
+  leaf_apply(Dinv.get(), R, N, ncols, Y, ldy, 1.0, 1.0);
 }
 
 // Leaf bases of one side.  Row side: range of A(I, I^c) (len x N with the
@@ -643,7 +650,8 @@ void HssEngine::factor() {
     return;
   }
   Ql.resize((size_t)N * std::max(rUmax, 1));
-  leaf_apply(Dinv.get(), Ul.get(), N, rUmax, Ql.get(), N);
+  lres_.reserve((size_t)N * std::max(rUmax, 1));
+  leaf_solve(Ul.get(), N, rUmax, Ql.get(), N, lres_.get());
   // K_i = V_i^T Q_i, stored with leading dimension ldv
   for (int id : tree.leaves) sn[id].oK = pool_alloc((int64_t)ldv * sn[id].rU);
   {
@@ -659,17 +667,20 @@ void HssEngine::factor() {
     for (int id : tree.by_depth[d])
       if (!tree.nodes[id].leaf) inner.push_back(id);
     // allocate M, G, K and two scratch blocks per node
-    std::vector<int64_t> oX(tree.nodes.size()), oC(tree.nodes.size()), oKG(tree.nodes.size());
+    std::vector<int64_t> oX(tree.nodes.size()), oXr(tree.nodes.size()), oC(tree.nodes.size()),
+        oKG(tree.nodes.size());
     for (int id : inner) {
       const TNode& nd = tree.nodes[id];
       const int ms = sn[nd.left].rV + sn[nd.right].rV;
       const int kR = sn[nd.left].rU + sn[nd.right].rU;
       sn[id].ms = ms;
       sn[id].oMinv = pool_alloc((int64_t)ms * ms);
+      sn[id].oM = pool_alloc((int64_t)ms * ms);
       if (d > 0) {
         sn[id].oG = pool_alloc((int64_t)kR * sn[id].rU);
         sn[id].oK = pool_alloc((int64_t)ldv * sn[id].rU);
         oX[id] = pool_alloc((int64_t)ms * kR);
+        oXr[id] = pool_alloc((int64_t)ms * kR);
         oC[id] = pool_alloc((int64_t)kR * kR);
         oKG[id] = pool_alloc((int64_t)ms * sn[id].rU);
       }
@@ -692,15 +703,23 @@ void HssEngine::factor() {
     kern::rect_axpby(zm, drc_, stream);
     kern::rect_add_identity(zm, drc_, stream);
     kern::bgemm(mb, dbg_, stream);
+    // keep M for the refinement step of every solve with it
+    std::vector<Rect> keepM;
+    for (int id : inner) {
+      const int m = sn[id].ms;
+      keepM.push_back(Rect{P + sn[id].oMinv, P + sn[id].oM, m, m, m, m, 1.0, 0.0});
+    }
+    kern::rect_axpby(keepM, drc_, stream);
     info_.reserve(std::max<size_t>(ms.size(), 1));
     kern::gj_inverse(ms, dsm_, P, info_.get(), stream);
     auto inf = download(info_.get(), ms.size(), stream);
     for (size_t i = 0; i < ms.size(); ++i)
       if (inf[i] != 0) throw Error(HMAT_ERR_SINGULAR, "singular HSS capacitance at depth " + std::to_string(d));
     if (d == 0) break;
-    // X = M^{-1} K (ms x kR);  CX = C X (kR x kR);  G = R - CX R;  K_p = W^T (K G)
-    std::vector<BGemm> s1, s2, s3, s4, s5;
-    std::vector<Rect> cpG;
+    // X = M^{-1} K (ms x kR), refined once: Xr = K - M X, X += M^{-1} Xr;
+    // CX = C X (kR x kR);  G = R - CX R;  K_p = W^T (K G)
+    std::vector<BGemm> s1, s1r, s1x, s2, s3, s4, s5;
+    std::vector<Rect> cpG, addK;
     for (int id : inner) {
       const TNode& nd = tree.nodes[id];
       const int va = sn[nd.left].rV, vb = sn[nd.right].rV, ua = sn[nd.left].rU, ub = sn[nd.right].rU;
@@ -712,6 +731,11 @@ void HssEngine::factor() {
       double* KG = P + oKG[id];
       s1.push_back(BGemm{Mi, P + sn[nd.left].oK, X, m, ua, va, m, ldv, m, 0, 0, 1.0, 0.0});
       s1.push_back(BGemm{Mi + (int64_t)va * m, P + sn[nd.right].oK, X + (int64_t)ua * m, m, ub, vb, m, ldv, m, 0, 0, 1.0, 0.0});
+      double* Xr = P + oXr[id];
+      s1r.push_back(BGemm{P + sn[id].oM, X, Xr, m, kR, m, m, m, m, 0, 0, -1.0, 0.0});
+      addK.push_back(Rect{P + sn[nd.left].oK, Xr, ldv, m, va, ua, 1.0, 1.0});
+      addK.push_back(Rect{P + sn[nd.right].oK, Xr + (int64_t)ua * m + va, ldv, m, vb, ub, 1.0, 1.0});
+      s1x.push_back(BGemm{Mi, Xr, X, m, kR, m, m, m, m, 0, 0, 1.0, 1.0});
       s2.push_back(BGemm{P + sn[id].oBab, X + va, CX, ua, kR, vb, sn[id].ldB, m, kR, 0, 0, 1.0, 0.0});
       s2.push_back(BGemm{P + sn[id].oBba, X, CX + ua, ub, kR, va, sn[id].ldB, m, kR, 0, 0, 1.0, 0.0});
       cpG.push_back(Rect{P + sn[id].oR, G, sn[id].ldR, kR, kR, up, 1.0, 0.0});
@@ -721,6 +745,9 @@ void HssEngine::factor() {
       s5.push_back(BGemm{P + sn[id].oW, KG, P + sn[id].oK, sn[id].rV, up, m, sn[id].ldW, m, ldv, 1, 0, 1.0, 0.0});
     }
     kern::bgemm(s1, dbg_, stream);
+    kern::bgemm(s1r, dbg_, stream);
+    kern::rect_axpby(addK, drc_, stream);
+    kern::bgemm(s1x, dbg_, stream);
     kern::bgemm(s2, dbg_, stream);
     kern::rect_axpby(cpG, drc_, stream);
     kern::bgemm(s3, dbg_, stream);
@@ -796,7 +823,8 @@ void HssEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64
 }
 
 void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
-  leaf_apply(Dinv.get(), dB, ldb, nrhs, dX, ldx);
+  lres_.reserve((size_t)N * nrhs);
+  leaf_solve(dB, ldb, nrhs, dX, ldx, lres_.get());
   if (tree.max_depth == 0) {
     HIP_CHECK(hipStreamSynchronize(stream));
     return;
@@ -808,7 +836,8 @@ void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* d
   vc_.reserve((size_t)nn * vs);  // s
   vd_.reserve((size_t)nn * vs);  // K s
   ve_.reserve((size_t)nn * vs);  // tau
-  double *H = va_.get(), *U = vb_.get(), *Sv = vc_.get(), *KS = vd_.get(), *T = ve_.get();
+  vf_.reserve((size_t)nn * vs);  // residual of the M solve
+  double *H = va_.get(), *U = vb_.get(), *Sv = vc_.get(), *KS = vd_.get(), *T = ve_.get(), *Rm = vf_.get();
   auto at = [&](double* b, int id) { return b + (int64_t)id * vs; };
   {
     std::vector<SegTN> tn;
@@ -820,7 +849,8 @@ void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* d
   }
   const double* P = pool.get();
   for (int d = tree.max_depth - 1; d >= 0; --d) {
-    std::vector<BGemm> u1, u2, s1, k1, h1, h2;
+    std::vector<BGemm> u1, u2, ur, ux, s1, k1, h1, h2;
+    std::vector<Rect> uh;
     for (int id : tree.by_depth[d]) {
       const TNode& nd = tree.nodes[id];
       if (nd.leaf) continue;
@@ -829,6 +859,11 @@ void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* d
       const double* Mi = P + sn[id].oMinv;
       u1.push_back(BGemm{Mi, at(H, nd.left), at(U, id), m, nrhs, va, m, ldv, ldv, 0, 0, 1.0, 0.0});
       u2.push_back(BGemm{Mi + (int64_t)va * m, at(H, nd.right), at(U, id), m, nrhs, vb, m, ldv, ldv, 0, 0, 1.0, 1.0});
+      // refinement of u = M^{-1} [h_a; h_b]: r = [h_a; h_b] - M u, u += M^{-1} r
+      ur.push_back(BGemm{P + sn[id].oM, at(U, id), at(Rm, id), m, nrhs, m, m, ldv, ldv, 0, 0, -1.0, 0.0});
+      uh.push_back(Rect{at(H, nd.left), at(Rm, id), ldv, ldv, va, nrhs, 1.0, 1.0});
+      uh.push_back(Rect{at(H, nd.right), at(Rm, id) + va, ldv, ldv, vb, nrhs, 1.0, 1.0});
+      ux.push_back(BGemm{Mi, at(Rm, id), at(U, id), m, nrhs, m, m, ldv, ldv, 0, 0, 1.0, 1.0});
       s1.push_back(BGemm{P + sn[id].oBab, at(U, id) + va, at(Sv, id), ua, nrhs, vb, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
       s1.push_back(BGemm{P + sn[id].oBba, at(U, id), at(Sv, id) + ua, ub, nrhs, va, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
       if (d > 0) {
@@ -842,6 +877,9 @@ void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* d
     }
     kern::bgemm(u1, dbg_, stream);
     kern::bgemm(u2, dbg_, stream);
+    kern::bgemm(ur, dbg_, stream);
+    kern::rect_axpby(uh, drc_, stream);
+    kern::bgemm(ux, dbg_, stream);
     kern::bgemm(s1, dbg_, stream);
     kern::bgemm(k1, dbg_, stream);
     kern::bgemm(h1, dbg_, stream);

@@ -28,6 +28,7 @@ struct SNode {
   int64_t oBab = -1, oBba = -1;  // couplings: rU_a x rV_b, rU_b x rV_a
   int ldR = 1, ldW = 1, ldB = 1;
   int64_t oMinv = -1, oG = -1, oK = -1;  // factor: M^{-1} (ms x ms), G (kR x rU), K_hat (rV x rU)
+  int64_t oM = -1;                        // factor: M itself, for the refinement step of M solves
   int ms = 0;                             // rV_a + rV_b
 };
 
@@ -73,7 +74,12 @@ class HssEngine {
   void build_internal();
   int64_t pool_alloc(int64_t n);
   void set_ldv();
-  void leaf_apply(const double* Dm, const double* B, int64_t ldb, int ncols, double* Y, int64_t ldy);
+  void leaf_apply(const double* Dm, const double* B, int64_t ldb, int ncols, double* Y, int64_t ldy,
+                  double alpha = 1.0, double beta = 0.0);
+  // Y = D^{-1} B over all leaves: the inverse, then one refinement step
+  // against D (residual in R, N x ncols, ld N), as the HODLR factorisation
+  // does: backward stable like an LU solve while every step stays a GEMM
+  void leaf_solve(const double* B, int64_t ldb, int ncols, double* Y, int64_t ldy, double* R);
   void apply_inverse(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
 
   // leaf generators
@@ -87,7 +93,8 @@ class HssEngine {
   int64_t pool_used = 0;
   int64_t pool_built = 0;  // end of the build's part (the factor's follows)
   int ldv = 1;  // leading dimension of every per-node vector in the sweeps
-  DevBuf<double> va_, vb_, vc_, vd_, ve_;
+  DevBuf<double> va_, vb_, vc_, vd_, ve_, vf_;
+  DevBuf<double> lres_;  // leaf-solve residual
   DevBuf<double> ir_y_, ir_r_, ir_n_, ir_p_;  // refinement: A x, residual, norms, norm partials
   DevBuf<double> ir_best_;                     // the iterate before the last correction
   // Build scratch, kept for the next build and grown only when a build

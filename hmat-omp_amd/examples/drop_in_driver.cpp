@@ -1,100 +1,116 @@
-// Driver-style use of the C++ drop-in classes (include/hmat_amd.hpp): the
-// flow of the reference drivers (src/main_hodlr.cpp:37-107,
-// src/main_hss.cpp:51-111) — generate the Cauchy test matrix, b = A x_true,
-// build, multiply, solve, and print the relative errors — with the
-// reference classes replaced by hmat_amd::HODLRMatrixOMPT / HSSMatrixOMPT.
+// A driver in the form of the reference drivers (src/main_hodlr.cpp,
+// src/main_hss.cpp), compiled against include/hmat_amd_dropin.hpp instead of
+// the reference headers: the same call sequence on the same names (Matrix,
+// Kernels::, HODLRMatrix / HODLRMatrixOMP, HSSMatrix / HSSMatrixOMP), the
+// dense baseline on the host, the hierarchical solvers on the MI355X.
 //
-//   drop_in_driver [N] [leaf] [tol]        (defaults 4096 512 1e-8)
+//   drop_in_driver hodlr [N leaf tol]   defaults 4096 512 1e-8, the drivers'
+//                                       Cauchy matrix (main_hodlr.cpp:38-49)
+//   drop_in_driver hss [N leaf tol]     defaults 2048 256 1e-9, Gaussian
+//                                       h 0.5, shift 1e-6 (main_hss.cpp:38-52)
 //
-// The dense matrix type below has the reference Matrix's interface
-// (rows(), cols(), data(), column-major, (r, c) constructor); the reference's
-// own Matrix works the same way.
+// Prints the reference's lines (build / GEMV / solve times, relative errors
+// of the product against b and of the solution against x_true) and, for
+// each solver, the residual ||b - A x|| / ||b||, which the reference does not
+// print (its HSS test matrix is too ill-conditioned for the error against
+// x_true to mean anything).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
-#include <random>
-#include <vector>
+#include <cstring>
+#include <string>
 
-#include "hmat_amd.hpp"
+#include "hmat_amd_dropin.hpp"
 
-struct Dense {
-  int r = 0, c = 0;
-  std::vector<double> v;
-  Dense() = default;
-  Dense(int rows, int cols) : r(rows), c(cols), v((size_t)rows * cols, 0.0) {}
-  int rows() const { return r; }
-  int cols() const { return c; }
-  double* data() { return v.data(); }
-  const double* data() const { return v.data(); }
-  double& operator()(int i, int j) { return v[(size_t)j * r + i]; }
-  double operator()(int i, int j) const { return v[(size_t)j * r + i]; }
+namespace {
+
+struct Stopwatch {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double lap() {
+    const auto t1 = std::chrono::steady_clock::now();
+    const double s = std::chrono::duration<double>(t1 - t0).count();
+    t0 = t1;
+    return s;
+  }
 };
 
-// The drivers' test matrix: 1/(x_i + x_j + 1) on the grid x_i = i/(N-1),
-// diagonal shifted by 1 (kernel.h:18-48, 92-106).
-static Dense cauchy_matrix(int n) {
-  Dense a(n, n);
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) a(i, j) = 1.0 / (i / double(n - 1) + j / double(n - 1) + 1.0) + (i == j ? 1.0 : 0.0);
-  return a;
-}
-
-static double rel_error(const Dense& ref, const Dense& res) {
+// Frobenius relative error, as calc_rel_error (main_hodlr.cpp:26-35)
+double rel_error(const Matrix& ref, const Matrix& res) {
   double num = 0.0, den = 0.0;
-  for (size_t i = 0; i < ref.v.size(); ++i) {
-    const double d = ref.v[i] - res.v[i];
+  for (size_t i = 0; i < ref.data_.size(); ++i) {
+    const double d = ref.data_[i] - res.data_[i];
     num += d * d;
-    den += ref.v[i] * ref.v[i];
+    den += ref.data_[i] * ref.data_[i];
   }
-  return den == 0.0 ? 0.0 : std::sqrt(num / den);
+  return den == 0.0 ? 0.0 : std::sqrt(num) / std::sqrt(den);
 }
 
-static Dense gemv(const Dense& a, const Dense& x) {
-  Dense y(a.rows(), 1);
-  for (int j = 0; j < a.cols(); ++j)
-    for (int i = 0; i < a.rows(); ++i) y.v[i] += a(i, j) * x.v[j];
-  return y;
-}
-
-template <class H>
-static void run(const char* name, H& h, const Dense& A, int leaf, const Dense& x_true, const Dense& b) {
-  using clk = std::chrono::steady_clock;
-  auto t0 = clk::now();
+template <class Solver>
+void run(const char* title, Solver& h, const Matrix& A, int leaf, const Matrix& x_true, const Matrix& b,
+         double* t_build, double* t_solve) {
+  std::printf("\n--- %s ---\n", title);
+  Stopwatch sw;
   h.build_from_dense(A, leaf);
-  auto t1 = clk::now();
-  Dense y = h.multiply(x_true);
-  auto t2 = clk::now();
-  Dense x = h.solve(b);
-  auto t3 = clk::now();
-  auto s = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
-  std::printf("--- %s ---\n Build: %.4fs\n GEMV:  %.4fs | Err: %.3e\n Solve: %.4fs | Err: %.3e\n", name,
-              s(t0, t1), s(t1, t2), rel_error(b, y), s(t2, t3), rel_error(x_true, x));
+  *t_build = sw.lap();
+  std::printf(" Build: %.4fs\n", *t_build);
+  Matrix y = h.multiply(x_true);
+  const double t_mv = sw.lap();
+  std::printf(" GEMV:  %.4fs | Err: %.6e\n", t_mv, rel_error(b, y));
+  Matrix x = h.solve(b);
+  *t_solve = sw.lap();
+  std::printf(" Solve: %.4fs | Err: %.6e\n", *t_solve, rel_error(x_true, x));
+  std::printf(" Resid: %.6e\n", rel_error(b, A * x));
 }
+
+}  // namespace
 
 int main(int argc, char** argv) {
-  const int n = argc > 1 ? std::atoi(argv[1]) : 4096;
-  const int leaf = argc > 2 ? std::atoi(argv[2]) : 512;
-  const double tol = argc > 3 ? std::atof(argv[3]) : 1e-8;
+  const std::string kind = argc > 1 ? argv[1] : "hodlr";
+  const bool hss = kind == "hss";
+  if (!hss && kind != "hodlr") {
+    std::fprintf(stderr, "usage: drop_in_driver hodlr|hss [N leaf tol]\n");
+    return 2;
+  }
+  const int N = argc > 2 ? std::atoi(argv[2]) : (hss ? 2048 : 4096);
+  const int leaf = argc > 3 ? std::atoi(argv[3]) : (hss ? 256 : 512);
+  const double tol = argc > 4 ? std::atof(argv[4]) : (hss ? 1e-9 : 1e-8);
   if (hmat_device_count() < 1) {
     std::fprintf(stderr, "no HIP device\n");
     return 2;
   }
-  std::printf("Matrix: %dx%d (Cauchy), leaf %d, tol %g\n", n, n, leaf, tol);
-  const Dense A = cauchy_matrix(n);
-  Dense x_true(n, 1);
-  std::mt19937 gen(12345);
-  std::uniform_real_distribution<double> u(0.0, 1.0);
-  for (auto& v : x_true.v) v = u(gen);
-  const Dense b = gemv(A, x_true);
+  std::printf("%s benchmark on the MI355X engine: N=%d leaf=%d tol=%g\n", hss ? "HSS" : "HODLR", N, leaf, tol);
+  Matrix A = hss ? Kernels::generate_gaussian(N, 0.5, 1e-6) : Kernels::generate_matrix(Kernels::CAUCHY, N);
+  Matrix x_true = Matrix::Random(N, 1);
+  Matrix b = A * x_true;
+
+  // the dense baseline (main_hodlr.cpp:53-65), on the host
+  Stopwatch sw;
+  Matrix b_dense = A * x_true;
+  const double t_dense_mv = sw.lap();
+  Matrix A_dense = A;
+  Matrix x_dense = b;
+  A_dense.solve_lu(x_dense);
+  const double t_dense_solve = sw.lap();
+  std::printf("\n--- Dense baseline (host) ---\n GEMV: %.4fs | Solve: %.4fs | Err: %.6e\n", t_dense_mv, t_dense_solve,
+              rel_error(x_true, x_dense));
+  double tb_s = 0, ts_s = 0, tb_p = 0, ts_p = 0;
   try {
-    hmat_amd::HODLRMatrixOMPT<Dense> hodlr(tol);
-    run("HODLR (MI355X)", hodlr, A, leaf, x_true, b);
-    hmat_amd::HSSMatrixOMPT<Dense> hss(tol);
-    run("HSS (MI355X, reference operator)", hss, A, leaf, x_true, b);
+    if (hss) {
+      HSSMatrix serial(tol);
+      run("HSSMatrix (MI355X)", serial, A, leaf, x_true, b, &tb_s, &ts_s);
+      HSSMatrixOMP omp(tol);
+      run("HSSMatrixOMP (MI355X)", omp, A, leaf, x_true, b, &tb_p, &ts_p);
+    } else {
+      HODLRMatrix serial(tol);
+      run("HODLRMatrix (MI355X)", serial, A, leaf, x_true, b, &tb_s, &ts_s);
+      HODLRMatrixOMP omp(tol);
+      run("HODLRMatrixOMP (MI355X)", omp, A, leaf, x_true, b, &tb_p, &ts_p);
+    }
   } catch (const hmat_amd::Error& e) {
     std::fprintf(stderr, "hmat_amd error %d: %s\n", (int)e.status, e.what());
     return 1;
   }
+  std::printf("\n>>> Speedup <<<\nSolve vs dense (host LU): %.2fx\n", t_dense_solve / ts_p);
   return 0;
 }

@@ -3,6 +3,9 @@
 //
 //   HODLRMatrixOMPT<M>  replaces HODLRMatrixOMP (include/hodlr_routines_omp.h:9-241)
 //   HSSMatrixOMPT<M>    replaces HSSMatrixOMP   (include/hss_routines_omp.h:11-375)
+//   HODLRMatrixT<M>, HSSMatrixT<M>  replace the serial HODLRMatrix / HSSMatrix
+// (include/hmat_amd_dropin.hpp adds the reference's Matrix, Kernels:: and
+// the four class names, for drivers written against the reference)
 //
 // M is the caller's dense matrix type.  It needs what the reference's Matrix
 // (include/matrix.h:28-150) provides: M(rows, cols) zero-initialised,
@@ -145,6 +148,26 @@ class HSSMatrixOMPT : public HandleBase<hmat_hss, HssOps> {
 
  private:
   bool nested_;
+};
+
+// The serial classes of the reference compute the same algebra as the
+// OpenMP ones (the engine is one GPU either way):
+//   HODLRMatrix  include/hodlr_routines.h:8-176 (solve takes every column)
+//   HSSMatrix    include/hss_routines.h:10-307  (solve takes every column,
+//                solve_multi_rhs, hss_routines.h:197-199 and 281-290)
+template <class M>
+using HODLRMatrixT = HODLRMatrixOMPT<M>;
+
+template <class M>
+class HSSMatrixT : public HSSMatrixOMPT<M> {
+ public:
+  explicit HSSMatrixT(double tol = 1e-9, bool nested = false) : HSSMatrixOMPT<M>(tol, nested) {}
+  // hss_routines.h:197-199 (every column of b)
+  M solve(const M& b) {
+    M x(this->N, b.cols());
+    check(hmat_hss_solve(this->h_, b.data(), b.rows(), b.cols(), x.data(), this->N));
+    return x;
+  }
 };
 
 }  // namespace hmat_amd

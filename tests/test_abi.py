@@ -74,12 +74,12 @@ def test_bad_options_rejected():
 
 
 def test_drop_in_driver_built_and_refuses_without_device():
-    """The C++ drop-in example (include/hmat_amd.hpp) is built by build() and
+    """The C++ drop-in example (include/hmat_amd_dropin.hpp) is built by build() and
     reports a missing device instead of falling back to anything."""
     import subprocess
     exe = os.path.join(ROOT, "hmat-omp_amd", "bin", "drop_in_driver")
     assert os.path.exists(exe)
     if hmat_amd.load().hmat_device_count() > 0:
         pytest.skip("a GPU is visible")
-    r = subprocess.run([exe, "256", "64"], capture_output=True, text=True, timeout=60)
+    r = subprocess.run([exe, "hodlr", "256", "64"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "no HIP device" in r.stderr
