@@ -340,6 +340,15 @@ hmat_status hmat_hodlr_set_solve_graph(hmat_hodlr h, int32_t on) {
   });
 }
 
+hmat_status hmat_hodlr_set_fused_solve(hmat_hodlr h, int32_t on) {
+  return guard([&] {
+    HMAT_REQUIRE(h, HMAT_ERR_ARG, "NULL handle");
+    h->eng.fused_solve = on != 0;
+  });
+}
+
+void hmat_set_fused_spin_limit(int64_t n) { kern::apply_fused_set_spin_limit(n); }
+
 hmat_status hmat_hodlr_set_group_bytes(hmat_hodlr h, double bytes) {
   return guard([&] {
     HMAT_REQUIRE(h && bytes >= 0.0, HMAT_ERR_ARG, "bad group scratch budget");

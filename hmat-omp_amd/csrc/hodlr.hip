@@ -691,11 +691,115 @@ void HodlrEngine::factor_t(HFactor<F>& fa) {
   }
   check_info();
   fa.ready = true;
+  plan_fused<F>(fa);
+}
+
+// Descriptors of the fused apply-inverse: per leaf its rows, per (leaf,
+// parent depth) the parent and the column counts of V and Q at the node
+// holding the leaf, per parent its leaf range and capacitance block.
+template <class F>
+void HodlrEngine::plan_fused(HFactor<F>& fa) {
+  kern::ApPlan<F>& ap = fa.ap;
+  ap.ok = false;
+  const int L = tree.max_depth;
+  const int nl = (int)tree.leaves.size();
+  std::vector<kern::ApLeaf> lf(nl);
+  std::vector<int64_t> starts(nl);
+  for (int i = 0; i < nl; ++i) {
+    const TNode& nd = tree.nodes[tree.leaves[i]];
+    lf[i] = kern::ApLeaf{nd.start, (int32_t)nd.len, 0};
+    starts[i] = nd.start;
+  }
+  auto leaf_range = [&](const TNode& nd, int& l0, int& l1) {
+    l0 = (int)(std::lower_bound(starts.begin(), starts.end(), nd.start) - starts.begin());
+    l1 = (int)(std::lower_bound(starts.begin(), starts.end(), nd.start + nd.len) - starts.begin());
+  };
+  std::vector<kern::ApLevel<F>> levels(std::max(L, 1));
+  std::vector<kern::ApEntry> ent((size_t)nl * std::max(L, 1), kern::ApEntry{-1, 0, 0, 0});
+  std::vector<kern::ApParent> pars;
+  int KM = 1, MM = 1;
+  for (int j = 0; j < L; ++j) {
+    const HParents& P = par[j];
+    const HLevel& C = lv[j + 1];
+    levels[j].V = fa.V[j + 1];
+    levels[j].Q = fa.Wall.get() + colofs[j + 1] * N;
+    levels[j].Minv = fa.Minv[j].get();
+    if constexpr (kRefineBlocks<F>)
+      levels[j].M = fa.Mmat[j].get();
+    else
+      levels[j].M = nullptr;
+    levels[j].mm = P.mmax;
+    KM = std::max(KM, C.rmax);
+    MM = std::max(MM, P.mmax);
+    for (size_t q = 0; q < P.ids.size(); ++q) {
+      const int pg = (int)pars.size();
+      const TNode& pn = tree.nodes[P.ids[q]];
+      const TNode& l = tree.nodes[C.ids[P.lpos[q]]];
+      const TNode& r = tree.nodes[C.ids[P.rpos[q]]];
+      const int rl = C.rank[P.lpos[q]], rr = C.rank[P.rpos[q]];
+      kern::ApParent ap_{(int64_t)q * P.mmax * P.mmax, 0, 0, 0, rr, rl, j, 0};
+      int a0, a1, b0, b1;
+      leaf_range(pn, a0, a1);
+      leaf_range(l, b0, b1);
+      ap_.leaf0 = a0;
+      ap_.lsplit = b1;
+      ap_.leaf1 = a1;
+      pars.push_back(ap_);
+      for (int i = b0; i < b1; ++i) ent[(size_t)i * L + j] = kern::ApEntry{pg, rr, rl, rr};
+      leaf_range(r, b0, b1);
+      for (int i = b0; i < b1; ++i) ent[(size_t)i * L + j] = kern::ApEntry{pg, rl, rr, 0};
+    }
+  }
+  ap.nleaf = nl;
+  ap.L = L;
+  ap.KM = KM;
+  ap.MM = MM;
+  ap.npar = (int)pars.size();
+  ap.lm = tree.lmax;
+  ap.leaves.upload(lf, stream);
+  ap.ent.upload(ent, stream);
+  ap.lv.upload(levels, stream);
+  if (!pars.empty()) ap.par.upload(pars, stream);
+  ap.part.reserve((size_t)nl * KM * 4 * sizeof(F));
+  ap.coef.reserve((size_t)std::max(ap.npar, 1) * MM * 4 * sizeof(F));
+  ap.sync.reserve(((size_t)2 * ap.npar + 1) * sizeof(int));
+  HIP_CHECK(hipMemsetAsync(ap.sync.template as<int>(), 0, ((size_t)2 * ap.npar + 1) * sizeof(int), stream));
+  ap.ok = true;
+}
+
+template <class F>
+bool HodlrEngine::fused_ready(const HFactor<F>& fa) const {
+  return fused_solve && fa.ap.ok && kern::apply_fused_grid<F>(fa.ap, kRefineBlocks<F>, 4) > 0;
+}
+
+void HodlrEngine::check_fused_active() {
+  if (opt.precision == HMAT_PREC_MIXED)
+    check_fused<float>(f32_);
+  else
+    check_fused<double>(f64_);
+}
+
+template <class F>
+void HodlrEngine::check_fused(HFactor<F>& fa) {
+  if (!fa.ap.ok) return;
+  int err = 0;
+  HIP_CHECK(hipMemcpyAsync(&err, fa.ap.sync.template as<int>() + 2 * fa.ap.npar, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+  if (err) {
+    HIP_CHECK(hipMemsetAsync(fa.ap.sync.template as<int>() + 2 * fa.ap.npar, 0, sizeof(int), stream));
+    throw Error(HMAT_ERR_DEVICE, "fused solve: a wait of the persistent kernel exceeded its spin limit");
+  }
 }
 
 template <class F>
 void HodlrEngine::apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int ncols, F* X,
                                   int64_t ldx) {
+  if (fused_ready(fa)) {
+    const F* Dref = nullptr;
+    if constexpr (kRefineBlocks<F>) Dref = D.get();
+    if (kern::apply_inverse_fused<F>(fa.ap, fa.Dinv.get(), Dref, B, ldb, ncols, X, ldx, N, stream)) return;
+    fa.ap.ok = false;  // refused by the runtime: the multi-launch path from now on
+  }
   if constexpr (kRefineBlocks<F>) fa.rtmp.reserve((size_t)N * ncols);
   leaf_solve_t<F>(fa, B, ldb, ncols, X, ldx, fa.rtmp.get());
   for (int j = tree.max_depth - 1; j >= 0; --j) correction_t<F>(fa, j, X, ldx, ncols);
@@ -742,7 +846,8 @@ void HodlrEngine::apply_inverse(const double* B, int64_t ldb, int ncols, double*
 
 void HodlrEngine::apply_inverse_impl(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx) {
   if (opt.precision != HMAT_PREC_MIXED) {
-    if (!graph_turn(f64_.plan, solve_graph, ncols)) {
+    // the fused apply is one launch per four columns: no graph to record
+    if (!graph_turn(f64_.plan, solve_graph && !fused_ready(f64_), ncols)) {
       apply_inverse_t<double>(f64_, B, ldb, ncols, X, ldx);
       return;
     }
@@ -752,7 +857,7 @@ void HodlrEngine::apply_inverse_impl(const double* B, int64_t ldb, int ncols, do
     kern::convert_cols<double, double>(N, ncols, f64_.x.get(), N, X, ldx, stream);
     return;
   }
-  const bool graph = graph_turn(f32_.plan, solve_graph, ncols);
+  const bool graph = graph_turn(f32_.plan, solve_graph && !fused_ready(f32_), ncols);
   if (graph) plan_solve<float>(f32_, ncols);
   f32_.b.reserve((size_t)N * ncols);
   f32_.x.reserve((size_t)N * ncols);
@@ -882,6 +987,7 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
   last_resid = -1.0;
   if (!(refine_rtol > 0.0)) {
     HIP_CHECK(hipStreamSynchronize(stream));
+    check_fused_active();
     return;
   }
   ir_y_.reserve((size_t)N * nrhs);
@@ -929,6 +1035,7 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
     kern::axpby_cols(N, nrhs, 1.0, dX, ldx, 1.0, ir_y_.get(), N, dX, ldx, stream);
   }
   HIP_CHECK(hipStreamSynchronize(stream));
+  check_fused_active();
   last_iters = it;
   last_resid = resid;
   // the target was asked for (by default in mixed precision, opt-in in FP64)

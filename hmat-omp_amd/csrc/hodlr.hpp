@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "apply.hpp"
 #include "engine.hpp"
 #include "matvec.hpp"
 
@@ -152,11 +153,13 @@ struct HFactor {
   DevBuf<F> tvec, coef, wtmp, b, x;
   DevBuf<F> rtmp;                 // leaf-solve residual (N x columns)
   SolvePlan<F> plan;
+  kern::ApPlan<F> ap;             // the fused apply-inverse (one persistent launch)
   bool ready = false;
   // invalidate; the device arrays are kept for the next factorisation
   void clear() {
     plan.clear();
     plan.uses.clear();
+    ap.clear();
     V.clear();
     ready = false;
   }
@@ -216,6 +219,7 @@ class HodlrEngine {
   double refine_rtol = 0.0;
   int refine_max = 30;
   bool solve_graph = true;  // replay the apply-inverse from a captured HIP graph
+  bool fused_solve = true;  // one persistent launch per apply-inverse where the device holds its grid
   double group_bytes = kGroupScratchBytes;  // scratch budget of one group of depths
   int last_iters = 0;
   double last_resid = -1.0;
@@ -238,6 +242,12 @@ class HodlrEngine {
                                        F* R);
   // graph-captured apply-inverse (fa.b -> fa.x, N x ncols, ld N)
   template <class F> void plan_solve(HFactor<F>& fa, int ncols);
+  // descriptors of the fused apply-inverse (after a factorisation)
+  template <class F> void plan_fused(HFactor<F>& fa);
+  template <class F> bool fused_ready(const HFactor<F>& fa) const;
+  // raise the fused kernel's error flag as an exception (after a sync)
+  template <class F> void check_fused(HFactor<F>& fa);
+  void check_fused_active();
   template <class F> void run_solve_graph(HFactor<F>& fa, int ncols);
   void apply_inverse(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx);
   void apply_inverse_impl(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx);

@@ -90,6 +90,8 @@ def _proto(lib):
         "hmat_hodlr_set_stream": (st, [vp, vp]),
         "hmat_hodlr_save": (st, [vp, C.c_char_p]),
         "hmat_hodlr_set_solve_graph": (st, [vp, i32]),
+        "hmat_hodlr_set_fused_solve": (st, [vp, i32]),
+        "hmat_set_fused_spin_limit": (None, [i64]),
         "hmat_hodlr_set_group_bytes": (st, [vp, C.c_double]),
         "hmat_hodlr_load": (st, [vp, C.c_char_p]),
         "hmat_hss_create": (st, [C.POINTER(vp), C.POINTER(Options)]),
@@ -179,6 +181,11 @@ def autotune_enable(on=True):
 
 def autotune_entries():
     return load().hmat_autotune_entries()
+
+
+def set_fused_spin_limit(n):
+    """Spin limit of the persistent apply-inverse's waits (0: default)."""
+    load().hmat_set_fused_spin_limit(int(n))
 
 
 def profile_enable(on=True):
@@ -288,6 +295,10 @@ class HODLRMatrixOMP(_Base):
     def set_solve_graph(self, on):
         """Replay solves from a captured HIP graph (default) or launch eagerly."""
         _check(self._lib.hmat_hodlr_set_solve_graph(self._h, int(bool(on))))
+
+    def set_fused_solve(self, on):
+        """Apply-inverse as one persistent launch (default) or as the multi-launch sequence."""
+        _check(self._lib.hmat_hodlr_set_fused_solve(self._h, int(bool(on))))
 
     def set_group_bytes(self, nbytes):
         """Scratch budget of one group of depths built together (0: default)."""

@@ -261,6 +261,18 @@ hmat_status hmat_hodlr_load(hmat_hodlr h, const char* path);
  * factorisation and column count (default on); off launches the same
  * kernels one by one.  No reference counterpart. */
 hmat_status hmat_hodlr_set_solve_graph(hmat_hodlr h, int32_t on);
+/* Apply-inverse as one persistent launch per four columns (default on):
+ * every workgroup owns leaves, solves their leaf blocks, then meets the
+ * other leaves of each parent through device-memory counters, deepest
+ * depth first; it is launched cooperatively (all workgroups resident) and
+ * every wait is bounded.  Off: the multi-launch apply (leaf GEMM, then
+ * three launches per depth, graph-recorded).  The two agree to rounding
+ * (different summation order).  No reference counterpart. */
+hmat_status hmat_hodlr_set_fused_solve(hmat_hodlr h, int32_t on);
+/* Diagnostic: spin limit of every wait of the persistent apply-inverse
+ * (process-wide; 0 restores the default, 2^22).  A wait that exceeds it
+ * ends its workgroup and the solve returns HMAT_ERR_DEVICE. */
+void hmat_set_fused_spin_limit(int64_t n);
 /* Scratch budget in bytes of one group of tree depths built together (the
  * build stacks a group's sketch bases: three N x k arrays per depth; every
  * launch covers all blocks of the group).  Groups hold at least one depth;
