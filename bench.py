@@ -136,6 +136,86 @@ def run_mode(hmat_amd, torch, mode, a, q, steps, warmup, dev):
     return out, h
 
 
+def batch_leg(hmat_amd, torch, steps, warmup, profile=True):
+    """The batched workload: 256 independent Gaussian-kernel systems,
+    A_s = exp(-(p_i - p_j)^2 / (2 sigma_s^2)) + 1e-2 delta_ij, N_s
+    log-uniform in [1024, 16384], sigma_s ~ U[0.05, 0.3], sorted U[0,1]
+    points (seeded), leaf 64, fixed rank 24, p 10, q 1, RHS counts uniform in
+    [1, 64] (zero-padded to 64 columns).  One step = build of the forest +
+    factor + FP64 solve with refinement to 1e-11 per column; the worst
+    per-system, per-column residual ||b - A_H x|| / ||b|| is reported."""
+    rng = np.random.default_rng(2024)
+    nsys, lam = 256, 1e-2
+    sizes = np.exp(rng.uniform(np.log(1024), np.log(16384), nsys)).astype(np.int64)
+    sigmas = rng.uniform(0.05, 0.3, nsys)
+    nrhs = rng.integers(1, 65, nsys)
+    pts = [np.sort(rng.random(int(s))) for s in sizes]
+    n = int(sizes.sum())
+    cols = int(nrhs.max())
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    bh = np.zeros((cols, n))  # column-major n x cols
+    for s in range(nsys):
+        bh[:nrhs[s], offs[s]:offs[s + 1]] = rng.standard_normal((nrhs[s], sizes[s]))
+    dp = torch.tensor(np.concatenate([p / sg for p, sg in zip(pts, sigmas)]), device="cuda")
+    db = torch.tensor(bh, device="cuda")
+    dx = torch.zeros_like(db)
+    o = hmat_amd.make_options(0.0, rank=24, fixed_rank=True, oversample=10, power_iters=1, exact_below=0,
+                              omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=99)
+    h = hmat_amd.HODLRMatrixOMP(opts=o)
+    h.set_refinement(1e-11, 10)
+    k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_GAUSSIAN, dp.data_ptr(), p0=1.0, p1=lam)
+    szl = [int(s) for s in sizes]
+
+    def step():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h.build_kernel_batch_device(k, szl, 64)
+        t1 = time.perf_counter()
+        h.factor()
+        t2 = time.perf_counter()
+        h.solve_device(db.data_ptr(), n, cols, dx.data_ptr(), n)
+        torch.cuda.synchronize()
+        return t0, t1, t2, time.perf_counter()
+
+    times, parts = [], []
+    for i in range(warmup + steps):
+        t0, t1, t2, t3 = step()
+        log(f"[batch] step {i} ({'timed' if i >= warmup else 'warmup'}): build {t1 - t0:.3f}s "
+            f"factor {t2 - t1:.3f}s solve {t3 - t2:.3f}s total {t3 - t0:.3f}s "
+            f"refinement {h.last_refinement()}")
+        if i >= warmup:
+            times.append(t3 - t0)
+            parts.append((t1 - t0, t2 - t1, t3 - t2))
+    dy = torch.empty_like(db)
+    h.multiply_device(dx.data_ptr(), n, cols, dy.data_ptr(), n)
+    torch.cuda.synchronize()
+    r = (db - dy).cpu().numpy()
+    worst = 0.0
+    for s in range(nsys):
+        sl = slice(offs[s], offs[s + 1])
+        for j in range(int(nrhs[s])):
+            worst = max(worst, float(np.linalg.norm(r[j, sl]) / np.linalg.norm(bh[j, sl])))
+    out = {"workload": "256 Gaussian HODLR systems, N log-uniform [1024,16384], sigma U[0.05,0.3], "
+                       "lambda 1e-2, leaf 64, rank 24, p 10, q 1, RHS U[1,64] (64 columns), FP64 + IR 1e-11",
+           "sum_N": n, "rhs_total": int(nrhs.sum()), "columns": cols,
+           "s_per_step": round(statistics.median(times), 5),
+           "build_factor_solve_s": [round(statistics.median(p[i] for p in parts), 5) for i in range(3)],
+           "refine_iters": h.last_refinement()[0], "worst_system_resid": worst}
+    if profile:
+        hmat_amd.profile_reset()
+        hmat_amd.profile_enable(True)
+        step()
+        hmat_amd.profile_enable(False)
+        prof = {}
+        for name in ("sketch_gen", "orth_gram", "orth_trsm", "solve_apply", "factor"):
+            pq = hmat_amd.profile_query(name)
+            if pq:
+                prof[name] = {"launches": pq[0], "ms": round(pq[1], 3), "work": pq[2]}
+        out["kernels"] = prof
+    del h
+    return out
+
+
 def true_residual(hmat_amd, torch, x, dev, n):
     """||b - A x|| / ||b|| with the exact generated A (FP64, never stored)."""
     k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_CAUCHY, dev["x"].data_ptr(), dev["y"].data_ptr())
@@ -265,6 +345,8 @@ def main():
                     help="skip the kernel-timed step (rooflines)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tilings (the autotuner runs in the warm-up step otherwise)")
+    ap.add_argument("--batch-steps", type=int, default=3,
+                    help="timed steps of the batched leg (256 systems); 0 skips it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (counter calibration)")
     a = ap.parse_args()
@@ -406,6 +488,8 @@ def main():
         probe = hbm_probe(torch)
         out["hbm_probe"] = probe
         out["matvec_roofline"]["frac_of_probe_read"] = round(gbs / probe["read_GBs"], 4)
+    if a.batch_steps > 0:
+        out["batch"] = batch_leg(hmat_amd, torch, a.batch_steps, 1, a.profile)
     if not a.no_cpu:
         cb = cpu_baseline(a)
         if cb:

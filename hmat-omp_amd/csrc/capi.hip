@@ -340,10 +340,11 @@ hmat_status hmat_hodlr_set_solve_graph(hmat_hodlr h, int32_t on) {
   });
 }
 
-hmat_status hmat_hodlr_set_fused_solve(hmat_hodlr h, int32_t on) {
+hmat_status hmat_hodlr_set_fused_solve(hmat_hodlr h, int32_t mode) {
   return guard([&] {
     HMAT_REQUIRE(h, HMAT_ERR_ARG, "NULL handle");
-    h->eng.fused_solve = on != 0;
+    HMAT_REQUIRE(mode >= 0 && mode <= 2, HMAT_ERR_ARG, "fused solve mode must be 0, 1 or 2");
+    h->eng.fused_solve = mode;
   });
 }
 

@@ -269,8 +269,10 @@ void Orthogonalizer::make_plan(const std::vector<OrthSeg>& segs, int kcap, bool 
   std::vector<SegNN> nn(nb), acc;
   std::vector<double> coef(nb), zero(nb, 0.0);
   plan_.nmax = 0;
+  plan_.panel = 0;
   for (size_t b = 0; b < nb; ++b) {
     const auto& g = segs[b];
+    plan_.panel += (double)g.len * g.k;
     tn[b] = SegTN{g.off, g.off, (int64_t)b * kk, g.len, g.k, g.k, 0};
     sm[b] = SmallMat{(int64_t)b * kk, g.k, kcap};
     // shift 11 (m n + n (n+1)) u ||X||^2 with ||X||_F^2 = trace(G) >= ||X||_2^2
@@ -311,13 +313,19 @@ void Orthogonalizer::make_plan(const std::vector<OrthSeg>& segs, int kcap, bool 
   ws_.part.reserve((size_t)std::max(plan_.nslots, 1) * 4096);
 }
 
+// Timed as "orth_gram" / "orth_trsm" when profiling, with their algorithmic
+// HBM bytes: the panel read once (Gram), read and written once (TRSM).
 void Orthogonalizer::gram(const double* Y, int64_t ld, int kcap, hipStream_t s) {
+  ProfScope prof("orth_gram", s, 8.0 * plan_.panel);
   kern::launch_tn_raw<double>(plan_.nitems, plan_.nouts, plan_.tn_segs, plan_.tn_items, plan_.tn_outs, Y, ld, Y,
                               ld, G_.get(), kcap, 1.0, 0.0, ws_.part.get(), s);
+  prof.finish();
 }
 
 void Orthogonalizer::trsm(const double* X, int64_t ld, const double* R, int kcap, double* Y, hipStream_t s) {
+  ProfScope prof("orth_trsm", s, 16.0 * plan_.panel);
   kern::launch_trsm_raw(plan_.kmax, plan_.tr_ntiles, plan_.tr_segs, plan_.tr_tiles, X, ld, R, kcap, Y, ld, s);
+  prof.finish();
 }
 
 // Racc <- R_ * Racc (upper triangular products)

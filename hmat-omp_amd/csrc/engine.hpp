@@ -156,6 +156,7 @@ class Orthogonalizer {
   // the launches of one call, planned once: Gram products, Cholesky
   // descriptors and shifts, triangular solves, R products
   struct Plan {
+    double panel = 0;      // sum of len * k over the segments (elements of the N x k panel)
     DevBuf<uint8_t> blob;  // every descriptor array below, uploaded with one copy
     const SegTN* tn_segs = nullptr;
     const kern::TNItem* tn_items = nullptr;

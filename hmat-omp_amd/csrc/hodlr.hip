@@ -767,9 +767,22 @@ void HodlrEngine::plan_fused(HFactor<F>& fa) {
   ap.ok = true;
 }
 
+// The fused apply takes four columns per launch, and each workgroup walks
+// its leaves one after another, each behind a barrier and an arrival
+// counter.  It wins for a few columns over a few leaves per workgroup (one
+// column, 1024 leaves of 256 rows: 0.90 ms against 2.17 ms); with many
+// columns or many small leaves the wide launches of the multi-launch path
+// win (64 columns over a forest of 20 480 leaves of 64 rows: 111 ms fused).
+constexpr int kFusedMaxCols = 4;
+constexpr int kFusedLeavesPerGroup = 4;
+
 template <class F>
-bool HodlrEngine::fused_ready(const HFactor<F>& fa) const {
-  return fused_solve && fa.ap.ok && kern::apply_fused_grid<F>(fa.ap, kRefineBlocks<F>, 4) > 0;
+bool HodlrEngine::fused_ready(const HFactor<F>& fa, int ncols) const {
+  if (fused_solve == 0 || !fa.ap.ok) return false;
+  const int grid = kern::apply_fused_grid<F>(fa.ap, kRefineBlocks<F>, 4);
+  if (grid <= 0) return false;
+  if (fused_solve == 2) return true;
+  return ncols <= kFusedMaxCols && (int64_t)fa.ap.nleaf <= (int64_t)kFusedLeavesPerGroup * grid;
 }
 
 void HodlrEngine::check_fused_active() {
@@ -794,7 +807,7 @@ void HodlrEngine::check_fused(HFactor<F>& fa) {
 template <class F>
 void HodlrEngine::apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int ncols, F* X,
                                   int64_t ldx) {
-  if (fused_ready(fa)) {
+  if (fused_ready(fa, ncols)) {
     const F* Dref = nullptr;
     if constexpr (kRefineBlocks<F>) Dref = D.get();
     if (kern::apply_inverse_fused<F>(fa.ap, fa.Dinv.get(), Dref, B, ldb, ncols, X, ldx, N, stream)) return;
@@ -807,6 +820,7 @@ void HodlrEngine::apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int n
 
 void HodlrEngine::factor() {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "factor before build");
+  ProfScope prof("factor", stream, 0.0);
   if (opt.precision == HMAT_PREC_MIXED) {
     f64_.clear();
     factor_t<float>(f32_);
@@ -814,6 +828,7 @@ void HodlrEngine::factor() {
     f32_.clear();
     factor_t<double>(f64_);
   }
+  prof.finish();
   factored = true;
 }
 
@@ -847,7 +862,7 @@ void HodlrEngine::apply_inverse(const double* B, int64_t ldb, int ncols, double*
 void HodlrEngine::apply_inverse_impl(const double* B, int64_t ldb, int ncols, double* X, int64_t ldx) {
   if (opt.precision != HMAT_PREC_MIXED) {
     // the fused apply is one launch per four columns: no graph to record
-    if (!graph_turn(f64_.plan, solve_graph && !fused_ready(f64_), ncols)) {
+    if (!graph_turn(f64_.plan, solve_graph && !fused_ready(f64_, ncols), ncols)) {
       apply_inverse_t<double>(f64_, B, ldb, ncols, X, ldx);
       return;
     }
@@ -857,7 +872,7 @@ void HodlrEngine::apply_inverse_impl(const double* B, int64_t ldb, int ncols, do
     kern::convert_cols<double, double>(N, ncols, f64_.x.get(), N, X, ldx, stream);
     return;
   }
-  const bool graph = graph_turn(f32_.plan, solve_graph && !fused_ready(f32_), ncols);
+  const bool graph = graph_turn(f32_.plan, solve_graph && !fused_ready(f32_, ncols), ncols);
   if (graph) plan_solve<float>(f32_, ncols);
   f32_.b.reserve((size_t)N * ncols);
   f32_.x.reserve((size_t)N * ncols);

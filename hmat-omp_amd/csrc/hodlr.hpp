@@ -219,7 +219,9 @@ class HodlrEngine {
   double refine_rtol = 0.0;
   int refine_max = 30;
   bool solve_graph = true;  // replay the apply-inverse from a captured HIP graph
-  bool fused_solve = true;  // one persistent launch per apply-inverse where the device holds its grid
+  // apply-inverse as one persistent launch per four columns: 0 never,
+  // 1 where it is the faster path, 2 whenever the device holds its grid
+  int fused_solve = 1;
   double group_bytes = kGroupScratchBytes;  // scratch budget of one group of depths
   int last_iters = 0;
   double last_resid = -1.0;
@@ -244,7 +246,7 @@ class HodlrEngine {
   template <class F> void plan_solve(HFactor<F>& fa, int ncols);
   // descriptors of the fused apply-inverse (after a factorisation)
   template <class F> void plan_fused(HFactor<F>& fa);
-  template <class F> bool fused_ready(const HFactor<F>& fa) const;
+  template <class F> bool fused_ready(const HFactor<F>& fa, int ncols) const;
   // raise the fused kernel's error flag as an exception (after a sync)
   template <class F> void check_fused(HFactor<F>& fa);
   void check_fused_active();

@@ -197,7 +197,10 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
                                                     const __bf16* __restrict__ Xs, int64_t ldx,
                                                     int64_t ps, double* __restrict__ Y, int64_t ldy,
                                                     int64_t split_stride) {
-  constexpr int RF = 2, NT = 4, NC = 64, KT = 64, KP = KT + 8;
+  // rows of X padded to 160 bytes: the B-fragment reads (ds_read_b128, lane
+  // l at row l % 16, byte 16 (l / 16)) hit distinct banks; with 144-byte rows
+  // they conflict (4 extra cycles per read, measured)
+  constexpr int RF = 2, NT = 4, NC = 64, KT = 64, KP = KT + 16;
   // the single-term pass keeps FP32 sums (its error is that of BF16 entries);
   // the three-term pass flushes them to FP64 every 8 K-tiles
   constexpr int FLUSH = SPLIT > 1 ? 8 : 0;

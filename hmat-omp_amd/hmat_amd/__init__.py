@@ -296,9 +296,11 @@ class HODLRMatrixOMP(_Base):
         """Replay solves from a captured HIP graph (default) or launch eagerly."""
         _check(self._lib.hmat_hodlr_set_solve_graph(self._h, int(bool(on))))
 
-    def set_fused_solve(self, on):
-        """Apply-inverse as one persistent launch (default) or as the multi-launch sequence."""
-        _check(self._lib.hmat_hodlr_set_fused_solve(self._h, int(bool(on))))
+    def set_fused_solve(self, mode):
+        """Apply-inverse as one persistent launch per four columns: 1 (or True,
+        the default) where that is the faster path, 0 (False) never, 2 whenever
+        the device holds its grid."""
+        _check(self._lib.hmat_hodlr_set_fused_solve(self._h, int(mode)))
 
     def set_group_bytes(self, nbytes):
         """Scratch budget of one group of depths built together (0: default)."""

@@ -261,14 +261,17 @@ hmat_status hmat_hodlr_load(hmat_hodlr h, const char* path);
  * factorisation and column count (default on); off launches the same
  * kernels one by one.  No reference counterpart. */
 hmat_status hmat_hodlr_set_solve_graph(hmat_hodlr h, int32_t on);
-/* Apply-inverse as one persistent launch per four columns (default on):
- * every workgroup owns leaves, solves their leaf blocks, then meets the
- * other leaves of each parent through device-memory counters, deepest
- * depth first; it is launched cooperatively (all workgroups resident) and
- * every wait is bounded.  Off: the multi-launch apply (leaf GEMM, then
- * three launches per depth, graph-recorded).  The two agree to rounding
- * (different summation order).  No reference counterpart. */
-hmat_status hmat_hodlr_set_fused_solve(hmat_hodlr h, int32_t on);
+/* Apply-inverse as one persistent launch per four columns: every
+ * workgroup owns leaves, solves their leaf blocks, then meets the other
+ * leaves of each parent through device-memory counters, deepest depth
+ * first; it is launched cooperatively (all workgroups resident) and every
+ * wait is bounded.  mode 1 (default): fused where it is the faster path (at
+ * most four columns and four leaves per resident workgroup), multi-launch
+ * otherwise; 0: always the multi-launch apply (leaf GEMM, then three
+ * launches per depth, graph-recorded); 2: fused whenever the device holds
+ * its grid (tests).  The two agree to rounding (different summation
+ * order).  No reference counterpart. */
+hmat_status hmat_hodlr_set_fused_solve(hmat_hodlr h, int32_t mode);
 /* Diagnostic: spin limit of every wait of the persistent apply-inverse
  * (process-wide; 0 restores the default, 2^22).  A wait that exceeds it
  * ends its workgroup and the solve returns HMAT_ERR_DEVICE. */

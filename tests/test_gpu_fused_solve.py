@@ -47,7 +47,7 @@ def test_fused_matches_multilaunch(n, leaf, mixed, kind):
         b = rng.standard_normal((n, ncols))
         h.set_fused_solve(False)
         xm = h.solve(b)
-        h.set_fused_solve(True)
+        h.set_fused_solve(2)  # fused for every column count (four per launch)
         xf = h.solve(b)
         xf2 = h.solve(b)
         e = rel(xf, xm)
@@ -59,7 +59,7 @@ def test_fused_matches_multilaunch(n, leaf, mixed, kind):
     b = rng.standard_normal((n, 2))
     h.set_fused_solve(False)
     tm = h.solve_transpose(b)
-    h.set_fused_solve(True)
+    h.set_fused_solve(2)
     assert rel(h.solve_transpose(b), tm) < bar
 
 
@@ -86,3 +86,22 @@ def test_fused_wait_limit_reports_error():
     finally:
         hmat_amd.set_fused_spin_limit(0)
     assert np.array_equal(h.solve(b), x0)
+
+
+def test_default_mode_dispatch():
+    """Mode 1 (default) takes the fused kernel for a few columns over a few
+    leaves per workgroup and the multi-launch apply for more than four
+    columns: bit-identical to the forced path in each case."""
+    n = 8192
+    h = _handle(n, 128, False)
+    rng = np.random.default_rng(3)
+    b1, b16 = rng.standard_normal((n, 1)), rng.standard_normal((n, 16))
+    h.set_fused_solve(2)
+    f1 = h.solve(b1)
+    h.set_fused_solve(0)
+    m16 = h.solve(b16)
+    h.set_fused_solve(1)
+    assert np.array_equal(h.solve(b1), f1)
+    assert np.array_equal(h.solve(b16), m16)
+    with pytest.raises(hmat_amd.HmatError):
+        h.set_fused_solve(3)
