@@ -2,6 +2,7 @@
 #pragma once
 #include <cstdint>
 #include <utility>
+#include <vector>
 
 #include "common.hpp"
 
@@ -33,8 +34,9 @@ struct ApParent {
   int64_t moff;   // offset of its capacitance matrix in the depth's array
   int32_t leaf0, lsplit, leaf1;
   int32_t rr, rl; // rows of the capacitance right-hand side from the left child (V_l^T x_l: rr) and right (rl)
-  int32_t level, pad_;
+  int32_t level, pad_[2];  // explicit padding: every byte of an uploaded descriptor is defined
 };
+static_assert(sizeof(ApParent) == 40, "ApParent has no implicit padding");
 
 template <class F>
 struct ApArgs {
@@ -106,6 +108,10 @@ struct ApPlan {
   int64_t lm = 0;
   bool ok = false;
   void clear() { ok = false; }
+  void addresses(std::vector<const void*>& v) const {
+    v.insert(v.end(), {leaves.get(), ent.get(), lv.get(), par.get(), part.as<void>(), coef.as<void>(),
+                       sync.as<void>()});
+  }
 };
 
 // X = A_H^{-1} B for ncols columns (groups of up to four per launch): the

@@ -319,6 +319,24 @@ hmat_status hmat_hodlr_build_kernel_batch_device(hmat_hodlr h, const hmat_kernel
   });
 }
 
+hmat_status hmat_hodlr_batch_solve_device(hmat_hodlr h, const hmat_kernel* k, const int64_t* sizes, int32_t nsys,
+                                          int32_t leaf, const double* B, int64_t ldb, int32_t ncols, double* X,
+                                          int64_t ldx, int32_t use_graph, int32_t* path) {
+  return guard([&] {
+    HMAT_REQUIRE(h && k && k->x && sizes && nsys > 0 && leaf > 0 && B && X && ncols > 0, HMAT_ERR_ARG,
+                 "bad batch_solve arguments");
+    HMAT_REQUIRE(k->type != HMAT_KERNEL_CAUCHY || k->y, HMAT_ERR_ARG, "Cauchy kernel needs y points");
+    HMAT_REQUIRE(k->type != HMAT_KERNEL_GAUSSIAN_DSIGMA, HMAT_ERR_ARG, "derivative kernel cannot be compressed");
+    std::vector<int64_t> sz(sizes, sizes + nsys);
+    for (int64_t n : sz) HMAT_REQUIRE(n > 0, HMAT_ERR_ARG, "every system size must be positive");
+    GenDesc g = gen_from_kernel(*k, k->x, k->y);
+    const int p = h->eng.batch_solve(g, sz, leaf, B, ldb, ncols, X, ldx, use_graph != 0);
+    if (path) *path = p;
+  });
+}
+
+const char* hmat_hodlr_batch_note(hmat_hodlr h) { return h ? h->eng.batch_note.c_str() : ""; }
+
 hmat_status hmat_hodlr_save(hmat_hodlr h, const char* path) {
   return guard([&] {
     HMAT_REQUIRE(h && path, HMAT_ERR_ARG, "bad save arguments");

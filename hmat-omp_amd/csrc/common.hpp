@@ -33,6 +33,38 @@ inline void hip_check(hipError_t e, const char* what, const char* file, int line
 #define HMAT_REQUIRE(cond, code, msg) \
   do { if (!(cond)) throw ::hmat::Error((code), (msg)); } while (0)
 
+// Device fills and copies as kernels (no memset or memcpy nodes where a
+// caller records the stream into a graph).
+void zero_bytes_async(void* p, size_t bytes, hipStream_t s);
+void copy_bytes_async(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// Descriptor record of a build that is replayed from a graph.  While a log
+// is attached to a stream's Stager in Persist mode, every staged region
+// gets a device region of its own that is never reused (its bytes kept in
+// the log), and an upload into a caller's buffer (DevBuf::upload) becomes a
+// device-to-device copy from such a region.  In Replay mode the same
+// sequence of staging calls returns the same regions and enqueues the same
+// copies, without host-to-device traffic, after checking that every array
+// holds the bytes recorded; a mismatch marks the log failed.  A stream
+// being recorded into a graph thus reads only device memory that stays
+// valid and unchanged for every replay.
+struct StageLog {
+  enum Mode { Persist, Replay };
+  Mode mode = Persist;
+  std::vector<std::vector<uint8_t>> bytes;  // per region
+  std::vector<char*> dev;                   // per region
+  std::vector<void*> dst;                   // per region: copy target, or null
+  size_t next = 0;                          // replay cursor
+  bool failed = false;
+  std::vector<char*> chunks;                // device arena
+  size_t chunk_off = 0, chunk_cap = 0;
+  StageLog() = default;
+  StageLog(const StageLog&) = delete;
+  StageLog& operator=(const StageLog&) = delete;
+  ~StageLog();
+  char* alloc(size_t bytes);  // a fresh 256-byte aligned region of the arena
+};
+
 // Per-stream staging arena for the small descriptor arrays that every
 // level-batched launch reads (segments, tiles, work items): a pinned host
 // ring and a device ring of the same size.  put_bytes() copies the host data
@@ -56,9 +88,14 @@ class Stager {
   char* put_gather(const std::pair<const void*, size_t>* items, int n, size_t* off);
   // DMA `bytes` from `h` (pageable, may be freed on return) to `dst`
   void copy_to(void* dst, const void* h, size_t bytes);
+  // attach a descriptor record (null detaches)
+  void attach(StageLog* log) { log_ = log; }
+  StageLog* log() const { return log_; }
 
  private:
   size_t reserve(size_t need);  // offset of `need` free bytes
+  char* logged(const void* h, size_t bytes, void* dst);
+  StageLog* log_ = nullptr;
   hipStream_t s_;
   char* host_ = nullptr;
   char* dev_ = nullptr;
@@ -135,7 +172,7 @@ class DevBuf {
   }
   // grow-only resize (keeps the allocation when large enough)
   void reserve(size_t n) { if (n > n_) resize(n); }
-  void zero(hipStream_t s) { if (n_) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(T), s)); }
+  void zero(hipStream_t s) { if (n_) zero_bytes_async(p_, n_ * sizeof(T), s); }
   // asynchronous through the stream's staging arena: `h` may be freed on
   // return, and no host synchronisation happens unless the buffer grows
   void upload(const T* h, size_t n, hipStream_t s) {
@@ -241,6 +278,9 @@ struct TuneKey {
 };
 bool autotune_enabled();
 void autotune_enable(bool on);
+// nested suspension of the autotuner / of profiling timers on this thread
+void autotune_suspend(bool on);
+void prof_suspend(bool on);
 bool tune_lookup(const TuneKey& k, int& choice);
 void tune_store(const TuneKey& k, int choice);
 size_t tune_entries();

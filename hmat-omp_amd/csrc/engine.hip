@@ -138,7 +138,54 @@ size_t Stager::reserve(size_t need) {
   return o;
 }
 
+StageLog::~StageLog() {
+  for (char* c : chunks) (void)hipFree(c);
+}
+
+char* StageLog::alloc(size_t bytes) {
+  const size_t need = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+  if (chunks.empty() || chunk_off + need > chunk_cap) {
+    chunk_cap = std::max<size_t>((size_t)16 << 20, need);
+    char* c = nullptr;
+    HIP_CHECK(hipMalloc((void**)&c, chunk_cap));
+    chunks.push_back(c);
+    chunk_off = 0;
+  }
+  char* p = chunks.back() + chunk_off;
+  chunk_off += need;
+  return p;
+}
+
+// A staged region (dst null) or an upload into `dst`, through the attached
+// log: Persist keeps the bytes and uploads them into a region of the log's
+// arena (from the kept copy, so the source may be freed at once), Replay
+// checks them against the record; both then copy the region into `dst`.
+char* Stager::logged(const void* h, size_t bytes, void* dst) {
+  StageLog& L = *log_;
+  char* d = nullptr;
+  if (L.mode == StageLog::Persist) {
+    d = L.alloc(bytes);
+    L.bytes.emplace_back(static_cast<const uint8_t*>(h), static_cast<const uint8_t*>(h) + bytes);
+    L.dev.push_back(d);
+    L.dst.push_back(dst);
+    HIP_CHECK(hipMemcpyAsync(d, L.bytes.back().data(), bytes, hipMemcpyHostToDevice, s_));
+  } else {
+    const size_t i = L.next++;
+    // the copy target may have moved since the record (a buffer that grew
+    // later in the recorded call): the current one is written
+    if (i >= L.dev.size() || L.bytes[i].size() != bytes || (L.dst[i] == nullptr) != (dst == nullptr) ||
+        (bytes && std::memcmp(L.bytes[i].data(), h, bytes) != 0)) {
+      L.failed = true;
+      return i < L.dev.size() ? L.dev[i] : nullptr;
+    }
+    d = L.dev[i];
+  }
+  if (dst && bytes) copy_bytes_async(dst, d, bytes, s_);
+  return d;
+}
+
 void* Stager::put_bytes(const void* h, size_t bytes) {
+  if (log_) return logged(h, bytes, nullptr);
   const size_t o = reserve((bytes + 255) & ~(size_t)255);
   std::memcpy(host_ + o, h, bytes);
   HIP_CHECK(hipMemcpyAsync(dev_ + o, host_ + o, bytes, hipMemcpyHostToDevice, s_));
@@ -152,6 +199,12 @@ char* Stager::put_gather(const std::pair<const void*, size_t>* items, int n, siz
     total += (items[i].second + 255) & ~(size_t)255;
   }
   if (total == 0) return nullptr;
+  if (log_) {
+    std::vector<uint8_t> cat(total, 0);
+    for (int i = 0; i < n; ++i)
+      if (items[i].second) std::memcpy(cat.data() + off[i], items[i].first, items[i].second);
+    return logged(cat.data(), total, nullptr);
+  }
   const size_t o = reserve(total);
   for (int i = 0; i < n; ++i)
     if (items[i].second) std::memcpy(host_ + o + off[i], items[i].first, items[i].second);
@@ -160,12 +213,23 @@ char* Stager::put_gather(const std::pair<const void*, size_t>* items, int n, siz
 }
 
 void Stager::copy_to(void* dst, const void* h, size_t bytes) {
+  if (log_) {
+    (void)logged(h, bytes, dst);
+    return;
+  }
   const size_t o = reserve((bytes + 255) & ~(size_t)255);
   std::memcpy(host_ + o, h, bytes);
   HIP_CHECK(hipMemcpyAsync(dst, host_ + o, bytes, hipMemcpyHostToDevice, s_));
 }
 
-bool autotune_enabled() { return g_tune_on; }
+namespace {
+// per thread: a recorded build keeps the default tilings (no timing runs)
+// and, while its launches are being recorded, takes no timing events
+thread_local int g_tune_suspend = 0, g_prof_suspend = 0;
+}  // namespace
+void autotune_suspend(bool on) { g_tune_suspend += on ? 1 : -1; }
+void prof_suspend(bool on) { g_prof_suspend += on ? 1 : -1; }
+bool autotune_enabled() { return g_tune_on && g_tune_suspend == 0; }
 void autotune_enable(bool on) { g_tune_on = on; }
 bool tune_lookup(const TuneKey& k, int& choice) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -183,7 +247,7 @@ size_t tune_entries() {
   return tune_map().size();
 }
 
-bool prof_enabled() { return g_prof_on; }
+bool prof_enabled() { return g_prof_on && g_prof_suspend == 0; }
 void prof_set(bool on) { g_prof_on = on; }
 void prof_reset() {
   std::lock_guard<std::mutex> lk(g_prof_mu);
@@ -375,9 +439,101 @@ void Orthogonalizer::check_once(hipStream_t s) {
   }
 }
 
+void Orthogonalizer::run_fixed(const std::vector<OrthSeg>& segs, double* Y, double* T, int64_t ld, int kcap,
+                               double* Rout, hipStream_t s) {
+  const size_t nb = segs.size();
+  const int64_t kk = (int64_t)kcap * kcap;
+  make_plan(segs, kcap, Rout != nullptr, s);
+  const size_t c = fixed_call_++;
+  if (c == fixed_info_.size()) {
+    fixed_info_.emplace_back();
+    fixed_gerr_.emplace_back();
+    fixed_nb_.push_back(0);
+    fixed_passes_.push_back(0);
+    fixed_dev_.push_back(0.0);
+  }
+  HMAT_REQUIRE(fixed_choose_ || fixed_nb_[c] == nb, HMAT_ERR_STATE, "recorded orthogonalisation differs");
+  fixed_nb_[c] = nb;
+  DevBuf<int>& st = fixed_info_[c];
+  DevBuf<double>& ge = fixed_gerr_[c];
+  st.reserve(2 * nb);
+  ge.reserve(nb);
+  const int nm = (int)nb;
+  // bases of at most 64 columns are solved in place; wider ones alternate
+  // between Y and T and are copied back at the end
+  const bool inplace = plan_.kmax <= 64;
+  // shifted pass: R_1 in Racc_
+  gram(Y, ld, kcap, s);
+  kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), Racc_.get(), plan_.shift_on, st.get(), ge.get(), s);
+  double* cur = inplace ? Y : T;
+  trsm(Y, ld, Racc_.get(), kcap, cur, s);
+  for (int p = 0;; ++p) {
+    // the Gram matrix of the current basis: its deviation from I checks it,
+    // its (fallback-shifted where needed) factor is the next pass's
+    gram(cur, ld, kcap, s);
+    kern::launch_chol_shifted(plan_.mats, nm, plan_.nmax, G_.get(), R_.get(), plan_.shift_off, st.get() + nb,
+                              ge.get(), s, plan_.shift_on);
+    bool more;
+    if (fixed_choose_) {
+      const auto hi = download(st.get(), 2 * nb, s);
+      const auto hg = download(ge.get(), nb, s);
+      double dev = 0.0;
+      bool clean = true;
+      for (size_t b = 0; b < nb; ++b) {
+        if (hi[b] < 0 || hi[nb + b] < 0) throw Error(HMAT_ERR_NONFINITE, "orthogonalisation: non-finite Gram matrix");
+        if (hi[b] > 0) throw Error(HMAT_ERR_NONFINITE, "orthogonalisation: shifted Cholesky broke down");
+        if (hi[nb + b] != 0) clean = false;
+        dev = std::max(dev, hg[b]);
+      }
+      more = !(clean && dev < kOrthTol) && p + 2 < kOrthMaxPasses;
+      if (!more) {
+        fixed_passes_[c] = p;
+        fixed_dev_[c] = dev;
+      }
+    } else {
+      more = p < fixed_passes_[c];
+    }
+    if (!more) break;
+    double* nxt = inplace ? cur : (cur == Y ? T : Y);
+    trsm(cur, ld, R_.get(), kcap, nxt, s);
+    if (Rout) acc_product(kcap, s);
+    cur = nxt;
+  }
+  if (!inplace) {
+    // always enqueued (a copy onto itself when the basis is in Y already), so
+    // that the staged descriptors do not depend on the number of passes
+    std::vector<SegNN> cp(nb);
+    for (size_t b = 0; b < nb; ++b) cp[b] = SegNN{segs[b].off, 0, segs[b].off, segs[b].len, 0, segs[b].k, 0};
+    kern::copy_segs(cp, dnn_, cur, ld, Y, ld, s);
+  }
+  if (Rout) copy_bytes_async(Rout, Racc_.get(), nb * kk * sizeof(double), s);
+}
+
+std::string Orthogonalizer::check_fixed(size_t ncalls, hipStream_t s) {
+  if (ncalls > fixed_info_.size()) return "orthogonalisation: statuses of a recorded build are missing";
+  for (size_t c = 0; c < ncalls; ++c) {
+    const size_t nb = fixed_nb_[c];
+    const auto h = download(fixed_info_[c].get(), 2 * nb, s);
+    const auto g = download(fixed_gerr_[c].get(), nb, s);
+    const double bound = std::max(kOrthTol, 10.0 * fixed_dev_[c]);
+    for (size_t b = 0; b < nb; ++b) {
+      if (h[b] < 0 || h[nb + b] < 0 || !std::isfinite(g[b])) return "orthogonalisation: non-finite Gram matrix";
+      if (h[b] > 0) return "orthogonalisation: shifted Cholesky broke down";
+      if (!(g[b] <= bound))
+        return "orthogonalisation: basis deviates from orthonormal by " + sci(g[b]) + " (recorded " +
+               sci(fixed_dev_[c]) + ")";
+    }
+  }
+  return std::string();
+}
+
 void Orthogonalizer::run(const std::vector<OrthSeg>& segs, double* Y, double* T, int64_t ld,
                          int kcap, double* Rout, hipStream_t s) {
   if (segs.empty()) return;
+  if (fixed_) {
+    run_fixed(segs, Y, T, ld, kcap, Rout, s);
+    return;
+  }
   const size_t nb = segs.size();
   const int64_t kk = (int64_t)kcap * kcap;
   make_plan(segs, kcap, Rout != nullptr, s);

@@ -1,6 +1,7 @@
 // Tree topology and shared level-batched building blocks.
 #pragma once
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "kernels.hpp"
@@ -151,8 +152,48 @@ class Orthogonalizer {
                 hipStream_t s);
   // Download and check the statuses of the run_once passes so far.
   void check_once(hipStream_t s);
+  // Schedule for builds recorded into a graph: run() becomes one shifted
+  // pass, then unshifted passes (an unshifted factor that breaks down is
+  // redone shifted in the same launch), each preceded by the Gram matrix
+  // and Cholesky factor that also check the current basis, and a final
+  // check.  With `choose`, the k-th call since set_fixed(true, ...) takes
+  // passes until its basis is orthonormal to kOrthTol (downloads; at most
+  // kOrthMaxPasses) and records their number; without, it takes the number
+  // recorded for the k-th call, with no download and no host decision.  The
+  // staged descriptors do not depend on that number.  Each call's statuses
+  // and final Gram deviations go to buffers of their own, so a replay of
+  // the recorded launches writes them again in place.
+  void set_fixed(bool on, bool choose = false) {
+    fixed_ = on;
+    fixed_choose_ = choose;
+    if (on) fixed_call_ = 0;
+  }
+  bool fixed() const { return fixed_; }
+  // every device buffer the calls use (a recorded graph holds these addresses)
+  void addresses(std::vector<const void*>& v) const {
+    v.insert(v.end(), {G_.get(), R_.get(), Rchk_.get(), Racc_.get(), Rtmp_.get(), gerr_.get(), info_.get(),
+                       once_info_.get(), chk_.get(), dnn_.get(), dsm_.get(), plan_.blob.get()});
+    for (const auto& b : fixed_info_) v.push_back(b.get());
+    for (const auto& b : fixed_gerr_) v.push_back(b.get());
+    ws_.addresses(v);
+  }
+  size_t fixed_calls() const { return fixed_call_; }
+  // The statuses of the first `ncalls` fixed-schedule calls since the last
+  // set_fixed(true): empty when every Gram matrix was finite, every shifted
+  // factor succeeded and every final basis is orthonormal to
+  // max(kOrthTol, 10 x the deviation recorded for the call), else what failed.
+  std::string check_fixed(size_t ncalls, hipStream_t s);
 
  private:
+  void run_fixed(const std::vector<OrthSeg>& segs, double* Y, double* T, int64_t ld, int kcap, double* Rout,
+                 hipStream_t s);
+  bool fixed_ = false, fixed_choose_ = false;
+  size_t fixed_call_ = 0;
+  std::vector<DevBuf<int>> fixed_info_;      // per call: shifted-pass statuses, then the final check's
+  std::vector<DevBuf<double>> fixed_gerr_;   // per call: the final check's Gram deviations
+  std::vector<size_t> fixed_nb_;              // per call: nseg
+  std::vector<int> fixed_passes_;             // per call: unshifted passes recorded
+  std::vector<double> fixed_dev_;             // per call: final deviation when recorded
   // the launches of one call, planned once: Gram products, Cholesky
   // descriptors and shifts, triangular solves, R products
   struct Plan {

@@ -58,6 +58,7 @@ std::vector<std::pair<int64_t, int>> HodlrEngine::omega_schedule(const hmat_opti
 
 void HodlrEngine::reset_for_build(int leaf_size) {
   HMAT_REQUIRE(leaf_size > 0, HMAT_ERR_ARG, "leaf_size must be positive");
+  if (!fixed_) bg_.drop();  // a recorded batch graph writes this handle's arrays
   HMAT_REQUIRE(opt.rank > 0 && opt.oversample >= 0 && opt.power_iters >= 0, HMAT_ERR_ARG,
                "bad rank/oversample/power_iters");
   built = factored = false;
@@ -158,7 +159,12 @@ void HodlrEngine::build_levels(const GenDesc& g, const double* host_omega, int64
     bd.push_back({nd.start, nd.start, (int64_t)i * lm * lm, (int32_t)nd.len, (int32_t)nd.len, (int32_t)lm, 0});
   }
   kern::gen_blocks(g, bd, dbd_, D.get(), stream);
-  HIP_CHECK(hipStreamSynchronize(stream));
+  if (diag_shift_) {
+    std::vector<int> lens;
+    for (int id : tree.leaves) lens.push_back((int)tree.nodes[id].len);
+    kern::add_diag(D.get(), lm, stage(lens, stream), (int)lens.size(), diag_shift_, stream);
+  }
+  if (!capturing_) HIP_CHECK(hipStreamSynchronize(stream));
   // the build scratch (three N x k arrays) is kept for the next build
   prepare_matvec();
   built = true;
@@ -295,8 +301,8 @@ void HodlrEngine::build_group(int c0, int c1, const GenDesc& g, const std::vecto
   auto cb = [&](int t) { return (int64_t)(lvl[t] - c0) * lvs; };
   const int64_t nk = lvs * ncl;
   Yb_.reserve(nk); Xb_.reserve(nk); Tb_.reserve(nk);
-  HIP_CHECK(hipMemsetAsync(Yb_.get(), 0, nk * sizeof(double), stream));
-  HIP_CHECK(hipMemsetAsync(Xb_.get(), 0, nk * sizeof(double), stream));
+  zero_bytes_async(Yb_.get(), nk * sizeof(double), stream);
+  zero_bytes_async(Xb_.get(), nk * sizeof(double), stream);
   const int64_t kk = (int64_t)kcap * kcap;
   Rb_.reserve(nt * kk); Vr_.reserve(nt * kk); Wk_.reserve(2 * nt * kk);
   TU_.reserve(nt * kk); TV_.reserve(nt * kk);
@@ -314,6 +320,7 @@ void HodlrEngine::build_group(int c0, int c1, const GenDesc& g, const std::vecto
     if (mode[t] == BlockMode::ExactWide)
       idx_row.push_back(SegNN{0, 0, st[t] + cb(t), (int32_t)ln[t], 0, k[t], 0});
   }
+  HMAT_REQUIRE(!(host_omega && fixed_), HMAT_ERR_ARG, "a recorded build draws Omega on the device");
   if (host_omega) {
     bool any = false;
     for (int t = 0; t < nt; ++t) any |= (mode[t] == BlockMode::RSVD);
@@ -405,7 +412,10 @@ void HodlrEngine::build_group(int c0, int c1, const GenDesc& g, const std::vecto
   // resolved against each other
   kern::jacobi_svd(sm, dsm_, Rb_.get(), Vr_.get(), Sv_.get(), kcap, Wk_.get(), stream,
                    lp ? kMixedJacobiFloor : 1e-30);
-  auto S = download(Sv_.get(), (size_t)nt * kcap, stream);
+  // a fixed-schedule build has fixed ranks and leaves its singular values
+  // on the device (a non-finite one shows in the solve's residual)
+  const std::vector<double> S =
+      fixed_ ? std::vector<double>((size_t)nt * kcap, 0.0) : download(Sv_.get(), (size_t)nt * kcap, stream);
 
   // rank per block (compression.h:42-46 or fixed)
   std::vector<int> rank(nt, 0);
@@ -413,7 +423,7 @@ void HodlrEngine::build_group(int c0, int c1, const GenDesc& g, const std::vecto
   for (int t = 0; t < nt; ++t) {
     const double* sv = &S[(size_t)t * kcap];
     int r = 0;
-    if (opt.fixed_rank) {
+    if (opt.fixed_rank || fixed_) {
       r = std::min(opt.rank, k[t]);
     } else {
       if (k[t] > 0 && sv[0] > 1e-15)
@@ -578,7 +588,7 @@ void HodlrEngine::correction_t(HFactor<F>& fa, int j, F* W, int64_t ldw, int nco
 template <class F>
 static void copy_or_convert(const double* src, F* dst, int64_t n, hipStream_t s) {
   if constexpr (std::is_same<F, double>::value) {
-    HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    copy_bytes_async(dst, src, n * sizeof(double), s);
   } else {
     kern::convert<F, double>(src, dst, n, s);
   }
@@ -624,8 +634,9 @@ void HodlrEngine::factor_t(HFactor<F>& fa) {
         if (info[cap_info[q].first + p] != 0)
           throw Error(HMAT_ERR_SINGULAR, "singular capacitance matrix at depth " + std::to_string(jj));
   };
+  last_ninfo_ = ninfo;
   if (dmax == 0) {
-    check_info();
+    if (!fixed_) check_info();
     fa.ready = true;
     return;
   }
@@ -680,8 +691,7 @@ void HodlrEngine::factor_t(HFactor<F>& fa) {
     kern::seg_gemm_tn<F>(kb, fa.V[j + 1], N, Q, N, fa.Minv[j].get(), mm, F(1), F(0), ws_, stream);
     if constexpr (kRefineBlocks<F>) {
       fa.Mmat[j].resize((size_t)np * mm * mm);
-      HIP_CHECK(hipMemcpyAsync(fa.Mmat[j].get(), fa.Minv[j].get(), (size_t)np * mm * mm * sizeof(F),
-                               hipMemcpyDeviceToDevice, stream));
+      copy_bytes_async(fa.Mmat[j].get(), fa.Minv[j].get(), (size_t)np * mm * mm * sizeof(F), stream);
     }
     kern::gj_inverse<F>(ms, dsm_, fa.Minv[j].get(), info_.get() + info_off, stream);
     cap_info.push_back({info_off, np});
@@ -689,7 +699,7 @@ void HodlrEngine::factor_t(HFactor<F>& fa) {
     // push the depth-j correction into the bases of shallower depths
     if (colofs[j + 1] > 0) correction_t<F>(fa, j, fa.Wall.get(), N, (int)colofs[j + 1]);
   }
-  check_info();
+  if (!fixed_) check_info();  // a fixed-schedule factorisation is checked by its caller
   fa.ready = true;
   plan_fused<F>(fa);
 }
@@ -763,7 +773,7 @@ void HodlrEngine::plan_fused(HFactor<F>& fa) {
   ap.part.reserve((size_t)nl * KM * 4 * sizeof(F));
   ap.coef.reserve((size_t)std::max(ap.npar, 1) * MM * 4 * sizeof(F));
   ap.sync.reserve(((size_t)2 * ap.npar + 1) * sizeof(int));
-  HIP_CHECK(hipMemsetAsync(ap.sync.template as<int>(), 0, ((size_t)2 * ap.npar + 1) * sizeof(int), stream));
+  zero_bytes_async(ap.sync.template as<int>(), ((size_t)2 * ap.npar + 1) * sizeof(int), stream);
   ap.ok = true;
 }
 
@@ -799,7 +809,7 @@ void HodlrEngine::check_fused(HFactor<F>& fa) {
   HIP_CHECK(hipMemcpyAsync(&err, fa.ap.sync.template as<int>() + 2 * fa.ap.npar, sizeof(int), hipMemcpyDeviceToHost, stream));
   HIP_CHECK(hipStreamSynchronize(stream));
   if (err) {
-    HIP_CHECK(hipMemsetAsync(fa.ap.sync.template as<int>() + 2 * fa.ap.npar, 0, sizeof(int), stream));
+    zero_bytes_async(fa.ap.sync.template as<int>() + 2 * fa.ap.npar, sizeof(int), stream);
     throw Error(HMAT_ERR_DEVICE, "fused solve: a wait of the persistent kernel exceeded its spin limit");
   }
 }
@@ -998,6 +1008,10 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
   HMAT_REQUIRE(nrhs > 0 && ldb >= N && ldx >= N, HMAT_ERR_ARG, "bad solve dimensions");
   if (!factored) factor();
   apply_inverse(dB, ldb, nrhs, dX, ldx);
+  refine_loop(dB, ldb, nrhs, dX, ldx);
+}
+
+void HodlrEngine::refine_loop(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
   last_iters = 0;
   last_resid = -1.0;
   if (!(refine_rtol > 0.0)) {
@@ -1062,6 +1076,260 @@ void HodlrEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int
                                             " iterations (target " + sci(refine_rtol) + ")");
 }
 
+// ---------------------------------------------------------------------------
+// Batches recorded into one graph
+
+void HodlrEngine::fixed_solve(int ncols, int& iters, bool adaptive, double& resid) {
+  double* B = bg_.b.get();
+  double* X = bg_.x_.get();
+  apply_inverse(B, N, ncols, X, N);
+  ir_y_.reserve((size_t)N * ncols);
+  ir_r_.reserve((size_t)N * ncols);
+  ir_n_.reserve((size_t)2 * ncols);
+  ir_p_.reserve((size_t)64 * ncols);
+  kern::col_norms2(B, N, N, ncols, ir_n_.get(), ir_p_.get(), stream);
+  std::vector<double> bn;
+  if (adaptive) bn = download(ir_n_.get(), ncols, stream);
+  resid = 0.0;
+  double prev = 0.0;
+  int it = 0;
+  for (;; ++it) {
+    matvec_enqueue(X, N, ncols, ir_y_.get(), N);
+    kern::axpby_cols(N, ncols, 1.0, B, N, -1.0, ir_y_.get(), N, ir_r_.get(), N, stream);
+    if (adaptive) {
+      // the refinement count that this batch takes (recorded with the graph)
+      kern::col_norms2(ir_r_.get(), N, N, ncols, ir_n_.get() + ncols, ir_p_.get(), stream);
+      const auto rn = download(ir_n_.get() + ncols, ncols, stream);
+      resid = 0.0;
+      for (int j = 0; j < ncols; ++j) {
+        const double rj = bn[j] > 0.0 ? std::sqrt(rn[j] / bn[j]) : std::sqrt(rn[j]);
+        resid = std::isfinite(rj) ? std::max(resid, rj) : HUGE_VAL;
+      }
+      if (!(refine_rtol > 0.0) || resid <= refine_rtol || it >= refine_max || (it > 0 && !(resid < prev))) break;
+      prev = resid;
+    } else if (it >= iters) {
+      kern::col_norms2(ir_r_.get(), N, N, ncols, ir_n_.get() + ncols, ir_p_.get(), stream);
+      break;
+    }
+    apply_inverse(ir_r_.get(), N, ncols, ir_y_.get(), N);
+    kern::axpby_cols(N, ncols, 1.0, X, N, 1.0, ir_y_.get(), N, X, N, stream);
+  }
+  iters = it;
+}
+
+std::string HodlrEngine::batch_check(int ncols, double& resid) {
+  std::string why = orth_.check_fixed(bg_.orth_calls, stream);
+  if (!why.empty()) return why;
+  const auto info = download(info_.get(), (size_t)bg_.ninfo, stream);
+  for (int v : info)
+    if (v != 0) return "singular leaf block or capacitance matrix";
+  const auto n = download(ir_n_.get(), (size_t)2 * ncols, stream);
+  resid = 0.0;
+  for (int j = 0; j < ncols; ++j) {
+    const double rj = n[j] > 0.0 ? std::sqrt(n[ncols + j] / n[j]) : std::sqrt(n[ncols + j]);
+    if (!std::isfinite(rj)) return "non-finite residual";
+    resid = std::max(resid, rj);
+  }
+  return std::string();
+}
+
+std::vector<const void*> HodlrEngine::batch_buffers() const {
+  std::vector<const void*> v{Yb_.get(), Xb_.get(), Tb_.get(), Rb_.get(), Vr_.get(), Sv_.get(), Wk_.get(),
+                             TU_.get(), TV_.get(), tvec_.get(), info_.get(), D.get(), dnn_.get(), dsm_.get(),
+                             dbd_.get(), mv_leaves_.get(), mv_lvls_.get(), mv_ent_.get(), mv_segs_.get(),
+                             mv_part_.get(), mv_t_.get(), ir_y_.get(), ir_r_.get(), ir_n_.get(), ir_p_.get(),
+                             f64_.Dinv.get(), f64_.Wall.get(), f64_.wtmp.get(), f64_.tvec.get(), f64_.coef.get(),
+                             f64_.rtmp.get(), f64_.b.get(), f64_.x.get(), bg_.b.get(), bg_.x_.get(),
+                             bg_.shift.get()};
+  for (const auto& m : f64_.Minv) v.push_back(m.get());
+  for (const auto& m : f64_.Mmat) v.push_back(m.get());
+  for (size_t c = 1; c < lv.size(); ++c) {
+    v.push_back(lv[c].US.get());
+    v.push_back(lv[c].V.get());
+  }
+  f64_.ap.addresses(v);
+  ws_.addresses(v);
+  orth_.addresses(v);
+  return v;
+}
+
+namespace {
+// the kernels whose parameter p1 is a diagonal shift (p1 on i == j only)
+bool has_diag_shift(GenKind k) {
+  return k == GenKind::Gaussian || k == GenKind::RefCauchy || k == GenKind::RefGaussian ||
+         k == GenKind::RefCoulomb;
+}
+}  // namespace
+
+int HodlrEngine::batch_solve(const GenDesc& g, const std::vector<int64_t>& sizes, int leaf_size, const double* dB,
+                             int64_t ldb, int ncols, double* dX, int64_t ldx, bool use_graph) {
+  int64_t ntot = 0;
+  for (int64_t n : sizes) ntot += n;
+  HMAT_REQUIRE(ncols > 0 && ldb >= ntot && ldx >= ntot, HMAT_ERR_ARG, "bad batch solve dimensions");
+  if (!use_graph) {
+    build_forest(g, sizes, leaf_size);
+    factor();
+    solve(dB, ldb, ncols, dX, ldx);
+    return kBatchEager;
+  }
+  HMAT_REQUIRE(opt.fixed_rank && opt.precision == HMAT_PREC_FP64, HMAT_ERR_ARG,
+               "a recorded batch needs fixed ranks and FP64");
+  HMAT_REQUIRE(g.kind != GenKind::Dense, HMAT_ERR_ARG, "a recorded batch generates its entries");
+  const double shift = has_diag_shift(g.kind) ? g.p1 : 0.0;
+  GenDesc g0 = g;
+  if (has_diag_shift(g.kind)) g0.p1 = 0.0;
+  auto same_opts = [](const hmat_options& a, const hmat_options& b) {
+    return a.rank == b.rank && a.fixed_rank == b.fixed_rank && a.oversample == b.oversample &&
+           a.power_iters == b.power_iters && a.exact_below == b.exact_below && a.precision == b.precision &&
+           a.omega_dist == b.omega_dist && a.seed == b.seed;
+  };
+  batch_note.clear();
+  bool redo = false;
+  const bool same = bg_.valid && bg_.graph.e && bg_.sizes == sizes && bg_.leaf == leaf_size &&
+                    bg_.ncols == ncols && bg_.kind == (int)g.kind && bg_.x == g.x && bg_.y == g.y &&
+                    bg_.p0 == g.p0 && same_opts(bg_.opt, opt) && bg_.bufs == batch_buffers();
+  if (same) {
+    kern::set_scalar(bg_.shift.get(), shift, stream);
+    kern::convert_cols<double, double>(N, ncols, dB, ldb, bg_.b.get(), N, stream);
+    HIP_CHECK(hipGraphLaunch(bg_.graph.e, stream));
+    kern::convert_cols<double, double>(N, ncols, bg_.x_.get(), N, dX, ldx, stream);
+    double resid = 0.0;
+    const std::string why = batch_check(ncols, resid);
+    if (why.empty()) {
+      last_iters = bg_.iters;
+      last_resid = resid;
+      if (!(refine_rtol > 0.0) || resid <= refine_rtol) return kBatchReplayed;
+      // the recorded refinement count fell short for these values: refine on
+      refine_loop(dB, ldb, ncols, dX, ldx);
+      return kBatchRedone;
+    }
+    // a status failed on these values (typically a basis that needs another
+    // CholeskyQR pass): the eager pass below redoes the work, choosing the
+    // pass counts for these values, and records them
+    batch_note = "replay rejected: " + why;
+    redo = true;
+  }
+
+  // First call for this shape: the fixed schedule eagerly, recording its
+  // descriptors; then the same schedule recorded into the graph.
+  bg_.drop();
+  bg_.sizes = sizes;
+  bg_.leaf = leaf_size;
+  bg_.ncols = ncols;
+  bg_.kind = (int)g.kind;
+  bg_.x = g.x;
+  bg_.y = g.y;
+  bg_.p0 = g.p0;
+  bg_.opt = opt;
+  bg_.b.reserve((size_t)ntot * ncols);
+  bg_.x_.reserve((size_t)ntot * ncols);
+  bg_.shift.reserve(1);
+  bg_.log.reset(new StageLog());
+  kern::set_scalar(bg_.shift.get(), shift, stream);
+  kern::convert_cols<double, double>(ntot, ncols, dB, ldb, bg_.b.get(), ntot, stream);
+  // the fixed-schedule state for the duration of one pass (restored on any exit)
+  struct Fixed {
+    HodlrEngine& e;
+    int fused;
+    bool graph;
+    bool capture;
+    Fixed(HodlrEngine& en, bool cap) : e(en), fused(en.fused_solve), graph(en.solve_graph), capture(cap) {
+      e.fixed_ = true;
+      e.capturing_ = cap;
+      e.diag_shift_ = e.bg_.shift.get();
+      e.orth_.set_fixed(true, !cap);  // the eager pass chooses the pass counts, the recording repeats them
+      e.fused_solve = 0;       // multi-launch applies (recordable, any column count)
+      e.solve_graph = false;   // no nested solve graph
+      stager(e.stream).attach(e.bg_.log.get());
+      autotune_suspend(true);
+      if (cap) prof_suspend(true);
+    }
+    ~Fixed() {
+      e.fixed_ = false;
+      e.capturing_ = false;
+      e.diag_shift_ = nullptr;
+      e.orth_.set_fixed(false);
+      e.fused_solve = fused;
+      e.solve_graph = graph;
+      stager(e.stream).attach(nullptr);
+      autotune_suspend(false);
+      if (capture) prof_suspend(false);
+    }
+  };
+  int iters = 0;
+  double resid = 0.0;
+  {
+    bg_.log->mode = StageLog::Persist;
+    Fixed fx(*this, false);
+    build_forest(g0, sizes, leaf_size);
+    factor();
+    fixed_solve(ncols, iters, true, resid);
+    bg_.orth_calls = orth_.fixed_calls();
+    bg_.ninfo = last_ninfo_;
+    bg_.iters = iters;
+  }
+  kern::convert_cols<double, double>(N, ncols, bg_.x_.get(), N, dX, ldx, stream);
+  std::string why = batch_check(ncols, resid);
+  if (!why.empty()) {
+    // this batch needs the adaptive schedule: no graph
+    batch_note = "fixed schedule failed: " + why;
+    bg_.drop();
+    build_forest(g, sizes, leaf_size);
+    factor();
+    solve(dB, ldb, ncols, dX, ldx);
+    return redo ? kBatchRedone : kBatchEager;
+  }
+  last_iters = iters;
+  last_resid = resid;
+  if (refine_rtol > 0.0 && resid > refine_rtol) {
+    // the fixed schedule does not reach the target here: no graph
+    batch_note = "the fixed schedule stops at residual " + sci(resid);
+    refine_loop(dB, ldb, ncols, dX, ldx);
+    bg_.drop();
+    return redo ? kBatchRedone : kBatchEager;
+  }
+  // record the same launches (nothing runs; the device keeps this call's results)
+  bool recorded = false;
+  {
+    bg_.log->mode = StageLog::Replay;
+    bg_.log->next = 0;
+    Fixed fx(*this, true);
+    HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      build_forest(g0, sizes, leaf_size);
+      factor();
+      int it = bg_.iters;
+      double r = 0.0;
+      fixed_solve(ncols, it, false, r);
+      HIP_CHECK(hipStreamEndCapture(stream, &bg_.graph.g));
+      recorded = !bg_.log->failed && bg_.log->next == bg_.log->dev.size() &&
+                 orth_.fixed_calls() == bg_.orth_calls && last_ninfo_ == bg_.ninfo;
+      if (!recorded)
+        batch_note = "recording differs from the eager pass (descriptors " + std::to_string(bg_.log->next) + "/" +
+                     std::to_string(bg_.log->dev.size()) + (bg_.log->failed ? ", mismatch" : "") + ")";
+    } catch (const std::exception& ex) {  // the stream leaves capture mode; this call's result stands
+      batch_note = std::string("recording failed: ") + ex.what();
+      hipGraph_t gg = nullptr;
+      (void)hipStreamEndCapture(stream, &gg);
+      if (gg) (void)hipGraphDestroy(gg);
+      (void)hipGetLastError();
+      recorded = false;
+    }
+  }
+  if (recorded) {
+    HIP_CHECK(hipGraphInstantiate(&bg_.graph.e, bg_.graph.g, nullptr, nullptr, 0));
+    bg_.bufs = batch_buffers();
+  }
+  bg_.valid = recorded;
+  if (!recorded) {
+    bg_.drop();
+    prof_add("batch_graph_refused", 0.0, 1.0);
+  }
+  built = factored = true;
+  if (redo) return kBatchRedone;
+  return recorded ? kBatchRecorded : kBatchEager;
+}
+
 void HodlrEngine::to_dense_host(double* A, int64_t lda) {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "to_dense before build");
   const int64_t lm = tree.lmax;
@@ -1098,6 +1366,7 @@ namespace hmat {
 // transpose the leaf blocks, and swap in the other factorisation.  Applying
 // it twice restores the original state (sib is an involution).
 void HodlrEngine::flip() {
+  bg_.drop();  // the recorded batch graph writes the arrays swapped here
   for (int c = 1; c <= tree.max_depth; ++c) {
     HLevel& L = lv[c];
     std::swap(L.US, L.V);

@@ -2,6 +2,7 @@
 // (include/hodlr_routines_omp.h:9-241).
 #pragma once
 #include <map>
+#include <memory>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -200,6 +201,28 @@ class HodlrEngine {
   void matvec_transpose(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
   void solve_transpose(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
 
+  // Build + factor + solve of independent systems (a forest, as
+  // build_forest) in one call, X = A^{-1} B for ncols columns.  With
+  // use_graph (fixed rank, FP64 only) the work follows a fixed schedule
+  // without host synchronisation (shifted CholeskyQR3, a refinement count
+  // fixed when the batch shape is first seen) and is replayed from one HIP
+  // graph: the first call for a shape runs that schedule eagerly while its
+  // descriptors are kept on the device, and records the same launches into
+  // the graph; a later call with the same shape (system sizes, leaf size,
+  // kernel kind, point arrays and width p0, column count, options) replays
+  // it.  New point values written in place and a new diagonal shift p1 (a
+  // device scalar added to the leaf blocks) replay the same graph.  One
+  // download after each replay checks it (Cholesky and pivot statuses, the
+  // residual of every column); a replay whose residual misses the target is
+  // refined further eagerly, one whose statuses fail is redone eagerly and
+  // recorded again.
+  // Any other build, load or transposed operation drops the graph.
+  enum BatchPath { kBatchEager = 0, kBatchRecorded = 1, kBatchReplayed = 2, kBatchRedone = 3 };
+  int batch_solve(const GenDesc& g, const std::vector<int64_t>& sizes, int leaf, const double* dB, int64_t ldb,
+                  int ncols, double* dX, int64_t ldx, bool use_graph);
+  // why the last batch_solve with use_graph did not record or replay (empty when it did)
+  std::string batch_note;
+
   static std::vector<std::pair<int64_t, int>> omega_schedule(const hmat_options& o, int64_t N,
                                                              int leaf);
 
@@ -282,6 +305,42 @@ class HodlrEngine {
   DevBuf<kern::MvSeg> mv_segs_;
   DevBuf<double> mv_part_, mv_t_;
   double mv_bytes_base_ = 0.0;  // D + (U S) + V bytes of one pass
+  int last_ninfo_ = 0;          // pivot statuses written by the last factorisation
+
+  // refinement of X toward A_H^{-1} B (the loop of solve() after the apply)
+  void refine_loop(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
+  // Fixed-schedule state of a recorded build: no host synchronisation or
+  // host decision (ranks fixed, statuses kept for a later check), the leaf
+  // blocks' diagonal shift read from a device scalar.
+  bool fixed_ = false, capturing_ = false;
+  const double* diag_shift_ = nullptr;
+  // X = A^{-1} B in the staged batch buffers with `iters` refinement steps;
+  // adaptive: as many as the target needs (host checks), `iters` receives the count
+  void fixed_solve(int ncols, int& iters, bool adaptive, double& resid);
+  // statuses and residual after a replay: "" when the statuses are good
+  std::string batch_check(int ncols, double& resid);
+  // every device buffer a recorded batch graph reads or writes: a replay
+  // needs each at the address it had when the graph was recorded
+  std::vector<const void*> batch_buffers() const;
+  struct BatchGraph {
+    std::vector<int64_t> sizes;
+    int leaf = 0, ncols = 0, kind = -1;
+    const double *x = nullptr, *y = nullptr;
+    double p0 = 0.0;
+    hmat_options opt{};
+    int iters = 0, ninfo = 0;
+    size_t orth_calls = 0;
+    std::vector<const void*> bufs;  // batch_buffers() when recorded
+    bool valid = false;
+    DevBuf<double> b, x_, shift;  // staged right-hand sides and solution, diagonal shift
+    std::unique_ptr<StageLog> log;
+    GraphExec graph;
+    void drop() {
+      graph.reset();
+      log.reset();
+      valid = false;
+    }
+  } bg_;
 };
 
 // Allocation-free matvec (noalloc.hip): the fused kernels with their

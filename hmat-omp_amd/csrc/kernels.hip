@@ -1681,6 +1681,85 @@ __global__ void k_zero_rects(const ZeroRect* __restrict__ items) {
     z.p[(z.r0 + e % z.len) + (z.c0 + e / z.len) * z.ld] = 0.0;
 }
 
+// Byte fills and copies in the widest unit the addresses and size allow.
+template <class U>
+__global__ void k_fill_units(U* __restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = U{};
+}
+template <class U>
+__global__ void k_copy_units(U* __restrict__ d, const U* __restrict__ src, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = src[i];
+}
+static unsigned unit_grid(int64_t n) { return (unsigned)std::min<int64_t>(std::max<int64_t>((n + kWG - 1) / kWG, 1), 4096); }
+
+}  // namespace kern
+
+void zero_bytes_async(void* p, size_t bytes, hipStream_t s) {
+  if (!bytes) return;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p) | bytes;
+  if ((a & 15) == 0) {
+    const int64_t n = (int64_t)(bytes / 16);
+    kern::k_fill_units<uint4><<<kern::unit_grid(n), kern::kWG, 0, s>>>(static_cast<uint4*>(p), n);
+  } else if ((a & 3) == 0) {
+    const int64_t n = (int64_t)(bytes / 4);
+    kern::k_fill_units<uint32_t><<<kern::unit_grid(n), kern::kWG, 0, s>>>(static_cast<uint32_t*>(p), n);
+  } else {
+    const int64_t n = (int64_t)bytes;
+    kern::k_fill_units<uint8_t><<<kern::unit_grid(n), kern::kWG, 0, s>>>(static_cast<uint8_t*>(p), n);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+void copy_bytes_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (!bytes) return;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | bytes;
+  if ((a & 15) == 0) {
+    const int64_t n = (int64_t)(bytes / 16);
+    kern::k_copy_units<uint4><<<kern::unit_grid(n), kern::kWG, 0, s>>>(static_cast<uint4*>(dst),
+                                                                        static_cast<const uint4*>(src), n);
+  } else if ((a & 3) == 0) {
+    const int64_t n = (int64_t)(bytes / 4);
+    kern::k_copy_units<uint32_t><<<kern::unit_grid(n), kern::kWG, 0, s>>>(
+        static_cast<uint32_t*>(dst), static_cast<const uint32_t*>(src), n);
+  } else {
+    const int64_t n = (int64_t)bytes;
+    kern::k_copy_units<uint8_t><<<kern::unit_grid(n), kern::kWG, 0, s>>>(static_cast<uint8_t*>(dst),
+                                                                          static_cast<const uint8_t*>(src), n);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+namespace kern {
+
+// D_i(j, j) += *shift for j < len_i over every leaf block (lmax x lmax each).
+__global__ void k_add_diag(double* __restrict__ D, int64_t lm, const int* __restrict__ lens, int nleaf,
+                           const double* __restrict__ shift) {
+  const double v = *shift;
+  const int64_t total = (int64_t)nleaf * lm;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / lm), j = (int)(e % lm);
+    if (j < lens[i]) D[(int64_t)i * lm * lm + (int64_t)j * lm + j] += v;
+  }
+}
+
+__global__ void k_set_scalar(double* p, double v) {
+  if (threadIdx.x == 0) p[threadIdx.x] = v;
+}
+
+void set_scalar(double* p, double v, hipStream_t s) {
+  k_set_scalar<<<1, 64, 0, s>>>(p, v);
+  HIP_CHECK(hipGetLastError());
+}
+
+void add_diag(double* D, int64_t lm, const int* dlens, int nleaf, const double* dshift, hipStream_t s) {
+  const int64_t n = (int64_t)nleaf * lm;
+  if (n == 0) return;
+  k_add_diag<<<unit_grid(n), kWG, 0, s>>>(D, lm, dlens, nleaf, dshift);
+  HIP_CHECK(hipGetLastError());
+}
+
 void zero_rects(const std::vector<ZeroRect>& items, DevBuf<ZeroRect>& d, hipStream_t s) {
   if (items.empty()) return;
   d.upload(items, s);

@@ -76,6 +76,9 @@ struct Workspace {
   DevBuf<__bf16> bf;  // BF16 planes of the mixed-precision sketch operand
   DevBuf<int64_t> aux;
   template <class T> DevBuf<T>& parts();
+  void addresses(std::vector<const void*>& v) const {
+    v.insert(v.end(), {part.get(), part32.get(), bf.get(), aux.get()});
+  }
 };
 template <> inline DevBuf<double>& Workspace::parts<double>() { return part; }
 template <> inline DevBuf<float>& Workspace::parts<float>() { return part32; }
@@ -260,6 +263,11 @@ struct ZeroRect {
   int32_t r0, len, c0, ncols;
 };
 void zero_rects(const std::vector<ZeroRect>& items, DevBuf<ZeroRect>& d, hipStream_t s);
+// D_i(j, j) += *dshift (a device scalar) for j < dlens[i] over nleaf blocks
+// of lm x lm
+void add_diag(double* D, int64_t lm, const int* dlens, int nleaf, const double* dshift, hipStream_t s);
+// *p = v on the stream (a kernel: recordable into a graph, value by argument)
+void set_scalar(double* p, double v, hipStream_t s);
 
 // Y = (To) X elementwise for n contiguous values.
 template <class To, class Ti>

@@ -71,6 +71,9 @@ def _proto(lib):
         "hmat_kernel_matvec_device": (st, [C.POINTER(Kernel), i64, vp, i64, i32, vp, i64]),
         "hmat_hodlr_omega_length": (i64, [C.POINTER(Options), i64, i32]),
         "hmat_hodlr_build_kernel_batch_device": (st, [vp, C.POINTER(Kernel), C.POINTER(i64), i32, i32]),
+        "hmat_hodlr_batch_note": (C.c_char_p, [vp]),
+        "hmat_hodlr_batch_solve_device": (st, [vp, C.POINTER(Kernel), C.POINTER(i64), i32, i32, vp, i64, i32,
+                                               vp, i64, i32, C.POINTER(i32)]),
         "hmat_hodlr_matvec": (st, [vp, _D, i64, i32, _D, i64]),
         "hmat_hodlr_matvec_device": (st, [vp, vp, i64, i32, vp, i64]),
         "hmat_hodlr_matvec_device_noalloc": (st, [vp, vp, i64, i32, vp, i64]),
@@ -323,6 +326,26 @@ class HODLRMatrixOMP(_Base):
         _check(self._lib.hmat_hodlr_build_kernel_batch_device(self._h, C.byref(kernel), sz, len(sizes),
                                                               leaf_size))
         self.N = int(sum(int(v) for v in sizes))
+
+    BATCH_PATHS = {0: "eager", 1: "recorded", 2: "replayed", 3: "redone"}
+
+    def batch_solve_device(self, kernel, sizes, leaf_size, b_ptr, ldb, ncols, x_ptr, ldx, graph=True):
+        """Build + factor + solve of a batch in one call (device arrays);
+        with graph=True the work is recorded into one HIP graph on the first
+        call for a shape and replayed on later ones.  Returns how the call
+        ran: "eager", "recorded", "replayed" or "redone"."""
+        self._sync_opts()
+        sz = (C.c_int64 * len(sizes))(*[int(v) for v in sizes])
+        path = C.c_int32(-1)
+        _check(self._lib.hmat_hodlr_batch_solve_device(self._h, C.byref(kernel), sz, len(sizes), leaf_size,
+                                                       b_ptr, ldb, ncols, x_ptr, ldx, int(bool(graph)),
+                                                       C.byref(path)))
+        self.N = int(sum(int(v) for v in sizes))
+        return self.BATCH_PATHS[path.value]
+
+    def batch_note(self):
+        """Why the last graph batch call neither recorded nor replayed ("" if it did)."""
+        return self._lib.hmat_hodlr_batch_note(self._h).decode()
 
     def multiply_device(self, x_ptr, ldx, nrhs, y_ptr, ldy):
         _check(self._lib.hmat_hodlr_matvec_device(self._h, x_ptr, ldx, nrhs, y_ptr, ldy))

@@ -179,6 +179,33 @@ hmat_status hmat_hodlr_build_kernel_batch_device(hmat_hodlr h,
                                                  const hmat_kernel* k_device_points,
                                                  const int64_t* sizes,
                                                  int32_t nsys, int32_t leaf_size);
+/* Build + factor + solve of such a batch in one call: X = A^{-1} B for
+ * ncols columns of sum(sizes) rows (device arrays; the handle's refinement
+ * target applies, per column).  use_graph = 0: build, factor and solve as
+ * the three calls above.  use_graph = 1 (fixed rank, FP64, generated
+ * entries): the work takes a fixed schedule without host synchronisation
+ * (shifted CholeskyQR3 bases, the refinement count the first call of the
+ * shape needed) and is replayed from one HIP graph.  The first call for a
+ * shape runs that schedule eagerly and records it; a later call with the
+ * same shape (sizes, leaf size, kernel type, point arrays, p0, column
+ * count, options) replays it, reading the current point values (rescale
+ * them in place for new widths) and the kernel's diagonal shift p1 (for
+ * the kernels that have one: Gaussian and the reference kernels).  One
+ * download after a replay checks it; a replay whose residual misses the
+ * target is refined further eagerly; one whose Cholesky or pivot statuses
+ * fail (a basis that needs more CholeskyQR passes for the new values) is
+ * redone eagerly and recorded again.  Any other build or load on the handle, or a
+ * transposed matvec or solve, drops the graph.  *path (optional): 0 eager,
+ * 1 eager and recorded, 2 replayed, 3 replayed and finished eagerly.  No
+ * reference counterpart. */
+hmat_status hmat_hodlr_batch_solve_device(hmat_hodlr h, const hmat_kernel* k_device_points,
+                                          const int64_t* sizes, int32_t nsys, int32_t leaf_size,
+                                          const double* B, int64_t ldb, int32_t ncols, double* X,
+                                          int64_t ldx, int32_t use_graph, int32_t* path);
+/* Why the handle's last hmat_hodlr_batch_solve_device with use_graph = 1
+ * neither recorded nor replayed a graph ("" when it did); valid until the
+ * next call on the handle. */
+const char* hmat_hodlr_batch_note(hmat_hodlr h);
 /* Number of doubles of host Omega that build_dense expects. */
 int64_t hmat_hodlr_omega_length(const hmat_options* opts, int64_t N,
                                 int32_t leaf_size);
