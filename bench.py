@@ -249,6 +249,75 @@ def batch_leg(hmat_amd, torch, steps, warmup, profile=True):
     return out
 
 
+def hss_leg(hmat_amd, torch, reps=10):
+    """The HSS + HODLR workload: Gaussian kernel exp(-(p_i - p_j)^2 / (2 sigma^2))
+    + lambda delta_ij, sigma 0.05, lambda 1e-2, N = 65536 sorted U[0,1] points
+    (seeded), leaf 128, fixed rank 32, p 10, q 2, 16 right-hand sides ~ N(0,1),
+    FP64.  Nested HSS and HODLR on the same matrix: build, factor, solve and
+    matvec times, the apply-inverse and matvec HBM rates (algorithmic bytes
+    over HIP-event times of `reps` calls each), residuals against A_H."""
+    n, sigma, lam, leaf, rank, nrhs = 65536, 0.05, 1e-2, 128, 32, 16
+    rng = np.random.default_rng(5)
+    pts = np.sort(rng.random(n))
+    db = torch.tensor(rng.standard_normal((nrhs, n)), device="cuda")  # column-major n x nrhs
+    dx = torch.empty_like(db)
+    dy = torch.empty_like(db)
+    dp = torch.tensor(pts, device="cuda")
+    o = hmat_amd.make_options(0.0, rank=rank, fixed_rank=True, oversample=10, power_iters=2, exact_below=0,
+                              omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=1)
+    out = {"workload": f"Gaussian sigma {sigma} lambda {lam}, N={n} sorted U[0,1], leaf {leaf}, rank {rank}, "
+                       f"p 10, q 2, {nrhs} RHS, FP64 (nested HSS and HODLR)"}
+    for name in ("hss", "hodlr"):
+        if name == "hss":
+            h = hmat_amd.HSSMatrixOMP(opts=o, nested=True)
+            k = hmat_amd.make_kernel(hmat_amd.KERNEL_GAUSSIAN, pts, p0=sigma, p1=lam)
+        else:
+            h = hmat_amd.HODLRMatrixOMP(opts=o)
+            k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_GAUSSIAN, dp.data_ptr(), p0=sigma, p1=lam)
+        r = {}
+        for it in range(2):  # the first build on a fresh handle allocates
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if name == "hss":
+                h.build_kernel(k, n, leaf)
+            else:
+                h.build_kernel_device(k, n, leaf)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            h.factor()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            h.solve_device(db.data_ptr(), n, nrhs, dx.data_ptr(), n)
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+        r.update(build_s=round(t1 - t0, 4), factor_s=round(t2 - t1, 4), solve_s=round(t3 - t2, 5))
+        h.multiply_device(dx.data_ptr(), n, nrhs, dy.data_ptr(), n)
+        torch.cuda.synchronize()
+        r["resid_AH"] = float(torch.linalg.norm(db - dy) / torch.linalg.norm(db))
+        if name == "hss":
+            r["root_rank"] = h.root_rank()
+        hmat_amd.profile_reset()
+        hmat_amd.profile_enable(True)
+        for _ in range(reps):
+            h.multiply_device(db.data_ptr(), n, nrhs, dy.data_ptr(), n)
+        for _ in range(reps):
+            h.solve_device(db.data_ptr(), n, nrhs, dx.data_ptr(), n)
+        torch.cuda.synchronize()
+        hmat_amd.profile_enable(False)
+        for key, pname in (("matvec", "hss_matvec" if name == "hss" else "matvec"),
+                           ("solve", "hss_solve_apply" if name == "hss" else "solve_apply")):
+            pq = hmat_amd.profile_query(pname)
+            if pq and pq[0]:
+                ms = pq[1] / pq[0]
+                gbs = pq[2] / pq[0] / (ms * 1e-3) / 1e9
+                r[key] = {"avg_ms": round(ms, 4), "bytes": pq[2] / pq[0], "GBs": round(gbs, 1),
+                          "frac_hbm": round(gbs / PEAK_HBM_GBS, 4), "calls": pq[0]}
+        log(f"[{name}] {r}")
+        out[name] = r
+        del h
+    return out
+
+
 def true_residual(hmat_amd, torch, x, dev, n):
     """||b - A x|| / ||b|| with the exact generated A (FP64, never stored)."""
     k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_CAUCHY, dev["x"].data_ptr(), dev["y"].data_ptr())
@@ -380,6 +449,7 @@ def main():
                     help="keep the default tilings (the autotuner runs in the warm-up step otherwise)")
     ap.add_argument("--batch-steps", type=int, default=3,
                     help="timed steps of the batched leg (256 systems); 0 skips it")
+    ap.add_argument("--no-hss", action="store_true", help="skip the HSS + HODLR leg (N=65536, 16 RHS)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (counter calibration)")
     a = ap.parse_args()
@@ -523,6 +593,8 @@ def main():
         out["matvec_roofline"]["frac_of_probe_read"] = round(gbs / probe["read_GBs"], 4)
     if a.batch_steps > 0:
         out["batch"] = batch_leg(hmat_amd, torch, a.batch_steps, 1, a.profile)
+    if not a.no_hss:
+        out["hss_hodlr_n65536"] = hss_leg(hmat_amd, torch)
     if not a.no_cpu:
         cb = cpu_baseline(a)
         if cb:

@@ -471,6 +471,23 @@ hmat_status hmat_hss_matvec(hmat_hss h, const double* X, int64_t ldx, int32_t nr
   });
 }
 
+hmat_status hmat_hss_matvec_device(hmat_hss h, const double* X, int64_t ldx, int32_t nrhs, double* Y,
+                                   int64_t ldy) {
+  return guard([&] {
+    HMAT_REQUIRE(h && X && Y && nrhs > 0, HMAT_ERR_ARG, "bad matvec arguments");
+    h->eng.matvec(X, ldx, nrhs, Y, ldy);
+  });
+}
+
+hmat_status hmat_hss_solve_device(hmat_hss h, const double* B, int64_t ldb, int32_t nrhs, double* X,
+                                  int64_t ldx) {
+  return guard([&] {
+    HMAT_REQUIRE(h && B && X && nrhs > 0, HMAT_ERR_ARG, "bad solve arguments");
+    h->eng.solve(B, ldb, nrhs, X, ldx);
+    HIP_CHECK(hipStreamSynchronize(h->eng.stream));
+  });
+}
+
 hmat_status hmat_hss_factor(hmat_hss h) {
   return guard([&] {
     HMAT_REQUIRE(h, HMAT_ERR_ARG, "NULL handle");

@@ -530,6 +530,7 @@ void HodlrEngine::matvec_enqueue(const double* dX, int64_t ldx, int nrhs, double
     prof.finish();
     return;
   }
+  ProfScope prof("matvec", stream, matvec_bytes(nrhs));
   leaf_apply_t<double>(D.get(), dX, ldx, nrhs, dY, ldy);
   for (int c = 1; c <= tree.max_depth; ++c) {
     HLevel& L = lv[c];
@@ -548,6 +549,7 @@ void HodlrEngine::matvec_enqueue(const double* dX, int64_t ldx, int nrhs, double
     kern::seg_gemm_tn<double>(tn, L.V.get(), N, dX, ldx, tvec_.get(), L.rmax, 1.0, 0.0, ws_, stream);
     kern::seg_gemm_nn<double>(nn, L.US.get(), N, tvec_.get(), L.rmax, dY, ldy, 1.0, 1.0, stream);
   }
+  prof.finish();
 }
 
 // W(rows of internal depth-j nodes, 0:ncols) -= Q C M^{-1} V^T W, with

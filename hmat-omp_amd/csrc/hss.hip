@@ -557,14 +557,36 @@ void HssEngine::set_ldv() {
   ldv = lv;
 }
 
+double HssEngine::matvec_bytes(int nrhs) const {
+  double leaf2 = 0.0, basis = 0.0;
+  for (int id : tree.leaves) {
+    const double len = (double)tree.nodes[id].len;
+    leaf2 += len * len;
+    basis += len * (double)(sn[id].rU + sn[id].rV);
+  }
+  return 8.0 * (leaf2 + basis + (double)pool_built) + 16.0 * (double)N * nrhs;
+}
+
+double HssEngine::solve_bytes(int nrhs) const {
+  double leaf2 = 0.0, basis = 0.0;
+  for (int id : tree.leaves) {
+    const double len = (double)tree.nodes[id].len;
+    leaf2 += len * len;
+    basis += len * (double)(sn[id].rU + sn[id].rV);
+  }
+  return 8.0 * (2.0 * leaf2 + basis + (double)pool_used) + 16.0 * (double)N * nrhs;
+}
+
 // ------------------------------------------------------------------ matvec
 // g_i = V_i^T x_i; g_p = W_p^T [g_a; g_b]; f_a = R_p(a) f_p + B_ab g_b;
 // y_i = D_i x_i + U_i f_i   (hss_routines_omp.h:154-230)
 void HssEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "matvec before build");
   HMAT_REQUIRE(nrhs > 0 && ldx >= N && ldy >= N, HMAT_ERR_ARG, "bad matvec dimensions");
+  ProfScope prof("hss_matvec", stream, matvec_bytes(nrhs));
   leaf_apply(D.get(), dX, ldx, nrhs, dY, ldy);
   if (tree.max_depth == 0) {
+    prof.finish();
     HIP_CHECK(hipStreamSynchronize(stream));
     return;
   }
@@ -621,6 +643,7 @@ void HssEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int6
     }
     kern::seg_gemm_nn(nn2, Ul.get(), N, F, ldv, dY, ldy, 1.0, 1.0, stream);
   }
+  prof.finish();
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 
@@ -823,9 +846,11 @@ void HssEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64
 }
 
 void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
+  ProfScope prof("hss_solve_apply", stream, solve_bytes(nrhs));
   lres_.reserve((size_t)N * nrhs);
   leaf_solve(dB, ldb, nrhs, dX, ldx, lres_.get());
   if (tree.max_depth == 0) {
+    prof.finish();
     HIP_CHECK(hipStreamSynchronize(stream));
     return;
   }
@@ -915,6 +940,7 @@ void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* d
     }
     kern::seg_gemm_nn(up, Ql.get(), N, T, ldv, dX, ldx, -1.0, 1.0, stream);
   }
+  prof.finish();
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 

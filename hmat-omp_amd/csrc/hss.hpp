@@ -48,6 +48,12 @@ class HssEngine {
   void save(const std::string& path);
   void load(const std::string& path);
   int root_rank() const;
+  // HBM bytes one matvec / one apply-inverse must move at least: leaf
+  // blocks (and, for the refined leaf solve, their inverses), the leaf bases
+  // at their ranks, the small generator (and factor) matrices, and the
+  // right-hand sides in and solutions out
+  double matvec_bytes(int nrhs) const;
+  double solve_bytes(int nrhs) const;
   static int64_t omega_length(const hmat_options& o, int64_t N, int leaf);
 
   hmat_options opt;

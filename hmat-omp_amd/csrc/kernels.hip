@@ -1623,20 +1623,83 @@ void trunc_prep(const std::vector<SmallMat>& mats, DevBuf<SmallMat>& dm, const d
   HIP_CHECK(hipGetLastError());
 }
 
+// Batched small GEMMs, one workgroup per item: C = alpha op(A) op(B) +
+// beta C in output tiles of 64 x 32, K in chunks of 32.  Both operands are
+// loaded with consecutive lanes on consecutive addresses (either transpose)
+// and stored k-major in LDS; each thread then accumulates a 4 x 2 block of
+// the tile from LDS.  A C that aliases A or B is not supported.
+constexpr int kBgTM = 64, kBgTN = 32, kBgKC = 32;
 __global__ __launch_bounds__(kWG) void k_bgemm(const BGemm* __restrict__ items) {
+  __shared__ double As[kBgKC][kBgTM + 1];
+  __shared__ double Bs[kBgKC][kBgTN + 1];
   const BGemm g = items[blockIdx.x];
-  const int64_t total = (int64_t)g.m * g.n;
-  for (int64_t e = threadIdx.x; e < total; e += kWG) {
-    const int i = (int)(e % g.m), j = (int)(e / g.m);
-    double acc = 0.0;
-    for (int l = 0; l < g.k; ++l) {
-      const double a = g.ta ? g.A[l + (int64_t)i * g.lda] : g.A[i + (int64_t)l * g.lda];
-      const double b = g.tb ? g.B[j + (int64_t)l * g.ldb] : g.B[l + (int64_t)j * g.ldb];
-      acc = fma(a, b, acc);
+  const int tid = threadIdx.x;
+  const int ti = tid % 16, tj = tid / 16;  // rows 4 ti .. 4 ti + 3, columns 2 tj, 2 tj + 1
+  for (int i0 = 0; i0 < g.m; i0 += kBgTM)
+    for (int j0 = 0; j0 < g.n; j0 += kBgTN) {
+      double acc[4][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+      for (int l0 = 0; l0 < g.k; l0 += kBgKC) {
+        // op(A)(i0 + i, l0 + l) -> As[l][i]
+#pragma unroll
+        for (int q = 0; q < kBgTM * kBgKC / kWG; ++q) {
+          const int e = tid + q * kWG;
+          int i, l;
+          if (g.ta) {
+            l = e % kBgKC;
+            i = e / kBgKC;
+          } else {
+            i = e % kBgTM;
+            l = e / kBgTM;
+          }
+          const int gi = i0 + i, gl = l0 + l;
+          double v = 0.0;
+          if (gi < g.m && gl < g.k) v = g.ta ? g.A[gl + (int64_t)gi * g.lda] : g.A[gi + (int64_t)gl * g.lda];
+          As[l][i] = v;
+        }
+        // op(B)(l0 + l, j0 + j) -> Bs[l][j]
+#pragma unroll
+        for (int q = 0; q < kBgTN * kBgKC / kWG; ++q) {
+          const int e = tid + q * kWG;
+          int l, j;
+          if (g.tb) {
+            j = e % kBgTN;
+            l = e / kBgTN;
+          } else {
+            l = e % kBgKC;
+            j = e / kBgKC;
+          }
+          const int gl = l0 + l, gj = j0 + j;
+          double v = 0.0;
+          if (gl < g.k && gj < g.n) v = g.tb ? g.B[gj + (int64_t)gl * g.ldb] : g.B[gl + (int64_t)gj * g.ldb];
+          Bs[l][j] = v;
+        }
+        __syncthreads();
+        const int lmax = min(kBgKC, g.k - l0);
+        for (int l = 0; l < lmax; ++l) {
+          double a[4], b[2];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a[r] = As[l][4 * ti + r];
+          b[0] = Bs[l][2 * tj];
+          b[1] = Bs[l][2 * tj + 1];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            acc[r][0] = fma(a[r], b[0], acc[r][0]);
+            acc[r][1] = fma(a[r], b[1], acc[r][1]);
+          }
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const int gi = i0 + 4 * ti + r, gj = j0 + 2 * tj + c;
+          if (gi < g.m && gj < g.n) {
+            double* cp = g.C + gi + (int64_t)gj * g.ldc;
+            *cp = g.alpha * acc[r][c] + (g.beta != 0.0 ? g.beta * *cp : 0.0);
+          }
+        }
     }
-    double* c = g.C + i + (int64_t)j * g.ldc;
-    *c = g.alpha * acc + (g.beta != 0.0 ? g.beta * *c : 0.0);
-  }
 }
 
 void bgemm(const std::vector<BGemm>& items, DevBuf<BGemm>& d, hipStream_t s) {

@@ -72,6 +72,8 @@ def _proto(lib):
         "hmat_hodlr_omega_length": (i64, [C.POINTER(Options), i64, i32]),
         "hmat_hodlr_build_kernel_batch_device": (st, [vp, C.POINTER(Kernel), C.POINTER(i64), i32, i32]),
         "hmat_hodlr_batch_note": (C.c_char_p, [vp]),
+        "hmat_hss_matvec_device": (st, [vp, vp, i64, i32, vp, i64]),
+        "hmat_hss_solve_device": (st, [vp, vp, i64, i32, vp, i64]),
         "hmat_hodlr_batch_solve_device": (st, [vp, C.POINTER(Kernel), C.POINTER(i64), i32, i32, vp, i64, i32,
                                                vp, i64, i32, C.POINTER(i32)]),
         "hmat_hodlr_matvec": (st, [vp, _D, i64, i32, _D, i64]),
@@ -420,6 +422,12 @@ class HSSMatrixOMP(_Base):
         _check(self._lib.hmat_hss_build_kernel(self._h, C.byref(kernel), n, leaf_size,
                                                int(self.nested)))
         self.N = n
+
+    def multiply_device(self, x_ptr, ldx, nrhs, y_ptr, ldy):
+        _check(self._lib.hmat_hss_matvec_device(self._h, x_ptr, ldx, nrhs, y_ptr, ldy))
+
+    def solve_device(self, b_ptr, ldb, nrhs, x_ptr, ldx):
+        _check(self._lib.hmat_hss_solve_device(self._h, b_ptr, ldb, nrhs, x_ptr, ldx))
 
     def omega_length(self, n, leaf_size):
         self._sync_opts()

@@ -332,10 +332,16 @@ int64_t hmat_hss_omega_length(const hmat_options* opts, int64_t N,
 /* HSSMatrixOMP::multiply                      hss_routines_omp.h:154-230 */
 hmat_status hmat_hss_matvec(hmat_hss h, const double* X, int64_t ldx,
                             int32_t nrhs, double* Y, int64_t ldy);
+/* The same on device arrays (inputs resident in HBM; ld >= N). */
+hmat_status hmat_hss_matvec_device(hmat_hss h, const double* X, int64_t ldx,
+                                   int32_t nrhs, double* Y, int64_t ldy);
 hmat_status hmat_hss_factor(hmat_hss h);
 /* HSSMatrixOMP::solve                         hss_routines_omp.h:235-358 */
 hmat_status hmat_hss_solve(hmat_hss h, const double* B, int64_t ldb,
                            int32_t nrhs, double* X, int64_t ldx);
+/* The same on device arrays (inputs resident in HBM; ld >= N). */
+hmat_status hmat_hss_solve_device(hmat_hss h, const double* B, int64_t ldb,
+                                  int32_t nrhs, double* X, int64_t ldx);
 int64_t hmat_hss_size(hmat_hss h);
 /* rank_U of the root's children summed (reference: root->rank_U). */
 int32_t hmat_hss_root_rank(hmat_hss h);
