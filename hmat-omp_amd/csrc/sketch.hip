@@ -281,59 +281,68 @@ __global__ __launch_bounds__(kWGs) void k_sketch_lp(Gen gen, const SegSk* __rest
     }
     if (threadIdx.x < KT) rp[buf][threadIdx.x] = ppt;
   };
-  // One K-tile: generate the A fragments (two entries per 32-bit word, word
-  // jp of term sp holding entries 2jp, 2jp+1) and multiply.
+  // Entries 2jp, 2jp+1 of this lane's eight in K-step ks, both row
+  // fragments: word jp of every term (two entries per 32-bit word).
+  auto gen_pair = [&](int buf, int kb, int ks, int jp, v4u (&wd)[RF][SPLIT]) {
+    const int k0 = ks * 32 + 8 * kq;
+    float pts[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) pts[h] = rp[buf][k0 + 2 * jp + h];
+#pragma unroll
+    for (int f = 0; f < RF; ++f) {
+      float val[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int jj = 2 * jp + h;
+        const int64_t jg = sg.red0 + min(kb + k0 + jj, tl.re - 1);
+        val[h] = TRANS ? gen.eval(pts[h], opt[f], jg, iglob[f]) : gen.eval(opt[f], pts[h], iglob[f], jg);
+      }
+      if constexpr (SPLIT == 1) {
+        // the leading term of the exact split (truncation): the upper
+        // halves of both words in one v_perm.  Rounding to nearest has
+        // the same cost and the same error on scattered points, but on
+        // an equispaced Cauchy grid (a Toeplitz matrix) it made the
+        // final basis 6x less accurate.
+        wd[f][0][jp] = __builtin_amdgcn_perm(__float_as_uint(val[1]), __float_as_uint(val[0]), 0x07060302u);
+      } else {
+        uint32_t o[2][3];
+        split_bf16_bits<SPLIT>(val[0], o[0]);
+        split_bf16_bits<SPLIT>(val[1], o[1]);
+#pragma unroll
+        for (int sp = 0; sp < SPLIT; ++sp) wd[f][sp][jp] = __builtin_amdgcn_perm(o[1][sp], o[0][sp], 0x07060302u);
+      }
+    }
+  };
+  // The products of K-step ks with column tile t.
+  auto mma_tile = [&](int buf, int ks, int t, const v4u (&wd)[RF][SPLIT]) {
+    const int k0 = ks * 32 + 8 * kq;
+    mfma::v8bf b[SPLIT];
+#pragma unroll
+    for (int sp = 0; sp < SPLIT; ++sp)
+      b[sp] = *reinterpret_cast<const mfma::v8bf*>(&xs[buf][sp][t * 16 + (lane % 16)][k0]);
+#pragma unroll
+    for (int sa = 0; sa < SPLIT; ++sa)
+#pragma unroll
+      for (int sb = 0; sb < SPLIT - sa; ++sb)
+#pragma unroll
+        for (int f = 0; f < RF; ++f)
+          acc32[f][t] = mfma::bf16_16x16x32(__builtin_bit_cast(mfma::v8bf, wd[f][sa]), b[sb], acc32[f][t]);
+  };
+  // One K-tile of two K-steps: each step's entries, then its products.
+  // (Interleaving the second step's entry generation with the first step's
+  // products, down to one MFMA per three vector instructions, measured no
+  // faster: 39.4 -> 39.7 ms for the three-term pass.  A wave's BF16 MFMAs
+  // and its vector instructions do not overlap here; the pass costs the
+  // sum of the two, 768 + ~670 cycles per K-step, as its counters show.)
   auto tile = [&](int buf, int kb) {
+    static_assert(KT == 64 && NT == 4, "two K-steps, four column tiles");
+    v4u w[RF][SPLIT];
 #pragma unroll
     for (int ks = 0; ks < KT / 32; ++ks) {
-      const int k0 = ks * 32 + 8 * kq;
-      float pts[8];
 #pragma unroll
-      for (int jj = 0; jj < 8; ++jj) pts[jj] = rp[buf][k0 + jj];
-      mfma::v8bf a[RF][SPLIT];
+      for (int jp = 0; jp < 4; ++jp) gen_pair(buf, kb, ks, jp, w);
 #pragma unroll
-      for (int f = 0; f < RF; ++f) {
-        v4u wd[SPLIT];
-#pragma unroll
-        for (int jp = 0; jp < 4; ++jp) {
-          float val[2];
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const int jj = 2 * jp + h;
-            const int64_t jg = sg.red0 + min(kb + k0 + jj, tl.re - 1);
-            val[h] = TRANS ? gen.eval(pts[jj], opt[f], jg, iglob[f]) : gen.eval(opt[f], pts[jj], iglob[f], jg);
-          }
-          if constexpr (SPLIT == 1) {
-            // the leading term of the exact split (truncation): the upper
-            // halves of both words in one v_perm.  Rounding to nearest has
-            // the same cost and the same error on scattered points, but on
-            // an equispaced Cauchy grid (a Toeplitz matrix) it made the
-            // final basis 6x less accurate.
-            wd[0][jp] = __builtin_amdgcn_perm(__float_as_uint(val[1]), __float_as_uint(val[0]), 0x07060302u);
-          } else {
-            uint32_t o[2][3];
-            split_bf16_bits<SPLIT>(val[0], o[0]);
-            split_bf16_bits<SPLIT>(val[1], o[1]);
-#pragma unroll
-            for (int sp = 0; sp < SPLIT; ++sp) wd[sp][jp] = __builtin_amdgcn_perm(o[1][sp], o[0][sp], 0x07060302u);
-          }
-        }
-#pragma unroll
-        for (int sp = 0; sp < SPLIT; ++sp) a[f][sp] = __builtin_bit_cast(mfma::v8bf, wd[sp]);
-      }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        mfma::v8bf b[SPLIT];
-#pragma unroll
-        for (int sp = 0; sp < SPLIT; ++sp)
-          b[sp] = *reinterpret_cast<const mfma::v8bf*>(&xs[buf][sp][t * 16 + (lane % 16)][k0]);
-#pragma unroll
-        for (int sa = 0; sa < SPLIT; ++sa)
-#pragma unroll
-          for (int sb = 0; sb < SPLIT - sa; ++sb)
-#pragma unroll
-            for (int f = 0; f < RF; ++f) acc32[f][t] = mfma::bf16_16x16x32(a[f][sa], b[sb], acc32[f][t]);
-      }
+      for (int t = 0; t < NT; ++t) mma_tile(buf, ks, t, w);
     }
   };
   commit(0, tl.rb, issue(tl.rb));
