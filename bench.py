@@ -440,7 +440,8 @@ def main():
     ap.add_argument("--cpu-sizes", default="4096,8192,16384")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--modes", default="fp64,mixed",
+                    help="precision legs of the headline workload ('' runs only the batch and HSS legs)")
     ap.add_argument("--matvec-reps", type=int, default=20)
     ap.add_argument("--solve-reps", type=int, default=10)
     ap.add_argument("--no-profile", dest="profile", action="store_false",
@@ -477,7 +478,17 @@ def main():
     dev = {"x": torch.tensor(np.arange(n) / n, dtype=torch.float64, device="cuda"),
            "y": torch.tensor((np.arange(n) + 0.5) / n, dtype=torch.float64, device="cuda"),
            "b": torch.tensor(np.random.default_rng(0).standard_normal(n), dtype=torch.float64, device="cuda")}
-    modes = a.modes.split(",")
+    modes = [m for m in a.modes.split(",") if m]
+    if not modes:
+        # only the batch and HSS legs (kernel profiles of those workloads)
+        out = {"metric": METRIC, "value": None, "unit": "s", "n_gpus": 1,
+               "note": "headline legs skipped (--modes '')"}
+        if a.batch_steps > 0:
+            out["batch"] = batch_leg(hmat_amd, torch, a.batch_steps, 1, a.profile)
+        if not a.no_hss:
+            out["hss_hodlr_n65536"] = hss_leg(hmat_amd, torch)
+        print(json.dumps(out), flush=True)
+        return
     res = {}
     for m in modes:
         r, h = run_mode(hmat_amd, torch, m, a, a.q, a.steps, a.warmup, dev)
