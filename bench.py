@@ -266,11 +266,15 @@ def hss_leg(hmat_amd, torch, reps=10):
     o = hmat_amd.make_options(0.0, rank=rank, fixed_rank=True, oversample=10, power_iters=2, exact_below=0,
                               omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=1)
     out = {"workload": f"Gaussian sigma {sigma} lambda {lam}, N={n} sorted U[0,1], leaf {leaf}, rank {rank}, "
-                       f"p 10, q 2, {nrhs} RHS, FP64 (nested HSS and HODLR)"}
-    for name in ("hss", "hodlr"):
+                       f"p 10, q 2, {nrhs} RHS, FP64 (nested HSS from N-wide leaf blocks, nested HSS from "
+                       f"one sample pass per side (q does not apply), and HODLR)"}
+    for name in ("hss", "hss_sampled", "hodlr"):
         if name == "hss":
             h = hmat_amd.HSSMatrixOMP(opts=o, nested=True)
             k = hmat_amd.make_kernel(hmat_amd.KERNEL_GAUSSIAN, pts, p0=sigma, p1=lam)
+        elif name == "hss_sampled":
+            h = hmat_amd.HSSMatrixOMP(opts=o, nested=hmat_amd.HSS_SAMPLED)
+            k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_GAUSSIAN, dp.data_ptr(), p0=sigma, p1=lam)
         else:
             h = hmat_amd.HODLRMatrixOMP(opts=o)
             k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_GAUSSIAN, dp.data_ptr(), p0=sigma, p1=lam)
@@ -294,7 +298,7 @@ def hss_leg(hmat_amd, torch, reps=10):
         h.multiply_device(dx.data_ptr(), n, nrhs, dy.data_ptr(), n)
         torch.cuda.synchronize()
         r["resid_AH"] = float(torch.linalg.norm(db - dy) / torch.linalg.norm(db))
-        if name == "hss":
+        if name != "hodlr":
             r["root_rank"] = h.root_rank()
         hmat_amd.profile_reset()
         hmat_amd.profile_enable(True)
@@ -304,8 +308,8 @@ def hss_leg(hmat_amd, torch, reps=10):
             h.solve_device(db.data_ptr(), n, nrhs, dx.data_ptr(), n)
         torch.cuda.synchronize()
         hmat_amd.profile_enable(False)
-        for key, pname in (("matvec", "hss_matvec" if name == "hss" else "matvec"),
-                           ("solve", "hss_solve_apply" if name == "hss" else "solve_apply")):
+        for key, pname in (("matvec", "matvec" if name == "hodlr" else "hss_matvec"),
+                           ("solve", "solve_apply" if name == "hodlr" else "hss_solve_apply")):
             pq = hmat_amd.profile_query(pname)
             if pq and pq[0]:
                 ms = pq[1] / pq[0]
@@ -316,6 +320,62 @@ def hss_leg(hmat_amd, torch, reps=10):
         out[name] = r
         del h
     return out
+
+
+def hss_cauchy_leg(hmat_amd, torch, dev, n, leaf, rank, p, rtol, reps=10):
+    """Nested HSS (one sample pass per side) on the precision benchmark's
+    system: 1/(x_i - y_j) on the grid, N = 2^18, leaf 256, fixed rank 48,
+    p 10, FP64 with refinement through the HSS matvec to rtol.  Build,
+    factor and solve times (second call on the handle), the residual
+    against A_H and the true A, and the matvec / apply-inverse HBM rates
+    (algorithmic bytes over HIP-event times of `reps` calls each)."""
+    o = hmat_amd.make_options(0.0, rank=rank, fixed_rank=True, oversample=p, power_iters=0, exact_below=0,
+                              omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=0x5EED)
+    h = hmat_amd.HSSMatrixOMP(opts=o, nested=hmat_amd.HSS_SAMPLED)
+    h.set_refinement(rtol, 10)
+    k = hmat_amd.make_kernel_device(hmat_amd.KERNEL_CAUCHY, dev["x"].data_ptr(), dev["y"].data_ptr())
+    b = dev["b"]
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    y = torch.zeros_like(x)
+    r = {"workload": f"HSS (sampled nested bases) Cauchy 1/(x_i-y_j), N={n}, leaf {leaf}, rank {rank}, p {p}, "
+                     f"1 RHS, FP64 + refinement to {rtol}"}
+    for it in range(2):  # the first build on a fresh handle allocates
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h.build_kernel_device(k, n, leaf)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        h.factor()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        h.solve_device(b.data_ptr(), n, 1, x.data_ptr(), n)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        if it == 0:
+            r["first_call_s"] = round(t3 - t0, 4)
+    r.update(build_s=round(t1 - t0, 4), factor_s=round(t2 - t1, 4), solve_s=round(t3 - t2, 5),
+             total_s=round(t3 - t0, 4), root_rank=h.root_rank())
+    its, res = h.last_refinement()
+    r.update(refine_iters=its, resid_AH=res, resid_trueA=true_residual(hmat_amd, torch, x, dev, n))
+    hmat_amd.profile_reset()
+    hmat_amd.profile_enable(True)
+    for _ in range(reps):
+        h.multiply_device(b.data_ptr(), n, 1, y.data_ptr(), n)
+    h.set_refinement(0.0)
+    for _ in range(reps):
+        h.solve_device(b.data_ptr(), n, 1, x.data_ptr(), n)
+    torch.cuda.synchronize()
+    hmat_amd.profile_enable(False)
+    for key, pname in (("matvec", "hss_matvec"), ("solve", "hss_solve_apply")):
+        pq = hmat_amd.profile_query(pname)
+        if pq and pq[0]:
+            ms = pq[1] / pq[0]
+            gbs = pq[2] / pq[0] / (ms * 1e-3) / 1e9
+            r[key] = {"avg_ms": round(ms, 4), "bytes": pq[2] / pq[0], "GBs": round(gbs, 1),
+                      "frac_hbm": round(gbs / PEAK_HBM_GBS, 4), "calls": pq[0]}
+    log(f"[hss_cauchy] {r}")
+    del h
+    return r
 
 
 def true_residual(hmat_amd, torch, x, dev, n):
@@ -450,7 +510,8 @@ def main():
                     help="keep the default tilings (the autotuner runs in the warm-up step otherwise)")
     ap.add_argument("--batch-steps", type=int, default=3,
                     help="timed steps of the batched leg (256 systems); 0 skips it")
-    ap.add_argument("--no-hss", action="store_true", help="skip the HSS + HODLR leg (N=65536, 16 RHS)")
+    ap.add_argument("--no-hss", action="store_true",
+                    help="skip the HSS legs (HSS + HODLR at N=65536 with 16 RHS; sampled HSS on the headline system)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (counter calibration)")
     a = ap.parse_args()
@@ -487,6 +548,7 @@ def main():
             out["batch"] = batch_leg(hmat_amd, torch, a.batch_steps, 1, a.profile)
         if not a.no_hss:
             out["hss_hodlr_n65536"] = hss_leg(hmat_amd, torch)
+            out["hss_cauchy_n262144"] = hss_cauchy_leg(hmat_amd, torch, dev, n, a.leaf, a.rank, a.oversample, a.rtol)
         print(json.dumps(out), flush=True)
         return
     res = {}
@@ -606,6 +668,7 @@ def main():
         out["batch"] = batch_leg(hmat_amd, torch, a.batch_steps, 1, a.profile)
     if not a.no_hss:
         out["hss_hodlr_n65536"] = hss_leg(hmat_amd, torch)
+        out["hss_cauchy_n262144"] = hss_cauchy_leg(hmat_amd, torch, dev, n, a.leaf, a.rank, a.oversample, a.rtol)
     if not a.no_cpu:
         cb = cpu_baseline(a)
         if cb:

@@ -440,7 +440,7 @@ hmat_status hmat_hss_build_dense(hmat_hss h, const double* A, int64_t lda, int64
     g.kind = GenKind::Dense;
     g.A = h->a.get();
     g.lda = N;
-    h->eng.build(g, N, leaf, nested != 0, omega, omega_len);
+    h->eng.build(g, N, leaf, nested, omega, omega_len);
     h->a.release();
   });
 }
@@ -453,8 +453,23 @@ hmat_status hmat_hss_build_kernel(hmat_hss h, const hmat_kernel* k, int64_t N, i
     upload_cols(h->eng.px, k->x, N, N, 1, h->eng.stream);
     if (k->y) upload_cols(h->eng.py, k->y, N, N, 1, h->eng.stream);
     GenDesc g = gen_from_kernel(*k, h->eng.px.get(), k->y ? h->eng.py.get() : nullptr);
-    h->eng.build(g, N, leaf, nested != 0, nullptr, 0);
+    h->eng.build(g, N, leaf, nested, nullptr, 0);
   });
+}
+
+hmat_status hmat_hss_build_kernel_device(hmat_hss h, const hmat_kernel* k, int64_t N, int32_t leaf,
+                                         int32_t nested) {
+  return guard([&] {
+    HMAT_REQUIRE(h && k && k->x && N > 0 && leaf > 0, HMAT_ERR_ARG, "bad build_kernel arguments");
+    HMAT_REQUIRE(k->type != HMAT_KERNEL_CAUCHY || k->y, HMAT_ERR_ARG, "Cauchy kernel needs y points");
+    GenDesc g = gen_from_kernel(*k, k->x, k->y);
+    h->eng.build(g, N, leaf, nested, nullptr, 0);
+  });
+}
+
+int64_t hmat_hss_sampled_omega_length(const hmat_options* opts, int64_t N) {
+  if (!opts || N <= 0) return -1;
+  return HssEngine::sampled_omega_length(*opts, N);
 }
 
 hmat_status hmat_hss_matvec(hmat_hss h, const double* X, int64_t ldx, int32_t nrhs, double* Y,

@@ -492,15 +492,431 @@ void HssEngine::build_internal() {
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 
-void HssEngine::build(const GenDesc& g, int64_t n, int leaf_size, bool nested_, const double* host_omega,
+int HssEngine::sampled_width(const hmat_options& o, int64_t N) {
+  return (int)std::min<int64_t>((int64_t)o.rank + o.oversample, N);
+}
+int64_t HssEngine::sampled_omega_length(const hmat_options& o, int64_t N) {
+  return 2 * N * (int64_t)sampled_width(o, N);
+}
+
+int64_t HssEngine::spool_alloc(int64_t n) {
+  SampleScratch& sc = ss_;
+  const int64_t off = sc.sp_used;
+  sc.sp_used += round_up((int)std::max<int64_t>(n, 1), 32);
+  if ((size_t)sc.sp_used > sc.sp.size()) {
+    DevBuf<double> bigger((size_t)std::max<int64_t>(sc.sp_used * 2, 1 << 16));
+    if (off > 0)
+      HIP_CHECK(hipMemcpyAsync(bigger.get(), sc.sp.get(), off * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    sc.sp = std::move(bigger);
+  }
+  return off;
+}
+
+// Nested generators from one sample pass per side (Martinsson 2011, SIMAX
+// 32(4), orthonormal bases).  With s = rank + oversample Gaussian columns:
+//   leaf i:   Yh_i = A(I, I^c) Omega(I^c)  (the sketch with the leaf's own
+//             columns excluded), U_i = leading left singular vectors of Yh_i,
+//             yh_i = U_i^T Yh_i; om_i = V_i^T Omega(I), ps_i = U_i^T Psi(I);
+//             the column side likewise from Zh_i = A(I^c, I)^T Psi(I^c);
+//   siblings: B_ab = U_a^T A(I_a, I_b) V_b (full bases; one generated-entry
+//             product per depth, then a segmented U^T P);
+//   parent p: M_p = [yh_a - B_ab om_b; yh_b - B_ba om_a] -> R_p (its leading
+//             left singular vectors), yh_p = R_p^T M_p, om_p = W_p^T [om_a; om_b],
+//             and the mirror image for W_p, zh_p, ps_p.
+// The reduced samples are kept transposed (s rows) in one device pool:
+// yhT, zhT, omT, psT of node p at spo[4p + 0..3], leading dimension s.
+// Singular vectors of a small M (m x s) come from one-sided Jacobi on M^T
+// zero-padded to max(m, s) square: M^T V = U S gives M = V S U^T, so the
+// basis is V(:m, :r) and (the reduced samples)^T = U(:s, :r) S.  Tall leaf
+// samples (len > s) are first factored Yh = Q R (CholeskyQR), Jacobi then
+// runs on R^T and the basis is Q V(:, :r).
+void HssEngine::build_sampled(const GenDesc& g, const double* host_omega, int64_t omega_len) {
+  (void)omega_len;
+  const int ks = sampled_width(opt, N);
+  const auto& L = tree.leaves;
+  const int nl = (int)L.size();
+  const int nn = (int)tree.nodes.size();
+  const int dmax = tree.max_depth;
+  SampleScratch& sc = ss_;
+  sc.sp_used = 0;
+  std::vector<int64_t> spo(4 * (size_t)nn, -1);
+  const int64_t nks = N * (int64_t)ks;
+  sc.Om.reserve(nks);
+  sc.Ps.reserve(nks);
+  sc.Y.reserve(nks);
+  sc.Z.reserve(nks);
+  sc.Tb.reserve(nks);
+  if (host_omega) {
+    HIP_CHECK(hipMemcpyAsync(sc.Om.get(), host_omega, nks * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(sc.Ps.get(), host_omega + nks, nks * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+  } else {
+    std::vector<SegNN> r0{SegNN{0, 0x5A301, 0, (int32_t)N, 0, ks, 0}};
+    std::vector<SegNN> r1{SegNN{0, 0x5A302, 0, (int32_t)N, 0, ks, 0}};
+    kern::fill_random_segs(r0, dnn_, sc.Om.get(), N, opt.seed, opt.omega_dist, stream);
+    kern::fill_random_segs(r1, dnn_, sc.Ps.get(), N, opt.seed, opt.omega_dist, stream);
+  }
+  // ---- leaf samples: Y(I) = A(I, I^c) Omega, Z(I) = A(I^c, I)^T Psi
+  {
+    std::vector<SegSk> sk;
+    for (int id : L) {
+      const TNode& nd = tree.nodes[id];
+      sk.push_back(SegSk{nd.start, 0, nd.start, 0, nd.start, (int32_t)nd.len, (int32_t)N, (int32_t)nd.len, ks});
+    }
+    kern::sketch(g, false, sk, sc.Om.get(), N, sc.Y.get(), N, ks, ws_, stream);
+    kern::sketch(g, true, sk, sc.Ps.get(), N, sc.Z.get(), N, ks, ws_, stream);
+  }
+  // rank rule of compression.h:42-46 on the singular values of a sample
+  auto pick_rank = [&](const double* sv, int nsv) {
+    for (int j = 0; j < nsv; ++j)
+      if (!std::isfinite(sv[j])) throw Error(HMAT_ERR_NONFINITE, "non-finite singular value in HSS build");
+    if (opt.fixed_rank) return std::max(1, std::min(opt.rank, nsv));
+    int r = 0;
+    if (sv[0] > 1e-15)
+      for (int j = 0; j < nsv; ++j) r += (sv[j] > opt.tol * sv[0]) ? 1 : 0;
+    return std::max(r, 1);
+  };
+  // ---- leaf bases, both sides
+  const int64_t kk = (int64_t)ks * ks;
+  sc.Sm.reserve((size_t)nl * kk);
+  sc.Rt.reserve((size_t)nl * kk);
+  sc.Vr.reserve((size_t)nl * kk);
+  sc.Sv.reserve((size_t)nl * ks);
+  sc.Wk.reserve((size_t)2 * nl * kk);
+  sc.TU.reserve((size_t)nl * kk);
+  sc.TV.reserve((size_t)nl * kk);
+  for (int side = 0; side < 2; ++side) {
+    double* Ys = side ? sc.Z.get() : sc.Y.get();
+    std::vector<OrthSeg> tall;
+    std::vector<int> tall_leaf;
+    std::vector<Rect> small;
+    for (int i = 0; i < nl; ++i) {
+      const TNode& nd = tree.nodes[L[i]];
+      if (nd.len > ks) {
+        tall.push_back(OrthSeg{nd.start, (int32_t)nd.len, ks});
+        tall_leaf.push_back(i);
+      } else {
+        small.push_back(Rect{Ys + nd.start, sc.Sm.get() + (int64_t)i * kk, N, ks, (int32_t)nd.len, ks, 1.0, 0.0});
+      }
+    }
+    if (!small.empty()) zero_bytes_async(sc.Sm.get(), (size_t)nl * kk * sizeof(double), stream);
+    if (!tall.empty()) {
+      // Yh = Q R in place (Q overwrites the samples), then R^T into the leaf's slot
+      orth_.run(tall, Ys, sc.Tb.get(), N, ks, sc.Rt.get(), stream);
+      std::vector<Rect> rt;
+      for (size_t b = 0; b < tall.size(); ++b)
+        rt.push_back(Rect{sc.Rt.get() + (int64_t)b * kk, sc.Sm.get() + (int64_t)tall_leaf[b] * kk, ks, ks, ks, ks,
+                          1.0, 0.0});
+      kern::rect_transpose(rt, drc_, stream);
+    }
+    kern::rect_transpose(small, drc_, stream);
+    std::vector<SmallMat> sm(nl);
+    for (int i = 0; i < nl; ++i) sm[i] = SmallMat{(int64_t)i * kk, ks, ks};
+    kern::jacobi_svd(sm, dsm_, sc.Sm.get(), sc.Vr.get(), sc.Sv.get(), ks, sc.Wk.get(), stream);
+    const auto S = download(sc.Sv.get(), (size_t)nl * ks, stream);
+    std::vector<int> rank(nl);
+    int rmax = 1;
+    for (int i = 0; i < nl; ++i) {
+      const int nsv = (int)std::min<int64_t>(tree.nodes[L[i]].len, ks);
+      rank[i] = pick_rank(&S[(size_t)i * ks], nsv);
+      rmax = std::max(rmax, rank[i]);
+    }
+    DevBuf<int> drank;
+    drank.upload(rank, stream);
+    // TU = V(:, :r) (basis coefficients), TV = U(:, :r) S (reduced samples^T)
+    kern::trunc_prep(sm, dsm_, sc.Sm.get(), sc.Vr.get(), sc.Sv.get(), ks, drank.get(), rmax, sc.TU.get(),
+                     sc.TV.get(), 1, stream);
+    DevBuf<double>& basis = side ? Vl : Ul;
+    basis.resize((size_t)N * rmax);
+    basis.zero(stream);
+    std::vector<SegNN> qt;
+    std::vector<Rect> cp, red;
+    for (int i = 0; i < nl; ++i) {
+      const int id = L[i];
+      const TNode& nd = tree.nodes[id];
+      if (nd.len > ks)
+        qt.push_back(SegNN{nd.start, (int64_t)i * kk, nd.start, (int32_t)nd.len, ks, rank[i], 0});
+      else
+        cp.push_back(Rect{sc.TU.get() + (int64_t)i * kk, basis.get() + nd.start, ks, N, (int32_t)nd.len, rank[i],
+                          1.0, 0.0});
+      spo[4 * id + side] = spool_alloc((int64_t)ks * rank[i]);
+      if (side) sn[id].rV = rank[i];
+      else sn[id].rU = rank[i];
+    }
+    for (int i = 0; i < nl; ++i)
+      red.push_back(Rect{sc.TV.get() + (int64_t)i * kk, sc.sp.get() + spo[4 * L[i] + side], ks, ks, ks, rank[i],
+                         1.0, 0.0});
+    kern::seg_gemm_nn(qt, Ys, N, sc.TU.get(), ks, basis.get(), N, 1.0, 0.0, stream);
+    kern::rect_axpby(cp, drc_, stream);
+    kern::rect_axpby(red, drc_, stream);
+    if (side) rVmax = rmax;
+    else rUmax = rmax;
+  }
+  // omT_i = Omega(I)^T V_i, psT_i = Psi(I)^T U_i
+  {
+    std::vector<SegTN> to, tp;
+    for (int id : L) {
+      spo[4 * id + 2] = spool_alloc((int64_t)ks * sn[id].rV);
+      spo[4 * id + 3] = spool_alloc((int64_t)ks * sn[id].rU);
+    }
+    for (int id : L) {
+      const TNode& nd = tree.nodes[id];
+      to.push_back(SegTN{nd.start, nd.start, spo[4 * id + 2], (int32_t)nd.len, ks, sn[id].rV, 0});
+      tp.push_back(SegTN{nd.start, nd.start, spo[4 * id + 3], (int32_t)nd.len, ks, sn[id].rU, 0});
+    }
+    kern::seg_gemm_tn(to, sc.Om.get(), N, Vl.get(), N, sc.sp.get(), ks, 1.0, 0.0, ws_, stream);
+    kern::seg_gemm_tn(tp, sc.Ps.get(), N, Ul.get(), N, sc.sp.get(), ks, 1.0, 0.0, ws_, stream);
+  }
+  // ---- full bases of one depth: leaf bases, or diag(F_a, F_b) T_p from the
+  // child depth's array Fc (T = R for side 0, W for side 1)
+  auto make_full = [&](const std::vector<int>& ids, int side, const double* Fc, DevBuf<double>& F) {
+    int r = 1, ld = 1;
+    for (int id : ids) {
+      r = std::max(r, side ? sn[id].rV : sn[id].rU);
+      if (!tree.nodes[id].leaf) ld = side ? sn[id].ldW : sn[id].ldR;
+    }
+    F.reserve((size_t)N * r);
+    std::vector<SegNN> cp, mm;
+    for (int id : ids) {
+      const TNode& nd = tree.nodes[id];
+      const int rp = side ? sn[id].rV : sn[id].rU;
+      if (nd.leaf) {
+        cp.push_back(SegNN{nd.start, 0, nd.start, (int32_t)nd.len, 0, rp, 0});
+      } else {
+        const TNode& a = tree.nodes[nd.left];
+        const TNode& b = tree.nodes[nd.right];
+        const int ra = side ? sn[nd.left].rV : sn[nd.left].rU;
+        const int rb = side ? sn[nd.right].rV : sn[nd.right].rU;
+        const int64_t o = side ? sn[id].oW : sn[id].oR;
+        mm.push_back(SegNN{a.start, o, a.start, (int32_t)a.len, ra, rp, 0});
+        mm.push_back(SegNN{b.start, o + ra, b.start, (int32_t)b.len, rb, rp, 0});
+      }
+    }
+    kern::copy_segs(cp, dnn_, side ? Vl.get() : Ul.get(), N, F.get(), N, stream);
+    kern::seg_gemm_nn(mm, Fc, N, pool.get(), ld, F.get(), N, 1.0, 0.0, stream);
+  };
+  DevBuf<double>*UFc = &sc.UF[0], *VFc = &sc.VF[0], *UFn = &sc.UF[1], *VFn = &sc.VF[1];
+  make_full(tree.by_depth[dmax], 0, nullptr, *UFc);
+  make_full(tree.by_depth[dmax], 1, nullptr, *VFc);
+  for (int d = dmax - 1; d >= 0; --d) {
+    std::vector<int> inner;
+    for (int id : tree.by_depth[d])
+      if (!tree.nodes[id].leaf) inner.push_back(id);
+    const int ni = (int)inner.size();
+    // ---- couplings of the children: P(I_a) = A(I_a, I_b) V_b, B_ab = U_a^T P(I_a)
+    int ldb = 1, kv = 1;
+    for (int id : inner) {
+      const TNode& nd = tree.nodes[id];
+      ldb = std::max(ldb, std::max(sn[nd.left].rU, sn[nd.right].rU));
+      kv = std::max(kv, std::max(sn[nd.left].rV, sn[nd.right].rV));
+    }
+    for (int id : inner) {
+      const TNode& nd = tree.nodes[id];
+      sn[id].ldB = ldb;
+      sn[id].oBab = pool_alloc((int64_t)ldb * sn[nd.right].rV);
+      sn[id].oBba = pool_alloc((int64_t)ldb * sn[nd.left].rV);
+    }
+    sc.P.reserve((size_t)N * kv);
+    {
+      std::vector<SegSk> sk;
+      std::vector<SegTN> tn;
+      for (int id : inner) {
+        const TNode& nd = tree.nodes[id];
+        const TNode& a = tree.nodes[nd.left];
+        const TNode& b = tree.nodes[nd.right];
+        const int ua = sn[nd.left].rU, ub = sn[nd.right].rU, va = sn[nd.left].rV, vb = sn[nd.right].rV;
+        sk.push_back(SegSk{a.start, b.start, 0, b.start, a.start, (int32_t)a.len, (int32_t)b.len, 0, vb});
+        sk.push_back(SegSk{b.start, a.start, 0, a.start, b.start, (int32_t)b.len, (int32_t)a.len, 0, va});
+        tn.push_back(SegTN{a.start, a.start, sn[id].oBab, (int32_t)a.len, ua, vb, 0});
+        tn.push_back(SegTN{b.start, b.start, sn[id].oBba, (int32_t)b.len, ub, va, 0});
+      }
+      kern::sketch(g, false, sk, VFc->get(), N, sc.P.get(), N, kv, ws_, stream);
+      kern::seg_gemm_tn(tn, UFc->get(), N, sc.P.get(), N, pool.get(), ldb, 1.0, 0.0, ws_, stream);
+    }
+    if (d == 0) break;  // the root has no transfer
+    // ---- local samples: side 0 M_p = [yh_a - B_ab om_b; yh_b - B_ba om_a],
+    // side 1 N_p = [zh_a - B_ba^T ps_b; zh_b - B_ab^T ps_a] (m x s).  Tall
+    // ones (m > s) are stored as they are in Mt (rows moff_j .., leading
+    // dimension Lt) and factored M = Q R; R^T goes to the entry's Jacobi
+    // slot.  Small ones go to the slot as M^T zero-padded to s x s.
+    const size_t nmat = 2 * (size_t)ni;
+    std::vector<int> mrows(nmat);
+    std::vector<int64_t> moff(nmat, -1);
+    int64_t Lt = 0;
+    int ldT = 1;
+    for (int i = 0; i < ni; ++i) {
+      const TNode& nd = tree.nodes[inner[i]];
+      mrows[i] = sn[nd.left].rU + sn[nd.right].rU;
+      mrows[ni + i] = sn[nd.left].rV + sn[nd.right].rV;
+    }
+    for (size_t j = 0; j < nmat; ++j) {
+      ldT = std::max(ldT, mrows[j]);
+      if (mrows[j] > ks) {
+        moff[j] = Lt;
+        Lt += mrows[j];
+      }
+    }
+    sc.Sm.reserve(nmat * kk);
+    sc.Rt.reserve(nmat * kk);
+    sc.Vr.reserve(nmat * kk);
+    sc.Sv.reserve(nmat * ks);
+    sc.Wk.reserve(2 * nmat * kk);
+    sc.TU.reserve(nmat * kk);
+    sc.TV.reserve(nmat * kk);
+    sc.Mt.reserve((size_t)std::max<int64_t>(Lt, 1) * ks);
+    sc.Tb.reserve((size_t)std::max<int64_t>(Lt, 1) * ks);
+    zero_bytes_async(sc.Sm.get(), nmat * kk * sizeof(double), stream);
+    double* SP = sc.sp.get();
+    double* PL = pool.get();
+    double* MT = sc.Mt.get();
+    std::vector<Rect> cpT, cpS;  // reduced samples into tall M (transposed copy) / small M^T (copy)
+    std::vector<BGemm> up;
+    for (int i = 0; i < ni; ++i) {
+      const int id = inner[i];
+      const TNode& nd = tree.nodes[id];
+      const int a = nd.left, b = nd.right;
+      const int ua = sn[a].rU, ub = sn[b].rU, va = sn[a].rV, vb = sn[b].rV;
+      const int64_t ldB = sn[id].ldB;
+      const double* Bab = PL + sn[id].oBab;
+      const double* Bba = PL + sn[id].oBba;
+      for (int side = 0; side < 2; ++side) {
+        const size_t j = side ? ni + i : i;
+        const int r0 = side ? va : ua, r1 = side ? vb : ub;
+        const double* s0 = SP + spo[4 * a + side];
+        const double* s1 = SP + spo[4 * b + side];
+        // subtracted products (first block, second block): side 0 B_ab om_b,
+        // B_ba om_a; side 1 B_ba^T ps_b, B_ab^T ps_a (om, ps stored transposed)
+        const double* f0 = side ? Bba : Bab;
+        const double* f1 = side ? Bab : Bba;
+        const double* g0 = SP + spo[4 * b + (side ? 3 : 2)];
+        const double* g1 = SP + spo[4 * a + (side ? 3 : 2)];
+        const int k0 = side ? ub : vb, k1 = side ? ua : va;
+        if (mrows[j] > ks) {
+          double* m = MT + moff[j];
+          cpT.push_back(Rect{s0, m, ks, Lt, ks, r0, 1.0, 0.0});
+          cpT.push_back(Rect{s1, m + r0, ks, Lt, ks, r1, 1.0, 0.0});
+          // M(rows 0:r0) -= op(f0) g0^T
+          up.push_back(BGemm{f0, g0, m, r0, ks, k0, (int32_t)ldB, ks, (int32_t)Lt, side, 1, -1.0, 1.0});
+          up.push_back(BGemm{f1, g1, m + r0, r1, ks, k1, (int32_t)ldB, ks, (int32_t)Lt, side, 1, -1.0, 1.0});
+        } else {
+          double* m = sc.Sm.get() + (int64_t)j * kk;
+          cpS.push_back(Rect{s0, m, ks, ks, ks, r0, 1.0, 0.0});
+          cpS.push_back(Rect{s1, m + (int64_t)r0 * ks, ks, ks, ks, r1, 1.0, 0.0});
+          // M^T(:, 0:r0) -= g0 op(f0)^T
+          up.push_back(BGemm{g0, f0, m, ks, r0, k0, ks, (int32_t)ldB, ks, 0, side ? 0 : 1, -1.0, 1.0});
+          up.push_back(BGemm{g1, f1, m + (int64_t)r0 * ks, ks, r1, k1, ks, (int32_t)ldB, ks, 0, side ? 0 : 1, -1.0,
+                             1.0});
+        }
+      }
+    }
+    kern::rect_transpose(cpT, drc_, stream);
+    kern::rect_axpby(cpS, drc_, stream);
+    kern::bgemm(up, dbg_, stream);
+    {
+      std::vector<OrthSeg> tall;
+      std::vector<Rect> rt;
+      for (size_t j = 0; j < nmat; ++j)
+        if (mrows[j] > ks) {
+          rt.push_back(Rect{sc.Rt.get() + (int64_t)tall.size() * kk, sc.Sm.get() + (int64_t)j * kk, ks, ks, ks, ks,
+                            1.0, 0.0});
+          tall.push_back(OrthSeg{moff[j], mrows[j], ks});
+        }
+      if (!tall.empty()) orth_.run(tall, MT, sc.Tb.get(), Lt, ks, sc.Rt.get(), stream);
+      kern::rect_transpose(rt, drc_, stream);
+    }
+    std::vector<SmallMat> sm(nmat);
+    for (size_t j = 0; j < nmat; ++j) sm[j] = SmallMat{(int64_t)j * kk, ks, ks};
+    kern::jacobi_svd(sm, dsm_, sc.Sm.get(), sc.Vr.get(), sc.Sv.get(), ks, sc.Wk.get(), stream);
+    const auto S = download(sc.Sv.get(), nmat * ks, stream);
+    std::vector<int> rk(nmat);
+    int rm = 1;
+    for (size_t j = 0; j < nmat; ++j) {
+      rk[j] = pick_rank(&S[j * ks], std::min(mrows[j], ks));
+      rm = std::max(rm, rk[j]);
+    }
+    DevBuf<int> drk;
+    drk.upload(rk, stream);
+    // TU = V(:, :r), TV = U(:, :r) S
+    kern::trunc_prep(sm, dsm_, sc.Sm.get(), sc.Vr.get(), sc.Sv.get(), ks, drk.get(), rm, sc.TU.get(), sc.TV.get(),
+                     1, stream);
+    // transfers (R_p, W_p: Q V(:, :r) for tall samples, V(:m, :r) for small
+    // ones) and reduced samples yhT_p, zhT_p = TV(:, :r)
+    for (int i = 0; i < ni; ++i) {
+      const int id = inner[i];
+      sn[id].rU = rk[i];
+      sn[id].rV = rk[ni + i];
+      sn[id].ldR = ldT;
+      sn[id].ldW = ldT;
+      sn[id].oR = pool_alloc((int64_t)ldT * rk[i]);
+      sn[id].oW = pool_alloc((int64_t)ldT * rk[ni + i]);
+      for (int q = 0; q < 4; ++q) spo[4 * id + q] = spool_alloc((int64_t)ks * ((q == 0 || q == 3) ? rk[i] : rk[ni + i]));
+    }
+    SP = sc.sp.get();
+    PL = pool.get();
+    std::vector<Rect> st;
+    std::vector<SegNN> qv;
+    for (int i = 0; i < ni; ++i) {
+      const int id = inner[i];
+      for (int side = 0; side < 2; ++side) {
+        const size_t j = side ? ni + i : i;
+        const int64_t ot = side ? sn[id].oW : sn[id].oR;
+        if (mrows[j] > ks)
+          qv.push_back(SegNN{moff[j], (int64_t)j * kk, ot, mrows[j], ks, rk[j], 0});
+        else
+          st.push_back(Rect{sc.TU.get() + (int64_t)j * kk, PL + ot, ks, ldT, mrows[j], rk[j], 1.0, 0.0});
+        st.push_back(Rect{sc.TV.get() + (int64_t)j * kk, SP + spo[4 * id + side], ks, ks, ks, rk[j], 1.0, 0.0});
+      }
+    }
+    kern::seg_gemm_nn(qv, (const double*)MT, Lt, (const double*)sc.TU.get(), ks, PL, (int64_t)ldT, 1.0, 0.0,
+                      stream);
+    kern::rect_axpby(st, drc_, stream);
+    // omT_p = [omT_a omT_b] W_p, psT_p = [psT_a psT_b] R_p (two launches: one C per item each)
+    std::vector<BGemm> r1, r2;
+    for (int i = 0; i < ni; ++i) {
+      const int id = inner[i];
+      const TNode& nd = tree.nodes[id];
+      const int a = nd.left, b = nd.right;
+      const int ua = sn[a].rU, ub = sn[b].rU, va = sn[a].rV, vb = sn[b].rV;
+      r1.push_back(BGemm{SP + spo[4 * a + 2], PL + sn[id].oW, SP + spo[4 * id + 2], ks, sn[id].rV, va, ks, ldT, ks, 0,
+                         0, 1.0, 0.0});
+      r1.push_back(BGemm{SP + spo[4 * a + 3], PL + sn[id].oR, SP + spo[4 * id + 3], ks, sn[id].rU, ua, ks, ldT, ks, 0,
+                         0, 1.0, 0.0});
+      r2.push_back(BGemm{SP + spo[4 * b + 2], PL + sn[id].oW + va, SP + spo[4 * id + 2], ks, sn[id].rV, vb, ks, ldT,
+                         ks, 0, 0, 1.0, 1.0});
+      r2.push_back(BGemm{SP + spo[4 * b + 3], PL + sn[id].oR + ua, SP + spo[4 * id + 3], ks, sn[id].rU, ub, ks, ldT,
+                         ks, 0, 0, 1.0, 1.0});
+    }
+    kern::bgemm(r1, dbg_, stream);
+    kern::bgemm(r2, dbg_, stream);
+    // ---- full bases of this depth
+    make_full(tree.by_depth[d], 0, UFc->get(), *UFn);
+    make_full(tree.by_depth[d], 1, VFc->get(), *VFn);
+    std::swap(UFc, UFn);
+    std::swap(VFc, VFn);
+  }
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void HssEngine::build(const GenDesc& g, int64_t n, int leaf_size, int mode_, const double* host_omega,
                       int64_t omega_len) {
   HMAT_REQUIRE(n > 0, HMAT_ERR_ARG, "N must be positive");
   HMAT_REQUIRE(leaf_size > 0, HMAT_ERR_ARG, "leaf_size must be positive");
   HMAT_REQUIRE(opt.precision == HMAT_PREC_FP64, HMAT_ERR_ARG, "HSS runs in FP64 only");
+  HMAT_REQUIRE(mode_ == kReference || mode_ == kNested || mode_ == kSampled, HMAT_ERR_ARG,
+               "HSS build mode must be 0 (reference), 1 (nested) or 2 (sampled)");
+  if (mode_ == kSampled) {
+    const int ks = sampled_width(opt, n);
+    HMAT_REQUIRE(ks <= 256, HMAT_ERR_ARG, "sampled HSS: rank + oversample above 256 is not supported");
+    HMAT_REQUIRE(!host_omega || omega_len == sampled_omega_length(opt, n), HMAT_ERR_ARG,
+                 "omega_len " + std::to_string(omega_len) + " != expected " +
+                     std::to_string(sampled_omega_length(opt, n)));
+  }
   built = factored = false;
   N = n;
   leaf = leaf_size;
-  nested = nested_;
+  mode = mode_;
+  nested = mode != kReference;
   tree.build(N, leaf);
   sn.assign(tree.nodes.size(), SNode{});
   pool_used = 0;
@@ -508,7 +924,7 @@ void HssEngine::build(const GenDesc& g, int64_t n, int leaf_size, bool nested_, 
   const int nl = (int)tree.leaves.size();
   // Omega offsets per leaf side in the serial draw order
   std::vector<int64_t> omega_off(2 * tree.nodes.size(), -1);
-  if (host_omega && tree.max_depth > 0) {
+  if (host_omega && tree.max_depth > 0 && mode != kSampled) {
     std::vector<int> order = tree.leaves;
     std::sort(order.begin(), order.end(),
               [&](int a, int b) { return tree.nodes[a].ref_id > tree.nodes[b].ref_id; });
@@ -524,7 +940,9 @@ void HssEngine::build(const GenDesc& g, int64_t n, int leaf_size, bool nested_, 
     HMAT_REQUIRE(pos == omega_len, HMAT_ERR_ARG,
                  "omega_len " + std::to_string(omega_len) + " != expected " + std::to_string(pos));
   }
-  if (tree.max_depth > 0) {
+  if (tree.max_depth > 0 && mode == kSampled) {
+    build_sampled(g, host_omega, omega_len);
+  } else if (tree.max_depth > 0) {
     compress_leaves(g, false, host_omega, omega_off);
     compress_leaves(g, true, host_omega, omega_off);
     build_internal();

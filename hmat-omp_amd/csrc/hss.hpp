@@ -8,10 +8,16 @@
 //   internal: transfers R_p ((rU_a + rU_b) x rU_p), W_p (same for V), and
 //             couplings B_ab = U_a^T A(I_a, I_b) V_b, B_ba = U_b^T A(I_b, I_a) V_a
 //             with the full (nested) bases of the children.
-// nested = false reproduces the reference operator: R = W = I and ranks add
-// up (hss_routines_omp.h:110-119).  nested = true compresses the transfers
-// (tolerance or fixed rank), so ranks stay bounded and the operator costs
-// O(N r) to apply.
+// Build modes: kReference reproduces the reference operator: R = W = I and
+// ranks add up (hss_routines_omp.h:110-119).  kNested compresses the
+// transfers (tolerance or fixed rank), so ranks stay bounded and the
+// operator costs O(N r) to apply; its leaf bases still come from the N-wide
+// row / column blocks, O(N^2 k / leaf) build memory.  kSampled builds the
+// same nested form from one sample pass per side, Y = A Omega and
+// Z = A^T Psi (Martinsson 2011, orthonormal bases): leaf bases from the
+// samples with the leaf's own block excluded, parents from the children's
+// reduced samples, couplings from one generated-entry product per depth;
+// O(N k) build memory.
 #pragma once
 #include <string>
 #include <vector>
@@ -37,7 +43,8 @@ class HssEngine {
   explicit HssEngine(const hmat_options& o);
   ~HssEngine();
 
-  void build(const GenDesc& g, int64_t N, int leaf, bool nested, const double* host_omega,
+  enum Mode { kReference = 0, kNested = 1, kSampled = 2 };
+  void build(const GenDesc& g, int64_t N, int leaf, int mode, const double* host_omega,
              int64_t omega_len);
   void matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
   void factor();
@@ -55,6 +62,10 @@ class HssEngine {
   double matvec_bytes(int nrhs) const;
   double solve_bytes(int nrhs) const;
   static int64_t omega_length(const hmat_options& o, int64_t N, int leaf);
+  // sample width of the sampled build (rank + oversample, at most N) and
+  // its host Omega length: [Omega | Psi], each N x width, column-major
+  static int sampled_width(const hmat_options& o, int64_t N);
+  static int64_t sampled_omega_length(const hmat_options& o, int64_t N);
 
   hmat_options opt;
   hipStream_t stream = nullptr;
@@ -62,6 +73,7 @@ class HssEngine {
   int64_t N = 0;
   int leaf = 0;
   bool nested = false;
+  int mode = kReference;
   bool built = false, factored = false;
   // FP64 iterative refinement of solves through the HSS matvec (off by
   // default: rtol 0); iterations and max relative residual of the last solve
@@ -78,6 +90,8 @@ class HssEngine {
   void compress_leaves(const GenDesc& g, bool col_side, const double* host_omega,
                        const std::vector<int64_t>& omega_off);
   void build_internal();
+  void build_sampled(const GenDesc& g, const double* host_omega, int64_t omega_len);
+  int64_t spool_alloc(int64_t n);
   int64_t pool_alloc(int64_t n);
   void set_ldv();
   void leaf_apply(const double* Dm, const double* B, int64_t ldb, int ncols, double* Y, int64_t ldy,
@@ -112,6 +126,12 @@ class HssEngine {
   struct NodeScratch {  // build_internal: wide arrays of the child and the current depth
     DevBuf<double> T[2], S[2], VF[2], Pt, Ps, Tmp, Pn[2], Rb, Vr, Sv, Wk, TU, TV;
   } ns_;
+  struct SampleScratch {  // build_sampled
+    DevBuf<double> Om, Ps, Y, Z, Tb, Rt, Sm, Vr, Sv, Wk, TU, TV, P, Mt;
+    DevBuf<double> UF[2], VF[2];  // full bases of the child depth and the current one
+    DevBuf<double> sp;            // reduced samples per node (see build_sampled)
+    int64_t sp_used = 0;
+  } ss_;
 
   kern::Workspace ws_;
   Orthogonalizer orth_;

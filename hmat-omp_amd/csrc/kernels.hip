@@ -1736,6 +1736,24 @@ void rect_add_identity(const std::vector<Rect>& items, DevBuf<Rect>& d, hipStrea
   k_rect_add_identity<<<dim3(4, (unsigned)items.size()), kWG, 0, s>>>(d.get());
   HIP_CHECK(hipGetLastError());
 }
+// dst(j, i) = alpha src(i, j): reads run down the source columns (small
+// blocks; the strided writes stay within a few KB)
+__global__ void k_rect_transpose(const Rect* __restrict__ items) {
+  const Rect r = items[blockIdx.y];
+  const int64_t total = (int64_t)r.m * r.n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % r.m), j = (int)(e / r.m);
+    double* d = r.dst + j + (int64_t)i * r.ldd;
+    const double sv = r.src[i + (int64_t)j * r.lds];
+    *d = r.alpha * sv + (r.beta != 0.0 ? r.beta * *d : 0.0);
+  }
+}
+void rect_transpose(const std::vector<Rect>& items, DevBuf<Rect>& d, hipStream_t s) {
+  if (items.empty()) return;
+  d.upload(items, s);
+  k_rect_transpose<<<dim3(16, (unsigned)items.size()), kWG, 0, s>>>(d.get());
+  HIP_CHECK(hipGetLastError());
+}
 
 __global__ void k_zero_rects(const ZeroRect* __restrict__ items) {
   const ZeroRect z = items[blockIdx.y];

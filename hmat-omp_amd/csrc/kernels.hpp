@@ -240,6 +240,8 @@ struct Rect {
 };
 void rect_axpby(const std::vector<Rect>& items, DevBuf<Rect>& d, hipStream_t s);
 void rect_add_identity(const std::vector<Rect>& items, DevBuf<Rect>& d, hipStream_t s);
+// dst(j, i) = alpha src(i, j) + beta dst(j, i) for i < m, j < n (src m x n)
+void rect_transpose(const std::vector<Rect>& items, DevBuf<Rect>& d, hipStream_t s);
 
 // Z(:, c) = alpha X(:, c) + beta Y(:, c) for n rows, ncols columns.
 void axpby_cols(int64_t n, int ncols, double alpha, const double* X, int64_t ldx, double beta,

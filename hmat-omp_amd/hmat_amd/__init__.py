@@ -10,6 +10,7 @@ CPU fallback: a missing library or device raises ``HmatError``.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -24,6 +25,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 OMEGA_UNIFORM, OMEGA_GAUSSIAN = 0, 1
 KERNEL_GAUSSIAN, KERNEL_CAUCHY, KERNEL_REF_CAUCHY, KERNEL_REF_GAUSSIAN, KERNEL_REF_COULOMB = 1, 2, 3, 4, 5
 KERNEL_GAUSSIAN_DSIGMA = 6
+# HSS build modes (the `nested` argument of hmat_hss_build_*)
+HSS_REFERENCE, HSS_NESTED, HSS_SAMPLED = 0, 1, 2
 
 
 class HmatError(RuntimeError):
@@ -104,6 +107,8 @@ def _proto(lib):
         "hmat_hss_build_dense": (st, [vp, _D, i64, i64, i32, i32, _D, i64]),
         "hmat_hss_build_kernel": (st, [vp, C.POINTER(Kernel), i64, i32, i32]),
         "hmat_hss_omega_length": (i64, [C.POINTER(Options), i64, i32]),
+        "hmat_hss_sampled_omega_length": (i64, [C.POINTER(Options), i64]),
+        "hmat_hss_build_kernel_device": (st, [vp, C.POINTER(Kernel), i64, i32, i32]),
         "hmat_hss_matvec": (st, [vp, _D, i64, i32, _D, i64]),
         "hmat_hss_factor": (st, [vp]),
         "hmat_hss_solve": (st, [vp, _D, i64, i32, _D, i64]),
@@ -129,8 +134,25 @@ def load(path=None):
         p = path or LIB_PATH
         if not os.path.exists(p):
             raise HmatError(2, f"{p} not built (run hmat-omp_amd/build.py)")
+        _one_hip_runtime()
         _lib = _proto(C.CDLL(p))
     return _lib
+
+
+def _one_hip_runtime():
+    """One HIP runtime per process.  torch's ROCm wheel carries its own
+    libamdhip64 (torch/lib, soname libamdhip64.so.7) and its libc10_hip asks
+    for the unversioned name; the engine links the system libamdhip64.so.7.
+    Loaded after torch, the engine binds to torch's copy (same soname); loaded
+    first, it makes torch open a second runtime later, and that one finds no
+    device.  So when torch is installed it is loaded first; device pointers
+    from torch tensors and from the engine then live in one runtime."""
+    if "torch" in sys.modules:
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def _check(st):
@@ -401,6 +423,9 @@ class HSSMatrixOMP(_Base):
     _prefix = "hss"
 
     def __init__(self, tol=1e-9, opts=None, nested=False):
+        """nested: False / HSS_REFERENCE (identity transfers, the reference
+        operator), True / HSS_NESTED (compressed transfers) or HSS_SAMPLED
+        (nested bases from one sample pass per side)."""
         if opts is None:
             opts = reference_options(tol)
         super().__init__(tol, opts)
@@ -423,6 +448,13 @@ class HSSMatrixOMP(_Base):
                                                int(self.nested)))
         self.N = n
 
+    def build_kernel_device(self, kernel, n, leaf_size):
+        """build_kernel with the kernel's points in device memory."""
+        self._sync_opts()
+        _check(self._lib.hmat_hss_build_kernel_device(self._h, C.byref(kernel), n, leaf_size,
+                                                      int(self.nested)))
+        self.N = n
+
     def multiply_device(self, x_ptr, ldx, nrhs, y_ptr, ldy):
         _check(self._lib.hmat_hss_matvec_device(self._h, x_ptr, ldx, nrhs, y_ptr, ldy))
 
@@ -431,6 +463,8 @@ class HSSMatrixOMP(_Base):
 
     def omega_length(self, n, leaf_size):
         self._sync_opts()
+        if int(self.nested) == HSS_SAMPLED:
+            return self._lib.hmat_hss_sampled_omega_length(C.byref(self.opts), n)
         return self._lib.hmat_hss_omega_length(C.byref(self.opts), n, leaf_size)
 
     def root_rank(self):

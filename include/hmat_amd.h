@@ -318,10 +318,19 @@ hmat_status hmat_hss_create(hmat_hss* h, const hmat_options* opts);
 hmat_status hmat_hss_destroy(hmat_hss h);
 /* HSSMatrixOMP::build_from_dense              hss_routines_omp.h:62-149
  *   nested = 0: identity transfers, ranks add up (the reference operator).
- *   nested = 1: compressed transfers (true nested bases).
- *   omega: optional host Omega blocks in the serial draw order (reverse
- *   node id over leaves, row block then column block, each N x k;
- *   hss_routines.h:57-77), or NULL. */
+ *   nested = 1: compressed transfers (true nested bases); leaf bases from
+ *     the N-wide off-diagonal row / column blocks as the reference's
+ *     (hss_routines_omp.h:69-95), O(N^2 k / leaf) device memory.
+ *   nested = 2: the same nested form from one sample pass per side,
+ *     Y = A Omega and Z = A^T Psi with s = rank + oversample columns
+ *     (randomized HSS construction, Martinsson 2011): O(N s) device memory,
+ *     power_iters does not apply.  Bases, transfers and couplings are new
+ *     with respect to the reference (which has no such construction).
+ *   omega: optional host Omega, or NULL.  nested = 0/1: blocks in the
+ *   serial draw order (reverse node id over leaves, row block then column
+ *   block, each N x k; hss_routines.h:57-77), omega_len =
+ *   hmat_hss_omega_length.  nested = 2: [Omega | Psi], each N x s
+ *   column-major, omega_len = hmat_hss_sampled_omega_length. */
 hmat_status hmat_hss_build_dense(hmat_hss h, const double* A, int64_t lda,
                                  int64_t N, int32_t leaf_size, int32_t nested,
                                  const double* omega, int64_t omega_len);
@@ -329,6 +338,13 @@ hmat_status hmat_hss_build_kernel(hmat_hss h, const hmat_kernel* k, int64_t N,
                                   int32_t leaf_size, int32_t nested);
 int64_t hmat_hss_omega_length(const hmat_options* opts, int64_t N,
                               int32_t leaf_size);
+/* Host Omega length of the sampled build (nested = 2): 2 N s doubles,
+ * s = min(rank + oversample, N).  No reference counterpart. */
+int64_t hmat_hss_sampled_omega_length(const hmat_options* opts, int64_t N);
+/* build_kernel with the points (k->x, k->y) already in device memory.
+ * No reference counterpart (the reference takes a dense host matrix). */
+hmat_status hmat_hss_build_kernel_device(hmat_hss h, const hmat_kernel* k, int64_t N,
+                                         int32_t leaf_size, int32_t nested);
 /* HSSMatrixOMP::multiply                      hss_routines_omp.h:154-230 */
 hmat_status hmat_hss_matvec(hmat_hss h, const double* X, int64_t ldx,
                             int32_t nrhs, double* Y, int64_t ldy);
