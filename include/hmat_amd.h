@@ -50,6 +50,13 @@ typedef enum {
   HMAT_OMEGA_GAUSSIAN = 1   /* N(0,1)                                      */
 } hmat_omega_dist;
 
+/* HSS build modes (the `nested` argument of hmat_hss_build_*) */
+typedef enum {
+  HMAT_HSS_REFERENCE = 0,   /* identity transfers (hss_routines_omp.h:110-119) */
+  HMAT_HSS_NESTED = 1,      /* compressed transfers, N-wide leaf blocks         */
+  HMAT_HSS_SAMPLED = 2      /* compressed transfers from one sample pass        */
+} hmat_hss_mode;
+
 /* Compression options.  hmat_options_reference() gives the reference's
  * behaviour: tolerance truncation S_i > tol*S_0 (compression.h:42-46),
  * rsvd_rank 64 (hodlr_routines_omp.h:17), k = rank+10 (compression.h:74),

@@ -122,15 +122,17 @@ class HODLRMatrixOMPT : public HandleBase<hmat_hodlr, HodlrOps> {
 template <class M>
 class HSSMatrixOMPT : public HandleBase<hmat_hss, HssOps> {
  public:
-  // nested = false reproduces the reference operator (identity transfers,
-  // hss_routines_omp.h:110-119); true compresses the transfers.
-  explicit HSSMatrixOMPT(double tol = 1e-9, bool nested = false) : HandleBase(tol), nested_(nested) {}
+  // mode HMAT_HSS_REFERENCE (0, false, the default) reproduces the reference
+  // operator (identity transfers, hss_routines_omp.h:110-119);
+  // HMAT_HSS_NESTED (1, true) compresses the transfers; HMAT_HSS_SAMPLED (2)
+  // builds the nested form from one sample pass per side, O(N s) memory.
+  explicit HSSMatrixOMPT(double tol = 1e-9, int mode = HMAT_HSS_REFERENCE) : HandleBase(tol), nested_(mode) {}
 
   // hss_routines_omp.h:62-149
   void build_from_dense(const M& A, int leaf_size) {
     sync();
     N = A.rows();
-    check(hmat_hss_build_dense(h_, A.data(), A.rows(), A.rows(), leaf_size, nested_ ? 1 : 0, nullptr, 0));
+    check(hmat_hss_build_dense(h_, A.data(), A.rows(), A.rows(), leaf_size, nested_, nullptr, 0));
   }
   // hss_routines_omp.h:154-160 (column 0 of x)
   M multiply(const M& x) {
@@ -147,7 +149,7 @@ class HSSMatrixOMPT : public HandleBase<hmat_hss, HssOps> {
   hmat_hss handle() const { return h_; }
 
  private:
-  bool nested_;
+  int nested_;
 };
 
 // The serial classes of the reference compute the same algebra as the
@@ -161,7 +163,7 @@ using HODLRMatrixT = HODLRMatrixOMPT<M>;
 template <class M>
 class HSSMatrixT : public HSSMatrixOMPT<M> {
  public:
-  explicit HSSMatrixT(double tol = 1e-9, bool nested = false) : HSSMatrixOMPT<M>(tol, nested) {}
+  explicit HSSMatrixT(double tol = 1e-9, int mode = HMAT_HSS_REFERENCE) : HSSMatrixOMPT<M>(tol, mode) {}
   // hss_routines.h:197-199 (every column of b)
   M solve(const M& b) {
     M x(this->N, b.cols());
