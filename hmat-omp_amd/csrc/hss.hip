@@ -29,6 +29,41 @@ using kern::BGemm;
 using kern::Rect;
 using kern::ZeroRect;
 
+namespace {
+// The per-node work of a sweep: phases (one launch each, in order) of tasks
+// (one workgroup each, independent within a phase) of ops (small GEMMs or
+// copies, run in order by the task's workgroup).  All descriptors go to the
+// device in one staged copy.
+struct SweepProgram {
+  std::vector<BGemm> ops;
+  std::vector<kern::NodeTask> tasks;
+  std::vector<std::pair<int, int>> phases;  // [first task, end)
+  void begin_phase() { phases.push_back({(int)tasks.size(), (int)tasks.size()}); }
+  void begin_task() { tasks.push_back(kern::NodeTask{(int32_t)ops.size(), 0}); }
+  void op(const BGemm& g) {
+    ops.push_back(g);
+    ++tasks.back().nops;
+  }
+  // C = alpha A + beta C for an m x n block
+  void copy(const double* A, int lda, double* C, int ldc, int m, int n, double alpha = 1.0, double beta = 0.0) {
+    op(BGemm{A, nullptr, C, m, n, 0, lda, 0, ldc, 0, 0, alpha, beta});
+  }
+  void end_task() {
+    if (tasks.back().nops == 0) tasks.pop_back();
+    phases.back().second = (int)tasks.size();
+  }
+  void run(hipStream_t s) {
+    if (tasks.empty()) return;
+    StageGroup g;
+    const int io = g.add(ops), it = g.add(tasks);
+    g.put(s);
+    const BGemm* dops = g.get<BGemm>(io);
+    const kern::NodeTask* dt = g.get<kern::NodeTask>(it);
+    for (const auto& p : phases) kern::node_tasks(dt + p.first, p.second - p.first, dops, s);
+  }
+};
+}  // namespace
+
 HssEngine::HssEngine(const hmat_options& o) : opt(o) {
   HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   own_stream = true;
@@ -1025,34 +1060,38 @@ void HssEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int6
     kern::seg_gemm_tn(tn, Vl.get(), N, dX, ldx, G, ldv, 1.0, 0.0, ws_, stream);
   }
   const double* P = pool.get();
+  SweepProgram sw;
+  // up-sweep g_p = W_p^T [g_a; g_b], one task per node and phase per depth
   for (int d = tree.max_depth - 1; d >= 1; --d) {
-    std::vector<BGemm> b1, b2;
+    sw.begin_phase();
     for (int id : tree.by_depth[d]) {
       const TNode& nd = tree.nodes[id];
       if (nd.leaf) continue;
       const int va = sn[nd.left].rV, vb = sn[nd.right].rV;
-      b1.push_back(BGemm{P + sn[id].oW, gv(nd.left), gv(id), sn[id].rV, nrhs, va, sn[id].ldW, ldv, ldv, 1, 0, 1.0, 0.0});
-      b2.push_back(BGemm{P + sn[id].oW + va, gv(nd.right), gv(id), sn[id].rV, nrhs, vb, sn[id].ldW, ldv, ldv, 1, 0, 1.0, 1.0});
+      sw.begin_task();
+      sw.op(BGemm{P + sn[id].oW, gv(nd.left), gv(id), sn[id].rV, nrhs, va, sn[id].ldW, ldv, ldv, 1, 0, 1.0, 0.0});
+      sw.op(BGemm{P + sn[id].oW + va, gv(nd.right), gv(id), sn[id].rV, nrhs, vb, sn[id].ldW, ldv, ldv, 1, 0, 1.0, 1.0});
+      sw.end_task();
     }
-    kern::bgemm(b1, dbg_, stream);
-    kern::bgemm(b2, dbg_, stream);
   }
+  // down-sweep f_a = B_ab g_b + R_p(a) f_p, f_b = B_ba g_a + R_p(b) f_p
   for (int d = 0; d < tree.max_depth; ++d) {
-    std::vector<BGemm> b1, b2;
+    sw.begin_phase();
     for (int id : tree.by_depth[d]) {
       const TNode& nd = tree.nodes[id];
       if (nd.leaf) continue;
       const int ua = sn[nd.left].rU, ub = sn[nd.right].rU;
-      b1.push_back(BGemm{P + sn[id].oBab, gv(nd.right), fv(nd.left), ua, nrhs, sn[nd.right].rV, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
-      b1.push_back(BGemm{P + sn[id].oBba, gv(nd.left), fv(nd.right), ub, nrhs, sn[nd.left].rV, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
+      sw.begin_task();
+      sw.op(BGemm{P + sn[id].oBab, gv(nd.right), fv(nd.left), ua, nrhs, sn[nd.right].rV, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
+      sw.op(BGemm{P + sn[id].oBba, gv(nd.left), fv(nd.right), ub, nrhs, sn[nd.left].rV, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
       if (d > 0) {
-        b2.push_back(BGemm{P + sn[id].oR, fv(id), fv(nd.left), ua, nrhs, sn[id].rU, sn[id].ldR, ldv, ldv, 0, 0, 1.0, 1.0});
-        b2.push_back(BGemm{P + sn[id].oR + ua, fv(id), fv(nd.right), ub, nrhs, sn[id].rU, sn[id].ldR, ldv, ldv, 0, 0, 1.0, 1.0});
+        sw.op(BGemm{P + sn[id].oR, fv(id), fv(nd.left), ua, nrhs, sn[id].rU, sn[id].ldR, ldv, ldv, 0, 0, 1.0, 1.0});
+        sw.op(BGemm{P + sn[id].oR + ua, fv(id), fv(nd.right), ub, nrhs, sn[id].rU, sn[id].ldR, ldv, ldv, 0, 0, 1.0, 1.0});
       }
+      sw.end_task();
     }
-    kern::bgemm(b1, dbg_, stream);
-    kern::bgemm(b2, dbg_, stream);
   }
+  sw.run(stream);
   {
     std::vector<SegNN> nn2;
     for (int id : tree.leaves) {
@@ -1291,65 +1330,56 @@ void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* d
     kern::seg_gemm_tn(tn, Vl.get(), N, dX, ldx, H, ldv, 1.0, 0.0, ws_, stream);
   }
   const double* P = pool.get();
+  SweepProgram sw;
+  // up-sweep, one task per node: u = M^{-1} [h_a; h_b] refined once against
+  // M, s = [B_ab u_b; B_ba u_a], and for non-root nodes
+  // h_p = W^T [h_a; h_b] - W^T [K_a s_a; K_b s_b]
   for (int d = tree.max_depth - 1; d >= 0; --d) {
-    std::vector<BGemm> u1, u2, ur, ux, s1, k1, h1, h2;
-    std::vector<Rect> uh;
+    sw.begin_phase();
     for (int id : tree.by_depth[d]) {
       const TNode& nd = tree.nodes[id];
       if (nd.leaf) continue;
       const int va = sn[nd.left].rV, vb = sn[nd.right].rV, ua = sn[nd.left].rU, ub = sn[nd.right].rU;
       const int m = sn[id].ms;
       const double* Mi = P + sn[id].oMinv;
-      u1.push_back(BGemm{Mi, at(H, nd.left), at(U, id), m, nrhs, va, m, ldv, ldv, 0, 0, 1.0, 0.0});
-      u2.push_back(BGemm{Mi + (int64_t)va * m, at(H, nd.right), at(U, id), m, nrhs, vb, m, ldv, ldv, 0, 0, 1.0, 1.0});
-      // refinement of u = M^{-1} [h_a; h_b]: r = [h_a; h_b] - M u, u += M^{-1} r
-      ur.push_back(BGemm{P + sn[id].oM, at(U, id), at(Rm, id), m, nrhs, m, m, ldv, ldv, 0, 0, -1.0, 0.0});
-      uh.push_back(Rect{at(H, nd.left), at(Rm, id), ldv, ldv, va, nrhs, 1.0, 1.0});
-      uh.push_back(Rect{at(H, nd.right), at(Rm, id) + va, ldv, ldv, vb, nrhs, 1.0, 1.0});
-      ux.push_back(BGemm{Mi, at(Rm, id), at(U, id), m, nrhs, m, m, ldv, ldv, 0, 0, 1.0, 1.0});
-      s1.push_back(BGemm{P + sn[id].oBab, at(U, id) + va, at(Sv, id), ua, nrhs, vb, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
-      s1.push_back(BGemm{P + sn[id].oBba, at(U, id), at(Sv, id) + ua, ub, nrhs, va, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
+      sw.begin_task();
+      sw.op(BGemm{Mi, at(H, nd.left), at(U, id), m, nrhs, va, m, ldv, ldv, 0, 0, 1.0, 0.0});
+      sw.op(BGemm{Mi + (int64_t)va * m, at(H, nd.right), at(U, id), m, nrhs, vb, m, ldv, ldv, 0, 0, 1.0, 1.0});
+      sw.copy(at(H, nd.left), ldv, at(Rm, id), ldv, va, nrhs);
+      sw.copy(at(H, nd.right), ldv, at(Rm, id) + va, ldv, vb, nrhs);
+      sw.op(BGemm{P + sn[id].oM, at(U, id), at(Rm, id), m, nrhs, m, m, ldv, ldv, 0, 0, -1.0, 1.0});
+      sw.op(BGemm{Mi, at(Rm, id), at(U, id), m, nrhs, m, m, ldv, ldv, 0, 0, 1.0, 1.0});
+      sw.op(BGemm{P + sn[id].oBab, at(U, id) + va, at(Sv, id), ua, nrhs, vb, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
+      sw.op(BGemm{P + sn[id].oBba, at(U, id), at(Sv, id) + ua, ub, nrhs, va, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
       if (d > 0) {
-        k1.push_back(BGemm{P + sn[nd.left].oK, at(Sv, id), at(KS, id), va, nrhs, ua, ldv, ldv, ldv, 0, 0, 1.0, 0.0});
-        k1.push_back(BGemm{P + sn[nd.right].oK, at(Sv, id) + ua, at(KS, id) + va, vb, nrhs, ub, ldv, ldv, ldv, 0, 0, 1.0, 0.0});
-        // h_p = W^T [h_a; h_b] - W^T [K_a s_a; K_b s_b]
-        h1.push_back(BGemm{P + sn[id].oW, at(H, nd.left), at(H, id), sn[id].rV, nrhs, va, sn[id].ldW, ldv, ldv, 1, 0, 1.0, 0.0});
-        h2.push_back(BGemm{P + sn[id].oW + va, at(H, nd.right), at(H, id), sn[id].rV, nrhs, vb, sn[id].ldW, ldv, ldv, 1, 0, 1.0, 1.0});
-        h2.push_back(BGemm{P + sn[id].oW, at(KS, id), at(H, id), sn[id].rV, nrhs, m, sn[id].ldW, ldv, ldv, 1, 0, -1.0, 1.0});
+        sw.op(BGemm{P + sn[nd.left].oK, at(Sv, id), at(KS, id), va, nrhs, ua, ldv, ldv, ldv, 0, 0, 1.0, 0.0});
+        sw.op(BGemm{P + sn[nd.right].oK, at(Sv, id) + ua, at(KS, id) + va, vb, nrhs, ub, ldv, ldv, ldv, 0, 0, 1.0, 0.0});
+        sw.op(BGemm{P + sn[id].oW, at(H, nd.left), at(H, id), sn[id].rV, nrhs, va, sn[id].ldW, ldv, ldv, 1, 0, 1.0, 0.0});
+        sw.op(BGemm{P + sn[id].oW + va, at(H, nd.right), at(H, id), sn[id].rV, nrhs, vb, sn[id].ldW, ldv, ldv, 1, 0, 1.0,
+                    1.0});
+        sw.op(BGemm{P + sn[id].oW, at(KS, id), at(H, id), sn[id].rV, nrhs, m, sn[id].ldW, ldv, ldv, 1, 0, -1.0, 1.0});
       }
+      sw.end_task();
     }
-    kern::bgemm(u1, dbg_, stream);
-    kern::bgemm(u2, dbg_, stream);
-    kern::bgemm(ur, dbg_, stream);
-    kern::rect_axpby(uh, drc_, stream);
-    kern::bgemm(ux, dbg_, stream);
-    kern::bgemm(s1, dbg_, stream);
-    kern::bgemm(k1, dbg_, stream);
-    kern::bgemm(h1, dbg_, stream);
-    // the second and third updates of h_p accumulate into the same block:
-    // one launch each keeps them ordered
-    std::vector<BGemm> h2a, h2b;
-    for (size_t i = 0; i < h2.size(); ++i) (i % 2 ? h2b : h2a).push_back(h2[i]);
-    kern::bgemm(h2a, dbg_, stream);
-    kern::bgemm(h2b, dbg_, stream);
   }
+  // down-sweep, one task per node: t_a = s_a + G_p(a) t_p, t_b = s_b + G_p(b) t_p
   for (int d = 0; d < tree.max_depth; ++d) {
-    std::vector<Rect> cp;
-    std::vector<BGemm> gt;
+    sw.begin_phase();
     for (int id : tree.by_depth[d]) {
       const TNode& nd = tree.nodes[id];
       if (nd.leaf) continue;
       const int ua = sn[nd.left].rU, ub = sn[nd.right].rU, kR = ua + ub;
-      cp.push_back(Rect{at(Sv, id), at(T, nd.left), ldv, ldv, ua, nrhs, 1.0, 0.0});
-      cp.push_back(Rect{at(Sv, id) + ua, at(T, nd.right), ldv, ldv, ub, nrhs, 1.0, 0.0});
+      sw.begin_task();
+      sw.copy(at(Sv, id), ldv, at(T, nd.left), ldv, ua, nrhs);
+      sw.copy(at(Sv, id) + ua, ldv, at(T, nd.right), ldv, ub, nrhs);
       if (d > 0) {
-        gt.push_back(BGemm{P + sn[id].oG, at(T, id), at(T, nd.left), ua, nrhs, sn[id].rU, kR, ldv, ldv, 0, 0, 1.0, 1.0});
-        gt.push_back(BGemm{P + sn[id].oG + ua, at(T, id), at(T, nd.right), ub, nrhs, sn[id].rU, kR, ldv, ldv, 0, 0, 1.0, 1.0});
+        sw.op(BGemm{P + sn[id].oG, at(T, id), at(T, nd.left), ua, nrhs, sn[id].rU, kR, ldv, ldv, 0, 0, 1.0, 1.0});
+        sw.op(BGemm{P + sn[id].oG + ua, at(T, id), at(T, nd.right), ub, nrhs, sn[id].rU, kR, ldv, ldv, 0, 0, 1.0, 1.0});
       }
+      sw.end_task();
     }
-    kern::rect_axpby(cp, drc_, stream);
-    kern::bgemm(gt, dbg_, stream);
   }
+  sw.run(stream);
   {
     std::vector<SegNN> up;
     for (int id : tree.leaves) {

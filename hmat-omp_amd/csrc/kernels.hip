@@ -1709,6 +1709,100 @@ void bgemm(const std::vector<BGemm>& items, DevBuf<BGemm>& d, hipStream_t s) {
   HIP_CHECK(hipGetLastError());
 }
 
+// Node tasks: one workgroup runs the ops of one task in order (per-node
+// work of the HSS sweeps, where each op reads what the previous one wrote,
+// so an op's cost is its latency).  An op is a small GEMM (m, k up to a few
+// hundred, n = right-hand sides): op(B) for up to kNtK reduction indices and
+// eight columns is staged in LDS (one load per thread, one barrier); each
+// thread owns one output row, loads its op(A) entries 16 at a time ahead of
+// the FMAs (coalesced down the rows for ta = 0; the small generator
+// matrices sit in L2), and the reduction is split over 256 / R thread groups
+// (R = 64, 128 or 256 rows per pass) and summed through LDS in group order.
+constexpr int kNtCols = 8, kNtK = 512, kNtA = 16;
+__global__ __launch_bounds__(kWG) void k_node_tasks(const NodeTask* __restrict__ tasks,
+                                                    const BGemm* __restrict__ ops) {
+  __shared__ double red[kWG * kNtCols];
+  __shared__ double bs[kNtK * kNtCols];  // op(B)(l, j0 + jj) at bs[l * kNtCols + jj]
+  const NodeTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x;
+  for (int o = 0; o < t.nops; ++o) {
+    const BGemm g = ops[t.op0 + o];
+    if (g.k == 0) {
+      // copy: C = alpha A + beta C (A m x n, leading dimension lda)
+      for (int e = tid; e < g.m * g.n; e += kWG) {
+        const int i = e % g.m, j = e / g.m;
+        double* c = g.C + i + (int64_t)j * g.ldc;
+        const double v = g.alpha * g.A[i + (int64_t)j * g.lda];
+        *c = g.beta != 0.0 ? v + g.beta * *c : v;
+      }
+      __syncthreads();  // the next op may read this op's C
+      continue;
+    }
+    const int R = g.m <= 64 ? 64 : (g.m <= 128 ? 128 : 256);
+    const int ns = kWG / R;
+    const int r = tid % R, sp = tid / R;
+    const int64_t as = g.ta ? 1 : g.lda;  // step of l in op(A)(i, l)
+    const int64_t bl = g.tb ? g.ldb : 1;  // step of l in op(B)(l, j)
+    const int64_t bj = g.tb ? 1 : g.ldb;  // step of j
+    for (int j0 = 0; j0 < g.n; j0 += kNtCols) {
+      const int nj = min(kNtCols, g.n - j0);
+      for (int i0 = 0; i0 < g.m; i0 += R) {
+        const int i = i0 + r;
+        const double* ap = g.A + (g.ta ? (int64_t)i * g.lda : i);
+        double acc[kNtCols];
+#pragma unroll
+        for (int jj = 0; jj < kNtCols; ++jj) acc[jj] = 0.0;
+        for (int k0 = 0; k0 < g.k; k0 += kNtK) {
+          const int kb = min(kNtK, g.k - k0);
+          for (int e = tid; e < kb * kNtCols; e += kWG) {
+            const int l = e / kNtCols, jj = e % kNtCols;
+            bs[e] = jj < nj ? g.B[(int64_t)(k0 + l) * bl + (int64_t)(j0 + jj) * bj] : 0.0;
+          }
+          __syncthreads();
+          const int kc = (kb + ns - 1) / ns;
+          const int l0 = min(kb, sp * kc), l1 = min(kb, l0 + kc);
+          if (i < g.m)
+            for (int lb = l0; lb < l1; lb += kNtA) {
+              double a[kNtA];
+#pragma unroll
+              for (int q = 0; q < kNtA; ++q) a[q] = lb + q < l1 ? ap[(int64_t)(k0 + lb + q) * as] : 0.0;
+#pragma unroll
+              for (int q = 0; q < kNtA; ++q) {
+                const double* b = bs + (lb + q) * kNtCols;
+#pragma unroll
+                for (int jj = 0; jj < kNtCols; ++jj) acc[jj] = fma(a[q], b[jj], acc[jj]);
+              }
+            }
+          __syncthreads();  // bs is rewritten next
+        }
+        if (ns > 1) {
+#pragma unroll
+          for (int jj = 0; jj < kNtCols; ++jj) red[(sp * R + r) * kNtCols + jj] = acc[jj];
+          __syncthreads();
+          if (sp == 0)
+            for (int q = 1; q < ns; ++q)
+#pragma unroll
+              for (int jj = 0; jj < kNtCols; ++jj) acc[jj] += red[(q * R + r) * kNtCols + jj];
+          __syncthreads();
+        }
+        if (sp == 0 && i < g.m)
+          for (int jj = 0; jj < nj; ++jj) {
+            double* c = g.C + i + (int64_t)(j0 + jj) * g.ldc;
+            const double v = g.alpha * acc[jj];
+            *c = g.beta != 0.0 ? v + g.beta * *c : v;
+          }
+      }
+    }
+    __syncthreads();  // the next op may read this op's C
+  }
+}
+
+void node_tasks(const NodeTask* dtasks, int ntasks, const BGemm* dops, hipStream_t s) {
+  if (ntasks <= 0) return;
+  k_node_tasks<<<(unsigned)ntasks, kWG, 0, s>>>(dtasks, dops);
+  HIP_CHECK(hipGetLastError());
+}
+
 __global__ void k_rect_axpby(const Rect* __restrict__ items) {
   const Rect r = items[blockIdx.y];
   const int64_t total = (int64_t)r.m * r.n;

@@ -258,6 +258,13 @@ struct BGemm {
   double alpha, beta;
 };
 void bgemm(const std::vector<BGemm>& items, DevBuf<BGemm>& d, hipStream_t s);
+// Node tasks: one workgroup per task runs ops[op0 .. op0 + nops) in order,
+// each op reading what the previous ones wrote.  An op is a BGemm; k == 0
+// makes it a copy C = alpha A + beta C (A m x n, leading dimension lda).
+struct NodeTask {
+  int32_t op0, nops;
+};
+void node_tasks(const NodeTask* dtasks, int ntasks, const BGemm* dops, hipStream_t s);
 // Zero rows [r0, r0 + len) of columns [c0, c0 + ncols) of a column-major array
 struct ZeroRect {
   double* p;
