@@ -1716,8 +1716,10 @@ void bgemm(const std::vector<BGemm>& items, DevBuf<BGemm>& d, hipStream_t s) {
 // eight columns is staged in LDS (one load per thread, one barrier); each
 // thread owns one output row, loads its op(A) entries 16 at a time ahead of
 // the FMAs (coalesced down the rows for ta = 0; the small generator
-// matrices sit in L2), and the reduction is split over 256 / R thread groups
-// (R = 64, 128 or 256 rows per pass) and summed through LDS in group order.
+// matrices sit in L2; the first batch is issued before op(B) is staged, so
+// the two loads overlap), and the reduction is split over 256 / R thread
+// groups (R = 64, 128 or 256 rows per pass) and summed through LDS in group
+// order.
 constexpr int kNtCols = 8, kNtK = 512, kNtA = 16;
 __global__ __launch_bounds__(kWG) void k_node_tasks(const NodeTask* __restrict__ tasks,
                                                     const BGemm* __restrict__ ops) {
@@ -1754,20 +1756,26 @@ __global__ __launch_bounds__(kWG) void k_node_tasks(const NodeTask* __restrict__
         for (int jj = 0; jj < kNtCols; ++jj) acc[jj] = 0.0;
         for (int k0 = 0; k0 < g.k; k0 += kNtK) {
           const int kb = min(kNtK, g.k - k0);
+          const int kc = (kb + ns - 1) / ns;
+          const int l0 = min(kb, sp * kc), l1 = min(kb, l0 + kc);
+          const bool row = i < g.m;
+          double a[kNtA];
+#pragma unroll
+          for (int q = 0; q < kNtA; ++q) a[q] = (row && l0 + q < l1) ? ap[(int64_t)(k0 + l0 + q) * as] : 0.0;
           for (int e = tid; e < kb * kNtCols; e += kWG) {
             const int l = e / kNtCols, jj = e % kNtCols;
             bs[e] = jj < nj ? g.B[(int64_t)(k0 + l) * bl + (int64_t)(j0 + jj) * bj] : 0.0;
           }
           __syncthreads();
-          const int kc = (kb + ns - 1) / ns;
-          const int l0 = min(kb, sp * kc), l1 = min(kb, l0 + kc);
-          if (i < g.m)
+          if (row)
             for (int lb = l0; lb < l1; lb += kNtA) {
-              double a[kNtA];
+              if (lb > l0) {
 #pragma unroll
-              for (int q = 0; q < kNtA; ++q) a[q] = lb + q < l1 ? ap[(int64_t)(k0 + lb + q) * as] : 0.0;
+                for (int q = 0; q < kNtA; ++q) a[q] = lb + q < l1 ? ap[(int64_t)(k0 + lb + q) * as] : 0.0;
+              }
 #pragma unroll
               for (int q = 0; q < kNtA; ++q) {
+                if (lb + q >= l1) break;
                 const double* b = bs + (lb + q) * kNtCols;
 #pragma unroll
                 for (int jj = 0; jj < kNtCols; ++jj) acc[jj] = fma(a[q], b[jj], acc[jj]);
