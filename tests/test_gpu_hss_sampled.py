@@ -181,3 +181,43 @@ def test_precision_benchmark_size():
     assert e_mv < 4e-10
     assert res <= 1e-10
     assert r_true < 3e-10
+
+
+def test_coulomb_reference_generator():
+    """The reference's Coulomb kernel 1/|x_i - x_j| with its 10 N diagonal
+    shift (kernel.h:69-89, 92-106) on its grid, generated on the device:
+    singular near the diagonal, so its block rows need higher ranks.
+    Tolerance mode reproduces the dense matrix to 10 x tol and the solve
+    matches dense LAPACK on the compressed operator."""
+    n, leaf, tol = 2048, 128, 1e-10
+    xg = kernels.generate_grid(n)
+    a = kernels.generate_coulomb(n, 10.0 * n)
+    h = sampled(64, 16, tol=tol, fixed=False)
+    h.build_kernel(hmat_amd.make_kernel(hmat_amd.KERNEL_REF_COULOMB, xg, p1=10.0 * n), n, leaf)
+    G = h.to_dense()
+    print(f"\nCoulomb: operator vs A {rel(G, a):.1e}, root rank {h.root_rank()}", flush=True)
+    assert rel(G, a) < 10 * tol
+    b = np.random.default_rng(4).standard_normal((n, 2))
+    assert rel(h.solve(b), np.linalg.solve(G, b)) < 1e-12
+
+
+def test_argument_errors():
+    """A host Omega of the wrong length, an unknown build mode and a sample
+    width above 256 report HMAT_ERR_ARG and leave no operator behind."""
+    n = 1024
+    a = cauchy(n)
+    h = sampled(16, 10)
+    with pytest.raises(hmat_amd.HmatError) as e:
+        h.build_from_dense(a, 128, omega=np.zeros(2 * n * 26 - 1))
+    assert e.value.status == 1  # HMAT_ERR_ARG
+    with pytest.raises(hmat_amd.HmatError) as e:
+        h.multiply(np.ones(n))
+    assert e.value.status == 6  # HMAT_ERR_STATE: the failed build left no operator
+    bad = hmat_amd.HSSMatrixOMP(opts=hmat_amd.make_options(1e-9), nested=3)
+    with pytest.raises(hmat_amd.HmatError) as e:
+        bad.build_from_dense(a, 128)
+    assert e.value.status == 1
+    wide = sampled(250, 10)
+    with pytest.raises(hmat_amd.HmatError) as e:
+        wide.build_from_dense(a, 128)
+    assert e.value.status == 1
