@@ -220,19 +220,29 @@ void HodlrEngine::prepare_matvec() {
   mv_.nseg = (int)segs.size();
   mv_.KM = KM;
   mv_bytes_base_ = bytes;
-  mv_part_.reserve(std::max<size_t>((size_t)nl * L * KM * 4, 1));
-  mv_t_.reserve(std::max<size_t>(segs.size() * KM * 4, 1));
+  // widest fused pass whose leaf-expand staging fits 64 KB of LDS
+  mv_width_ = 0;
+  for (int w : {16, 8, 4})
+    if (kern::mv_lds_bytes(mv_, tree.lmax, w) <= 64 * 1024) {
+      mv_width_ = w;
+      break;
+    }
+  const size_t wr = (size_t)std::max(mv_width_, 4);
+  mv_part_.reserve(std::max<size_t>((size_t)nl * L * KM * wr, 1));
+  mv_t_.reserve(std::max<size_t>(segs.size() * KM * wr, 1));
 }
 
 double HodlrEngine::matvec_bytes(int nrhs) const { return mv_bytes_base_ + 16.0 * (double)N * nrhs; }
 
-// Fused path for up to 4 right-hand sides per pass (1, 2 or 4 at a time).
+// Fused path, passes of 16, 8, 4, 2 or 1 right-hand sides (at most
+// mv_width_ at a time): every pass reads the generators once.
 void HodlrEngine::matvec_async(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
   const int64_t lm = tree.lmax;
   int done = 0;
   while (done < nrhs) {
     const int rem = nrhs - done;
-    const int nr = rem >= 4 ? 4 : (rem >= 2 ? 2 : 1);
+    int nr = 1;
+    while (nr * 2 <= std::min(rem, mv_width_)) nr *= 2;
     kern::hodlr_matvec_fused(mv_, D.get(), lm, dX + (int64_t)done * ldx, ldx, nr, mv_part_.get(), mv_t_.get(),
                              dY + (int64_t)done * ldy, ldy, stream);
     done += nr;
@@ -492,7 +502,7 @@ void HodlrEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, in
 }
 
 size_t HodlrEngine::matvec_scratch_need(int nrhs) const {
-  const int nr = nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1);  // widest pass of matvec_async
+  const int nr = nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1);  // widest pass of matvec_with_scratch
   return ((size_t)mv_.nleaf * mv_.L * mv_.KM + (size_t)mv_.nseg * mv_.KM) * nr;
 }
 
@@ -524,7 +534,7 @@ void HodlrEngine::matvec_with_scratch(const double* dX, int64_t ldx, int nrhs, d
 void HodlrEngine::matvec_enqueue(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy) {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "matvec before build");
   HMAT_REQUIRE(nrhs > 0 && ldx >= N && ldy >= N, HMAT_ERR_ARG, "bad matvec dimensions");
-  if (nrhs <= 4 && kern::mv_lds_bytes(mv_, tree.lmax, 4) <= 64 * 1024) {
+  if (mv_width_ >= 4 && nrhs <= mv_width_) {
     ProfScope prof("matvec", stream, matvec_bytes(nrhs));
     matvec_async(dX, ldx, nrhs, dY, ldy);
     prof.finish();

@@ -304,6 +304,7 @@ class HodlrEngine {
   DevBuf<kern::MvEntry> mv_ent_;
   DevBuf<kern::MvSeg> mv_segs_;
   DevBuf<double> mv_part_, mv_t_;
+  int mv_width_ = 0;  // widest fused matvec pass (16, 8 or 4 columns; 0: none fits LDS)
   double mv_bytes_base_ = 0.0;  // D + (U S) + V bytes of one pass
   int last_ninfo_ = 0;          // pivot statuses written by the last factorisation
 

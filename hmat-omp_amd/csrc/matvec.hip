@@ -9,6 +9,7 @@
 //            V_c^T x over its rows (lanes run down the rows, so V is read
 //            coalesced once), reduced across lanes by recursive halving
 //            (63 shuffles for 64 values) -> one partial per (leaf, depth);
+//            passes of 8 or 16 right-hand sides run it on FP64 MFMA;
 //   reduce   one workgroup per segment: t_s = sum of its leaves' partials;
 //   expand   one workgroup per leaf: y = D_leaf x_leaf + sum_c (U S)_c t_sib,
 //            x and the t vectors staged in LDS, D and every (U S)_c read
@@ -21,6 +22,7 @@
 
 #include "kernels.hpp"
 #include "matvec.hpp"
+#include "mfma.hpp"
 
 namespace hmat {
 namespace kern {
@@ -58,7 +60,7 @@ __device__ inline double wave_reduce_scatter16(double (&v)[16]) {
 }
 
 // partial[((leaf * L + c) * KM + k) * nrhs + col].  Per depth, groups of
-// KG = 16 / NR columns of V: every lane loads its row of the group
+// KG = 16 / NR columns of V (one column for NR = 16): every lane loads its row of the group
 // (unconditionally, the column index clamped), the wave reduces, and the
 // four waves meet in LDS once per depth.
 template <int NR>
@@ -87,52 +89,60 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project(const MvLeaf* __restrict__
     __syncthreads();
   };
   if (lf.len <= kMvWG) {
-    // One row per thread.  The steps (depth c, column group kb) run as a
+    // One row per thread.  A step loads 16 columns of V (the same depth of
+    // loads for every NR) and reduces them in 16 / KG rounds of KG columns
+    // x NR right-hand sides.  The steps (depth c, column block kb) run as a
     // pipeline: the next step's loads are issued before this step's
-    // reduction, so every wave keeps loads in flight across its shuffles.
+    // reductions, so every wave keeps loads in flight across its shuffles.
+    constexpr int LU = 16;
     const int r = threadIdx.x;
     const bool live = r < lf.len;
     const int64_t row = lf.start + min(r, lf.len - 1);
     double xr[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) xr[q] = live ? Xg[row + (int64_t)q * ldx] : 0.0;
-    auto load = [&](double (&a)[KG], int c, int vcols, int kb) {
+    auto load = [&](double (&a)[LU], int c, int vcols, int kb) {
       gdouble* V = (gdouble*)lvls[c].V;
       const int64_t N = lvls[c].ld;
 #pragma unroll
-      for (int kk = 0; kk < KG; ++kk) a[kk] = V[row + (int64_t)min(kb + kk, vcols - 1) * N];
+      for (int kk = 0; kk < LU; ++kk) a[kk] = V[row + (int64_t)min(kb + kk, vcols - 1) * N];
     };
     int c = 0;
     while (c < L && le[c].vcols <= 0) ++c;
     if (c == L) return;
     int vc = le[c].vcols, kb = 0;
-    double a[KG];
+    double a[LU];
     load(a, c, vc, kb);
     while (true) {
-      int nc = c, nkb = kb + KG, nvc = vc;
+      int nc = c, nkb = kb + LU, nvc = vc;
       if (nkb >= vc) {
         nkb = 0;
         ++nc;
         while (nc < L && le[nc].vcols <= 0) ++nc;
         if (nc < L) nvc = le[nc].vcols;
       }
-      double an[KG];
+      double an[LU];
       if (nc < L) load(an, nc, nvc, nkb);
-      double v[16];
 #pragma unroll
-      for (int kk = 0; kk < KG; ++kk) {
-        const double ak = (live && kb + kk < vc) ? a[kk] : 0.0;
+      for (int g = 0; g < LU / KG; ++g) {
+        if (kb + g * KG < vc) {  // the same for the whole workgroup
+          double v[16];
 #pragma unroll
-        for (int q = 0; q < NR; ++q) v[kk * NR + q] = ak * xr[q];
+          for (int kk = 0; kk < KG; ++kk) {
+            const double ak = (live && kb + g * KG + kk < vc) ? a[g * KG + kk] : 0.0;
+#pragma unroll
+            for (int q = 0; q < NR; ++q) v[kk * NR + q] = ak * xr[q];
+          }
+          const double sum = wave_reduce_scatter16(v);  // lane & 15 = kk * NR + q
+          if (lane < 16) red[w * KMP * NR + (kb + g * KG) * NR + lane] = sum;
+        }
       }
-      const double sum = wave_reduce_scatter16(v);  // lane & 15 = kk * NR + q
-      if (lane < 16) red[w * KMP * NR + kb * NR + lane] = sum;
       if (nkb == 0) flush(c, vc);
       if (nc >= L) break;
       c = nc;
       kb = nkb;
       vc = nvc;
-      for (int kk = 0; kk < KG; ++kk) a[kk] = an[kk];
+      for (int kk = 0; kk < LU; ++kk) a[kk] = an[kk];
     }
     return;
   }
@@ -164,6 +174,64 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project(const MvLeaf* __restrict__
       if (lane < 16) red[w * KMP * NR + kb * NR + lane] = s;
     }
     flush(c, e.vcols);
+  }
+}
+
+// Wide passes (NR = 8, 16; leaves of at most 256 rows): the per-leaf
+// projection is a small GEMM, (V_c^T)(rows of the leaf) X(rows, 0:NR), run
+// on v_mfma_f64_16x16x4f64.  The leaf's X rows are staged in LDS as 16
+// columns (zero beyond NR and past the leaf's rows); each wave takes whole
+// (depth, 16-column block) items, streams its 16 V columns four rows per
+// MFMA (eight MFMAs' loads in flight) and writes the 16 x NR block of
+// partials.  Same partial layout as k_mv_project.
+template <int NR>
+__global__ __launch_bounds__(kMvWG) void k_mv_project_mfma(const MvLeaf* __restrict__ leaves,
+                                                           const MvLevel* __restrict__ lvls,
+                                                           const MvEntry* __restrict__ ent, int L, int KM,
+                                                           const double* __restrict__ X, int64_t ldx,
+                                                           double* __restrict__ part) {
+  static_assert(NR == 8 || NR == 16, "wide passes only");
+  extern __shared__ double xs[];  // lenp x 16
+  const MvLeaf lf = leaves[blockIdx.x];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lenp = (lf.len + 3) & ~3;
+  for (int e = threadIdx.x; e < lenp * 16; e += kMvWG) {
+    const int r = e / 16, q = e % 16;
+    xs[e] = (r < lf.len && q < NR) ? X[lf.start + r + (int64_t)q * ldx] : 0.0;
+  }
+  __syncthreads();
+  const MvEntry* le = ent + (int64_t)blockIdx.x * L;
+  const int kr = lane >> 4, ci = lane & 15;
+  int item = 0;
+  for (int c = 0; c < L; ++c) {
+    const int vc = le[c].vcols;
+    if (vc <= 0) continue;
+    gdouble* V = (gdouble*)lvls[c].V;
+    const int64_t N = lvls[c].ld;
+    for (int cb = 0; cb < vc; cb += 16, ++item) {
+      if ((item & 3) != w) continue;
+      const bool cok = cb + ci < vc;
+      gdouble* vcol = V + lf.start + (int64_t)min(cb + ci, vc - 1) * N;
+      mfma::v4d acc = {0.0, 0.0, 0.0, 0.0};
+      for (int r0 = 0; r0 < lenp; r0 += 32) {
+        double a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int r = r0 + 4 * u + kr;
+          const double v = vcol[min(r, lf.len - 1)];
+          a[u] = (cok && r < lf.len) ? v : 0.0;
+          b[u] = r < lenp ? xs[r * 16 + ci] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (r0 + 4 * u < lenp) acc = mfma::f64_16x16x4(a[u], b[u], acc);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = mfma::f64_row(lane, v), q = mfma::f64_col(lane, v);
+        if (cb + i < vc && q < NR) part[(((int64_t)blockIdx.x * L + c) * KM + cb + i) * NR + q] = acc[v];
+      }
+    }
   }
 }
 
@@ -231,32 +299,42 @@ __global__ __launch_bounds__(kMvWG) void k_mv_expand(const MvLeaf* __restrict__ 
   }
   __syncthreads();
   gdouble* Dl = (gdouble*)(Dm + (int64_t)blockIdx.x * lm * lm);
-  for (int r = threadIdx.x; r < lf.len; r += kMvWG) {
+  // Short leaves leave threads without a row: there each row's products
+  // are split over tpr adjacent lanes (contiguous column ranges), summed by
+  // shuffles, so every wave streams loads.
+  const int tpr = lf.len <= 64 ? 4 : (lf.len <= 128 ? 2 : 1);
+  const int sub = threadIdx.x % tpr;
+  const int rpp = kMvWG / tpr;  // rows per pass
+  for (int r0 = 0; r0 < lf.len; r0 += rpp) {
+    const int r = r0 + threadIdx.x / tpr;
+    const bool live = r < lf.len;
     double acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = 0.0;
     // acc += M(row, j0 .. j0+n) v(j0 .. j0+n): batches of 16 and 8
-    // independent loads, then single ones; M column j at M + j * ldm
+    // independent loads (8 or 4 for the wide passes, which keep NR
+    // accumulators), then single ones; M column j at M + j * ldm
     auto dot = [&](gdouble* M, int64_t ldm, const double* v, int n) {
       int j = 0;
-      for (; j + 16 <= n; j += 16) {
-        double m[16];
+      if constexpr (NR <= 4)
+        for (; j + 16 <= n; j += 16) {
+          double m[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) m[u] = M[(int64_t)(j + u) * ldm];
+          for (int u = 0; u < 16; ++u) m[u] = M[(int64_t)(j + u) * ldm];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
+          for (int u = 0; u < 16; ++u)
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q] = fma(m[u], v[(j + u) * NR + q], acc[q]);
+        }
+      constexpr int B = NR <= 8 ? 8 : 4;
+      for (; j + B <= n; j += B) {
+        double m[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) m[u] = M[(int64_t)(j + u) * ldm];
+#pragma unroll
+        for (int u = 0; u < B; ++u)
 #pragma unroll
           for (int q = 0; q < NR; ++q) acc[q] = fma(m[u], v[(j + u) * NR + q], acc[q]);
-      }
-      if (j + 8 <= n) {
-        double m[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) m[u] = M[(int64_t)(j + u) * ldm];
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-#pragma unroll
-          for (int q = 0; q < NR; ++q) acc[q] = fma(m[u], v[(j + u) * NR + q], acc[q]);
-        j += 8;
       }
       for (; j < n; ++j) {
         const double m = M[(int64_t)j * ldm];
@@ -264,14 +342,25 @@ __global__ __launch_bounds__(kMvWG) void k_mv_expand(const MvLeaf* __restrict__ 
         for (int q = 0; q < NR; ++q) acc[q] = fma(m, v[j * NR + q], acc[q]);
       }
     };
-    dot(Dl + r, lm, xs, lf.len);
-    const int64_t row = lf.start + r;
-    for (int c = 0; c < L; ++c) {
-      const MvEntry en = ent[(int64_t)blockIdx.x * L + c];
-      dot((gdouble*)lvls[c].US + row, lvls[c].ld, ts + (int64_t)c * KM * NR, en.ucols);
+    // this lane's share [j0, j1) of n columns
+    auto part_dot = [&](gdouble* M, int64_t ldm, const double* v, int n) {
+      const int j0 = (int)((int64_t)n * sub / tpr), j1 = (int)((int64_t)n * (sub + 1) / tpr);
+      dot(M + (int64_t)j0 * ldm, ldm, v + (int64_t)j0 * NR, j1 - j0);
+    };
+    const int64_t row = lf.start + (live ? r : 0);
+    if (live) {
+      part_dot(Dl + r, lm, xs, lf.len);
+      for (int c = 0; c < L; ++c) {
+        const MvEntry en = ent[(int64_t)blockIdx.x * L + c];
+        part_dot((gdouble*)lvls[c].US + row, lvls[c].ld, ts + (int64_t)c * KM * NR, en.ucols);
+      }
     }
+    for (int o = 1; o < tpr; o <<= 1)
 #pragma unroll
-    for (int q = 0; q < NR; ++q) Y[row + (int64_t)q * ldy] = acc[q];
+      for (int q = 0; q < NR; ++q) acc[q] += __shfl_xor(acc[q], o);
+    if (live && sub == 0)
+#pragma unroll
+      for (int q = 0; q < NR; ++q) Y[row + (int64_t)q * ldy] = acc[q];
   }
 }
 
@@ -280,8 +369,18 @@ static void launch_mv(const MvPlan& p, const double* Dm, int64_t lm, const doubl
                       double* part, double* t, double* Y, int64_t ldy, hipStream_t s) {
   const unsigned nl = (unsigned)p.nleaf;
   if (p.L > 0) {
-    const size_t plds = (size_t)(kMvWG / 64) * ((p.KM + 15) / 16 * 16) * NR * sizeof(double);
-    k_mv_project<NR><<<nl, kMvWG, plds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
+    if constexpr (NR >= 8) {
+      if (lm <= kMvWG) {
+        const size_t xl = (size_t)((lm + 3) & ~3) * 16 * sizeof(double);
+        k_mv_project_mfma<NR><<<nl, kMvWG, xl, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
+      } else {
+        const size_t plds = (size_t)(kMvWG / 64) * ((p.KM + 15) / 16 * 16) * NR * sizeof(double);
+        k_mv_project<NR><<<nl, kMvWG, plds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
+      }
+    } else {
+      const size_t plds = (size_t)(kMvWG / 64) * ((p.KM + 15) / 16 * 16) * NR * sizeof(double);
+      k_mv_project<NR><<<nl, kMvWG, plds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
+    }
     HIP_CHECK(hipGetLastError());
     k_mv_reduce<NR><<<(unsigned)p.nseg, kMvWG, 0, s>>>(p.segs, p.L, p.KM, part, t);
     HIP_CHECK(hipGetLastError());
@@ -301,7 +400,9 @@ void hodlr_matvec_fused(const MvPlan& p, const double* Dm, int64_t lm, const dou
     case 1: launch_mv<1>(p, Dm, lm, X, ldx, part, t, Y, ldy, s); break;
     case 2: launch_mv<2>(p, Dm, lm, X, ldx, part, t, Y, ldy, s); break;
     case 4: launch_mv<4>(p, Dm, lm, X, ldx, part, t, Y, ldy, s); break;
-    default: throw Error(HMAT_ERR_ARG, "fused matvec takes 1, 2 or 4 right-hand sides");
+    case 8: launch_mv<8>(p, Dm, lm, X, ldx, part, t, Y, ldy, s); break;
+    case 16: launch_mv<16>(p, Dm, lm, X, ldx, part, t, Y, ldy, s); break;
+    default: throw Error(HMAT_ERR_ARG, "fused matvec takes 1, 2, 4, 8 or 16 right-hand sides");
   }
 }
 

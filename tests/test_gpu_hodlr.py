@@ -549,3 +549,22 @@ def test_sketch_width_64():
     assert rel(ad, a) < 1e-10
     b = np.random.default_rng(8).standard_normal((n, 2))
     assert rel(h.solve(b), np.linalg.solve(ad, b)) < 1e-10
+
+
+@pytest.mark.parametrize("ncols", [5, 12, 16, 20])
+def test_fused_matvec_wide_passes(ncols):
+    """Column blocks go through the fused matvec in passes of 16, 8, 4, 2 or
+    1 columns (20 columns take the segmented path): every column equals its
+    single-column product and the dense reconstruction's to rounding."""
+    n = 4096
+    x = kernels.sorted_uniform_points(n, 31)
+    o = hmat_amd.make_options(0.0, rank=16, fixed_rank=True, oversample=10, power_iters=1, exact_below=0,
+                              omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=9)
+    h = hmat_amd.HODLRMatrixOMP(opts=o)
+    h.build_kernel(hmat_amd.make_kernel(hmat_amd.KERNEL_GAUSSIAN, x, p0=0.05, p1=1e-2), n, 64)
+    ad = h.to_dense()
+    v = np.random.default_rng(ncols).standard_normal((n, ncols))
+    y = h.multiply(v)
+    assert rel(y, ad @ v) < 1e-13
+    for j in range(ncols):
+        assert rel(y[:, j:j + 1], h.multiply(v[:, j])) < 1e-14
