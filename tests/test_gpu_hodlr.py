@@ -551,17 +551,18 @@ def test_sketch_width_64():
     assert rel(h.solve(b), np.linalg.solve(ad, b)) < 1e-10
 
 
-@pytest.mark.parametrize("ncols", [5, 12, 16, 20])
-def test_fused_matvec_wide_passes(ncols):
+@pytest.mark.parametrize("ncols,leaf", [(5, 64), (12, 64), (16, 64), (20, 64), (12, 300), (16, 512)])
+def test_fused_matvec_wide_passes(ncols, leaf):
     """Column blocks go through the fused matvec in passes of 16, 8, 4, 2 or
-    1 columns (20 columns take the segmented path): every column equals its
+    1 columns (20 columns take the segmented path; leaves above 256 rows
+    project by shuffles and take narrower passes): every column equals its
     single-column product and the dense reconstruction's to rounding."""
     n = 4096
     x = kernels.sorted_uniform_points(n, 31)
     o = hmat_amd.make_options(0.0, rank=16, fixed_rank=True, oversample=10, power_iters=1, exact_below=0,
                               omega_dist=hmat_amd.OMEGA_GAUSSIAN, seed=9)
     h = hmat_amd.HODLRMatrixOMP(opts=o)
-    h.build_kernel(hmat_amd.make_kernel(hmat_amd.KERNEL_GAUSSIAN, x, p0=0.05, p1=1e-2), n, 64)
+    h.build_kernel(hmat_amd.make_kernel(hmat_amd.KERNEL_GAUSSIAN, x, p0=0.05, p1=1e-2), n, leaf)
     ad = h.to_dense()
     v = np.random.default_rng(ncols).standard_normal((n, ncols))
     y = h.multiply(v)
