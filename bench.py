@@ -373,6 +373,38 @@ def hss_cauchy_leg(hmat_amd, torch, dev, n, leaf, rank, p, rtol, reps=10):
             gbs = pq[2] / pq[0] / (ms * 1e-3) / 1e9
             r[key] = {"avg_ms": round(ms, 4), "bytes": pq[2] / pq[0], "GBs": round(gbs, 1),
                       "frac_hbm": round(gbs / PEAK_HBM_GBS, 4), "calls": pq[0]}
+    # ULV factorisation of the same operator (orthogonal eliminations,
+    # backward stable): factor and solve times (second call), residuals of
+    # the unrefined solve, and the apply-inverse rate
+    h.set_solver(hmat_amd.HSS_ULV)
+    u = {}
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h.factor()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        h.solve_device(b.data_ptr(), n, 1, x.data_ptr(), n)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+    u.update(factor_s=round(t1 - t0, 4), solve_s=round(t2 - t1, 5))
+    h.multiply_device(x.data_ptr(), n, 1, y.data_ptr(), n)
+    torch.cuda.synchronize()
+    u.update(resid_AH=float(torch.linalg.norm(b - y) / torch.linalg.norm(b)),
+             resid_trueA=true_residual(hmat_amd, torch, x, dev, n))
+    hmat_amd.profile_reset()
+    hmat_amd.profile_enable(True)
+    for _ in range(reps):
+        h.solve_device(b.data_ptr(), n, 1, x.data_ptr(), n)
+    torch.cuda.synchronize()
+    hmat_amd.profile_enable(False)
+    pq = hmat_amd.profile_query("hss_ulv_apply")
+    if pq and pq[0]:
+        ms = pq[1] / pq[0]
+        gbs = pq[2] / pq[0] / (ms * 1e-3) / 1e9
+        u["solve"] = {"avg_ms": round(ms, 4), "bytes": pq[2] / pq[0], "GBs": round(gbs, 1),
+                      "frac_hbm": round(gbs / PEAK_HBM_GBS, 4), "calls": pq[0]}
+    r["ulv"] = u
     log(f"[hss_cauchy] {r}")
     del h
     return r

@@ -535,6 +535,15 @@ hmat_status hmat_hss_set_refinement(hmat_hss h, double rtol, int32_t max_iters) 
   });
 }
 
+hmat_status hmat_hss_set_solver(hmat_hss h, int32_t solver) {
+  return guard([&] {
+    HMAT_REQUIRE(h, HMAT_ERR_ARG, "NULL handle");
+    HMAT_REQUIRE(solver == HMAT_HSS_WOODBURY || solver == HMAT_HSS_ULV, HMAT_ERR_ARG, "unknown HSS solver");
+    if (h->eng.solver != solver) h->eng.factored = false;
+    h->eng.solver = solver;
+  });
+}
+
 hmat_status hmat_hss_last_refinement(hmat_hss h, int32_t* iters, double* resid) {
   return guard([&] {
     HMAT_REQUIRE(h, HMAT_ERR_ARG, "NULL handle");

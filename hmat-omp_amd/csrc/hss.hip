@@ -22,47 +22,13 @@
 #include <cstring>
 
 #include "hss.hpp"
+#include "sweep.hpp"
 
 namespace hmat {
 
 using kern::BGemm;
 using kern::Rect;
 using kern::ZeroRect;
-
-namespace {
-// The per-node work of a sweep: phases (one launch each, in order) of tasks
-// (one workgroup each, independent within a phase) of ops (small GEMMs or
-// copies, run in order by the task's workgroup).  All descriptors go to the
-// device in one staged copy.
-struct SweepProgram {
-  std::vector<BGemm> ops;
-  std::vector<kern::NodeTask> tasks;
-  std::vector<std::pair<int, int>> phases;  // [first task, end)
-  void begin_phase() { phases.push_back({(int)tasks.size(), (int)tasks.size()}); }
-  void begin_task() { tasks.push_back(kern::NodeTask{(int32_t)ops.size(), 0}); }
-  void op(const BGemm& g) {
-    ops.push_back(g);
-    ++tasks.back().nops;
-  }
-  // C = alpha A + beta C for an m x n block
-  void copy(const double* A, int lda, double* C, int ldc, int m, int n, double alpha = 1.0, double beta = 0.0) {
-    op(BGemm{A, nullptr, C, m, n, 0, lda, 0, ldc, 0, 0, alpha, beta});
-  }
-  void end_task() {
-    if (tasks.back().nops == 0) tasks.pop_back();
-    phases.back().second = (int)tasks.size();
-  }
-  void run(hipStream_t s) {
-    if (tasks.empty()) return;
-    StageGroup g;
-    const int io = g.add(ops), it = g.add(tasks);
-    g.put(s);
-    const BGemm* dops = g.get<BGemm>(io);
-    const kern::NodeTask* dt = g.get<kern::NodeTask>(it);
-    for (const auto& p : phases) kern::node_tasks(dt + p.first, p.second - p.first, dops, s);
-  }
-};
-}  // namespace
 
 HssEngine::HssEngine(const hmat_options& o) : opt(o) {
   HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -1112,6 +1078,12 @@ void HssEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int6
 void HssEngine::factor() {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "factor before build");
   factored = false;
+  if (solver == kUlv) {
+    ulv_factor();
+    HIP_CHECK(hipStreamSynchronize(stream));
+    factored = true;
+    return;
+  }
   pool_used = pool_built;  // a refactorisation reuses the factor's part of the pool
   const int64_t lm = tree.lmax;
   const int nl = (int)tree.leaves.size();
@@ -1303,6 +1275,10 @@ void HssEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64
 }
 
 void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
+  if (solver == kUlv) {
+    ulv_apply(dB, ldb, nrhs, dX, ldx);
+    return;
+  }
   ProfScope prof("hss_solve_apply", stream, solve_bytes(nrhs));
   lres_.reserve((size_t)N * nrhs);
   leaf_solve(dB, ldb, nrhs, dX, ldx, lres_.get());

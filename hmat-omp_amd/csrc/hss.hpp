@@ -44,6 +44,10 @@ class HssEngine {
   ~HssEngine();
 
   enum Mode { kReference = 0, kNested = 1, kSampled = 2 };
+  // factor()/solve(): the reference's push-through Woodbury recursion, or a
+  // ULV factorisation (orthogonal eliminations, backward stable; nested
+  // bases with every reduced block at most 1024 rows)
+  enum Solver { kWoodbury = 0, kUlv = 1 };
   void build(const GenDesc& g, int64_t N, int leaf, int mode, const double* host_omega,
              int64_t omega_len);
   void matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int64_t ldy);
@@ -75,6 +79,7 @@ class HssEngine {
   bool nested = false;
   int mode = kReference;
   bool built = false, factored = false;
+  int solver = kWoodbury;
   // FP64 iterative refinement of solves through the HSS matvec (off by
   // default: rtol 0); iterations and max relative residual of the last solve
   double refine_rtol = 0.0;
@@ -101,6 +106,9 @@ class HssEngine {
   // does: backward stable like an LU solve while every step stays a GEMM
   void leaf_solve(const double* B, int64_t ldb, int ncols, double* Y, int64_t ldy, double* R);
   void apply_inverse(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
+  void ulv_factor();
+  void ulv_apply(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
+  double ulv_bytes(int nrhs) const;
 
   // leaf generators
   DevBuf<double> D, Dinv, Ul, Vl, Ql;  // D/Dinv: nleaf x lmax^2; Ul, Vl, Ql: N x rmax
@@ -132,6 +140,21 @@ class HssEngine {
     DevBuf<double> sp;            // reduced samples per node (see build_sampled)
     int64_t sp_used = 0;
   } ss_;
+
+  // ULV factor, per node: n block rows (leaf size, or the children's ranks
+  // summed), r = rank of U, m = n - r rows eliminated, s = rank of V; offsets
+  // into upool_ of the Householder vectors of U (n x r) with their tau and
+  // signs, of C[r:]^T = H_P [Rq; 0] (n x m, reflectors below Rq) with tau,
+  // of G = H_P^T C[:r]^T (n x r), of H_P^T V (n x s) and of Dt (r x r); all
+  // leading dimensions n (Dt: r)
+  struct UNode {
+    int n = 0, r = 0, m = 0, s = 0;
+    int64_t oHU = -1, oTU = -1, oSU = -1, oHP = -1, oTP = -1, oG = -1, oV = -1, oDt = -1;
+  };
+  std::vector<UNode> un_;
+  DevBuf<double> upool_, uC_;           // factor; reduced blocks of one depth
+  DevBuf<double> ubeta_, ugk_, uzero_;  // solve: right-hand sides and known parts per node
+  DevBuf<int> uinfo_;
 
   kern::Workspace ws_;
   Orthogonalizer orth_;

@@ -27,6 +27,7 @@ KERNEL_GAUSSIAN, KERNEL_CAUCHY, KERNEL_REF_CAUCHY, KERNEL_REF_GAUSSIAN, KERNEL_R
 KERNEL_GAUSSIAN_DSIGMA = 6
 # HSS build modes (the `nested` argument of hmat_hss_build_*)
 HSS_REFERENCE, HSS_NESTED, HSS_SAMPLED = 0, 1, 2
+HSS_WOODBURY, HSS_ULV = 0, 1  # HSS solvers
 
 
 class HmatError(RuntimeError):
@@ -116,6 +117,7 @@ def _proto(lib):
         "hmat_hss_root_rank": (i32, [vp]),
         "hmat_hss_to_dense": (st, [vp, _D, i64]),
         "hmat_hss_set_refinement": (st, [vp, C.c_double, i32]),
+        "hmat_hss_set_solver": (st, [vp, i32]),
         "hmat_hss_last_refinement": (st, [vp, C.POINTER(i32), C.POINTER(C.c_double)]),
         "hmat_hss_save": (st, [vp, C.c_char_p]),
         "hmat_hss_load": (st, [vp, C.c_char_p]),
@@ -478,6 +480,12 @@ class HSSMatrixOMP(_Base):
         it, res = C.c_int32(), C.c_double()
         _check(self._lib.hmat_hss_last_refinement(self._h, C.byref(it), C.byref(res)))
         return it.value, res.value
+
+    def set_solver(self, solver):
+        """HSS_WOODBURY (the reference's push-through solve, default) or
+        HSS_ULV (orthogonal ULV factorisation, backward stable); drops the
+        factorisation."""
+        _check(self._lib.hmat_hss_set_solver(self._h, solver))
 
     def save(self, path):
         """Write the built operator to `path` (see hmat_hss_save)."""

@@ -380,6 +380,19 @@ hmat_status hmat_hss_to_dense(hmat_hss h, double* A, int64_t lda);
  * refinement was off). */
 hmat_status hmat_hss_set_refinement(hmat_hss h, double rtol, int32_t max_iters);
 hmat_status hmat_hss_last_refinement(hmat_hss h, int32_t* iters, double* resid);
+/* Solver of hmat_hss_factor / hmat_hss_solve (no reference counterpart for
+ * ULV: the reference's solve is the push-through Woodbury recursion,
+ * hss_routines_omp.h:247-346, HMAT_HSS_WOODBURY, the default).
+ * HMAT_HSS_ULV factors the operator by orthogonal eliminations (Householder
+ * transforms and triangular solves; Chandrasekaran, Gu and Pals 2006):
+ * backward stable without refinement.  It needs every reduced block to
+ * have at most 1024 rows (leaves of at most 1024 points, and the children's
+ * ranks summed at every node; the nested builds keep them bounded, the
+ * reference-mode build's ranks add up) and returns HMAT_ERR_ARG from the
+ * factorisation otherwise.  Changing the solver drops the factorisation;
+ * refinement (above) runs on top of either solver. */
+typedef enum { HMAT_HSS_WOODBURY = 0, HMAT_HSS_ULV = 1 } hmat_hss_solver;
+hmat_status hmat_hss_set_solver(hmat_hss h, int32_t solver);
 /* Operator files (no reference counterpart: the reference has no
  * serialisation).  save writes the built operator (options, N, leaf size,
  * nested flag, per-node ranks, transfers, couplings, leaf bases and leaf
