@@ -172,29 +172,35 @@ __device__ __forceinline__ bool ap_wait(int* flag, int p, int* err, int64_t limi
   return ok;
 }
 
-// dst[i] = sum over leaves [l0, l1) of part[leaf][i], i < T: thread slot s
-// of an i-chunk sums leaves l0 + s, l0 + s + S, ... in order (eight loads in
-// flight), then the S slots are added in order, so the result does not
-// depend on scheduling.  red: kApWG entries of LDS.
+// The parent's right-hand side from its leaves' partials, both children in
+// one pass: dst[i] for i < T1 sums part[leaf][i] over leaves [l0, l1) (the
+// left child), dst[T1 + i] for i < T2 sums part[leaf][i] over [l1, l2).
+// Thread slot s of a value chunk sums the leaves a0 + s, a0 + s + S, ... in
+// order (16 loads in flight), then the S slots are added in order, so the
+// result does not depend on scheduling.  red: kApWG entries of LDS.
 template <class F>
-__device__ void ap_sum_leaves(F* part, int64_t pst, int l0, int l1, int T, F* dst, F* red) {
+__device__ void ap_sum_leaves(F* part, int64_t pst, int l0, int l1, int l2, int T1, int T2, F* dst, F* red) {
   const int tid = threadIdx.x;
+  const int T = T1 + T2;
   for (int i0 = 0; i0 < T; i0 += kApWG) {
     const int C = min(T - i0, kApWG);
     const int S = kApWG / C;
     const int ic = tid % C, sl = tid / C;
+    const int i = i0 + ic;
+    const bool left = i < T1;
+    const int a1 = left ? l1 : l2;
     F acc = F(0);
     if (sl < S) {
-      F* pp = part + i0 + ic;
-      int l = l0 + sl;
-      for (; l + 7 * S < l1; l += 8 * S) {
-        F v[8];
+      F* pp = part + (left ? i : i - T1);
+      int l = (left ? l0 : l1) + sl;
+      for (; l + 15 * S < a1; l += 16 * S) {
+        F v[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = ld_dev(pp + (int64_t)(l + u * S) * pst);
+        for (int u = 0; u < 16; ++u) v[u] = ld_dev(pp + (int64_t)(l + u * S) * pst);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc += v[u];
+        for (int u = 0; u < 16; ++u) acc += v[u];
       }
-      for (; l < l1; l += S) acc += ld_dev(pp + (int64_t)l * pst);
+      for (; l < a1; l += S) acc += ld_dev(pp + (int64_t)l * pst);
     }
     red[tid] = acc;
     __syncthreads();
@@ -207,45 +213,47 @@ __device__ void ap_sum_leaves(F* part, int64_t pst, int l0, int l1, int T, F* ds
   }
 }
 
-// y = base + sign * A v for an n x n matrix A (ld lda, global) and v in LDS
-// (n x NR, row-major k * NR + q): the four waves take quarters of k, lanes
-// rows, eight loads in flight; the quarters are added in order.  y may be
-// LDS or global; red2: 4 n NR entries of LDS.
+// y = base + sign * A v for an m x n matrix A (ld lda, global) and v in LDS
+// (n x NR, row-major k * NR + q).  Thread t < S m takes row t % m and the
+// k-slice t / m of S = min(4, kApWG / m) slices, so that every thread
+// streams a run of one row: 32 loads in flight for one FP32 right-hand
+// side, 16 otherwise (more held the FP64 kernels above 128 VGPRs).  Above
+// kApWG rows: blocks of kApWG rows with one slice.  The slices are added in
+// order.  y may be LDS or global; red2: 4 m NR entries of LDS.
 template <class F, int NR>
-__device__ void ap_matvec(gptr<F> A, int64_t lda, int n, const F* v, const F* base, F sign, F* y, F* red2) {
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int kq = (n + kApWaves - 1) / kApWaves;
-  const int k0 = w * kq, k1 = min(n, k0 + kq);
-  for (int i0 = 0; i0 < n; i0 += 64) {
-    const int i = i0 + lane;
+__device__ void ap_matvec(gptr<F> A, int64_t lda, int m, int n, const F* v, const F* base, F sign, F* y,
+                          F* red2) {
+  constexpr int KB = (NR == 1 && sizeof(F) == 4) ? 32 : 16;  // loads in flight per thread
+  const int tid = threadIdx.x;
+  const int S = (m <= 0 || m >= kApWG) ? 1 : min(4, kApWG / m);
+  const int kq = (n + S - 1) / S;
+  for (int i0 = 0; i0 < m; i0 += kApWG) {
+    const int rows = min(m - i0, kApWG);
+    const int sl = tid / rows, i = i0 + tid % rows;
+    if (sl >= S) continue;
+    const int k0 = sl * kq, k1 = min(n, k0 + kq);
     F acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = F(0);
-    if (i < n) {
-      int k = k0;
-      for (; k + 8 <= k1; k += 8) {
-        F m[8];
+    gptr<F> Ai = A + i;
+    for (int k = k0; k < k1; k += KB) {
+      const int nk = min(KB, k1 - k);
+      F e[KB];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) m[u] = A[i + (int64_t)(k + u) * lda];
+      for (int u = 0; u < KB; ++u) e[u] = Ai[(int64_t)(k + min(u, nk - 1)) * lda];  // in bounds
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < KB; ++u)
+        if (u < nk)
 #pragma unroll
-          for (int q = 0; q < NR; ++q) acc[q] = fma(m[u], v[(k + u) * NR + q], acc[q]);
-      }
-      for (; k < k1; ++k) {
-        const F m = A[i + (int64_t)k * lda];
-#pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] = fma(m, v[k * NR + q], acc[q]);
-      }
-#pragma unroll
-      for (int q = 0; q < NR; ++q) red2[((int64_t)w * n + i) * NR + q] = acc[q];
+          for (int q = 0; q < NR; ++q) acc[q] = fma(e[u], v[(k + u) * NR + q], acc[q]);
     }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) red2[((int64_t)sl * m + i) * NR + q] = acc[q];
   }
   __syncthreads();
-  for (int e = tid; e < n * NR; e += kApWG) {
+  for (int e = tid; e < m * NR; e += kApWG) {
     F sum = F(0);
-#pragma unroll
-    for (int u = 0; u < kApWaves; ++u) sum += red2[(int64_t)u * n * NR + e];
+    for (int u = 0; u < S; ++u) sum += red2[(int64_t)u * m * NR + e];
     y[e] = (base ? base[e] : F(0)) + sign * sum;
   }
   __syncthreads();
@@ -259,25 +267,91 @@ __device__ void ap_leader(const ApArgs<F>& a, const ApLevel<F>& lv, int p, F* ts
   const ApParent P = a.par[p];
   const int msize = P.rr + P.rl;
   const int64_t pst = (int64_t)a.KM * NR;
-  ap_sum_leaves<F>(a.part, pst, P.leaf0, P.lsplit, P.rr * NR, ts, red);
-  ap_sum_leaves<F>(a.part, pst, P.lsplit, P.leaf1, P.rl * NR, ts + P.rr * NR, red);
   gptr<F> Mi = (gptr<F>)(lv.Minv + P.moff);
+  gptr<F> M = REFINE ? (gptr<F>)(lv.M + P.moff) : nullptr;
+  ap_sum_leaves<F>(a.part, pst, P.leaf0, P.lsplit, P.leaf1, P.rr * NR, P.rl * NR, ts, red);
   F* out = a.coef + (int64_t)p * a.MM * NR;
   F* c = cs;
   if constexpr (REFINE) {
-    gptr<F> M = (gptr<F>)(lv.M + P.moff);
-    ap_matvec<F, NR>(Mi, lv.mm, msize, ts, nullptr, F(1), cs, red2);   // c = M^{-1} t
-    ap_matvec<F, NR>(M, lv.mm, msize, cs, ts, F(-1), us, red2);        // r = t - M c
-    ap_matvec<F, NR>(Mi, lv.mm, msize, us, cs, F(1), ts, red2);        // c + M^{-1} r
+    ap_matvec<F, NR>(Mi, lv.mm, msize, msize, ts, nullptr, F(1), cs, red2);  // c = M^{-1} t
+    ap_matvec<F, NR>(M, lv.mm, msize, msize, cs, ts, F(-1), us, red2);       // r = t - M c
+    ap_matvec<F, NR>(Mi, lv.mm, msize, msize, us, cs, F(1), ts, red2);       // c + M^{-1} r
     c = ts;
   } else {
-    ap_matvec<F, NR>(Mi, lv.mm, msize, ts, nullptr, F(1), cs, red2);
+    ap_matvec<F, NR>(Mi, lv.mm, msize, msize, ts, nullptr, F(1), cs, red2);
   }
   for (int e = threadIdx.x; e < msize * NR; e += kApWG) st_dev(out + e, c[e]);
 }
 
+// dst[a * NR + q] = sum over the leaf's rows r of V(r, a) X(r, q), a < vc
+// (dst in LDS): 16 / NR columns of V per step, each wave's sums by shuffles,
+// the four waves' sums added in order.  red: kApWaves x KMP x NR.
+template <class F, int NR>
+__device__ void ap_vtx(gptr<F> V, int64_t ldv, const ApLeaf& lf, int vc, const F* X, int64_t ldx, int rpt, int KMP,
+                       F* red, F* dst) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  constexpr int KG = 16 / NR;
+  for (int kb = 0; kb < vc; kb += KG) {
+    F v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = F(0);
+    for (int t = 0; t < rpt; ++t) {
+      const int r = tid + t * kApWG;
+      const bool live = r < lf.len;
+      const int64_t row = lf.start + (live ? r : 0);
+      F x[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) x[q] = live ? X[row + (int64_t)q * ldx] : F(0);
+#pragma unroll
+      for (int kk = 0; kk < KG; ++kk) {
+        const int k = kb + kk;
+        const F vk = (live && k < vc) ? V[row + (int64_t)k * ldv] : F(0);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) v[kk * NR + q] = fma(vk, x[q], v[kk * NR + q]);
+      }
+    }
+    const F sum = ap_reduce16(v);  // lane & 15 = kk * NR + q
+    if (lane < 16) red[w * KMP * NR + kb * NR + lane] = sum;
+  }
+  __syncthreads();
+  for (int i = tid; i < vc * NR; i += kApWG) {
+    F s = F(0);
+#pragma unroll
+    for (int u = 0; u < kApWaves; ++u) s += red[u * KMP * NR + i];
+    dst[i] = s;
+  }
+  __syncthreads();
+}
+
+// Publishes a leaf's partials (src: LDS, vc * NR entries) for parent en.p
+// and counts the leaf in; the last arrival sums the parent's partials,
+// solves its capacitance system and raises the parent's flag.
 template <class F, int NR, bool REFINE>
-__global__ __launch_bounds__(kApWG) void k_apply_inverse(ApArgs<F> a) {
+__device__ void ap_publish(const ApArgs<F>& a, const ApLevel<F>& lv, const ApEntry& en, int leaf, const F* src,
+                           F* ts, F* cs, F* us, F* red, F* red2, int* sh) {
+  const int tid = threadIdx.x;
+  F* out = a.part + (int64_t)leaf * a.KM * NR;
+  for (int i = tid; i < en.vcols * NR; i += kApWG) st_dev(out + i, src[i]);
+  stores_complete();
+  __syncthreads();  // the workgroup's partials, then its arrival
+  if (tid == 0) {
+    const ApParent P = a.par[en.p];
+    const int old = __hip_atomic_fetch_add(a.cnt + en.p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *sh = (old == P.leaf1 - P.leaf0 - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  const bool leader = *sh != 0;
+  __syncthreads();
+  if (leader) {
+    ap_leader<F, NR, REFINE>(a, lv, en.p, ts, cs, us, red, red2);
+    stores_complete();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(a.flag + en.p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <class F, int NR, bool REFINE>
+__global__ __launch_bounds__(kApWG, 4) void k_apply_inverse(ApArgs<F> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   F* sm = reinterpret_cast<F*>(smem_raw);
   const int KMP = (a.KM + 15) / 16 * 16;
@@ -290,8 +364,9 @@ __global__ __launch_bounds__(kApWG) void k_apply_inverse(ApArgs<F> a) {
   F* us = cs + (int64_t)a.MM * NR;  // MM x NR
   F* red2 = us + (int64_t)a.MM * NR;  // waves x MM x NR
   __shared__ int sh;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x;
   const int rpt = (int)((a.lm + kApWG - 1) / kApWG);
+  const int L = a.L;
 
   // 1. leaf solves
   for (int leaf = blockIdx.x; leaf < a.nleaf; leaf += gridDim.x) {
@@ -356,66 +431,19 @@ __global__ __launch_bounds__(kApWG) void k_apply_inverse(ApArgs<F> a) {
   }
 
   // 2. corrections, deepest parent depth first
-  constexpr int KG = 16 / NR;
-  for (int j = a.L - 1; j >= 0; --j) {
+  for (int j = L - 1; j >= 0; --j) {
     const ApLevel<F> lv = a.lv[j];
-    gptr<F> V = (gptr<F>)lv.V;
     gptr<F> Q = (gptr<F>)lv.Q;
     // 2a. partials of the owned leaves; the last arrival at a parent solves it
     for (int leaf = blockIdx.x; leaf < a.nleaf; leaf += gridDim.x) {
-      const ApEntry en = a.ent[(int64_t)leaf * a.L + j];
+      const ApEntry en = a.ent[(int64_t)leaf * L + j];
       if (en.p < 0) continue;
-      const ApLeaf lf = a.leaves[leaf];
-      for (int kb = 0; kb < en.vcols; kb += KG) {
-        F v[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) v[i] = F(0);
-        for (int t = 0; t < rpt; ++t) {
-          const int r = tid + t * kApWG;
-          const bool live = r < lf.len;
-          const int64_t row = lf.start + (live ? r : 0);
-          F x[NR];
-#pragma unroll
-          for (int q = 0; q < NR; ++q) x[q] = live ? a.X[row + (int64_t)q * a.ldx] : F(0);
-#pragma unroll
-          for (int kk = 0; kk < KG; ++kk) {
-            const int k = kb + kk;
-            const F vk = (live && k < en.vcols) ? V[row + (int64_t)k * a.ldv] : F(0);
-#pragma unroll
-            for (int q = 0; q < NR; ++q) v[kk * NR + q] = fma(vk, x[q], v[kk * NR + q]);
-          }
-        }
-        const F sum = ap_reduce16(v);  // lane & 15 = kk * NR + q
-        if (lane < 16) red[w * KMP * NR + kb * NR + lane] = sum;
-      }
-      __syncthreads();
-      F* out = a.part + (int64_t)leaf * a.KM * NR;
-      for (int i = tid; i < en.vcols * NR; i += kApWG) {
-        F s = F(0);
-#pragma unroll
-        for (int u = 0; u < kApWaves; ++u) s += red[u * KMP * NR + i];
-        st_dev(out + i, s);
-      }
-      stores_complete();
-      __syncthreads();  // the workgroup's partials, then its arrival
-      if (tid == 0) {
-        const ApParent P = a.par[en.p];
-        const int old = __hip_atomic_fetch_add(a.cnt + en.p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sh = (old == P.leaf1 - P.leaf0 - 1) ? 1 : 0;
-      }
-      __syncthreads();
-      const bool leader = sh != 0;
-      __syncthreads();
-      if (leader) {
-        ap_leader<F, NR, REFINE>(a, lv, en.p, ts, cs, us, red, red2);
-        stores_complete();
-        __syncthreads();
-        if (tid == 0) __hip_atomic_store(a.flag + en.p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      ap_vtx<F, NR>((gptr<F>)lv.V, a.ldv, a.leaves[leaf], en.vcols, a.X, a.ldx, rpt, KMP, red, us);
+      ap_publish<F, NR, REFINE>(a, lv, en, leaf, us, ts, cs, us, red, red2, &sh);
     }
     // 2b. wait for the parents' coefficients, X -= Q c over the owned leaves
     for (int leaf = blockIdx.x; leaf < a.nleaf; leaf += gridDim.x) {
-      const ApEntry en = a.ent[(int64_t)leaf * a.L + j];
+      const ApEntry en = a.ent[(int64_t)leaf * L + j];
       if (en.p < 0) continue;
       if (!ap_wait(a.flag, en.p, a.err, a.spin_limit, &sh)) return;
       F* cp = a.coef + (int64_t)en.p * a.MM * NR + (int64_t)en.coff * NR;
