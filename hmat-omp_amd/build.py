@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["kernels.hip", "sketch.hip", "matvec.hip", "engine.hip", "hodlr.hip", "hss.hip", "capi.hip",
-           "noalloc.hip", "apply.hip", "ulv.hip"]
+           "noalloc.hip", "apply.hip", "ulv.hip", "hss_mv.hip"]
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -960,6 +960,7 @@ void HssEngine::build(const GenDesc& g, int64_t n, int leaf_size, int mode_, con
   pool_built = pool_used;
   set_ldv();
   HIP_CHECK(hipStreamSynchronize(stream));
+  mv_.ok = false;
   built = true;
 }
 
@@ -1003,6 +1004,15 @@ void HssEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int6
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "matvec before build");
   HMAT_REQUIRE(nrhs > 0 && ldx >= N && ldy >= N, HMAT_ERR_ARG, "bad matvec dimensions");
   ProfScope prof("hss_matvec", stream, matvec_bytes(nrhs));
+  if (nrhs == 1 && plan_matvec()) {
+    if (kern::hss_mv_launch(mv_, D.get(), Ul.get(), Vl.get(), pool.get(), N, dX, dY, stream)) {
+      prof.finish();
+      if (prof_enabled()) prof_add("hss_matvec_one_launch", 0.0, 1.0);
+      kern::hss_mv_finish(mv_, stream);
+      return;
+    }
+    mv_.ok = false;  // refused by the runtime: the multi-launch path
+  }
   leaf_apply(D.get(), dX, ldx, nrhs, dY, ldy);
   if (tree.max_depth == 0) {
     prof.finish();
@@ -1068,6 +1078,55 @@ void HssEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int6
   }
   prof.finish();
   HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+// One-launch matvec descriptors: the leaves, then the sweep's node tasks
+// in dependency order (up-sweep deepest depth first, down-sweep from the
+// root), as matvec() runs them depth by depth.
+bool HssEngine::plan_matvec() {
+  if (mv_.ok) return true;
+  if (tree.max_depth == 0 || tree.lmax > 1024) return false;
+  int rmax = 1;
+  for (const SNode& q : sn) rmax = std::max({rmax, q.rU, q.rV});
+  if (rmax > 64) return false;
+  std::vector<kern::HmvLeaf> lf;
+  for (int id : tree.leaves) {
+    const TNode& nd = tree.nodes[id];
+    lf.push_back(kern::HmvLeaf{nd.start, (int32_t)nd.len, id, sn[id].rU, sn[id].rV});
+  }
+  std::vector<kern::HmvTask> tk;
+  for (int d = tree.max_depth - 1; d >= 1; --d)
+    for (int id : tree.by_depth[d]) {
+      const TNode& nd = tree.nodes[id];
+      if (nd.leaf) continue;
+      tk.push_back(kern::HmvTask{0, id, nd.left, nd.right, sn[nd.left].rV, sn[nd.right].rV, sn[id].rV, 0, 0,
+                                 sn[id].ldW, 0, 0, sn[id].oW, 0, 0});
+    }
+  for (int d = 0; d < tree.max_depth; ++d)
+    for (int id : tree.by_depth[d]) {
+      const TNode& nd = tree.nodes[id];
+      if (nd.leaf) continue;
+      const int rp = d > 0 ? sn[id].rU : 0;
+      tk.push_back(kern::HmvTask{1, id, nd.left, nd.right, sn[nd.left].rU, sn[nd.right].rU, rp, sn[nd.left].rV,
+                                 sn[nd.right].rV, sn[id].ldB, sn[id].ldR, 0, sn[id].oBab, sn[id].oBba,
+                                 d > 0 ? sn[id].oR : 0});
+    }
+  mv_.nleaf = (int)lf.size();
+  mv_.ntasks = (int)tk.size();
+  mv_.nn = (int)tree.nodes.size();
+  mv_.ldv = rmax;
+  mv_.lm = tree.lmax;
+  mv_.leaves.upload(lf, stream);
+  mv_.tasks.upload(tk, stream);
+  mv_.vec.reserve((size_t)2 * mv_.nn * mv_.ldv * sizeof(double));
+  mv_.sync.reserve(((size_t)2 * mv_.nn + 2) * sizeof(int));
+  mv_.ok = true;
+  mv_.grid = kern::hss_mv_grid(mv_);
+  if (mv_.grid <= 0) {
+    mv_.ok = false;
+    return false;
+  }
+  return true;
 }
 
 // -------------------------------------------------------------- factor/solve
@@ -1590,6 +1649,7 @@ void HssEngine::load(const std::string& path) {
   put_dev(D, hd);
   HIP_CHECK(hipStreamSynchronize(stream));
   set_ldv();
+  mv_.ok = false;
   built = true;
 }
 
