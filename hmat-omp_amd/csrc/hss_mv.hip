@@ -36,7 +36,7 @@ constexpr int kMvWG = 256;
 constexpr int kMvWaves = kMvWG / 64;
 constexpr int kMvMaxRpt = 4;  // leaves of up to 1024 rows
 constexpr int kMvMaxRank = 64;
-constexpr int kMvPre = 24;    // matrix entries per thread loaded before a down task's wait
+constexpr int kMvPre = 32;  // entries of R per thread loaded before a down task's last wait
 std::atomic<int64_t> g_mv_spin{(int64_t)1 << 22};
 
 using gp = const __attribute__((address_space(1))) double*;
@@ -139,62 +139,63 @@ __device__ bool mv_up(const HmvArgs& a, const HmvTask& T, int* sh) {
   return true;
 }
 
-// Down task: rows i < ra + rb of [f_a; f_b]; thread (i, s) with i < 128:
-// s = 0 the coupling part (B_ab g_b or B_ba g_a), s = 1 the transfer part
-// R f_p.  The first kMvPre matrix entries of each thread are loaded before
-// the wait, the rest after it.  vs: 3 x 64 doubles, red: 256 doubles of LDS.
+// Down task: rows i < m = ra + rb of [f_a; f_b], thread (i, s) for the
+// k-slice s of S = min(4, 256 / m).  The coupling part (B_ab g_b or B_ba
+// g_a) is formed as soon as the children's g are flagged, which is long
+// before f_p; then the thread's slice of R is loaded into registers, and
+// only f_p itself is read after the last wait.  vs: 3 x 64 doubles, red:
+// 256 doubles of LDS.
 __device__ bool mv_down(const HmvArgs& a, const HmvTask& T, double* vs, double* red, int* sh) {
   const int tid = threadIdx.x;
-  const int i = tid & 127, s = tid >> 7;
   const int m = T.ra + T.rb;
-  gp M = nullptr;
-  int64_t ld = 1;
-  int nk = 0, voff = 0;
-  if (i < m) {
-    if (s == 0) {
-      if (i < T.ra) {
-        M = (gp)(a.pool + T.o1) + i;
-        nk = T.gb;
-        voff = 0;  // g_b
-      } else {
-        M = (gp)(a.pool + T.o2) + (i - T.ra);
-        nk = T.ga;
-        voff = 64;  // g_a
-      }
-      ld = T.ld1;
-    } else {
-      M = (gp)(a.pool + T.o3) + i;
-      nk = T.rp;
-      ld = T.ld2;
-      voff = 128;  // f_p
-    }
-  }
-  double pf[kMvPre];
-#pragma unroll
-  for (int u = 0; u < kMvPre; ++u) pf[u] = u < nk ? M[(int64_t)u * ld] : 0.0;
-  if (!mv_wait(a.gflag + T.a, a.gflag + T.b, T.rp > 0 ? a.fflag + T.p : nullptr, a.err, a.spin_limit, sh))
-    return false;
+  const int S = min(4, kMvWG / max(m, 1));
+  const int i = tid % max(m, 1), s = tid / max(m, 1);
+  const bool live = m > 0 && s < S;
+  if (!mv_wait(a.gflag + T.a, a.gflag + T.b, nullptr, a.err, a.spin_limit, sh)) return false;
   if (tid < T.gb) vs[tid] = ld_rel(a.G + (int64_t)T.b * a.ldv + tid);
   if (tid >= 64 && tid - 64 < T.ga) vs[tid] = ld_rel(a.G + (int64_t)T.a * a.ldv + (tid - 64));
-  if (tid >= 128 && tid - 128 < T.rp) vs[tid] = ld_rel(a.F + (int64_t)T.p * a.ldv + (tid - 128));
   __syncthreads();
   double acc = 0.0;
+  if (live) {
+    const bool top = i < T.ra;
+    gp M = top ? (gp)(a.pool + T.o1) + i : (gp)(a.pool + T.o2) + (i - T.ra);
+    const int nk = top ? T.gb : T.ga, voff = top ? 0 : 64;
+    const int kq = (nk + S - 1) / S, k0 = s * kq, k1 = min(nk, k0 + kq);
+    for (int k = k0; k < k1; k += 16) {
+      const int n = min(16, k1 - k);
+      double e[16];
 #pragma unroll
-  for (int u = 0; u < kMvPre; ++u)
-    if (u < nk) acc = fma(pf[u], vs[voff + u], acc);
-  for (int k = kMvPre; k < nk; k += 16) {
-    const int n = min(16, nk - k);
-    double e[16];
+      for (int u = 0; u < 16; ++u) e[u] = M[(int64_t)(k + min(u, n - 1)) * T.ld1];  // in bounds
 #pragma unroll
-    for (int u = 0; u < 16; ++u) e[u] = M[(int64_t)(k + min(u, n - 1)) * ld];  // in bounds
-#pragma unroll
-    for (int u = 0; u < 16; ++u)
-      if (u < n) acc = fma(e[u], vs[voff + k + u], acc);
+      for (int u = 0; u < 16; ++u)
+        if (u < n) acc = fma(e[u], vs[voff + k + u], acc);
+    }
   }
-  red[tid] = acc;
+  if (T.rp > 0) {
+    // R's slice (at most 32 entries: rp <= 64 and S >= 2) before f_p's wait
+    int k0 = 0, nr = 0;
+    gp R = nullptr;
+    if (live) {
+      const int kq = (T.rp + S - 1) / S;
+      k0 = s * kq;
+      nr = max(0, min(T.rp, k0 + kq) - k0);
+      R = (gp)(a.pool + T.o3) + i;
+    }
+    double pf[kMvPre];
+#pragma unroll
+    for (int u = 0; u < kMvPre; ++u) pf[u] = u < nr ? R[(int64_t)(k0 + u) * T.ld2] : 0.0;
+    if (!mv_wait(a.fflag + T.p, nullptr, nullptr, a.err, a.spin_limit, sh)) return false;
+    if (tid >= 128 && tid - 128 < T.rp) vs[tid] = ld_rel(a.F + (int64_t)T.p * a.ldv + (tid - 128));
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kMvPre; ++u)
+      if (u < nr) acc = fma(pf[u], vs[128 + k0 + u], acc);
+  }
+  red[tid] = live ? acc : 0.0;
   __syncthreads();
   if (tid < m) {
-    const double f = red[tid] + red[128 + tid];
+    double f = 0.0;
+    for (int u = 0; u < S; ++u) f += red[u * m + tid];
     if (tid < T.ra)
       st_rel(a.F + (int64_t)T.a * a.ldv + tid, f);
     else
@@ -228,12 +229,12 @@ __global__ __launch_bounds__(kMvWG, 4) void k_hss_matvec(HmvArgs a) {
       if (r >= lf.len) break;
       double acc = 0.0;
       int k = 0;
-      for (; k + 8 <= lf.len; k += 8) {
-        double d[8];
+      for (; k + 16 <= lf.len; k += 16) {
+        double d[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) d[u] = D[r + (int64_t)(k + u) * a.lm];
+        for (int u = 0; u < 16; ++u) d[u] = D[r + (int64_t)(k + u) * a.lm];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc = fma(d[u], xs[k + u], acc);
+        for (int u = 0; u < 16; ++u) acc = fma(d[u], xs[k + u], acc);
       }
       for (; k < lf.len; ++k) acc = fma(D[r + (int64_t)k * a.lm], xs[k], acc);
       a.Y[lf.start + r] = acc;
@@ -281,18 +282,29 @@ __global__ __launch_bounds__(kMvWG, 4) void k_hss_matvec(HmvArgs a) {
     __syncthreads();
   }
 
-  // 3. y += U f over the owned leaves
+  // 3. y += U f over the owned leaves (the first 16 columns of U of the
+  // first row block loaded before the wait)
   for (int li = blockIdx.x; li < a.nleaf; li += gridDim.x) {
     const HmvLeaf lf = a.leaves[li];
+    gp U = (gp)a.U + lf.start;
+    const int np = (tid < lf.len) ? min(16, lf.rU) : 0;
+    double up[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) up[u] = u < np ? U[tid + (int64_t)u * a.N] : 0.0;
     if (!mv_wait(a.fflag + lf.id, nullptr, nullptr, a.err, a.spin_limit, &sh)) return;
     if (tid < lf.rU) vs[tid] = ld_rel(a.F + (int64_t)lf.id * a.ldv + tid);
     __syncthreads();
-    gp U = (gp)a.U + lf.start;
     for (int t = 0; t < rpt; ++t) {
       const int r = tid + t * kMvWG;
       if (r >= lf.len) break;
       double acc = 0.0;
       int k = 0;
+      if (t == 0) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          if (u < np) acc = fma(up[u], vs[u], acc);
+        k = np;
+      }
       for (; k + 8 <= lf.rU; k += 8) {
         double e[8];
 #pragma unroll
