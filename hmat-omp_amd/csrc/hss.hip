@@ -1004,8 +1004,8 @@ void HssEngine::matvec(const double* dX, int64_t ldx, int nrhs, double* dY, int6
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "matvec before build");
   HMAT_REQUIRE(nrhs > 0 && ldx >= N && ldy >= N, HMAT_ERR_ARG, "bad matvec dimensions");
   ProfScope prof("hss_matvec", stream, matvec_bytes(nrhs));
-  if (nrhs == 1 && plan_matvec()) {
-    if (kern::hss_mv_launch(mv_, D.get(), Ul.get(), Vl.get(), pool.get(), N, dX, dY, stream)) {
+  if (plan_matvec()) {
+    if (kern::hss_mv_launch(mv_, D.get(), Ul.get(), Vl.get(), pool.get(), N, dX, ldx, nrhs, dY, ldy, stream)) {
       prof.finish();
       if (prof_enabled()) prof_add("hss_matvec_one_launch", 0.0, 1.0);
       kern::hss_mv_finish(mv_, stream);
@@ -1118,11 +1118,11 @@ bool HssEngine::plan_matvec() {
   mv_.lm = tree.lmax;
   mv_.leaves.upload(lf, stream);
   mv_.tasks.upload(tk, stream);
-  mv_.vec.reserve((size_t)2 * mv_.nn * mv_.ldv * sizeof(double));
+  mv_.vec.reserve((size_t)2 * mv_.nn * mv_.ldv * 16 * sizeof(double));
   mv_.sync.reserve(((size_t)2 * mv_.nn + 2) * sizeof(int));
   mv_.ok = true;
-  mv_.grid = kern::hss_mv_grid(mv_);
-  if (mv_.grid <= 0) {
+  for (int l = 0; l < 5; ++l) mv_.grid[l] = kern::hss_mv_grid(mv_, l);
+  if (mv_.grid[0] <= 0) {
     mv_.ok = false;
     return false;
   }

@@ -107,9 +107,9 @@ class HssEngine {
   // does: backward stable like an LU solve while every step stays a GEMM
   void leaf_solve(const double* B, int64_t ldb, int ncols, double* Y, int64_t ldy, double* R);
   void apply_inverse(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
-  // descriptors of the one-launch matvec (one column); false when the
-  // operator's shape is beyond it (ranks above 64, leaves above 1024 rows,
-  // a single leaf) or the device cannot hold its grid
+  // descriptors of the one-launch matvec; false when the operator's shape
+  // is beyond it (ranks above 64, leaves above 1024 rows, a single leaf) or
+  // the device cannot hold its grid
   bool plan_matvec();
   void ulv_factor();
   void ulv_apply(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);

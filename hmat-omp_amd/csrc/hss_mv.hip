@@ -1,4 +1,5 @@
-// One-launch HSS matvec, y = A_HSS x for one right-hand side.
+// One-launch HSS matvec, Y = A_HSS X for up to 16 right-hand sides per
+// launch (column groups of 16, 8, 4, 2, 1).
 //
 // Reference: HSSMatrixOMP::multiply (include/hss_routines_omp.h:154-230):
 // up-sweep g = V^T x at the leaves and g_p = W_p^T [g_a; g_b] above,
@@ -7,13 +8,14 @@
 // depth and sweep direction (19 at N = 2^18, leaf 256) plus three leaf
 // passes, each launch a few microseconds of latency-bound work.  Here one
 // persistent cooperative launch does all of it:
-//   1. every workgroup, for its leaves: y = D x and g = V^T x in one pass
-//      over the leaf's rows (x staged in LDS), g published with a flag;
+//   1. every workgroup, for its leaves: Y = D X and g = V^T X in one pass
+//      over the leaf's rows (X staged in LDS), g published with a flag;
 //   2. the workgroups take the sweep's node tasks from a queue in
 //      dependency order (up-sweep deepest first, then down-sweep from the
 //      root), each task loading its transfer or coupling matrices into
 //      registers before it waits for the flags of its inputs;
-//   3. every workgroup, for its leaves: y += U f once f is flagged.
+//   3. every workgroup, for its leaves: Y += U F once F is flagged.
+// A task's matrices stay in registers for all the launch's columns.
 // A task only waits for tasks earlier in the queue or for phase 1, and
 // every workgroup is resident (cooperative launch), so the queue drains.
 // Every wait is bounded: a wait past its spin limit raises `err` and the
@@ -105,11 +107,17 @@ __device__ __forceinline__ double mv_reduce16(double (&v)[16]) {
   return s;
 }
 
+// Per-node vectors of a launch over NR columns: node id's g (or f) is an
+// (ldv x NR) block at G + id * ldv * NR, entry (c, q) at c * NR + q.
+
 // Up task: g_p = W^T [g_a; g_b].  Wave w takes outputs w*16 .. w*16+15,
-// lane l the reduction indices l and l + 64.
-__device__ bool mv_up(const HmvArgs& a, const HmvTask& T, int* sh) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+// lane l the reduction indices l and l + 64; W stays in registers for the
+// NR columns, whose [g_a; g_b] are staged in LDS (vs: 128 x NR).
+template <int NR>
+__device__ bool mv_up(const HmvArgs& a, const HmvTask& T, double* vs, int* sh) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int nk = T.ra + T.rb;
+  const int64_t vst = (int64_t)a.ldv * NR;
   gp W = (gp)(a.pool + T.o1);
   double wv[2][16];
 #pragma unroll
@@ -122,19 +130,21 @@ __device__ bool mv_up(const HmvArgs& a, const HmvTask& T, int* sh) {
     }
   }
   if (!mv_wait(a.gflag + T.a, a.gflag + T.b, nullptr, a.err, a.spin_limit, sh)) return false;
-  double gk[2];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int k = lane + 64 * h;
-    gk[h] = k < T.ra ? ld_rel(a.G + (int64_t)T.a * a.ldv + k)
-                     : (k < nk ? ld_rel(a.G + (int64_t)T.b * a.ldv + (k - T.ra)) : 0.0);
+  for (int e = tid; e < nk * NR; e += kMvWG) {
+    const int k = e / NR, q = e % NR;
+    vs[e] = k < T.ra ? ld_rel(a.G + T.a * vst + k * NR + q) : ld_rel(a.G + T.b * vst + (k - T.ra) * NR + q);
   }
-  double v[16];
-#pragma unroll
-  for (int u = 0; u < 16; ++u) v[u] = fma(wv[1][u], gk[1], wv[0][u] * gk[0]);
-  const double s = mv_reduce16(v);
+  __syncthreads();
   const int c = w * 16 + (lane & 15);
-  if (lane < 16 && c < T.rp) st_rel(a.G + (int64_t)T.p * a.ldv + c, s);
+  for (int q = 0; q < NR; ++q) {
+    const double g0 = lane < nk ? vs[lane * NR + q] : 0.0;
+    const double g1 = lane + 64 < nk ? vs[(lane + 64) * NR + q] : 0.0;
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = fma(wv[1][u], g1, wv[0][u] * g0);
+    const double s = mv_reduce16(v);
+    if (lane < 16 && c < T.rp) st_rel(a.G + T.p * vst + c * NR + q, s);
+  }
   mv_flag(a.gflag + T.p);
   return true;
 }
@@ -143,19 +153,26 @@ __device__ bool mv_up(const HmvArgs& a, const HmvTask& T, int* sh) {
 // k-slice s of S = min(4, 256 / m).  The coupling part (B_ab g_b or B_ba
 // g_a) is formed as soon as the children's g are flagged, which is long
 // before f_p; then the thread's slice of R is loaded into registers, and
-// only f_p itself is read after the last wait.  vs: 3 x 64 doubles, red:
-// 256 doubles of LDS.
+// only f_p itself is read after the last wait.  vs: 3 x 64 x NR doubles
+// (g_b, g_a, f_p), red: 256 x NR doubles of LDS.
+template <int NR>
 __device__ bool mv_down(const HmvArgs& a, const HmvTask& T, double* vs, double* red, int* sh) {
   const int tid = threadIdx.x;
   const int m = T.ra + T.rb;
   const int S = min(4, kMvWG / max(m, 1));
   const int i = tid % max(m, 1), s = tid / max(m, 1);
   const bool live = m > 0 && s < S;
+  const int64_t vst = (int64_t)a.ldv * NR;
   if (!mv_wait(a.gflag + T.a, a.gflag + T.b, nullptr, a.err, a.spin_limit, sh)) return false;
-  if (tid < T.gb) vs[tid] = ld_rel(a.G + (int64_t)T.b * a.ldv + tid);
-  if (tid >= 64 && tid - 64 < T.ga) vs[tid] = ld_rel(a.G + (int64_t)T.a * a.ldv + (tid - 64));
+  for (int e = tid; e < 64 * NR; e += kMvWG) {
+    const int k = e / NR, q = e % NR;
+    if (k < T.gb) vs[e] = ld_rel(a.G + T.b * vst + k * NR + q);
+    if (k < T.ga) vs[64 * NR + e] = ld_rel(a.G + T.a * vst + k * NR + q);
+  }
   __syncthreads();
-  double acc = 0.0;
+  double acc[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) acc[q] = 0.0;
   if (live) {
     const bool top = i < T.ra;
     gp M = top ? (gp)(a.pool + T.o1) + i : (gp)(a.pool + T.o2) + (i - T.ra);
@@ -168,7 +185,9 @@ __device__ bool mv_down(const HmvArgs& a, const HmvTask& T, double* vs, double* 
       for (int u = 0; u < 16; ++u) e[u] = M[(int64_t)(k + min(u, n - 1)) * T.ld1];  // in bounds
 #pragma unroll
       for (int u = 0; u < 16; ++u)
-        if (u < n) acc = fma(e[u], vs[voff + k + u], acc);
+        if (u < n)
+#pragma unroll
+          for (int q = 0; q < NR; ++q) acc[q] = fma(e[u], vs[(voff + k + u) * NR + q], acc[q]);
     }
   }
   if (T.rp > 0) {
@@ -185,21 +204,26 @@ __device__ bool mv_down(const HmvArgs& a, const HmvTask& T, double* vs, double* 
 #pragma unroll
     for (int u = 0; u < kMvPre; ++u) pf[u] = u < nr ? R[(int64_t)(k0 + u) * T.ld2] : 0.0;
     if (!mv_wait(a.fflag + T.p, nullptr, nullptr, a.err, a.spin_limit, sh)) return false;
-    if (tid >= 128 && tid - 128 < T.rp) vs[tid] = ld_rel(a.F + (int64_t)T.p * a.ldv + (tid - 128));
+    for (int e = tid; e < T.rp * NR; e += kMvWG) vs[128 * NR + e] = ld_rel(a.F + T.p * vst + e);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kMvPre; ++u)
-      if (u < nr) acc = fma(pf[u], vs[128 + k0 + u], acc);
+      if (u < nr)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = fma(pf[u], vs[(128 + k0 + u) * NR + q], acc[q]);
   }
-  red[tid] = live ? acc : 0.0;
+  if (live)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) red[(s * m + i) * NR + q] = acc[q];
   __syncthreads();
-  if (tid < m) {
+  for (int e = tid; e < m * NR; e += kMvWG) {
+    const int r = e / NR, q = e % NR;
     double f = 0.0;
-    for (int u = 0; u < S; ++u) f += red[u * m + tid];
-    if (tid < T.ra)
-      st_rel(a.F + (int64_t)T.a * a.ldv + tid, f);
+    for (int u = 0; u < S; ++u) f += red[(u * m + r) * NR + q];
+    if (r < T.ra)
+      st_rel(a.F + T.a * vst + r * NR + q, f);
     else
-      st_rel(a.F + (int64_t)T.b * a.ldv + (tid - T.ra), f);
+      st_rel(a.F + T.b * vst + (r - T.ra) * NR + q, f);
   }
   stores_done();
   __syncthreads();
@@ -210,60 +234,82 @@ __device__ bool mv_down(const HmvArgs& a, const HmvTask& T, double* vs, double* 
   return true;
 }
 
-__global__ __launch_bounds__(kMvWG, 4) void k_hss_matvec(HmvArgs a) {
-  __shared__ double xs[kMvMaxRpt * kMvWG];
-  __shared__ double red[kMvWG];
-  __shared__ double vs[3 * kMvMaxRank];
+// Dynamic LDS of the launch over NR columns (doubles): phase 1 stages the
+// leaf's X (lm x NR) and the waves' partial sums of V^T X (4 x 64 x NR);
+// the tasks use 3 x 64 x NR vector entries and 256 x NR partial sums.
+inline int64_t mv_lds_doubles(int64_t lm, int nr) { return (int64_t)nr * std::max<int64_t>(lm + 256, 448); }
+
+template <int NR>
+__global__ __launch_bounds__(kMvWG, NR == 1 ? 4 : 2) void k_hss_matvec(HmvArgs a) {
+  extern __shared__ __align__(16) double sm[];
   __shared__ int sh;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int rpt = (int)((a.lm + kMvWG - 1) / kMvWG);
+  const int64_t vst = (int64_t)a.ldv * NR;
+  double* xs = sm;                       // phase 1: lm x NR
+  double* red = sm + a.lm * NR;          // phase 1: 4 x 64 x NR
+  double* vs = sm;                       // tasks, phase 3: 3 x 64 x NR
+  double* red2 = sm + 3 * 64 * NR;       // tasks: 256 x NR
+  constexpr int KG = 16 / NR;            // columns of V per step of V^T X
 
-  // 1. y = D x, g = V^T x over the owned leaves
+  // 1. Y = D X, g = V^T X over the owned leaves
   for (int li = blockIdx.x; li < a.nleaf; li += gridDim.x) {
     const HmvLeaf lf = a.leaves[li];
-    for (int e = tid; e < lf.len; e += kMvWG) xs[e] = a.X[lf.start + e];
+    for (int e = tid; e < lf.len * NR; e += kMvWG) {
+      const int r = e / NR, q = e % NR;
+      xs[e] = a.X[lf.start + r + (int64_t)q * a.ldx];
+    }
     __syncthreads();
     gp D = (gp)(a.D + (int64_t)li * a.lm * a.lm);
     for (int t = 0; t < rpt; ++t) {
       const int r = tid + t * kMvWG;
       if (r >= lf.len) break;
-      double acc = 0.0;
+      double acc[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) acc[q] = 0.0;
       int k = 0;
       for (; k + 16 <= lf.len; k += 16) {
         double d[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) d[u] = D[r + (int64_t)(k + u) * a.lm];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) acc = fma(d[u], xs[k + u], acc);
+        for (int u = 0; u < 16; ++u)
+#pragma unroll
+          for (int q = 0; q < NR; ++q) acc[q] = fma(d[u], xs[(k + u) * NR + q], acc[q]);
       }
-      for (; k < lf.len; ++k) acc = fma(D[r + (int64_t)k * a.lm], xs[k], acc);
-      a.Y[lf.start + r] = acc;
+      for (; k < lf.len; ++k) {
+        const double dk = D[r + (int64_t)k * a.lm];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = fma(dk, xs[k * NR + q], acc[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < NR; ++q) a.Y[lf.start + r + (int64_t)q * a.ldy] = acc[q];
     }
     gp V = (gp)a.V + lf.start;
-    for (int kb = 0; kb < lf.rV; kb += 16) {
+    for (int kb = 0; kb < lf.rV; kb += KG) {
       double v[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) v[u] = 0.0;
       for (int t = 0; t < rpt; ++t) {
         const int r = tid + t * kMvWG;
         const bool live = r < lf.len;
-        const double xr = live ? xs[r] : 0.0;
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int c = kb + u;
+        for (int kk = 0; kk < KG; ++kk) {
+          const int c = kb + kk;
           const double vc = (live && c < lf.rV) ? V[(live ? r : 0) + (int64_t)c * a.N] : 0.0;
-          v[u] = fma(vc, xr, v[u]);
+#pragma unroll
+          for (int q = 0; q < NR; ++q) v[kk * NR + q] = fma(vc, live ? xs[r * NR + q] : 0.0, v[kk * NR + q]);
         }
       }
-      const double sum = mv_reduce16(v);
-      if (lane < 16) red[w * kMvMaxRank + kb + lane] = sum;
+      const double sum = mv_reduce16(v);  // lane & 15 = kk * NR + q
+      if (lane < 16) red[w * kMvMaxRank * NR + kb * NR + lane] = sum;
     }
     __syncthreads();
-    for (int c = tid; c < lf.rV; c += kMvWG) {
+    for (int e = tid; e < lf.rV * NR; e += kMvWG) {
       double s = 0.0;
 #pragma unroll
-      for (int u = 0; u < kMvWaves; ++u) s += red[u * kMvMaxRank + c];
-      st_rel(a.G + (int64_t)lf.id * a.ldv + c, s);
+      for (int u = 0; u < kMvWaves; ++u) s += red[u * kMvMaxRank * NR + e];
+      st_rel(a.G + lf.id * vst + e, s);
     }
     mv_flag(a.gflag + lf.id);
     __syncthreads();  // xs and red are reused by the next leaf
@@ -277,12 +323,12 @@ __global__ __launch_bounds__(kMvWG, 4) void k_hss_matvec(HmvArgs a) {
     __syncthreads();
     if (t >= a.ntasks) break;
     const HmvTask T = a.tasks[t];
-    const bool ok = T.kind == 0 ? mv_up(a, T, &sh) : mv_down(a, T, vs, red, &sh);
+    const bool ok = T.kind == 0 ? mv_up<NR>(a, T, vs, &sh) : mv_down<NR>(a, T, vs, red2, &sh);
     if (!ok) return;
     __syncthreads();
   }
 
-  // 3. y += U f over the owned leaves (the first 16 columns of U of the
+  // 3. Y += U F over the owned leaves (the first 16 columns of U of the
   // first row block loaded before the wait)
   for (int li = blockIdx.x; li < a.nleaf; li += gridDim.x) {
     const HmvLeaf lf = a.leaves[li];
@@ -292,17 +338,21 @@ __global__ __launch_bounds__(kMvWG, 4) void k_hss_matvec(HmvArgs a) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) up[u] = u < np ? U[tid + (int64_t)u * a.N] : 0.0;
     if (!mv_wait(a.fflag + lf.id, nullptr, nullptr, a.err, a.spin_limit, &sh)) return;
-    if (tid < lf.rU) vs[tid] = ld_rel(a.F + (int64_t)lf.id * a.ldv + tid);
+    for (int e = tid; e < lf.rU * NR; e += kMvWG) vs[e] = ld_rel(a.F + lf.id * vst + e);
     __syncthreads();
     for (int t = 0; t < rpt; ++t) {
       const int r = tid + t * kMvWG;
       if (r >= lf.len) break;
-      double acc = 0.0;
+      double acc[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) acc[q] = 0.0;
       int k = 0;
       if (t == 0) {
 #pragma unroll
         for (int u = 0; u < 16; ++u)
-          if (u < np) acc = fma(up[u], vs[u], acc);
+          if (u < np)
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q] = fma(up[u], vs[u * NR + q], acc[q]);
         k = np;
       }
       for (; k + 8 <= lf.rU; k += 8) {
@@ -310,10 +360,17 @@ __global__ __launch_bounds__(kMvWG, 4) void k_hss_matvec(HmvArgs a) {
 #pragma unroll
         for (int u = 0; u < 8; ++u) e[u] = U[r + (int64_t)(k + u) * a.N];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc = fma(e[u], vs[k + u], acc);
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int q = 0; q < NR; ++q) acc[q] = fma(e[u], vs[(k + u) * NR + q], acc[q]);
       }
-      for (; k < lf.rU; ++k) acc = fma(U[r + (int64_t)k * a.N], vs[k], acc);
-      a.Y[lf.start + r] += acc;
+      for (; k < lf.rU; ++k) {
+        const double ek = U[r + (int64_t)k * a.N];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = fma(ek, vs[k * NR + q], acc[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < NR; ++q) a.Y[lf.start + r + (int64_t)q * a.ldy] += acc[q];
     }
     __syncthreads();
   }
@@ -330,58 +387,95 @@ HmvPlan::~HmvPlan() {
   if (err_host) (void)hipHostFree(err_host);
 }
 
-int hss_mv_grid(const HmvPlan& p) {
-  if (!p.ok || p.lm > (int64_t)kMvWG * kMvMaxRpt || p.ldv > kMvMaxRank) return 0;
+template <int NR>
+static int mv_grid_for(const HmvPlan& p) {
+  const size_t lds = (size_t)mv_lds_doubles(p.lm, NR) * sizeof(double);
+  if (lds > 60 * 1024) return 0;
   int dev = 0, ncu = 0, coop = 0, nb = 0;
   HIP_CHECK(hipGetDevice(&dev));
   HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   HIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
   if (!coop) return 0;
-  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_hss_matvec, kMvWG, 0));
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_hss_matvec<NR>, kMvWG, lds));
   return std::min(p.nleaf, nb * ncu);
 }
 
+int hss_mv_grid(const HmvPlan& p, int l) {
+  if (!p.ok || p.lm > (int64_t)kMvWG * kMvMaxRpt || p.ldv > kMvMaxRank) return 0;
+  switch (l) {
+    case 0: return mv_grid_for<1>(p);
+    case 1: return mv_grid_for<2>(p);
+    case 2: return mv_grid_for<4>(p);
+    case 3: return mv_grid_for<8>(p);
+    default: return mv_grid_for<16>(p);
+  }
+}
+
+template <int NR>
+static hipError_t mv_launch_nr(HmvArgs a, int grid, hipStream_t s) {
+  void* args[] = {&a};
+  const size_t lds = (size_t)mv_lds_doubles(a.lm, NR) * sizeof(double);
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_hss_matvec<NR>), dim3(grid), dim3(kMvWG), args,
+                                    (unsigned)lds, s);
+}
+
 bool hss_mv_launch(HmvPlan& p, const double* D, const double* U, const double* V, const double* pool, int64_t N,
-                   const double* X, double* Y, hipStream_t s) {
-  const int grid = p.grid;
-  if (!p.ok || grid <= 0) return false;
+                   const double* X, int64_t ldx, int ncols, double* Y, int64_t ldy, hipStream_t s) {
+  if (!p.ok) return false;
   if (!p.err_host) {
     HIP_CHECK(hipHostMalloc((void**)&p.err_host, sizeof(int), hipHostMallocDefault));
     *p.err_host = 0;
   }
-  HmvArgs a;
-  a.leaves = p.leaves.get();
-  a.tasks = p.tasks.get();
-  a.D = D;
-  a.U = U;
-  a.V = V;
-  a.pool = pool;
-  a.X = X;
-  a.Y = Y;
-  a.G = p.vec.as<double>();
-  a.F = a.G + (int64_t)p.nn * p.ldv;
   int* sy = p.sync.as<int>();
-  a.gflag = sy;
-  a.fflag = sy + p.nn;
-  a.counter = sy + 2 * p.nn;
-  a.err = sy + 2 * p.nn + 1;
-  a.N = N;
-  a.lm = p.lm;
-  a.spin_limit = g_mv_spin.load();
-  a.nleaf = p.nleaf;
-  a.ntasks = p.ntasks;
-  a.ldv = p.ldv;
-  a.pad_ = 0;
-  // flags, counter and err start at zero (a kernel, not a memset)
-  const int nz = 2 * p.nn + 2;
-  k_mv_zero<<<(unsigned)std::max(1, std::min(64, (nz + 255) / 256)), 256, 0, s>>>(sy, nz);
-  HIP_CHECK(hipGetLastError());
-  void* args[] = {&a};
-  const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_hss_matvec), dim3(grid),
-                                                  dim3(kMvWG), args, 0, s);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
+  int done = 0;
+  while (done < ncols) {
+    // the widest launch that fits the remaining columns and the device
+    int l = 4;
+    while (l > 0 && ((1 << l) > ncols - done || p.grid[l] <= 0)) --l;
+    const int nr = 1 << l, grid = p.grid[l];
+    if (grid <= 0) return false;
+    HmvArgs a;
+    a.leaves = p.leaves.get();
+    a.tasks = p.tasks.get();
+    a.D = D;
+    a.U = U;
+    a.V = V;
+    a.pool = pool;
+    a.X = X + (int64_t)done * ldx;
+    a.Y = Y + (int64_t)done * ldy;
+    a.G = p.vec.as<double>();
+    a.F = a.G + (int64_t)p.nn * p.ldv * nr;
+    a.gflag = sy;
+    a.fflag = sy + p.nn;
+    a.counter = sy + 2 * p.nn;
+    a.err = sy + 2 * p.nn + 1;
+    a.N = N;
+    a.lm = p.lm;
+    a.spin_limit = g_mv_spin.load();
+    a.ldx = ldx;
+    a.ldy = ldy;
+    a.nleaf = p.nleaf;
+    a.ntasks = p.ntasks;
+    a.ldv = p.ldv;
+    a.pad_ = 0;
+    // flags and counter start at zero (a kernel, not a memset); err is
+    // cleared once per call, so a failed group is reported after the call
+    const int nz = done == 0 ? 2 * p.nn + 2 : 2 * p.nn + 1;
+    k_mv_zero<<<(unsigned)std::max(1, std::min(64, (nz + 255) / 256)), 256, 0, s>>>(sy, nz);
+    HIP_CHECK(hipGetLastError());
+    hipError_t e;
+    switch (l) {
+      case 0: e = mv_launch_nr<1>(a, grid, s); break;
+      case 1: e = mv_launch_nr<2>(a, grid, s); break;
+      case 2: e = mv_launch_nr<4>(a, grid, s); break;
+      case 3: e = mv_launch_nr<8>(a, grid, s); break;
+      default: e = mv_launch_nr<16>(a, grid, s); break;
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    done += nr;
   }
   return true;
 }

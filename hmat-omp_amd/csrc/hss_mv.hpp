@@ -1,4 +1,5 @@
-// One-launch HSS matvec for one right-hand side (persistent, cooperative).
+// One-launch HSS matvec for up to 16 right-hand sides (persistent,
+// cooperative).
 #pragma once
 #include <cstdint>
 
@@ -34,13 +35,13 @@ struct HmvArgs {
   const double* pool;
   const double* X;
   double* Y;
-  double* G;  // per node g (up-sweep), nn x ldv, uncached
-  double* F;  // per node f (down-sweep), nn x ldv, uncached
+  double* G;  // per node g (up-sweep), nn x ldv x NR, uncached
+  double* F;  // per node f (down-sweep), nn x ldv x NR, uncached
   int* gflag;
   int* fflag;
   int* counter;  // next task
   int* err;      // a wait exceeded its spin limit
-  int64_t N, lm, spin_limit;
+  int64_t N, lm, spin_limit, ldx, ldy;
   int nleaf, ntasks, ldv, pad_;
 };
 
@@ -49,11 +50,11 @@ struct HmvArgs {
 struct HmvPlan {
   DevBuf<HmvLeaf> leaves;
   DevBuf<HmvTask> tasks;
-  UcBuf vec;   // G then F, 2 x nn x ldv doubles
+  UcBuf vec;   // G then F, 2 x nn x ldv x 16 doubles
   UcBuf sync;  // gflag (nn), fflag (nn), counter, err
   int* err_host = nullptr;  // pinned copy of err after each call
   int nleaf = 0, ntasks = 0, nn = 0, ldv = 0;
-  int grid = 0;  // hss_mv_grid, kept by the plan
+  int grid[5] = {0, 0, 0, 0, 0};  // hss_mv_grid for 1, 2, 4, 8, 16 columns
   int64_t lm = 0;
   bool ok = false;
   HmvPlan() = default;
@@ -62,12 +63,14 @@ struct HmvPlan {
   ~HmvPlan();
 };
 
-// Largest grid the device holds at once for this plan (0: not available).
-int hss_mv_grid(const HmvPlan& p);
-// Y = A_HSS X for one column (X, Y: N entries, device), enqueued on s.
-// Returns false when the launch was refused (nothing ran).
+// Largest grid the device holds at once for this plan and 2^l columns per
+// launch (0: not available).
+int hss_mv_grid(const HmvPlan& p, int l);
+// Y = A_HSS X for ncols columns (X, Y: device, column-major), enqueued on
+// s, up to 16 columns per launch.  Returns false when a launch was refused
+// (the columns before it are done).
 bool hss_mv_launch(HmvPlan& p, const double* D, const double* U, const double* V, const double* pool, int64_t N,
-                   const double* X, double* Y, hipStream_t s);
+                   const double* X, int64_t ldx, int ncols, double* Y, int64_t ldy, hipStream_t s);
 // Waits for the launch and throws when one of its waits exceeded its spin
 // limit (the error flag is cleared by the next launch).
 void hss_mv_finish(HmvPlan& p, hipStream_t s);

@@ -1,13 +1,13 @@
 """The one-launch HSS matvec against the dense reconstruction of the same
-operator and against the multi-launch sweeps.
+operator (hss_routines_omp.h:154-230).
 
-One column runs as one persistent launch (leaf products and g = V^T x,
-both sweeps as a queue of node tasks, then y += U f); several columns take
-the per-depth launches.  Both are the same algebra
-(hss_routines_omp.h:154-230), so each column of a multi-column product
-equals the one-column product of that column to rounding, and both equal
-the dense reconstruction.  The profiler's one-launch counter shows that
-the single-column calls took the persistent kernel.
+Up to 16 columns run as one persistent launch (leaf products and g = V^T x,
+both sweeps as a queue of node tasks, then y += U f), wider calls as groups
+of 16, 8, 4, 2 and 1 columns.  Every column of every group width equals the
+dense reconstruction's product, and the one-column product of a column
+equals that column of a wider product.  The profiler's one-launch counter
+shows which calls took the persistent kernel; operators beyond its limits
+(ranks above 64) keep the per-depth launches.
 """
 import numpy as np
 import pytest
@@ -55,19 +55,20 @@ def test_one_launch_matvec(case):
     h = hmat_amd.HSSMatrixOMP(opts=o, nested=mode)
     h.build_from_dense(a, leaf)
     G = h.to_dense()
-    X = np.random.default_rng(5).standard_normal((n, 3))
+    X = np.random.default_rng(5).standard_normal((n, 19))
     hmat_amd.profile_reset()
     hmat_amd.profile_enable(True)
     Y1 = np.column_stack([h.multiply(X[:, [j]]) for j in range(3)])
     hmat_amd.profile_enable(False)
     one = hmat_amd.profile_query("hss_matvec_one_launch")
-    Y3 = h.multiply(X)
-    e_dense, e_cols = rel(Y1, G @ X), rel(Y1, Y3)
-    print(f"\n{case}: root rank {h.root_rank()}, one-launch calls {one and one[0]}, "
-          f"vs dense {e_dense:.1e}, vs multi-launch {e_cols:.1e}", flush=True)
+    Y19 = h.multiply(X)  # groups of 16, 2 and 1 columns (8 and 4 where 16 does not fit)
+    e1, e19, ec = rel(Y1, G @ X[:, :3]), rel(Y19, G @ X), rel(Y1, Y19[:, :3])
+    print(f"\n{case}: root rank {h.root_rank()}, one-launch calls {one and one[0]}, one column vs dense {e1:.1e}, "
+          f"19 columns vs dense {e19:.1e}, one vs grouped {ec:.1e}", flush=True)
     assert one is not None and one[0] == 3
-    assert e_dense < 1e-13
-    assert e_cols < 1e-14
+    assert e1 < 1e-13
+    assert e19 < 1e-13
+    assert ec < 1e-14
 
 
 def test_reference_ranks_take_the_sweeps():
