@@ -366,7 +366,10 @@ hmat_status hmat_hodlr_set_fused_solve(hmat_hodlr h, int32_t mode) {
   });
 }
 
-void hmat_set_fused_spin_limit(int64_t n) { kern::apply_fused_set_spin_limit(n); }
+void hmat_set_fused_spin_limit(int64_t n) {
+  kern::apply_fused_set_spin_limit(n);
+  kern::hss_mv_set_spin_limit(n);
+}
 
 hmat_status hmat_hodlr_set_group_bytes(hmat_hodlr h, double bytes) {
   return guard([&] {

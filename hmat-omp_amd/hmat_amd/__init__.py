@@ -213,7 +213,8 @@ def autotune_entries():
 
 
 def set_fused_spin_limit(n):
-    """Spin limit of the persistent apply-inverse's waits (0: default)."""
+    """Spin limit of the persistent kernels' waits: the HODLR apply-inverse
+    and the HSS matvec (0: default)."""
     load().hmat_set_fused_spin_limit(int(n))
 
 

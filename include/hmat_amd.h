@@ -306,9 +306,10 @@ hmat_status hmat_hodlr_set_solve_graph(hmat_hodlr h, int32_t on);
  * its grid (tests).  The two agree to rounding (different summation
  * order).  No reference counterpart. */
 hmat_status hmat_hodlr_set_fused_solve(hmat_hodlr h, int32_t mode);
-/* Diagnostic: spin limit of every wait of the persistent apply-inverse
- * (process-wide; 0 restores the default, 2^22).  A wait that exceeds it
- * ends its workgroup and the solve returns HMAT_ERR_DEVICE. */
+/* Diagnostic: spin limit of every wait of the persistent kernels, the
+ * HODLR apply-inverse and the HSS matvec (process-wide; 0 restores the
+ * default, 2^22).  A wait that exceeds it ends its workgroup and the solve
+ * or matvec returns HMAT_ERR_DEVICE. */
 void hmat_set_fused_spin_limit(int64_t n);
 /* Scratch budget in bytes of one group of tree depths built together (the
  * build stacks a group's sketch bases: three N x k arrays per depth; every

@@ -88,3 +88,23 @@ def test_reference_ranks_take_the_sweeps():
     hmat_amd.profile_enable(False)
     assert hmat_amd.profile_query("hss_matvec_one_launch") is None
     assert rel(y, G @ x) < 1e-13
+
+
+def test_wait_limit_reports_error():
+    """A spin limit of one poll, set from the host, makes the persistent
+    kernel's waits give up: every workgroup exits, the call returns
+    HMAT_ERR_DEVICE, and the next call with the default limit gives the
+    same product as before."""
+    n = 16384
+    h = hmat_amd.HSSMatrixOMP(opts=opts(32, 10), nested=hmat_amd.HSS_SAMPLED)
+    h.build_from_dense(cauchy(n), 128)
+    x = np.random.default_rng(7).standard_normal((n, 1))
+    y0 = h.multiply(x)
+    hmat_amd.set_fused_spin_limit(1)
+    try:
+        with pytest.raises(hmat_amd.HmatError) as e:
+            h.multiply(x)
+        assert e.value.status == 2  # HMAT_ERR_DEVICE
+    finally:
+        hmat_amd.set_fused_spin_limit(0)
+    assert np.array_equal(h.multiply(x), y0)
