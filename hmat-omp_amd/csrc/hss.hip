@@ -1351,10 +1351,9 @@ void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* d
   va_.reserve((size_t)nn * vs);  // h
   vb_.reserve((size_t)nn * vs);  // u (internal) / tau (all)
   vc_.reserve((size_t)nn * vs);  // s
-  vd_.reserve((size_t)nn * vs);  // K s
   ve_.reserve((size_t)nn * vs);  // tau
-  vf_.reserve((size_t)nn * vs);  // residual of the M solve
-  double *H = va_.get(), *U = vb_.get(), *Sv = vc_.get(), *KS = vd_.get(), *T = ve_.get(), *Rm = vf_.get();
+  vf_.reserve((size_t)nn * vs);  // [h_a; h_b], then the residual of the M solve
+  double *H = va_.get(), *U = vb_.get(), *Sv = vc_.get(), *T = ve_.get(), *Rm = vf_.get();
   auto at = [&](double* b, int id) { return b + (int64_t)id * vs; };
   {
     std::vector<SegTN> tn;
@@ -1367,8 +1366,10 @@ void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* d
   const double* P = pool.get();
   SweepProgram sw;
   // up-sweep, one task per node: u = M^{-1} [h_a; h_b] refined once against
-  // M, s = [B_ab u_b; B_ba u_a], and for non-root nodes
-  // h_p = W^T [h_a; h_b] - W^T [K_a s_a; K_b s_b]
+  // M, s = [B_ab u_b; B_ba u_a], and for non-root nodes h_p = W^T u.  The
+  // last is the push-through formula's h_p = W^T ([h_a; h_b] - K s) with
+  // K s = K C u = (M - I) u and M u = [h_a; h_b]: three dependent products
+  // per node fewer than forming K s.
   for (int d = tree.max_depth - 1; d >= 0; --d) {
     sw.begin_phase();
     for (int id : tree.by_depth[d]) {
@@ -1378,22 +1379,15 @@ void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* d
       const int m = sn[id].ms;
       const double* Mi = P + sn[id].oMinv;
       sw.begin_task();
-      sw.op(BGemm{Mi, at(H, nd.left), at(U, id), m, nrhs, va, m, ldv, ldv, 0, 0, 1.0, 0.0});
-      sw.op(BGemm{Mi + (int64_t)va * m, at(H, nd.right), at(U, id), m, nrhs, vb, m, ldv, ldv, 0, 0, 1.0, 1.0});
       sw.copy(at(H, nd.left), ldv, at(Rm, id), ldv, va, nrhs);
       sw.copy(at(H, nd.right), ldv, at(Rm, id) + va, ldv, vb, nrhs);
+      sw.op(BGemm{Mi, at(Rm, id), at(U, id), m, nrhs, m, m, ldv, ldv, 0, 0, 1.0, 0.0});
       sw.op(BGemm{P + sn[id].oM, at(U, id), at(Rm, id), m, nrhs, m, m, ldv, ldv, 0, 0, -1.0, 1.0});
       sw.op(BGemm{Mi, at(Rm, id), at(U, id), m, nrhs, m, m, ldv, ldv, 0, 0, 1.0, 1.0});
       sw.op(BGemm{P + sn[id].oBab, at(U, id) + va, at(Sv, id), ua, nrhs, vb, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
       sw.op(BGemm{P + sn[id].oBba, at(U, id), at(Sv, id) + ua, ub, nrhs, va, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
-      if (d > 0) {
-        sw.op(BGemm{P + sn[nd.left].oK, at(Sv, id), at(KS, id), va, nrhs, ua, ldv, ldv, ldv, 0, 0, 1.0, 0.0});
-        sw.op(BGemm{P + sn[nd.right].oK, at(Sv, id) + ua, at(KS, id) + va, vb, nrhs, ub, ldv, ldv, ldv, 0, 0, 1.0, 0.0});
-        sw.op(BGemm{P + sn[id].oW, at(H, nd.left), at(H, id), sn[id].rV, nrhs, va, sn[id].ldW, ldv, ldv, 1, 0, 1.0, 0.0});
-        sw.op(BGemm{P + sn[id].oW + va, at(H, nd.right), at(H, id), sn[id].rV, nrhs, vb, sn[id].ldW, ldv, ldv, 1, 0, 1.0,
-                    1.0});
-        sw.op(BGemm{P + sn[id].oW, at(KS, id), at(H, id), sn[id].rV, nrhs, m, sn[id].ldW, ldv, ldv, 1, 0, -1.0, 1.0});
-      }
+      if (d > 0)
+        sw.op(BGemm{P + sn[id].oW, at(U, id), at(H, id), sn[id].rV, nrhs, m, sn[id].ldW, ldv, ldv, 1, 0, 1.0, 0.0});
       sw.end_task();
     }
   }

@@ -126,7 +126,7 @@ class HssEngine {
   int64_t pool_used = 0;
   int64_t pool_built = 0;  // end of the build's part (the factor's follows)
   int ldv = 1;  // leading dimension of every per-node vector in the sweeps
-  DevBuf<double> va_, vb_, vc_, vd_, ve_, vf_;
+  DevBuf<double> va_, vb_, vc_, ve_, vf_;
   DevBuf<double> lres_;  // leaf-solve residual
   DevBuf<double> ir_y_, ir_r_, ir_n_, ir_p_;  // refinement: A x, residual, norms, norm partials
   DevBuf<double> ir_best_;                     // the iterate before the last correction
