@@ -245,6 +245,15 @@ def batch_leg(hmat_amd, torch, steps, warmup, profile=True):
             if pq:
                 prof[name] = {"launches": pq[0], "ms": round(pq[1], 3), "work": pq[2]}
         out["kernels"] = prof
+        sk = prof.get("sketch_gen")
+        if sk and sk["ms"] > 0:
+            # the batch's dominant kernel: the FP64 generated-entry sketches
+            # (Gaussian entries), 2 k (2 + 2 q) sum(m n) flops per step
+            achieved = sk["work"] / (sk["ms"] * 1e-3) / 1e12
+            out["roofline"] = {"kernel": "k_sketch (FP64 MFMA, generated Gaussian entries)", "bound": "mfma",
+                               "achieved": round(achieved, 2), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                               "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "launches": sk["launches"],
+                               "step_share": round(sk["ms"] / 1000 / statistics.median(times), 3)}
     del h
     return out
 
