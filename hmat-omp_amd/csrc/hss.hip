@@ -1137,6 +1137,7 @@ bool HssEngine::plan_matvec() {
 void HssEngine::factor() {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "factor before build");
   factored = false;
+  apl_.ok = false;
   if (solver == kUlv) {
     ulv_factor();
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -1333,89 +1334,119 @@ void HssEngine::solve(const double* dB, int64_t ldb, int nrhs, double* dX, int64
                                             sci(refine_rtol) + ")");
 }
 
+// Descriptors of the Woodbury apply-inverse for nrhs columns: the leaf
+// blocks' segments, V^T X over the leaves, the sweep program (up-sweep
+// then down-sweep, one phase per depth) and the leaf expansion, built and
+// uploaded once and kept while the column count and the per-node vectors
+// stay the same (rebuilt after every factorisation).
+void HssEngine::plan_apply(int nrhs) {
+  apl_.ok = false;
+  const int64_t lm = tree.lmax;
+  std::vector<SegNN> leafs;
+  for (size_t i = 0; i < tree.leaves.size(); ++i) {
+    const TNode& nd = tree.nodes[tree.leaves[i]];
+    leafs.push_back(SegNN{(int64_t)i * lm * lm, nd.start, nd.start, (int32_t)nd.len, (int32_t)nd.len, nrhs, 0});
+  }
+  kern::plan_nn(leafs, apl_.leaf, stream);
+  apl_.phases.clear();
+  if (tree.max_depth > 0) {
+    const int64_t vs = (int64_t)ldv * nrhs;
+    double *H = va_.get(), *U = vb_.get(), *Sv = vc_.get(), *T = ve_.get(), *Rm = vf_.get();
+    auto at = [&](double* b, int id) { return b + (int64_t)id * vs; };
+    std::vector<SegTN> tn;
+    std::vector<SegNN> up;
+    for (int id : tree.leaves) {
+      const TNode& nd = tree.nodes[id];
+      tn.push_back(SegTN{nd.start, nd.start, (int64_t)id * vs, (int32_t)nd.len, sn[id].rV, nrhs, 0});
+      up.push_back(SegNN{nd.start, (int64_t)id * vs, nd.start, (int32_t)nd.len, sn[id].rU, nrhs, 0});
+    }
+    kern::plan_tn(tn, apl_.vt, stream);
+    kern::plan_nn(up, apl_.expand, stream);
+    const double* P = pool.get();
+    SweepProgram sw;
+    // up-sweep, one task per node: u = M^{-1} [h_a; h_b] refined once against
+    // M, s = [B_ab u_b; B_ba u_a], and for non-root nodes h_p = W^T u.  The
+    // last is the push-through formula's h_p = W^T ([h_a; h_b] - K s) with
+    // K s = K C u = (M - I) u and M u = [h_a; h_b]: three dependent products
+    // per node fewer than forming K s.
+    for (int d = tree.max_depth - 1; d >= 0; --d) {
+      sw.begin_phase();
+      for (int id : tree.by_depth[d]) {
+        const TNode& nd = tree.nodes[id];
+        if (nd.leaf) continue;
+        const int va = sn[nd.left].rV, vb = sn[nd.right].rV, ua = sn[nd.left].rU, ub = sn[nd.right].rU;
+        const int m = sn[id].ms;
+        const double* Mi = P + sn[id].oMinv;
+        sw.begin_task();
+        sw.copy(at(H, nd.left), ldv, at(Rm, id), ldv, va, nrhs);
+        sw.copy(at(H, nd.right), ldv, at(Rm, id) + va, ldv, vb, nrhs);
+        sw.op(BGemm{Mi, at(Rm, id), at(U, id), m, nrhs, m, m, ldv, ldv, 0, 0, 1.0, 0.0});
+        sw.op(BGemm{P + sn[id].oM, at(U, id), at(Rm, id), m, nrhs, m, m, ldv, ldv, 0, 0, -1.0, 1.0});
+        sw.op(BGemm{Mi, at(Rm, id), at(U, id), m, nrhs, m, m, ldv, ldv, 0, 0, 1.0, 1.0});
+        sw.op(BGemm{P + sn[id].oBab, at(U, id) + va, at(Sv, id), ua, nrhs, vb, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
+        sw.op(BGemm{P + sn[id].oBba, at(U, id), at(Sv, id) + ua, ub, nrhs, va, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
+        if (d > 0)
+          sw.op(BGemm{P + sn[id].oW, at(U, id), at(H, id), sn[id].rV, nrhs, m, sn[id].ldW, ldv, ldv, 1, 0, 1.0, 0.0});
+        sw.end_task();
+      }
+    }
+    // down-sweep, one task per node: t_a = s_a + G_p(a) t_p, t_b = s_b + G_p(b) t_p
+    for (int d = 0; d < tree.max_depth; ++d) {
+      sw.begin_phase();
+      for (int id : tree.by_depth[d]) {
+        const TNode& nd = tree.nodes[id];
+        if (nd.leaf) continue;
+        const int ua = sn[nd.left].rU, ub = sn[nd.right].rU, kR = ua + ub;
+        sw.begin_task();
+        sw.copy(at(Sv, id), ldv, at(T, nd.left), ldv, ua, nrhs);
+        sw.copy(at(Sv, id) + ua, ldv, at(T, nd.right), ldv, ub, nrhs);
+        if (d > 0) {
+          sw.op(BGemm{P + sn[id].oG, at(T, id), at(T, nd.left), ua, nrhs, sn[id].rU, kR, ldv, ldv, 0, 0, 1.0, 1.0});
+          sw.op(BGemm{P + sn[id].oG + ua, at(T, id), at(T, nd.right), ub, nrhs, sn[id].rU, kR, ldv, ldv, 0, 0, 1.0, 1.0});
+        }
+        sw.end_task();
+      }
+    }
+    apl_.ops.upload(sw.ops, stream);
+    apl_.tasks.upload(sw.tasks, stream);
+    apl_.phases = sw.phases;
+  }
+  apl_.nrhs = nrhs;
+  apl_.addrs = {pool.get(), va_.get(), vb_.get(), vc_.get(), ve_.get(), vf_.get()};
+  apl_.ok = true;
+}
+
 void HssEngine::apply_inverse(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx) {
   if (solver == kUlv) {
     ulv_apply(dB, ldb, nrhs, dX, ldx);
     return;
   }
   ProfScope prof("hss_solve_apply", stream, solve_bytes(nrhs));
-  lres_.reserve((size_t)N * nrhs);
-  leaf_solve(dB, ldb, nrhs, dX, ldx, lres_.get());
-  if (tree.max_depth == 0) {
-    prof.finish();
-    HIP_CHECK(hipStreamSynchronize(stream));
-    return;
-  }
+  const int64_t lm = tree.lmax;
   const int nn = (int)tree.nodes.size();
   const int64_t vs = (int64_t)ldv * nrhs;
-  va_.reserve((size_t)nn * vs);  // h
-  vb_.reserve((size_t)nn * vs);  // u (internal) / tau (all)
-  vc_.reserve((size_t)nn * vs);  // s
-  ve_.reserve((size_t)nn * vs);  // tau
-  vf_.reserve((size_t)nn * vs);  // [h_a; h_b], then the residual of the M solve
-  double *H = va_.get(), *U = vb_.get(), *Sv = vc_.get(), *T = ve_.get(), *Rm = vf_.get();
-  auto at = [&](double* b, int id) { return b + (int64_t)id * vs; };
-  {
-    std::vector<SegTN> tn;
-    for (int id : tree.leaves) {
-      const TNode& nd = tree.nodes[id];
-      tn.push_back(SegTN{nd.start, nd.start, (int64_t)id * vs, (int32_t)nd.len, sn[id].rV, nrhs, 0});
-    }
-    kern::seg_gemm_tn(tn, Vl.get(), N, dX, ldx, H, ldv, 1.0, 0.0, ws_, stream);
+  lres_.reserve((size_t)N * nrhs);
+  if (tree.max_depth > 0) {
+    va_.reserve((size_t)nn * vs);  // h
+    vb_.reserve((size_t)nn * vs);  // u
+    vc_.reserve((size_t)nn * vs);  // s
+    ve_.reserve((size_t)nn * vs);  // tau
+    vf_.reserve((size_t)nn * vs);  // [h_a; h_b], then the residual of the M solve
   }
-  const double* P = pool.get();
-  SweepProgram sw;
-  // up-sweep, one task per node: u = M^{-1} [h_a; h_b] refined once against
-  // M, s = [B_ab u_b; B_ba u_a], and for non-root nodes h_p = W^T u.  The
-  // last is the push-through formula's h_p = W^T ([h_a; h_b] - K s) with
-  // K s = K C u = (M - I) u and M u = [h_a; h_b]: three dependent products
-  // per node fewer than forming K s.
-  for (int d = tree.max_depth - 1; d >= 0; --d) {
-    sw.begin_phase();
-    for (int id : tree.by_depth[d]) {
-      const TNode& nd = tree.nodes[id];
-      if (nd.leaf) continue;
-      const int va = sn[nd.left].rV, vb = sn[nd.right].rV, ua = sn[nd.left].rU, ub = sn[nd.right].rU;
-      const int m = sn[id].ms;
-      const double* Mi = P + sn[id].oMinv;
-      sw.begin_task();
-      sw.copy(at(H, nd.left), ldv, at(Rm, id), ldv, va, nrhs);
-      sw.copy(at(H, nd.right), ldv, at(Rm, id) + va, ldv, vb, nrhs);
-      sw.op(BGemm{Mi, at(Rm, id), at(U, id), m, nrhs, m, m, ldv, ldv, 0, 0, 1.0, 0.0});
-      sw.op(BGemm{P + sn[id].oM, at(U, id), at(Rm, id), m, nrhs, m, m, ldv, ldv, 0, 0, -1.0, 1.0});
-      sw.op(BGemm{Mi, at(Rm, id), at(U, id), m, nrhs, m, m, ldv, ldv, 0, 0, 1.0, 1.0});
-      sw.op(BGemm{P + sn[id].oBab, at(U, id) + va, at(Sv, id), ua, nrhs, vb, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
-      sw.op(BGemm{P + sn[id].oBba, at(U, id), at(Sv, id) + ua, ub, nrhs, va, sn[id].ldB, ldv, ldv, 0, 0, 1.0, 0.0});
-      if (d > 0)
-        sw.op(BGemm{P + sn[id].oW, at(U, id), at(H, id), sn[id].rV, nrhs, m, sn[id].ldW, ldv, ldv, 1, 0, 1.0, 0.0});
-      sw.end_task();
-    }
-  }
-  // down-sweep, one task per node: t_a = s_a + G_p(a) t_p, t_b = s_b + G_p(b) t_p
-  for (int d = 0; d < tree.max_depth; ++d) {
-    sw.begin_phase();
-    for (int id : tree.by_depth[d]) {
-      const TNode& nd = tree.nodes[id];
-      if (nd.leaf) continue;
-      const int ua = sn[nd.left].rU, ub = sn[nd.right].rU, kR = ua + ub;
-      sw.begin_task();
-      sw.copy(at(Sv, id), ldv, at(T, nd.left), ldv, ua, nrhs);
-      sw.copy(at(Sv, id) + ua, ldv, at(T, nd.right), ldv, ub, nrhs);
-      if (d > 0) {
-        sw.op(BGemm{P + sn[id].oG, at(T, id), at(T, nd.left), ua, nrhs, sn[id].rU, kR, ldv, ldv, 0, 0, 1.0, 1.0});
-        sw.op(BGemm{P + sn[id].oG + ua, at(T, id), at(T, nd.right), ub, nrhs, sn[id].rU, kR, ldv, ldv, 0, 0, 1.0, 1.0});
-      }
-      sw.end_task();
-    }
-  }
-  sw.run(stream);
-  {
-    std::vector<SegNN> up;
-    for (int id : tree.leaves) {
-      const TNode& nd = tree.nodes[id];
-      up.push_back(SegNN{nd.start, (int64_t)id * vs, nd.start, (int32_t)nd.len, sn[id].rU, nrhs, 0});
-    }
-    kern::seg_gemm_nn(up, Ql.get(), N, T, ldv, dX, ldx, -1.0, 1.0, stream);
+  const std::vector<const void*> addrs = {pool.get(), va_.get(), vb_.get(), vc_.get(), ve_.get(), vf_.get()};
+  if (!apl_.ok || apl_.nrhs != nrhs || apl_.addrs != addrs) plan_apply(nrhs);
+  // leaf solves, refined once against the leaf blocks (as leaf_solve)
+  kern::launch_nn<double>(apl_.leaf, Dinv.get(), lm, dB, ldb, dX, ldx, 1.0, 0.0, stream);
+  kern::convert_cols<double, double>(N, nrhs, dB, ldb, lres_.get(), N, stream);
+  kern::launch_nn<double>(apl_.leaf, D.get(), lm, dX, ldx, lres_.get(), N, -1.0, 1.0, stream);
+  kern::launch_nn<double>(apl_.leaf, Dinv.get(), lm, lres_.get(), N, dX, ldx, 1.0, 1.0, stream);
+  if (tree.max_depth > 0) {
+    DevBuf<double>& part = ws_.parts<double>();
+    part.reserve((size_t)std::max(apl_.vt.nslots, 1) * 4096);
+    kern::launch_tn<double>(apl_.vt, Vl.get(), N, dX, ldx, va_.get(), ldv, 1.0, 0.0, part.get(), stream);
+    for (const auto& ph : apl_.phases)
+      kern::node_tasks(apl_.tasks.get() + ph.first, ph.second - ph.first, apl_.ops.get(), stream);
+    kern::launch_nn<double>(apl_.expand, Ql.get(), N, ve_.get(), ldv, dX, ldx, -1.0, 1.0, stream);
   }
   prof.finish();
   HIP_CHECK(hipStreamSynchronize(stream));

@@ -20,6 +20,7 @@
 // O(N k) build memory.
 #pragma once
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "hodlr.hpp"
@@ -107,6 +108,7 @@ class HssEngine {
   // does: backward stable like an LU solve while every step stays a GEMM
   void leaf_solve(const double* B, int64_t ldb, int ncols, double* Y, int64_t ldy, double* R);
   void apply_inverse(const double* dB, int64_t ldb, int nrhs, double* dX, int64_t ldx);
+  void plan_apply(int nrhs);
   // descriptors of the one-launch matvec; false when the operator's shape
   // is beyond it (ranks above 64, leaves above 1024 rows, a single leaf) or
   // the device cannot hold its grid
@@ -162,6 +164,18 @@ class HssEngine {
   DevBuf<int> uinfo_;
 
   kern::HmvPlan mv_;  // one-launch matvec (valid for the built operator)
+  // descriptors of the Woodbury apply-inverse for one column count (valid
+  // for the factorisation and the per-node vectors they were built for)
+  struct ApplyPlan {
+    int nrhs = 0;
+    bool ok = false;
+    kern::NNPlan leaf, expand;
+    kern::TNPlan vt;
+    DevBuf<kern::BGemm> ops;
+    DevBuf<kern::NodeTask> tasks;
+    std::vector<std::pair<int, int>> phases;
+    std::vector<const void*> addrs;
+  } apl_;
   kern::Workspace ws_;
   Orthogonalizer orth_;
   DevBuf<SegNN> dnn_;
