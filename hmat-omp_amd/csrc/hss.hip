@@ -1138,6 +1138,7 @@ void HssEngine::factor() {
   HMAT_REQUIRE(built, HMAT_ERR_STATE, "factor before build");
   factored = false;
   apl_.ok = false;
+  ulvp_.ok = false;
   if (solver == kUlv) {
     ulv_factor();
     HIP_CHECK(hipStreamSynchronize(stream));

@@ -176,6 +176,23 @@ class HssEngine {
     std::vector<std::pair<int, int>> phases;
     std::vector<const void*> addrs;
   } apl_;
+  // the ULV apply-inverse's launches for one column count and one pair of
+  // right-hand side / solution buffers: descriptors on the device, replayed
+  // by a list of steps (0: reflector applications, 1: triangular solves,
+  // 2: node tasks; rows and cols size the launch)
+  struct UlvStep {
+    int32_t kind, off, count, rows, cols;
+  };
+  struct UlvPlan {
+    bool ok = false;
+    int nrhs = 0;
+    std::vector<const void*> key;
+    std::vector<UlvStep> steps;
+    DevBuf<unsigned char> items;  // reflector descriptors, then (at tr_off) triangular-solve descriptors
+    int64_t tr_off = 0;
+    DevBuf<kern::BGemm> ops;
+    DevBuf<kern::NodeTask> tasks;
+  } ulvp_;
   kern::Workspace ws_;
   Orthogonalizer orth_;
   DevBuf<SegNN> dnn_;

@@ -30,6 +30,7 @@
 // node tasks.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "hss.hpp"
 #include "sweep.hpp"
@@ -286,16 +287,11 @@ void launch_qr(const std::vector<HQr>& v, hipStream_t s) {
   HIP_CHECK(hipGetLastError());
 }
 
-void launch_apply(const std::vector<HApply>& v, hipStream_t s) {
-  if (v.empty()) return;
-  int rmax = 0, cmax = 0;
-  for (const auto& x : v) {
-    rmax = std::max(rmax, (int)x.rows);
-    cmax = std::max(cmax, (int)x.ncols);
-  }
-  if (cmax == 0) return;
-  const HApply* d = stage(v, s);
-  const dim3 g((unsigned)((cmax + kUlvWG / 64 - 1) / (kUlvWG / 64)), (unsigned)v.size());
+// Reflector applications of `count` descriptors already on the device, up to
+// rmax rows and cmax columns each.
+void launch_apply_dev(const HApply* d, int count, int rmax, int cmax, hipStream_t s) {
+  if (count == 0 || cmax == 0) return;
+  const dim3 g((unsigned)((cmax + kUlvWG / 64 - 1) / (kUlvWG / 64)), (unsigned)count);
   switch (rows_per_lane(rmax)) {
     case 4: k_house_apply<4><<<g, kUlvWG, 0, s>>>(d); break;
     case 8: k_house_apply<8><<<g, kUlvWG, 0, s>>>(d); break;
@@ -304,15 +300,25 @@ void launch_apply(const std::vector<HApply>& v, hipStream_t s) {
   HIP_CHECK(hipGetLastError());
 }
 
-void launch_trsv(const std::vector<TrItem>& v, hipStream_t s) {
-  if (v.empty()) return;
-  int mmax = 0, cmax = 0;
+void apply_extent(const std::vector<HApply>& v, int& rmax, int& cmax) {
+  rmax = cmax = 0;
   for (const auto& x : v) {
-    mmax = std::max(mmax, (int)x.m);
+    rmax = std::max(rmax, (int)x.rows);
     cmax = std::max(cmax, (int)x.ncols);
   }
-  const TrItem* d = stage(v, s);
-  const dim3 g((unsigned)((cmax + kUlvWG / 64 - 1) / (kUlvWG / 64)), (unsigned)v.size());
+}
+
+void launch_apply(const std::vector<HApply>& v, hipStream_t s) {
+  int rmax, cmax;
+  apply_extent(v, rmax, cmax);
+  if (v.empty() || cmax == 0) return;
+  launch_apply_dev(stage(v, s), (int)v.size(), rmax, cmax, s);
+}
+
+// Forward substitutions of `count` descriptors already on the device.
+void launch_trsv_dev(const TrItem* d, int count, int mmax, int cmax, hipStream_t s) {
+  if (count == 0) return;
+  const dim3 g((unsigned)((cmax + kUlvWG / 64 - 1) / (kUlvWG / 64)), (unsigned)count);
   switch (rows_per_lane(mmax)) {
     case 4: k_ulv_trsv<4><<<g, kUlvWG, 0, s>>>(d); break;
     case 8: k_ulv_trsv<8><<<g, kUlvWG, 0, s>>>(d); break;
@@ -483,81 +489,141 @@ void HssEngine::ulv_apply(const double* dB, int64_t ldb, int nrhs, double* dX, i
   ugk_.reserve((size_t)tg);
   double* UP = upool_.get();
   const double* P = pool.get();
-  auto beta = [&](int id) { return ubeta_.get() + ob[id]; };
-  auto gk = [&](int id) { return ugk_.get() + og[id]; };
-  // up the tree
-  for (int d = tree.max_depth; d >= 0; --d) {
-    std::vector<HApply> au;
-    std::vector<TrItem> ts;
-    for (int id : tree.by_depth[d]) {
-      const TNode& nd = tree.nodes[id];
-      const UNode& u = un_[id];
-      if (nd.leaf)
-        au.push_back(HApply{UP + u.oHU, UP + u.oTU, UP + u.oSU, dB + nd.start, nullptr, beta(id), u.n, u.r, u.n, u.n,
-                            (int)ldb, 0, u.n, nrhs, 1, 0});
-      else if (u.r > 0)
-        au.push_back(HApply{UP + u.oHU, UP + u.oTU, UP + u.oSU, beta(id), nullptr, beta(id), u.n, u.r, u.n, u.n, u.n, 0,
-                            u.n, nrhs, 1, 0});
-      if (u.m > 0) ts.push_back(TrItem{UP + u.oHP, beta(id) + u.r, u.m, u.n, u.n, nrhs});
-    }
-    launch_apply(au, stream);  // gamma = Qt beta
-    launch_trsv(ts, stream);   // z = Rq^{-T} gamma[r:]
-    if (d == 0) break;
-    // one task per parent: finish both children (bt, gk), then the
-    // parent's right-hand side
-    SweepProgram sw;
-    sw.begin_phase();
-    for (int p : tree.by_depth[d - 1]) {
-      const TNode& np = tree.nodes[p];
-      if (np.leaf) continue;
-      sw.begin_task();
-      for (int c : {np.left, np.right}) {
-        const TNode& nc = tree.nodes[c];
-        const UNode& u = un_[c];
-        if (u.m > 0)  // bt = gamma[:r] - E z
-          sw.op(BGemm{UP + u.oG, beta(c) + u.r, beta(c), u.r, nrhs, u.m, u.n, u.n, u.n, 1, 0, -1.0, 1.0});
-        bool first = true;
-        if (!nc.leaf) {  // gk = W^T [gk_a; gk_b] + Vh^T z
-          const UNode &ua = un_[nc.left], &ub = un_[nc.right];
-          sw.op(BGemm{P + sn[c].oW, gk(nc.left), gk(c), u.s, nrhs, ua.s, sn[c].ldW, ua.s, u.s, 1, 0, 1.0, 0.0});
-          sw.op(BGemm{P + sn[c].oW + ua.s, gk(nc.right), gk(c), u.s, nrhs, ub.s, sn[c].ldW, ub.s, u.s, 1, 0, 1.0, 1.0});
-          first = false;
-        }
-        if (u.m > 0) {
-          sw.op(BGemm{UP + u.oV, beta(c) + u.r, gk(c), u.s, nrhs, u.m, u.n, u.n, u.s, 1, 0, 1.0, first ? 0.0 : 1.0});
-          first = false;
-        }
-        if (first) sw.copy(uzero_.get(), 0, gk(c), u.s, u.s, nrhs);  // nothing known below a full-rank leaf
+  // the launches' descriptors are built on the first call for these
+  // buffers and column count, and replayed afterwards
+  const std::vector<const void*> key = {dB, dX, reinterpret_cast<const void*>(static_cast<intptr_t>(ldb)),
+                                        reinterpret_cast<const void*>(static_cast<intptr_t>(ldx)), UP, P,
+                                        ubeta_.get(), ugk_.get(), uzero_.get()};
+  if (!ulvp_.ok || ulvp_.nrhs != nrhs || ulvp_.key != key) {
+    ulvp_.ok = false;
+    auto beta = [&](int id) { return ubeta_.get() + ob[id]; };
+    auto gk = [&](int id) { return ugk_.get() + og[id]; };
+    std::vector<HApply> ha;
+    std::vector<TrItem> tr;
+    std::vector<BGemm> ops;
+    std::vector<kern::NodeTask> tasks;
+    std::vector<UlvStep> steps;
+    auto add_apply = [&](const std::vector<HApply>& v) {
+      int rmax, cmax;
+      apply_extent(v, rmax, cmax);
+      if (v.empty() || cmax == 0) return;
+      steps.push_back(UlvStep{0, (int32_t)ha.size(), (int32_t)v.size(), rmax, cmax});
+      ha.insert(ha.end(), v.begin(), v.end());
+    };
+    // up the tree
+    for (int d = tree.max_depth; d >= 0; --d) {
+      std::vector<HApply> au;
+      std::vector<TrItem> ts;
+      for (int id : tree.by_depth[d]) {
+        const TNode& nd = tree.nodes[id];
+        const UNode& u = un_[id];
+        if (nd.leaf)
+          au.push_back(HApply{UP + u.oHU, UP + u.oTU, UP + u.oSU, dB + nd.start, nullptr, beta(id), u.n, u.r, u.n,
+                              u.n, (int)ldb, 0, u.n, nrhs, 1, 0});
+        else if (u.r > 0)
+          au.push_back(HApply{UP + u.oHU, UP + u.oTU, UP + u.oSU, beta(id), nullptr, beta(id), u.n, u.r, u.n, u.n,
+                              u.n, 0, u.n, nrhs, 1, 0});
+        if (u.m > 0) ts.push_back(TrItem{UP + u.oHP, beta(id) + u.r, u.m, u.n, u.n, nrhs});
       }
-      const UNode &ua = un_[np.left], &ub = un_[np.right], &up = un_[p];
-      double* bp = beta(p);
-      sw.copy(beta(np.left), ua.n, bp, up.n, ua.r, nrhs);
-      sw.op(BGemm{P + sn[p].oBab, gk(np.right), bp, ua.r, nrhs, ub.s, sn[p].ldB, ub.s, up.n, 0, 0, -1.0, 1.0});
-      sw.copy(beta(np.right), ub.n, bp + ua.r, up.n, ub.r, nrhs);
-      sw.op(BGemm{P + sn[p].oBba, gk(np.left), bp + ua.r, ub.r, nrhs, ua.s, sn[p].ldB, ua.s, up.n, 0, 0, -1.0, 1.0});
-      sw.end_task();
+      add_apply(au);  // gamma = Qt beta
+      if (!ts.empty()) {  // z = Rq^{-T} gamma[r:]
+        int mmax = 0, cmax = 0;
+        for (const auto& x : ts) {
+          mmax = std::max(mmax, (int)x.m);
+          cmax = std::max(cmax, (int)x.ncols);
+        }
+        steps.push_back(UlvStep{1, (int32_t)tr.size(), (int32_t)ts.size(), mmax, cmax});
+        tr.insert(tr.end(), ts.begin(), ts.end());
+      }
+      if (d == 0) break;
+      // one task per parent: finish both children (bt, gk), then the
+      // parent's right-hand side
+      SweepProgram sw;
+      sw.begin_phase();
+      for (int p : tree.by_depth[d - 1]) {
+        const TNode& np = tree.nodes[p];
+        if (np.leaf) continue;
+        sw.begin_task();
+        for (int c : {np.left, np.right}) {
+          const TNode& nc = tree.nodes[c];
+          const UNode& u = un_[c];
+          if (u.m > 0)  // bt = gamma[:r] - E z
+            sw.op(BGemm{UP + u.oG, beta(c) + u.r, beta(c), u.r, nrhs, u.m, u.n, u.n, u.n, 1, 0, -1.0, 1.0});
+          bool first = true;
+          if (!nc.leaf) {  // gk = W^T [gk_a; gk_b] + Vh^T z
+            const UNode &ua = un_[nc.left], &ub = un_[nc.right];
+            sw.op(BGemm{P + sn[c].oW, gk(nc.left), gk(c), u.s, nrhs, ua.s, sn[c].ldW, ua.s, u.s, 1, 0, 1.0, 0.0});
+            sw.op(BGemm{P + sn[c].oW + ua.s, gk(nc.right), gk(c), u.s, nrhs, ub.s, sn[c].ldW, ub.s, u.s, 1, 0, 1.0,
+                        1.0});
+            first = false;
+          }
+          if (u.m > 0) {
+            sw.op(BGemm{UP + u.oV, beta(c) + u.r, gk(c), u.s, nrhs, u.m, u.n, u.n, u.s, 1, 0, 1.0,
+                        first ? 0.0 : 1.0});
+            first = false;
+          }
+          if (first) sw.copy(uzero_.get(), 0, gk(c), u.s, u.s, nrhs);  // nothing known below a full-rank leaf
+        }
+        const UNode &ua = un_[np.left], &ub = un_[np.right], &up = un_[p];
+        double* bp = beta(p);
+        sw.copy(beta(np.left), ua.n, bp, up.n, ua.r, nrhs);
+        sw.op(BGemm{P + sn[p].oBab, gk(np.right), bp, ua.r, nrhs, ub.s, sn[p].ldB, ub.s, up.n, 0, 0, -1.0, 1.0});
+        sw.copy(beta(np.right), ub.n, bp + ua.r, up.n, ub.r, nrhs);
+        sw.op(BGemm{P + sn[p].oBba, gk(np.left), bp + ua.r, ub.r, nrhs, ua.s, sn[p].ldB, ua.s, up.n, 0, 0, -1.0, 1.0});
+        sw.end_task();
+      }
+      if (!sw.tasks.empty()) {
+        const int op0 = (int)ops.size();
+        steps.push_back(UlvStep{2, (int32_t)tasks.size(), (int32_t)sw.tasks.size(), 0, 0});
+        for (kern::NodeTask t : sw.tasks) {
+          t.op0 += op0;
+          tasks.push_back(t);
+        }
+        ops.insert(ops.end(), sw.ops.begin(), sw.ops.end());
+      }
     }
-    sw.run(stream);
+    // down the tree: y = H_P [z; xt]
+    for (int d = 0; d <= tree.max_depth; ++d) {
+      std::vector<HApply> ad;
+      for (int id : tree.by_depth[d]) {
+        const TNode& nd = tree.nodes[id];
+        const UNode& u = un_[id];
+        const double* xt = nullptr;
+        int ldxt = 0;
+        if (id != 0) {
+          const TNode& npar = tree.nodes[nd.parent];
+          xt = beta(nd.parent) + (npar.right == id ? un_[npar.left].r : 0);
+          ldxt = un_[nd.parent].n;
+        }
+        double* y = nd.leaf ? dX + nd.start : beta(id);
+        const int ldy = nd.leaf ? (int)ldx : u.n;
+        ad.push_back(HApply{UP + u.oHP, UP + u.oTP, nullptr, beta(id) + u.r, xt, y, u.n, u.m, u.n, u.m, u.n, ldxt,
+                            ldy, nrhs, 0, 0});
+      }
+      add_apply(ad);
+    }
+    const size_t hab = ha.size() * sizeof(HApply);
+    ulvp_.tr_off = (int64_t)((hab + 255) / 256 * 256);
+    std::vector<unsigned char> bytes((size_t)ulvp_.tr_off + tr.size() * sizeof(TrItem));
+    if (!ha.empty()) std::memcpy(bytes.data(), ha.data(), hab);
+    if (!tr.empty()) std::memcpy(bytes.data() + ulvp_.tr_off, tr.data(), tr.size() * sizeof(TrItem));
+    ulvp_.items.upload(bytes, stream);
+    if (!ops.empty()) ulvp_.ops.upload(ops, stream);
+    if (!tasks.empty()) ulvp_.tasks.upload(tasks, stream);
+    ulvp_.steps = steps;
+    ulvp_.nrhs = nrhs;
+    ulvp_.key = key;
+    ulvp_.ok = true;
   }
-  // down the tree: y = H_P [z; xt]
-  for (int d = 0; d <= tree.max_depth; ++d) {
-    std::vector<HApply> ad;
-    for (int id : tree.by_depth[d]) {
-      const TNode& nd = tree.nodes[id];
-      const UNode& u = un_[id];
-      const double* xt = nullptr;
-      int ldxt = 0;
-      if (id != 0) {
-        const TNode& npar = tree.nodes[nd.parent];
-        xt = beta(nd.parent) + (npar.right == id ? un_[npar.left].r : 0);
-        ldxt = un_[nd.parent].n;
-      }
-      double* y = nd.leaf ? dX + nd.start : beta(id);
-      const int ldy = nd.leaf ? (int)ldx : u.n;
-      ad.push_back(HApply{UP + u.oHP, UP + u.oTP, nullptr, beta(id) + u.r, xt, y, u.n, u.m, u.n, u.m, u.n, ldxt, ldy,
-                          nrhs, 0, 0});
-    }
-    launch_apply(ad, stream);
+  const HApply* dha = reinterpret_cast<const HApply*>(ulvp_.items.get());
+  const TrItem* dtr = reinterpret_cast<const TrItem*>(ulvp_.items.get() + ulvp_.tr_off);
+  for (const UlvStep& st : ulvp_.steps) {
+    if (st.kind == 0)
+      launch_apply_dev(dha + st.off, st.count, st.rows, st.cols, stream);
+    else if (st.kind == 1)
+      launch_trsv_dev(dtr + st.off, st.count, st.rows, st.cols, stream);
+    else
+      kern::node_tasks(ulvp_.tasks.get() + st.off, st.count, ulvp_.ops.get(), stream);
   }
   prof.finish();
   HIP_CHECK(hipStreamSynchronize(stream));
