@@ -180,9 +180,10 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project(const MvLeaf* __restrict__
 // Wide passes (NR = 8, 16; leaves of at most 256 rows): the per-leaf
 // projection is a small GEMM, (V_c^T)(rows of the leaf) X(rows, 0:NR), run
 // on v_mfma_f64_16x16x4f64.  The leaf's X rows are staged in LDS as 16
-// columns (zero beyond NR and past the leaf's rows); each wave takes whole
-// (depth, 16-column block) items, streams its 16 V columns four rows per
-// MFMA (eight MFMAs' loads in flight) and writes the 16 x NR block of
+// columns (zero beyond NR and past the leaf's rows); the leaf's (depth,
+// 16-column block) items are dealt round-robin to the waves of gridDim.y
+// workgroups, and each wave streams an item's 16 V columns four rows per
+// MFMA (sixteen MFMAs' loads in flight) and writes the 16 x NR block of
 // partials.  Same partial layout as k_mv_project.
 template <int NR>
 __global__ __launch_bounds__(kMvWG) void k_mv_project_mfma(const MvLeaf* __restrict__ leaves,
@@ -202,6 +203,7 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project_mfma(const MvLeaf* __restr
   __syncthreads();
   const MvEntry* le = ent + (int64_t)blockIdx.x * L;
   const int kr = lane >> 4, ci = lane & 15;
+  const int waves = (kMvWG / 64) * (int)gridDim.y, mine = (int)blockIdx.y * (kMvWG / 64) + w;
   int item = 0;
   for (int c = 0; c < L; ++c) {
     const int vc = le[c].vcols;
@@ -209,21 +211,21 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project_mfma(const MvLeaf* __restr
     gdouble* V = (gdouble*)lvls[c].V;
     const int64_t N = lvls[c].ld;
     for (int cb = 0; cb < vc; cb += 16, ++item) {
-      if ((item & 3) != w) continue;
+      if (item % waves != mine) continue;
       const bool cok = cb + ci < vc;
       gdouble* vcol = V + lf.start + (int64_t)min(cb + ci, vc - 1) * N;
       mfma::v4d acc = {0.0, 0.0, 0.0, 0.0};
-      for (int r0 = 0; r0 < lenp; r0 += 32) {
-        double a[8], b[8];
+      for (int r0 = 0; r0 < lenp; r0 += 64) {
+        double a[16], b[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < 16; ++u) {
           const int r = r0 + 4 * u + kr;
           const double v = vcol[min(r, lf.len - 1)];
           a[u] = (cok && r < lf.len) ? v : 0.0;
-          b[u] = r < lenp ? xs[r * 16 + ci] : 0.0;
+          b[u] = r < lenp ? xs[min(r, lenp - 1) * 16 + ci] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < 16; ++u)
           if (r0 + 4 * u < lenp) acc = mfma::f64_16x16x4(a[u], b[u], acc);
       }
 #pragma unroll
@@ -247,7 +249,8 @@ __global__ __launch_bounds__(kMvWG) void k_mv_reduce(const MvSeg* __restrict__ s
   constexpr int WS = kMvWG / 64;  // wave w sums leaves w, w + WS, w + 2 WS, ...
   const int64_t lstride = (int64_t)L * KM * NR;
   const int nlf = sg.leaf1 - sg.leaf0;
-  for (int i0 = 0; i0 < P; i0 += 64) {
+  // gridDim.y workgroups share the segment's values, 64 at a time
+  for (int i0 = (int)blockIdx.y * 64; i0 < P; i0 += 64 * (int)gridDim.y) {
     const int i = i0 + lane;
     double acc = 0.0;
     if (i < P) {
@@ -364,6 +367,84 @@ __global__ __launch_bounds__(kMvWG) void k_mv_expand(const MvLeaf* __restrict__ 
   }
 }
 
+// Wide passes (NR = 8, 16; leaves of at most 256 rows): the same product
+// as k_mv_expand, y(leaf rows, 0:NR) = [D_leaf | (U S)_c ...] [x; t_sib ...],
+// as a small GEMM on v_mfma_f64_16x16x4f64 with the same LDS staging.  Each
+// wave takes two 16-row tiles at a time and streams their rows of every
+// matrix four columns per MFMA, sixteen loads per lane in flight (the VALU
+// form keeps four per lane for these widths).
+template <int NR>
+__global__ __launch_bounds__(kMvWG) void k_mv_expand_mfma(const MvLeaf* __restrict__ leaves,
+                                                          const MvLevel* __restrict__ lvls,
+                                                          const MvEntry* __restrict__ ent, int L, int KM,
+                                                          const double* __restrict__ Dm, int64_t lm,
+                                                          const double* __restrict__ X, int64_t ldx,
+                                                          const double* __restrict__ t,
+                                                          double* __restrict__ Y, int64_t ldy) {
+  static_assert(NR == 8 || NR == 16, "wide passes only");
+  extern __shared__ double sm[];  // x: lm x NR, then per depth KM x NR
+  const MvLeaf lf = leaves[blockIdx.x];
+  double* xs = sm;
+  double* ts = sm + lm * NR;
+  for (int e = threadIdx.x; e < lf.len * NR; e += kMvWG) {
+    const int r = e / NR, q = e % NR;
+    xs[e] = X[lf.start + r + (int64_t)q * ldx];
+  }
+  for (int c = 0; c < L; ++c) {
+    const MvEntry en = ent[(int64_t)blockIdx.x * L + c];
+    for (int e = threadIdx.x; e < en.ucols * NR; e += kMvWG)
+      ts[(int64_t)c * KM * NR + e] = t[(int64_t)en.tin * KM * NR + e];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, kr = lane >> 4;
+  const int qb = min(i, NR - 1);  // B column of this lane (zero beyond NR)
+  const int ntile = (lf.len + 15) / 16;
+  gdouble* Dl = (gdouble*)(Dm + (int64_t)blockIdx.x * lm * lm);
+  for (int t0 = w; t0 < ntile; t0 += 2 * (kMvWG / 64)) {
+    const int t1 = t0 + kMvWG / 64;  // past the last tile: computed on clamped rows, not stored
+    const int ra = min(t0 * 16 + i, lf.len - 1), rb = min(t1 * 16 + i, lf.len - 1);
+    mfma::v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    // acc += M(rows, 0:n) v(0:n, 0:NR); M column j at M + j * ldm, rows
+    // relative to the leaf start
+    auto seg = [&](gdouble* M, int64_t ldm, const double* v, int n) {
+      for (int k0 = 0; k0 < n; k0 += 32) {
+        double a0[8], a1[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int k = k0 + 4 * u + kr;
+          const int kc = min(k, n - 1);
+          const double m0 = M[ra + (int64_t)kc * ldm];
+          const double m1 = M[rb + (int64_t)kc * ldm];
+          a0[u] = k < n ? m0 : 0.0;
+          a1[u] = k < n ? m1 : 0.0;
+          b[u] = (k < n && i < NR) ? v[kc * NR + qb] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (k0 + 4 * u < n) {
+            acc0 = mfma::f64_16x16x4(a0[u], b[u], acc0);
+            acc1 = mfma::f64_16x16x4(a1[u], b[u], acc1);
+          }
+      }
+    };
+    seg(Dl, lm, xs, lf.len);
+    for (int c = 0; c < L; ++c) {
+      const MvEntry en = ent[(int64_t)blockIdx.x * L + c];
+      if (en.ucols > 0) seg((gdouble*)lvls[c].US + lf.start, lvls[c].ld, ts + (int64_t)c * KM * NR, en.ucols);
+    }
+    if (i < NR) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = mfma::f64_row(lane, v);
+        const int r0 = t0 * 16 + r, r1 = t1 * 16 + r;
+        if (r0 < lf.len) Y[lf.start + r0 + (int64_t)i * ldy] = acc0[v];
+        if (t1 < ntile && r1 < lf.len) Y[lf.start + r1 + (int64_t)i * ldy] = acc1[v];
+      }
+    }
+  }
+}
+
 template <int NR>
 static void launch_mv(const MvPlan& p, const double* Dm, int64_t lm, const double* X, int64_t ldx,
                       double* part, double* t, double* Y, int64_t ldy, hipStream_t s) {
@@ -372,7 +453,10 @@ static void launch_mv(const MvPlan& p, const double* Dm, int64_t lm, const doubl
     if constexpr (NR >= 8) {
       if (lm <= kMvWG) {
         const size_t xl = (size_t)((lm + 3) & ~3) * 16 * sizeof(double);
-        k_mv_project_mfma<NR><<<nl, kMvWG, xl, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
+        // up to four workgroups per leaf, so that a few hundred leaves
+        // still put about 1024 workgroups in flight
+        const unsigned sy = (unsigned)std::max(1, std::min(4, (1024 + p.nleaf - 1) / std::max(p.nleaf, 1)));
+        k_mv_project_mfma<NR><<<dim3(nl, sy), kMvWG, xl, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
       } else {
         const size_t plds = (size_t)(kMvWG / 64) * ((p.KM + 15) / 16 * 16) * NR * sizeof(double);
         k_mv_project<NR><<<nl, kMvWG, plds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
@@ -382,10 +466,19 @@ static void launch_mv(const MvPlan& p, const double* Dm, int64_t lm, const doubl
       k_mv_project<NR><<<nl, kMvWG, plds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
     }
     HIP_CHECK(hipGetLastError());
-    k_mv_reduce<NR><<<(unsigned)p.nseg, kMvWG, 0, s>>>(p.segs, p.L, p.KM, part, t);
+    // one workgroup per 64 values of a segment's t
+    const unsigned vy = (unsigned)std::max(1, (p.KM * NR + 63) / 64);
+    k_mv_reduce<NR><<<dim3((unsigned)p.nseg, vy), kMvWG, 0, s>>>(p.segs, p.L, p.KM, part, t);
     HIP_CHECK(hipGetLastError());
   }
   const size_t lds = (size_t)(lm + (int64_t)p.L * p.KM) * NR * sizeof(double);
+  if constexpr (NR >= 8) {
+    if (lm <= kMvWG) {
+      k_mv_expand_mfma<NR><<<nl, kMvWG, lds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, Dm, lm, X, ldx, t, Y, ldy);
+      HIP_CHECK(hipGetLastError());
+      return;
+    }
+  }
   k_mv_expand<NR><<<nl, kMvWG, lds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, Dm, lm, X, ldx, t, Y, ldy);
   HIP_CHECK(hipGetLastError());
 }
