@@ -9,11 +9,12 @@
 //            V_c^T x over its rows (lanes run down the rows, so V is read
 //            coalesced once), reduced across lanes by recursive halving
 //            (63 shuffles for 64 values) -> one partial per (leaf, depth);
-//            passes of 8 or 16 right-hand sides run it on FP64 MFMA;
-//   reduce   one workgroup per segment: t_s = sum of its leaves' partials;
+//            passes of 4, 8 or 16 right-hand sides run it on FP64 MFMA;
+//   reduce   one workgroup per 64 values of a segment: t_s = sum of its
+//            leaves' partials;
 //   expand   one workgroup per leaf: y = D_leaf x_leaf + sum_c (U S)_c t_sib,
 //            x and the t vectors staged in LDS, D and every (U S)_c read
-//            coalesced once.
+//            coalesced once (on FP64 MFMA from four right-hand sides).
 // HBM traffic is the algorithmic 8 (N leaf + 2 N r L) + 16 N nrhs bytes
 // plus the O(leaves x depths x r) partials.
 #include <hip/hip_runtime.h>
@@ -177,24 +178,41 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project(const MvLeaf* __restrict__
   }
 }
 
-// Wide passes (NR = 8, 16; leaves of at most 256 rows): the per-leaf
+// Orders this wave's LDS writes before its later LDS reads of other
+// lanes' data (and the reads before later writes): a wave's LDS operations
+// run in issue order, so a compiler barrier suffices.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wide passes (NR = 4, 8, 16; leaves of at most 256 rows): the per-leaf
 // projection is a small GEMM, (V_c^T)(rows of the leaf) X(rows, 0:NR), run
 // on v_mfma_f64_16x16x4f64.  The leaf's X rows are staged in LDS as 16
 // columns (zero beyond NR and past the leaf's rows); the leaf's (depth,
 // 16-column block) items are dealt round-robin to the waves of gridDim.y
-// workgroups, and each wave streams an item's 16 V columns four rows per
-// MFMA (sixteen MFMAs' loads in flight) and writes the 16 x NR block of
-// partials.  Same partial layout as k_mv_project.
+// workgroups.  A wave reads its item's 16 V columns 64 rows at a time with
+// the lanes down the rows (coalesced, sixteen loads per lane in flight
+// during the previous chunk's products), transposes them through its own
+// LDS tile into the MFMA operand layout (lane l takes column l % 16, rows
+// l / 16 + 4u), and writes the 16 x NR block of partials.  Same partial
+// layout as k_mv_project.
+constexpr int kMvVS = 66;  // LDS stride of a staged V column (doubles): the operand reads of one
+                           // half-wave (columns 0..15, two rows) touch 64 distinct banks
+constexpr size_t kMvProjTile = (size_t)(kMvWG / 64) * 16 * kMvVS;  // doubles
 template <int NR>
 __global__ __launch_bounds__(kMvWG) void k_mv_project_mfma(const MvLeaf* __restrict__ leaves,
                                                            const MvLevel* __restrict__ lvls,
                                                            const MvEntry* __restrict__ ent, int L, int KM,
                                                            const double* __restrict__ X, int64_t ldx,
                                                            double* __restrict__ part) {
-  static_assert(NR == 8 || NR == 16, "wide passes only");
-  extern __shared__ double xs[];  // lenp x 16
+  static_assert(NR >= 4, "wide passes only");
+  extern __shared__ double sm[];  // per wave 16 x kMvVS, then x: lenp x 16
   const MvLeaf lf = leaves[blockIdx.x];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double* vsw = sm + (size_t)w * 16 * kMvVS;
+  double* xs = sm + kMvProjTile;
   const int lenp = (lf.len + 3) & ~3;
   for (int e = threadIdx.x; e < lenp * 16; e += kMvWG) {
     const int r = e / 16, q = e % 16;
@@ -204,6 +222,7 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project_mfma(const MvLeaf* __restr
   const MvEntry* le = ent + (int64_t)blockIdx.x * L;
   const int kr = lane >> 4, ci = lane & 15;
   const int waves = (kMvWG / 64) * (int)gridDim.y, mine = (int)blockIdx.y * (kMvWG / 64) + w;
+  const int nch = (lf.len + 63) / 64;
   int item = 0;
   for (int c = 0; c < L; ++c) {
     const int vc = le[c].vcols;
@@ -212,21 +231,28 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project_mfma(const MvLeaf* __restr
     const int64_t N = lvls[c].ld;
     for (int cb = 0; cb < vc; cb += 16, ++item) {
       if (item % waves != mine) continue;
-      const bool cok = cb + ci < vc;
-      gdouble* vcol = V + lf.start + (int64_t)min(cb + ci, vc - 1) * N;
-      mfma::v4d acc = {0.0, 0.0, 0.0, 0.0};
-      for (int r0 = 0; r0 < lenp; r0 += 64) {
-        double a[16], b[16];
+      double g[16];
+      auto load = [&](int ch) {  // rows and columns clamped: every load in bounds
+        const int64_t row = lf.start + min(ch * 64 + lane, lf.len - 1);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int r = r0 + 4 * u + kr;
-          const double v = vcol[min(r, lf.len - 1)];
-          a[u] = (cok && r < lf.len) ? v : 0.0;
-          b[u] = r < lenp ? xs[min(r, lenp - 1) * 16 + ci] : 0.0;
-        }
+        for (int j = 0; j < 16; ++j) g[j] = V[row + (int64_t)min(cb + j, vc - 1) * N];
+      };
+      load(0);
+      mfma::v4d acc = {0.0, 0.0, 0.0, 0.0};
+      for (int ch = 0; ch < nch; ++ch) {
+        const bool rok = ch * 64 + lane < lf.len;
+        wave_lds_sync();  // the previous chunk's operand reads are done
+#pragma unroll
+        for (int j = 0; j < 16; ++j) vsw[j * kMvVS + lane] = (rok && cb + j < vc) ? g[j] : 0.0;
+        wave_lds_sync();
+        if (ch + 1 < nch) load(ch + 1);
+        const int r0 = ch * 64;
 #pragma unroll
         for (int u = 0; u < 16; ++u)
-          if (r0 + 4 * u < lenp) acc = mfma::f64_16x16x4(a[u], b[u], acc);
+          if (r0 + 4 * u < lenp) {  // rows up to lenp: x is staged (zero past the leaf)
+            const int r = 4 * u + kr;
+            acc = mfma::f64_16x16x4(vsw[ci * kMvVS + r], xs[(r0 + r) * 16 + ci], acc);
+          }
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -367,7 +393,7 @@ __global__ __launch_bounds__(kMvWG) void k_mv_expand(const MvLeaf* __restrict__ 
   }
 }
 
-// Wide passes (NR = 8, 16; leaves of at most 256 rows): the same product
+// Wide passes (NR = 4, 8, 16; leaves of at most 256 rows): the same product
 // as k_mv_expand, y(leaf rows, 0:NR) = [D_leaf | (U S)_c ...] [x; t_sib ...],
 // as a small GEMM on v_mfma_f64_16x16x4f64 with the same LDS staging.  Each
 // wave takes two 16-row tiles at a time and streams their rows of every
@@ -381,7 +407,7 @@ __global__ __launch_bounds__(kMvWG) void k_mv_expand_mfma(const MvLeaf* __restri
                                                           const double* __restrict__ X, int64_t ldx,
                                                           const double* __restrict__ t,
                                                           double* __restrict__ Y, int64_t ldy) {
-  static_assert(NR == 8 || NR == 16, "wide passes only");
+  static_assert(NR >= 4, "wide passes only");
   extern __shared__ double sm[];  // x: lm x NR, then per depth KM x NR
   const MvLeaf lf = leaves[blockIdx.x];
   double* xs = sm;
@@ -450,9 +476,9 @@ static void launch_mv(const MvPlan& p, const double* Dm, int64_t lm, const doubl
                       double* part, double* t, double* Y, int64_t ldy, hipStream_t s) {
   const unsigned nl = (unsigned)p.nleaf;
   if (p.L > 0) {
-    if constexpr (NR >= 8) {
+    if constexpr (NR >= 4) {
       if (lm <= kMvWG) {
-        const size_t xl = (size_t)((lm + 3) & ~3) * 16 * sizeof(double);
+        const size_t xl = (kMvProjTile + (size_t)((lm + 3) & ~3) * 16) * sizeof(double);
         // up to four workgroups per leaf, so that a few hundred leaves
         // still put about 1024 workgroups in flight
         const unsigned sy = (unsigned)std::max(1, std::min(4, (1024 + p.nleaf - 1) / std::max(p.nleaf, 1)));
@@ -472,7 +498,7 @@ static void launch_mv(const MvPlan& p, const double* Dm, int64_t lm, const doubl
     HIP_CHECK(hipGetLastError());
   }
   const size_t lds = (size_t)(lm + (int64_t)p.L * p.KM) * NR * sizeof(double);
-  if constexpr (NR >= 8) {
+  if constexpr (NR >= 4) {
     if (lm <= kMvWG) {
       k_mv_expand_mfma<NR><<<nl, kMvWG, lds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, Dm, lm, X, ldx, t, Y, ldy);
       HIP_CHECK(hipGetLastError());
