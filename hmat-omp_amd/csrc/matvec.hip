@@ -187,7 +187,7 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Wide passes (NR = 4, 8, 16; leaves of at most 256 rows): the per-leaf
+// MFMA passes (leaves of at most 256 rows; see launch_mv): the per-leaf
 // projection is a small GEMM, (V_c^T)(rows of the leaf) X(rows, 0:NR), run
 // on v_mfma_f64_16x16x4f64.  The leaf's X rows are staged in LDS as 16
 // columns (zero beyond NR and past the leaf's rows); the leaf's (depth,
@@ -207,8 +207,7 @@ __global__ __launch_bounds__(kMvWG) void k_mv_project_mfma(const MvLeaf* __restr
                                                            const MvEntry* __restrict__ ent, int L, int KM,
                                                            const double* __restrict__ X, int64_t ldx,
                                                            double* __restrict__ part) {
-  static_assert(NR >= 4, "wide passes only");
-  extern __shared__ double sm[];  // per wave 16 x kMvVS, then x: lenp x 16
+    extern __shared__ double sm[];  // per wave 16 x kMvVS, then x: lenp x 16
   const MvLeaf lf = leaves[blockIdx.x];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double* vsw = sm + (size_t)w * 16 * kMvVS;
@@ -393,12 +392,12 @@ __global__ __launch_bounds__(kMvWG) void k_mv_expand(const MvLeaf* __restrict__ 
   }
 }
 
-// Wide passes (NR = 4, 8, 16; leaves of at most 256 rows): the same product
+// MFMA passes (leaves of at most 256 rows; see launch_mv): the same product
 // as k_mv_expand, y(leaf rows, 0:NR) = [D_leaf | (U S)_c ...] [x; t_sib ...],
 // as a small GEMM on v_mfma_f64_16x16x4f64 with the same LDS staging.  Each
 // wave takes two 16-row tiles at a time and streams their rows of every
 // matrix four columns per MFMA, sixteen loads per lane in flight (the VALU
-// form keeps four per lane for these widths).
+// form keeps four per lane from four right-hand sides).
 template <int NR>
 __global__ __launch_bounds__(kMvWG) void k_mv_expand_mfma(const MvLeaf* __restrict__ leaves,
                                                           const MvLevel* __restrict__ lvls,
@@ -407,8 +406,7 @@ __global__ __launch_bounds__(kMvWG) void k_mv_expand_mfma(const MvLeaf* __restri
                                                           const double* __restrict__ X, int64_t ldx,
                                                           const double* __restrict__ t,
                                                           double* __restrict__ Y, int64_t ldy) {
-  static_assert(NR >= 4, "wide passes only");
-  extern __shared__ double sm[];  // x: lm x NR, then per depth KM x NR
+    extern __shared__ double sm[];  // x: lm x NR, then per depth KM x NR
   const MvLeaf lf = leaves[blockIdx.x];
   double* xs = sm;
   double* ts = sm + lm * NR;
@@ -475,18 +473,19 @@ template <int NR>
 static void launch_mv(const MvPlan& p, const double* Dm, int64_t lm, const double* X, int64_t ldx,
                       double* part, double* t, double* Y, int64_t ldy, hipStream_t s) {
   const unsigned nl = (unsigned)p.nleaf;
+  // MFMA passes from four columns, and at every width on leaves of at most
+  // 128 rows, where the row-per-thread projection leaves half of its
+  // workgroup idle (one column, 65536 rows in leaves of 128: 0.097 ms
+  // against 0.108 ms; in leaves of 256 the VALU kernels stay ahead for one
+  // column, 0.471 ms against 0.492 ms at 2^18 rows)
+  const bool mf = lm <= kMvWG && (NR >= 4 || lm <= 128);
   if (p.L > 0) {
-    if constexpr (NR >= 4) {
-      if (lm <= kMvWG) {
-        const size_t xl = (kMvProjTile + (size_t)((lm + 3) & ~3) * 16) * sizeof(double);
-        // up to four workgroups per leaf, so that a few hundred leaves
-        // still put about 1024 workgroups in flight
-        const unsigned sy = (unsigned)std::max(1, std::min(4, (1024 + p.nleaf - 1) / std::max(p.nleaf, 1)));
-        k_mv_project_mfma<NR><<<dim3(nl, sy), kMvWG, xl, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
-      } else {
-        const size_t plds = (size_t)(kMvWG / 64) * ((p.KM + 15) / 16 * 16) * NR * sizeof(double);
-        k_mv_project<NR><<<nl, kMvWG, plds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
-      }
+    if (mf) {
+      const size_t xl = (kMvProjTile + (size_t)((lm + 3) & ~3) * 16) * sizeof(double);
+      // up to four workgroups per leaf, so that a few hundred leaves still
+      // put about 1024 workgroups in flight
+      const unsigned sy = (unsigned)std::max(1, std::min(4, (1024 + p.nleaf - 1) / std::max(p.nleaf, 1)));
+      k_mv_project_mfma<NR><<<dim3(nl, sy), kMvWG, xl, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
     } else {
       const size_t plds = (size_t)(kMvWG / 64) * ((p.KM + 15) / 16 * 16) * NR * sizeof(double);
       k_mv_project<NR><<<nl, kMvWG, plds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, X, ldx, part);
@@ -498,14 +497,10 @@ static void launch_mv(const MvPlan& p, const double* Dm, int64_t lm, const doubl
     HIP_CHECK(hipGetLastError());
   }
   const size_t lds = (size_t)(lm + (int64_t)p.L * p.KM) * NR * sizeof(double);
-  if constexpr (NR >= 4) {
-    if (lm <= kMvWG) {
-      k_mv_expand_mfma<NR><<<nl, kMvWG, lds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, Dm, lm, X, ldx, t, Y, ldy);
-      HIP_CHECK(hipGetLastError());
-      return;
-    }
-  }
-  k_mv_expand<NR><<<nl, kMvWG, lds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, Dm, lm, X, ldx, t, Y, ldy);
+  if (mf)
+    k_mv_expand_mfma<NR><<<nl, kMvWG, lds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, Dm, lm, X, ldx, t, Y, ldy);
+  else
+    k_mv_expand<NR><<<nl, kMvWG, lds, s>>>(p.leaves, p.lvls, p.ent, p.L, p.KM, Dm, lm, X, ldx, t, Y, ldy);
   HIP_CHECK(hipGetLastError());
 }
 
