@@ -568,7 +568,8 @@ void HodlrEngine::matvec_enqueue(const double* dX, int64_t ldx, int nrhs, double
 // W(rows of internal depth-j nodes, 0:ncols) -= Q C M^{-1} V^T W, with
 //   M = [[I, V_l^T Q_l], [V_r^T Q_r, I]] and C swapping the two halves.
 template <class F>
-void HodlrEngine::correction_t(HFactor<F>& fa, int j, F* W, int64_t ldw, int ncols) {
+void HodlrEngine::correction_t(HFactor<F>& fa, int j, F* W, int64_t ldw, int ncols, const kern::NodeTask* cap,
+                               const kern::BGemm* capops) {
   HParents& P = par[j];
   const HLevel& C = lv[j + 1];
   const int np = (int)P.ids.size();
@@ -591,11 +592,17 @@ void HodlrEngine::correction_t(HFactor<F>& fa, int j, F* W, int64_t ldw, int nco
     up.push_back(SegNN{r.start, (int64_t)p * tst, r.start, (int32_t)r.len, rr, ncols, 0});
   }
   kern::seg_gemm_tn<F>(tn, fa.V[j + 1], N, W, ldw, fa.tvec.get(), mm, F(1), F(0), ws_, stream);
-  kern::seg_gemm_nn<F>(mn, fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(0), stream);
-  if constexpr (kRefineBlocks<F>) {
-    // tvec <- tvec - M coef, coef += M^{-1} tvec
-    kern::seg_gemm_nn<F>(mn, fa.Mmat[j].get(), mm, fa.coef.get(), mm, fa.tvec.get(), mm, F(-1), F(1), stream);
-    kern::seg_gemm_nn<F>(mn, fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(1), stream);
+  if (cap) {
+    // the refined capacitance solves as node tasks (one workgroup per
+    // parent, its three products in order): one launch instead of three
+    kern::node_tasks(cap, np, capops, stream);
+  } else {
+    kern::seg_gemm_nn<F>(mn, fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(0), stream);
+    if constexpr (kRefineBlocks<F>) {
+      // tvec <- tvec - M coef, coef += M^{-1} tvec
+      kern::seg_gemm_nn<F>(mn, fa.Mmat[j].get(), mm, fa.coef.get(), mm, fa.tvec.get(), mm, F(-1), F(1), stream);
+      kern::seg_gemm_nn<F>(mn, fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(1), stream);
+    }
   }
   kern::seg_gemm_nn<F>(up, Q, N, fa.coef.get(), mm, W, ldw, F(-1), F(1), stream);
 }
@@ -829,6 +836,36 @@ void HodlrEngine::check_fused(HFactor<F>& fa) {
   }
 }
 
+// Every parent depth's refined capacitance solves of an FP64 apply,
+// coef = M^{-1} t, t -= M coef, coef += M^{-1} t, as node tasks (one
+// workgroup per parent running its three products in order): one launch per
+// depth instead of three segmented products, each of which was a ~15 us
+// launch at rank 32 and 16 columns whatever the number of parents.  tvec and
+// coef must already hold np x mmax x ncols values at every depth.
+static void cap_program(const std::vector<HParents>& par, const HFactor<double>& fa, int ncols,
+                        std::vector<kern::BGemm>& ops, std::vector<kern::NodeTask>& tasks, std::vector<int>& first) {
+  ops.clear();
+  tasks.clear();
+  first.assign(std::max<size_t>(par.size(), 1), 0);
+  for (int j = (int)par.size() - 1; j >= 0; --j) {
+    const HParents& P = par[j];
+    const int mm = P.mmax;
+    const int64_t tst = (int64_t)mm * ncols;
+    first[j] = (int)tasks.size();
+    for (size_t p = 0; p < P.ids.size(); ++p) {
+      const double* Mi = fa.Minv[j].get() + (int64_t)p * mm * mm;
+      const double* Mm = fa.Mmat[j].get() + (int64_t)p * mm * mm;
+      double* t = fa.tvec.get() + (int64_t)p * tst;
+      double* cf = fa.coef.get() + (int64_t)p * tst;
+      const int ms = P.msize[p];
+      tasks.push_back(kern::NodeTask{(int32_t)ops.size(), 3});
+      ops.push_back(kern::BGemm{Mi, t, cf, ms, ncols, ms, mm, mm, mm, 0, 0, 1.0, 0.0});
+      ops.push_back(kern::BGemm{Mm, cf, t, ms, ncols, ms, mm, mm, mm, 0, 0, -1.0, 1.0});
+      ops.push_back(kern::BGemm{Mi, t, cf, ms, ncols, ms, mm, mm, mm, 0, 0, 1.0, 1.0});
+    }
+  }
+}
+
 template <class F>
 void HodlrEngine::apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int ncols, F* X,
                                   int64_t ldx) {
@@ -840,6 +877,25 @@ void HodlrEngine::apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int n
   }
   if constexpr (kRefineBlocks<F>) fa.rtmp.reserve((size_t)N * ncols);
   leaf_solve_t<F>(fa, B, ldb, ncols, X, ldx, fa.rtmp.get());
+  if constexpr (kRefineBlocks<F>) {
+    // the capacitance solves as node tasks, all depths' descriptors staged
+    // in one copy (the recorded solve graph runs the same tasks)
+    const int dmax = tree.max_depth;
+    size_t need = 1;
+    for (int j = 0; j < dmax; ++j) need = std::max(need, par[j].ids.size() * (size_t)par[j].mmax * ncols);
+    fa.tvec.reserve(need);
+    fa.coef.reserve(need);
+    std::vector<kern::BGemm> ops;
+    std::vector<kern::NodeTask> tasks;
+    std::vector<int> first;
+    cap_program(par, fa, ncols, ops, tasks, first);
+    StageGroup g;
+    const int io = g.add(ops), it = g.add(tasks);
+    g.put(stream);
+    for (int j = dmax - 1; j >= 0; --j)
+      correction_t<F>(fa, j, X, ldx, ncols, g.get<kern::NodeTask>(it) + first[j], g.get<kern::BGemm>(io));
+    return;
+  }
   for (int j = tree.max_depth - 1; j >= 0; --j) correction_t<F>(fa, j, X, ldx, ncols);
 }
 
@@ -974,7 +1030,16 @@ void HodlrEngine::plan_solve(HFactor<F>& fa, int ncols) {
   }
   fa.tvec.reserve(tv);
   fa.coef.reserve(tv);
-  if constexpr (kRefineBlocks<F>) fa.rtmp.reserve((size_t)N * ncols);
+  if constexpr (kRefineBlocks<F>) {
+    fa.rtmp.reserve((size_t)N * ncols);
+    std::vector<kern::BGemm> ops;
+    std::vector<kern::NodeTask> tasks;
+    cap_program(par, fa, ncols, ops, tasks, sp.cap_first);
+    if (!ops.empty()) {
+      sp.cap_ops.upload(ops, stream);
+      sp.cap_tasks.upload(tasks, stream);
+    }
+  }
   sp.part.reserve(slots * 4096);
   HIP_CHECK(hipStreamSynchronize(stream));
   // record
@@ -991,12 +1056,10 @@ void HodlrEngine::plan_solve(HFactor<F>& fa, int ncols) {
     const F* Q = fa.Wall.get() + colofs[j + 1] * N;
     kern::launch_tn<F>(sp.tn[j], fa.V[j + 1], N, fa.x.get(), N, fa.tvec.get(), mm, F(1), F(0), sp.part.get(),
                        stream);
-    kern::launch_nn<F>(sp.mn[j], fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(0), stream);
-    if constexpr (kRefineBlocks<F>) {
-      kern::launch_nn<F>(sp.mn[j], fa.Mmat[j].get(), mm, fa.coef.get(), mm, fa.tvec.get(), mm, F(-1), F(1),
-                         stream);
-      kern::launch_nn<F>(sp.mn[j], fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(1), stream);
-    }
+    if constexpr (kRefineBlocks<F>)
+      kern::node_tasks(sp.cap_tasks.get() + sp.cap_first[j], (int)par[j].ids.size(), sp.cap_ops.get(), stream);
+    else
+      kern::launch_nn<F>(sp.mn[j], fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(0), stream);
     kern::launch_nn<F>(sp.up[j], Q, N, fa.coef.get(), mm, fa.x.get(), N, F(-1), F(1), stream);
   }
   HIP_CHECK(hipStreamEndCapture(stream, &sp.graph.g));

@@ -110,6 +110,10 @@ struct SolvePlan {
   std::vector<kern::NNPlan> mn, up;
   std::vector<int64_t> mm;  // per parent depth: capacitance leading dimension
   DevBuf<F> part;
+  // FP64: every parent depth's capacitance solves as node tasks
+  DevBuf<kern::BGemm> cap_ops;
+  DevBuf<kern::NodeTask> cap_tasks;
+  std::vector<int> cap_first;  // per parent depth: its first task
   GraphExec graph;
   // device addresses of the factorisation's buffers that the graph reads
   // and writes, as recorded: eager applies with more columns may reallocate
@@ -121,6 +125,7 @@ struct SolvePlan {
     mn.clear();
     up.clear();
     mm.clear();
+    cap_first.clear();
     part.release();
     bufs.clear();
     ncols = 0;
@@ -256,7 +261,10 @@ class HodlrEngine {
                    const double* host_omega);
   int group_kcap(int c0, int c1) const;
   template <class F> void factor_t(HFactor<F>& fa);
-  template <class F> void correction_t(HFactor<F>& fa, int j, F* W, int64_t ldw, int ncols);
+  // cap / capops: the depth's capacitance solves as prebuilt node tasks
+  // (FP64 applies); null: three segmented products
+  template <class F> void correction_t(HFactor<F>& fa, int j, F* W, int64_t ldw, int ncols,
+                                       const kern::NodeTask* cap = nullptr, const kern::BGemm* capops = nullptr);
   template <class F> void apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int ncols, F* X,
                                           int64_t ldx);
   template <class F> void leaf_apply_t(const F* Dm, const F* B, int64_t ldb, int ncols, F* Y,
