@@ -269,8 +269,9 @@ def test_state_and_argument_errors():
                                          (1000, 60, 4), (5000, 256, 5), (4000, 512, 2)])
 def test_fused_matvec_matches_dense(n, leaf, nrhs):
     """The fused leaf-organised matvec (1, 2 or 4 columns per pass) equals the
-    dense reconstruction of the operator, uneven trees included; leaves above
-    256 rows take the projection's several-rows-per-thread loop."""
+    dense reconstruction of the operator, uneven trees included; leaves of up
+    to 128 rows take the MFMA passes at every width, leaves above 256 rows
+    the projection's several-rows-per-thread loop."""
     x = kernels.sorted_uniform_points(n, 9)
     a = kernels.gaussian_points(x, 0.1, 1.0)
     o = hmat_amd.make_options(1e-10, rank=24, oversample=10, power_iters=1, exact_below=0,
@@ -551,12 +552,16 @@ def test_sketch_width_64():
     assert rel(h.solve(b), np.linalg.solve(ad, b)) < 1e-10
 
 
-@pytest.mark.parametrize("ncols,leaf", [(5, 64), (12, 64), (16, 64), (20, 64), (12, 300), (16, 512)])
+@pytest.mark.parametrize("ncols,leaf", [(5, 64), (12, 64), (16, 64), (20, 64), (40, 64), (16, 100), (7, 200),
+                                         (12, 300), (16, 512)])
 def test_fused_matvec_wide_passes(ncols, leaf):
     """Column blocks go through the fused matvec in passes of 16, 8, 4, 2 or
-    1 columns (20 columns take the segmented path; leaves above 256 rows
-    project by shuffles and take narrower passes): every column equals its
-    single-column product and the dense reconstruction's to rounding."""
+    1 columns, up to two passes of the widest width (20 columns run as 16 +
+    4, 40 take the segmented path); leaves of up to 256 rows (ragged ones
+    included) take the MFMA projection and expansion from four columns,
+    leaves above 256 rows project by shuffles and take narrower passes.
+    Every column equals its single-column product and the dense
+    reconstruction's to rounding."""
     n = 4096
     x = kernels.sorted_uniform_points(n, 31)
     o = hmat_amd.make_options(0.0, rank=16, fixed_rank=True, oversample=10, power_iters=1, exact_below=0,
