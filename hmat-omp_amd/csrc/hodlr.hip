@@ -836,19 +836,25 @@ void HodlrEngine::check_fused(HFactor<F>& fa) {
   }
 }
 
-// Every parent depth's refined capacitance solves of an FP64 apply,
-// coef = M^{-1} t, t -= M coef, coef += M^{-1} t, as node tasks (one
-// workgroup per parent running its three products in order): one launch per
-// depth instead of three segmented products, each of which was a ~15 us
-// launch at rank 32 and 16 columns whatever the number of parents.  tvec and
-// coef must already hold np x mmax x ncols values at every depth.
+// The refined capacitance solves of an FP64 apply, coef = M^{-1} t,
+// t -= M coef, coef += M^{-1} t, as node tasks (one workgroup per parent
+// running its three products in order) at the parent depths where that is
+// the cheaper form: one launch per depth instead of three segmented
+// products, each of which is a ~15 us launch at rank 32 and 16 columns
+// whatever the number of parents.  A workgroup's products run on the vector
+// units, so with many parents or wide column blocks the segmented MFMA
+// products win (64 columns over 7481 parents of rank 24: 1.34 ms per depth
+// as node tasks); those depths get first[j] = -1.  tvec and coef must
+// already hold np x mmax x ncols values at every depth.
+constexpr int kCapTaskCols = 16, kCapTaskParents = 512;
 static void cap_program(const std::vector<HParents>& par, const HFactor<double>& fa, int ncols,
                         std::vector<kern::BGemm>& ops, std::vector<kern::NodeTask>& tasks, std::vector<int>& first) {
   ops.clear();
   tasks.clear();
-  first.assign(std::max<size_t>(par.size(), 1), 0);
+  first.assign(std::max<size_t>(par.size(), 1), -1);
   for (int j = (int)par.size() - 1; j >= 0; --j) {
     const HParents& P = par[j];
+    if (ncols > kCapTaskCols || P.ids.size() > (size_t)kCapTaskParents) continue;
     const int mm = P.mmax;
     const int64_t tst = (int64_t)mm * ncols;
     first[j] = (int)tasks.size();
@@ -878,8 +884,9 @@ void HodlrEngine::apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int n
   if constexpr (kRefineBlocks<F>) fa.rtmp.reserve((size_t)N * ncols);
   leaf_solve_t<F>(fa, B, ldb, ncols, X, ldx, fa.rtmp.get());
   if constexpr (kRefineBlocks<F>) {
-    // the capacitance solves as node tasks, all depths' descriptors staged
-    // in one copy (the recorded solve graph runs the same tasks)
+    // the capacitance solves as node tasks where cap_program takes that
+    // form, all depths' descriptors staged in one copy (the recorded solve
+    // graph runs the same tasks)
     const int dmax = tree.max_depth;
     size_t need = 1;
     for (int j = 0; j < dmax; ++j) need = std::max(need, par[j].ids.size() * (size_t)par[j].mmax * ncols);
@@ -893,7 +900,8 @@ void HodlrEngine::apply_inverse_t(HFactor<F>& fa, const F* B, int64_t ldb, int n
     const int io = g.add(ops), it = g.add(tasks);
     g.put(stream);
     for (int j = dmax - 1; j >= 0; --j)
-      correction_t<F>(fa, j, X, ldx, ncols, g.get<kern::NodeTask>(it) + first[j], g.get<kern::BGemm>(io));
+      correction_t<F>(fa, j, X, ldx, ncols, first[j] >= 0 ? g.get<kern::NodeTask>(it) + first[j] : nullptr,
+                      g.get<kern::BGemm>(io));
     return;
   }
   for (int j = tree.max_depth - 1; j >= 0; --j) correction_t<F>(fa, j, X, ldx, ncols);
@@ -1056,10 +1064,19 @@ void HodlrEngine::plan_solve(HFactor<F>& fa, int ncols) {
     const F* Q = fa.Wall.get() + colofs[j + 1] * N;
     kern::launch_tn<F>(sp.tn[j], fa.V[j + 1], N, fa.x.get(), N, fa.tvec.get(), mm, F(1), F(0), sp.part.get(),
                        stream);
-    if constexpr (kRefineBlocks<F>)
+    bool tasks = false;
+    if constexpr (kRefineBlocks<F>) tasks = sp.cap_first[j] >= 0;
+    if (tasks) {
       kern::node_tasks(sp.cap_tasks.get() + sp.cap_first[j], (int)par[j].ids.size(), sp.cap_ops.get(), stream);
-    else
+    } else {
       kern::launch_nn<F>(sp.mn[j], fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(0), stream);
+      if constexpr (kRefineBlocks<F>) {
+        kern::launch_nn<F>(sp.mn[j], fa.Mmat[j].get(), mm, fa.coef.get(), mm, fa.tvec.get(), mm, F(-1), F(1),
+                           stream);
+        kern::launch_nn<F>(sp.mn[j], fa.Minv[j].get(), mm, fa.tvec.get(), mm, fa.coef.get(), mm, F(1), F(1),
+                           stream);
+      }
+    }
     kern::launch_nn<F>(sp.up[j], Q, N, fa.coef.get(), mm, fa.x.get(), N, F(-1), F(1), stream);
   }
   HIP_CHECK(hipStreamEndCapture(stream, &sp.graph.g));
